@@ -1,0 +1,169 @@
+/* tensormol_amd.h -- C ABI of libtmamd.so, the MI355X (gfx950) BP+EE engine.
+ *
+ * Conventions
+ *   - Every pointer argument is a device pointer unless documented as a host
+ *     array; sizes are element counts; `stream` is a hipStream_t.
+ *   - Calls only enqueue work on `stream` (no host synchronisation), so they
+ *     can be captured into a HIP graph.  The exception is tm_cell_build, which
+ *     clears its counters with hipMemsetAsync.
+ *   - Return value: 0 on success, negative on error; tm_last_error() then
+ *     holds a message (thread-local).
+ *   - Packed atoms: molecule m owns atoms [mol_start[m], mol_start[m+1]);
+ *     mol[i] is the molecule of atom i; elem[i] is the index of atom i's
+ *     element in the model's ascending element list; pos is float64 [N,3] in A;
+ *     box is float64 [B,3] orthorhombic edges (NULL = open boundaries).
+ *   - Neighbour rows use an ELL layout with stride ldr (4.6 A list) and lda
+ *     (3.1 A sub-list), see tm_nl_build.
+ *
+ * Reference paths below are relative to the TensorMol repository.
+ */
+#ifndef TENSORMOL_AMD_H
+#define TENSORMOL_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+int tm_version(void);
+const char* tm_last_error(void);
+int tm_zero_i32(int n, int* p, void* stream);
+
+/* ---- neighbour lists ---------------------------------------------------
+ * Replaces MolEmb.Make_NListNaive (C_API/MolEmb.cpp:1180-1247) and the pair /
+ * triple assembly of NeighborList.buildPairsAndTriples and
+ * NeighborListSet.buildPairsAndTriplesWithEleIndex
+ * (TensorMol/ForceModifiers/Neighbors.py:117-201, 344-423).
+ *
+ * tm_cell_build bins atoms into an nx*ny*nz grid of cells with origin o and
+ * edge c (periodic: positions wrapped; open: clamped, flag 2 raised) and
+ * sorts each cell's atoms by index.
+ */
+int tm_cell_build(int N, const double* pos, double ox, double oy, double oz, double cx, double cy,
+                  double cz, int nx, int ny, int nz, int periodic, int* cell_of, int* cell_count,
+                  int* cell_start, int* cell_fill, int* cell_atoms, int* flags, void* stream);
+
+/* Builds, for every atom i, the full (DoPerms=1) list of neighbours j with
+ * squared distance s < s_r and marks those with s < s_a.  s_r/s_a are the
+ * first doubles failing sqrt(s) + 1e-13 < rc (MolEmb.cpp:1214-1218), so the
+ * sets are bit-exact.  Rows are sorted by (element of j, j).  Candidates come
+ * from the molecule (cell_atoms == NULL) or from the 27-cell stencil.
+ * Outputs: nbr[i*ldr+n], ncnt[i], rseg[i*(nele+1)+e], ang[i*lda+a] (position
+ * in row i), acnt[i], aseg[i*(nele+1)+e], rev[i*ldr+n] (mirror slot), flags
+ * (0: row overflow, 1: sub-list overflow, 3: non-finite input). */
+int tm_nl_build(int N, const double* pos, const int* elem, const int* mol, const int* mol_start,
+                const double* box, const int* cell_start, const int* cell_atoms,
+                const int* cell_of, int nx, int ny, int nz, int periodic, int nele, double s_r,
+                double s_a, int ldr, int lda, int* nbr, int* ncnt, int* rseg, int* ang, int* acnt,
+                int* aseg, int* rev, int* flags, void* stream);
+
+/* Materialises the sorted triples of every centre (Neighbors.py:369-420):
+ * out_jkpr[4*t] = (j, k, element-pair index, rank in its (i, pair) group),
+ * ordered by (i, pair, k, j); trip_start[i] = exclusive prefix of
+ * acnt[i]*(acnt[i]-1)/2 (int64). */
+int tm_triples(int N, int ldr, int lda, int nele, const int* nbr, const int* ang, const int* acnt,
+               const int* aseg, const long* trip_start, int* out_jkpr, void* stream);
+
+/* ---- descriptors -------------------------------------------------------
+ * TFSymRSet_Linear_WithEle + TFSymASet_Linear_WithEle concatenated as in
+ * TFSymSet_Scattered_Linear_WithEle (TensorMol/TFDescriptors/RawSymFunc.py:
+ * 1696-1755, 868-962, 2223-2265).  G float32 [N, ldg], column l*nr + s
+ * (radial) then nele*nr + p*64 + t*8 + s (angular).  Host arrays: rs_r[nr],
+ * rs_a[8], thetas[8]. */
+int tm_sym_forward(int N, const double* pos, const double* box, const int* mol, const int* elem,
+                   const int* nbr, const int* ncnt, const int* rseg, const int* ang,
+                   const int* acnt, const int* aseg, int ldr, int lda, int nele, int nr,
+                   float eta, float zeta, float rr, float ra, const float* rs_r,
+                   const float* rs_a, const float* thetas, float* G, int ldg, void* stream);
+
+/* Descriptor adjoint (replaces tf.gradients through the descriptor graph,
+ * TFMolInstanceDirect.py:5761): with g_i = gE_i + u_i * gQ_i (gQ may be NULL),
+ * writes wslot[(i*ldr+n)*4 + 0..2] = d(sum_f g_if G_if)/d(x_j - x_i). */
+int tm_sym_adjoint(int N, const double* pos, const double* box, const int* mol, const int* elem,
+                   const int* nbr, const int* ncnt, const int* rseg, const int* ang,
+                   const int* acnt, const int* aseg, int ldr, int lda, int nele, int nr,
+                   float eta, float zeta, float rr, float ra, const float* rs_r,
+                   const float* rs_a, const float* thetas, const float* gE, const float* gQ,
+                   const float* u, int ldg, float* wslot, void* stream);
+
+/* grad_a = sum_n (w[rev(a,n)] - w[(a,n)]) + extra_a (Ha/A);
+ * force = -scale * grad (scale = JOULEPERHARTREE gives J/mol/A as returned by
+ * TFMolManage.EvalBPDirectEEUpdateSingle, TFMolManage.py:1320). */
+int tm_force_gather(int N, int ldr, const int* ncnt, const int* rev, const float* wslot,
+                    const double* extra, double scale, double* grad, double* force,
+                    void* stream);
+
+/* ---- atomic networks ---------------------------------------------------
+ * Energy / charge branches of energy_inference and dipole_inference
+ * (TFMolInstanceDirect.py:5164-5212, 5219-5272) with act = log(1+e^{a z})/a
+ * (Util.py:200-201) and their input gradient.
+ *
+ * tm_mlp_f32: plain FP32 path, one (net, element) segment of M atoms
+ * perm[0..M); W = float32 [W1, b1, ..., W_nh, b_nh, wo, bo]; writes y[atom]
+ * and dG[atom*ldg + f]; ws holds (2*nh+2)*M*max(hid) floats. */
+int tm_mlp_f32(int M, int D, int nh, const int* hid, const float* G, int ldg, const int* perm,
+               const float* W, float alpha, float* y, float* dG, float* ws, void* stream);
+
+/* tm_mlp_mfma: fused BF16/FP16 MFMA path, FP32 accumulation, all (net,
+ * element) segments in one launch.  seg (device int32 [nseg*4]) =
+ * (net, element, first row, rows); wpack: weights packed by
+ * tensormol_amd.mlp_mfma; mode 0 = bf16, 1 = fp16, 2 = bf16 hi/lo split.
+ * y[net*N + atom], dG[net][atom*ldg + f]. */
+int tm_mlp_mfma(int N, int D, int H, int nseg, const int* seg, int ldg, const float* G,
+                const int* perm, const void* wpack, const float* bias, float alpha, int mode,
+                float* y, float* dG, int* flags, void* ws, void* stream);
+int tm_mlp_mfma_pack_info(int D, int H, int nh, int mode, long* out);
+
+/* ---- electrostatics ----------------------------------------------------
+ * Neutral charges q = qraw - mean_m(qraw) and dipole terms
+ * (TFMolInstanceDirect.py:5274-5279); xq = float32 (x, y, z, q). */
+int tm_charges(int N, const int* mol, const double* qmean, const float* qraw, const double* pos,
+               float* xq, double* q, double* qx, void* stream);
+
+/* DSF-elu Coulomb (RawSymFunc.py:1307-1359) and Grimme-C6 vdW
+ * (RawSymFunc.py:1361-1414) over all pairs within rc_ee, evaluated from the
+ * cell grid (or the molecule).  consts (host f64[7]) = alpha', R_sr, R_lr, ZZ,
+ * YY, elu_shift, elu_alpha in bohr units; c6/r0 host f64[nele].  Outputs
+ * phi[i] = sum_j K(r_ij) q_j, grad (dE/dx at fixed q, Ha/A), eatom[2i] =
+ * q_i phi_i / 2, eatom[2i+1] = vdW energy share. */
+int tm_coulomb(int N, const float* xq, const int* elem, const int* mol, const int* mol_start,
+               const double* box, const int* cell_start, const int* cell_atoms, const int* cell_of,
+               int nx, int ny, int nz, int periodic, int reach, const double* consts,
+               double rc_ee, double width, int add_ecc, int has_vdw, int nele, const double* c6,
+               const double* r0, double* phi, double* grad, double* eatom, void* stream);
+
+/* u_i = phi_i - mean_m(phi): dE_cc/dqraw_i through the neutrality step. */
+int tm_charge_upstream(int N, const int* mol, const double* phi, const double* phimean, float* u,
+                       void* stream);
+
+/* Per-molecule column sums (or means, mean != 0) of an [N, C] array:
+ * the sparse_reduce_sum / reduce_sum per molecule of RawSymFunc.py:1354-1359
+ * and TFMolInstanceDirect.py:5212, in a fixed FP64 order. */
+int tm_mol_sum_f32(int B, const int* mol_start, int C, const float* in, int ldin, double* out,
+                   int mean, void* stream);
+int tm_mol_sum_f64(int B, const int* mol_start, int C, const double* in, int ldin, double* out,
+                   int mean, void* stream);
+
+/* ---- integrators (TensorMol/Simulations/SimpleMD.py) --------------------
+ * VelocityVerletStep (SimpleMD.py:14-38) split into drift and kick;
+ * NoseThermostat.step (SimpleMD.py:104-129) with one eta per molecule;
+ * FIRE (Bitzek et al. 2006; no reference counterpart, the optimiser of the
+ * serving workload). */
+int tm_vv_drift(int N, double dt, const double* v, const double* a, double* x, void* stream);
+int tm_vv_kick(int N, double dt, const double* f, const double* m, double* v, double* a,
+               double* ke, void* stream);
+int tm_nh_drift(int N, double dt, const int* mol, const double* eta, const double* m, double* x,
+                const double* v, const double* a, double* vh, double* ke2, void* stream);
+int tm_nh_eta(int B, const int* mol_start, double dt, const double* kT, const double* Q,
+              const double* ke2sum, double* eta, void* stream);
+int tm_nh_kick(int N, double dt, const int* mol, const double* eta, const double* f,
+               const double* m, const double* vh, double* v, double* a, double* ke, void* stream);
+int tm_fire_dots(int N, const double* f, const double* v, double* dots, void* stream);
+int tm_fire_update(int N, int B, const int* mol, const double* dots_sum, double* state,
+                   double* mix, const double* f, const double* m, double* v, double* x,
+                   double dtmax, int nmin, double finc, double fdec, double alpha0,
+                   double falpha, double maxstep, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

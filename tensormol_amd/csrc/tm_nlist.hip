@@ -1,0 +1,422 @@
+// Neighbour lists for the BP+EE path.
+//
+// Replaces MolEmb.Make_NListNaive (C_API/MolEmb.cpp:1180-1247) and the Python
+// pair/triple assembly of NeighborList.buildPairsAndTriples and
+// NeighborListSet.buildPairsAndTriplesWithEleIndex
+// (TensorMol/ForceModifiers/Neighbors.py:117-201, 344-423).
+//
+// Pair predicate: the reference keeps j when sqrt(dx*dx+dy*dy+dz*dz)+1e-13 < rc
+// in float64.  With correctly rounded sqrt and add that predicate is monotone
+// in s = (dx*dx+dy*dy)+dz*dz, so the host computes the first double s* that
+// fails it and the kernel tests s < s* with s evaluated without contraction:
+// the same set, bit for bit, and no dependence on the device sqrt.
+//
+// Output (ELL rows, one per centre atom, stride ldr):
+//   nbr[i*ldr + n]   neighbour atom j (global packed index), sorted by
+//                    (element index of j, j) = the (m,i,l,j) order of trpE
+//   ncnt[i]          row length
+//   rseg[i*(nele+1)+e] start of element e's segment in the row (last = ncnt)
+//   ang[i*lda + a]   position (in row i) of the a-th neighbour closer than
+//                    the angular cutoff; the sub-list keeps the row order
+//   acnt[i], aseg[i*(nele+1)+e]  as above for the sub-list
+//   rev[i*ldr + n]   slot (j*ldr + n') of the mirror entry j -> i
+//
+// Candidate atoms come either from the atom's molecule (batches of small
+// molecules) or from a cell grid (one large open or periodic system).
+#include "tm_common.h"
+
+namespace {
+
+constexpr int NL_WAVES = 4;           // waves per block, one centre per wave
+constexpr int NL_LDS_CAP = 256;       // candidate buffer per wave
+
+struct CellGrid {
+  const int* cell_start;   // ncell+1, offsets into cell_atoms
+  const int* cell_atoms;   // atoms sorted by cell
+  const int* cell_of;      // cell of each atom
+  int nc[3];               // cells per axis
+  int periodic;            // 1: wrap stencil, 0: clamp
+};
+
+__device__ __forceinline__ double sq_dist_nocontract(const double* __restrict__ pos, int i, int j,
+                                                     double Lx, double Ly, double Lz) {
+#pragma clang fp contract(off)
+  double dx = tm_min_image(pos[3 * i + 0] - pos[3 * j + 0], Lx);
+  double dy = tm_min_image(pos[3 * i + 1] - pos[3 * j + 1], Ly);
+  double dz = tm_min_image(pos[3 * i + 2] - pos[3 * j + 2], Lz);
+  double s = dx * dx + dy * dy;
+  s = s + dz * dz;
+  return s;
+}
+
+// Visit the candidate ranges of atom i.  Calls f(lo, hi, mapped) for each
+// contiguous range of candidate positions; mapped != 0 means positions index
+// cell_atoms, else they are atom indices.  The loop structure is uniform
+// across the wave (depends only on i).
+template <class F>
+__device__ __forceinline__ void for_candidate_ranges(int i, const int* __restrict__ mol,
+                                                     const int* __restrict__ mol_start,
+                                                     const CellGrid& g, int reach, F&& f) {
+  if (g.cell_atoms == nullptr) {
+    int m = mol[i];
+    f(mol_start[m], mol_start[m + 1], 0);
+    return;
+  }
+  int c = g.cell_of[i];
+  int cx = c % g.nc[0];
+  int cy = (c / g.nc[0]) % g.nc[1];
+  int cz = c / (g.nc[0] * g.nc[1]);
+  int k[3] = {reach, reach, reach};
+  bool full[3];
+  for (int d = 0; d < 3; ++d) full[d] = g.periodic ? (2 * k[d] + 1 >= g.nc[d]) : false;
+  int zlo = full[2] ? 0 : cz - k[2], zhi = full[2] ? g.nc[2] - 1 : cz + k[2];
+  int ylo = full[1] ? 0 : cy - k[1], yhi = full[1] ? g.nc[1] - 1 : cy + k[1];
+  if (!g.periodic) {
+    zlo = max(zlo, 0); zhi = min(zhi, g.nc[2] - 1);
+    ylo = max(ylo, 0); yhi = min(yhi, g.nc[1] - 1);
+  }
+  for (int zz = zlo; zz <= zhi; ++zz) {
+    int z = zz;
+    if (g.periodic) z = ((zz % g.nc[2]) + g.nc[2]) % g.nc[2];
+    for (int yy = ylo; yy <= yhi; ++yy) {
+      int y = yy;
+      if (g.periodic) y = ((yy % g.nc[1]) + g.nc[1]) % g.nc[1];
+      int base = g.nc[0] * (y + g.nc[1] * z);
+      if (full[0] || (!g.periodic && (cx - k[0] <= 0 && cx + k[0] >= g.nc[0] - 1))) {
+        f(g.cell_start[base], g.cell_start[base + g.nc[0]], 1);
+      } else if (!g.periodic) {
+        int xlo = max(cx - k[0], 0), xhi = min(cx + k[0], g.nc[0] - 1);
+        f(g.cell_start[base + xlo], g.cell_start[base + xhi + 1], 1);
+      } else {
+        int xlo = cx - k[0], xhi = cx + k[0];
+        if (xlo < 0) {
+          f(g.cell_start[base + xlo + g.nc[0]], g.cell_start[base + g.nc[0]], 1);
+          f(g.cell_start[base], g.cell_start[base + xhi + 1], 1);
+        } else if (xhi >= g.nc[0]) {
+          f(g.cell_start[base + xlo], g.cell_start[base + g.nc[0]], 1);
+          f(g.cell_start[base], g.cell_start[base + xhi - g.nc[0] + 1], 1);
+        } else {
+          f(g.cell_start[base + xlo], g.cell_start[base + xhi + 1], 1);
+        }
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(NL_WAVES * 64)
+nl_rows_kernel(int N, const double* __restrict__ pos, const int* __restrict__ elem,
+               const int* __restrict__ mol, const int* __restrict__ mol_start,
+               const double* __restrict__ box, CellGrid grid, int nele,
+               double s_r, double s_a, int ldr, int lda,
+               int* __restrict__ nbr, int* __restrict__ ncnt, int* __restrict__ rseg,
+               int* __restrict__ ang, int* __restrict__ acnt, int* __restrict__ aseg,
+               int* __restrict__ flags) {
+  __shared__ unsigned long long keys[NL_WAVES][NL_LDS_CAP];
+  __shared__ unsigned long long sorted[NL_WAVES][NL_LDS_CAP];
+  const int w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * NL_WAVES + w;
+  const bool active = i < N;   // wave-uniform; inactive waves still reach every barrier
+  int count = 0;               // wave-uniform
+  if (active) {
+    double Lx = 0, Ly = 0, Lz = 0;
+    const int mi = mol[i];
+    if (box != nullptr) {
+      Lx = box[3 * mi]; Ly = box[3 * mi + 1]; Lz = box[3 * mi + 2];
+    }
+    auto visit = [&](int lo, int hi, int mapped) {
+      for (int base = lo; base < hi; base += 64) {
+        int p = base + lane;
+        bool ok = false;
+        int j = -1;
+        bool in_a = false;
+        if (p < hi) {
+          j = mapped ? grid.cell_atoms[p] : p;
+          if (j != i && (!mapped || mol[j] == mi)) {
+            double s = sq_dist_nocontract(pos, i, j, Lx, Ly, Lz);
+            ok = s < s_r;
+            in_a = s < s_a;
+          }
+        }
+        unsigned long long bal = __ballot(ok);
+        int before = __popcll(bal & ((1ull << lane) - 1ull));
+        if (ok) {
+          int slot = count + before;
+          if (slot < NL_LDS_CAP)
+            keys[w][slot] = ((unsigned long long)(unsigned)elem[j] << 40) |
+                            ((unsigned long long)(in_a ? 1u : 0u) << 32) | (unsigned)j;
+        }
+        count += __popcll(bal);
+      }
+    };
+    for_candidate_ranges(i, mol, mol_start, grid, 1, visit);
+    if (count > NL_LDS_CAP || count > ldr) {
+      if (lane == 0) atomicOr(&flags[TM_FLAG_NB_OVERFLOW], 1);
+      count = min(count, min(NL_LDS_CAP, ldr));
+    }
+  }
+  __syncthreads();
+  // rank sort by (element, atom index); the in_a bit sits below the element
+  // and above the index and never decides the order of distinct atoms
+  for (int a = lane; a < count; a += 64) {
+    unsigned long long ka = keys[w][a];
+    unsigned long long ord_a = (ka >> 40 << 32) | (ka & 0xffffffffull);
+    int rank = 0;
+    for (int b = 0; b < count; ++b) {
+      unsigned long long kb = keys[w][b];
+      unsigned long long ord_b = (kb >> 40 << 32) | (kb & 0xffffffffull);
+      rank += ord_b < ord_a;
+    }
+    sorted[w][rank] = ka;
+  }
+  __syncthreads();
+  if (!active) return;   // no barriers below
+  long rowr = (long)i * ldr;
+  long rowa = (long)i * lda;
+  int acount = 0;
+  for (int base = 0; base < count; base += 64) {
+    int a = base + lane;
+    bool have = a < count;
+    unsigned long long k = have ? sorted[w][a] : 0ull;
+    if (have) nbr[rowr + a] = (int)(k & 0xffffffffull);
+    bool isang = have && ((k >> 32) & 1ull);
+    unsigned long long bal = __ballot(isang);
+    int before = __popcll(bal & ((1ull << lane) - 1ull));
+    if (isang) {
+      int slot = acount + before;
+      if (slot < lda) ang[rowa + slot] = a;
+    }
+    acount += __popcll(bal);
+  }
+  if (acount > lda) {
+    if (lane == 0) atomicOr(&flags[TM_FLAG_ANG_OVERFLOW], 1);
+    acount = lda;
+  }
+  if (lane == 0) { ncnt[i] = count; acnt[i] = acount; }
+  // element segment starts: number of entries with element < e
+  if (lane <= nele) {
+    int e = lane;
+    int cr = 0, ca = 0;
+    for (int a = 0; a < count; ++a) {
+      unsigned long long k = sorted[w][a];
+      int el = (int)(k >> 40);
+      if (el < e) { cr++; if ((k >> 32) & 1ull) ca++; }
+    }
+    if (ca > acount) ca = acount;
+    rseg[(long)i * (nele + 1) + e] = cr;
+    aseg[(long)i * (nele + 1) + e] = ca;
+  }
+}
+
+// Mirror slot of every entry: rev[i*ldr+n] = j*ldr + position of i in row j.
+__global__ void nl_rev_kernel(int N, int ldr, int nele, const int* __restrict__ elem,
+                              const int* __restrict__ nbr, const int* __restrict__ ncnt,
+                              const int* __restrict__ rseg, int* __restrict__ rev,
+                              int* __restrict__ flags) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i = (int)(t / ldr);
+  int n = (int)(t % ldr);
+  if (i >= N || n >= ncnt[i]) return;
+  int j = nbr[(long)i * ldr + n];
+  int e = elem[i];
+  int lo = rseg[(long)j * (nele + 1) + e];
+  int hi = rseg[(long)j * (nele + 1) + e + 1];
+  const int* row = nbr + (long)j * ldr;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (row[mid] < i) lo = mid + 1; else hi = mid;
+  }
+  if (lo < ncnt[j] && row[lo] == i) {
+    rev[(long)i * ldr + n] = (int)((long)j * ldr + lo);
+  } else {
+    rev[(long)i * ldr + n] = (int)((long)i * ldr + n);   // unmatched (truncated row)
+    atomicOr(&flags[TM_FLAG_NB_OVERFLOW], 2);
+  }
+}
+
+// ---------------------------------------------------------------- cell grid
+__global__ void cell_assign_kernel(int N, const double* __restrict__ pos, double ox, double oy,
+                                   double oz, double cx, double cy, double cz, int nx, int ny,
+                                   int nz, int periodic, int* __restrict__ cell_of,
+                                   int* __restrict__ cell_count, int* __restrict__ flags) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double p[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+  if (!isfinite(p[0]) || !isfinite(p[1]) || !isfinite(p[2])) {
+    atomicOr(&flags[TM_FLAG_NONFINITE], 1);
+    p[0] = p[1] = p[2] = 0.0;
+  }
+  double o[3] = {ox, oy, oz}, cs[3] = {cx, cy, cz};
+  int n[3] = {nx, ny, nz}, c[3];
+  for (int d = 0; d < 3; ++d) {
+    long q = (long)floor((p[d] - o[d]) / cs[d]);
+    if (periodic) {
+      q %= n[d];
+      if (q < 0) q += n[d];
+    } else if (q < 0 || q >= n[d]) {
+      atomicOr(&flags[TM_FLAG_CELL_OVERFLOW], 1);
+      q = q < 0 ? 0 : n[d] - 1;
+    }
+    c[d] = (int)q;
+  }
+  int cell = c[0] + nx * (c[1] + ny * c[2]);
+  cell_of[i] = cell;
+  atomicAdd(&cell_count[cell], 1);
+}
+
+// exclusive scan of cell_count into cell_start (single block, any length)
+__global__ void __launch_bounds__(1024) scan_kernel(int n, const int* __restrict__ in,
+                                                    int* __restrict__ out) {
+  __shared__ int part[1024];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < n; base += 1024) {
+    int idx = base + threadIdx.x;
+    int v = idx < n ? in[idx] : 0;
+    part[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+      int t = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
+      __syncthreads();
+      part[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (idx < n) out[idx] = carry + part[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += part[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[n] = carry;
+}
+
+__global__ void cell_fill_kernel(int N, const int* __restrict__ cell_of,
+                                 const int* __restrict__ cell_start, int* __restrict__ cell_fill,
+                                 int* __restrict__ cell_atoms) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int c = cell_of[i];
+  int slot = cell_start[c] + atomicAdd(&cell_fill[c], 1);
+  cell_atoms[slot] = i;
+}
+
+// order atoms inside each cell by index, so every later sweep is deterministic
+__global__ void cell_sort_kernel(int ncell, const int* __restrict__ cell_start,
+                                 int* __restrict__ cell_atoms) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  int lo = cell_start[c], hi = cell_start[c + 1];
+  for (int a = lo + 1; a < hi; ++a) {
+    int v = cell_atoms[a];
+    int b = a - 1;
+    while (b >= lo && cell_atoms[b] > v) {
+      cell_atoms[b + 1] = cell_atoms[b];
+      --b;
+    }
+    cell_atoms[b + 1] = v;
+  }
+}
+
+// Materialise the sorted triple list of one row set (API parity with
+// buildPairsAndTriplesWithEleIndex): for each centre, triples grouped by
+// element pair p, inside a group ordered by (k, j); rank = position in group.
+// trip_start[i] is the exclusive prefix of per-centre triple counts.
+__global__ void triples_kernel(int N, int ldr, int lda, int nele, const int* __restrict__ nbr,
+                               const int* __restrict__ ang, const int* __restrict__ acnt,
+                               const int* __restrict__ aseg, const long* __restrict__ trip_start,
+                               int* __restrict__ out_jkpr) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  long o = trip_start[i];
+  const int* seg = aseg + (long)i * (nele + 1);
+  const int* arow = ang + (long)i * lda;
+  const int* row = nbr + (long)i * ldr;
+  int p = 0;
+  for (int ea = 0; ea < nele; ++ea) {
+    for (int eb = ea; eb < nele; ++eb, ++p) {
+      int rank = 0;
+      // order (k, j): k runs over the second element's segment ascending,
+      // j over the first's ascending with j < k when the elements agree
+      for (int kb = seg[eb]; kb < seg[eb + 1]; ++kb) {
+        int jhi = (ea == eb) ? kb : seg[ea + 1];
+        for (int ja = seg[ea]; ja < jhi; ++ja) {
+          int j = row[arow[ja]], k = row[arow[kb]];
+          out_jkpr[4 * o + 0] = j;
+          out_jkpr[4 * o + 1] = k;
+          out_jkpr[4 * o + 2] = p;
+          out_jkpr[4 * o + 3] = rank++;
+          ++o;
+        }
+      }
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int tm_cell_build(int N, const double* pos, double ox, double oy, double oz, double cx, double cy,
+                  double cz, int nx, int ny, int nz, int periodic, int* cell_of, int* cell_count,
+                  int* cell_start, int* cell_fill, int* cell_atoms, int* flags, void* stream) {
+  TM_REQUIRE(N >= 0 && nx > 0 && ny > 0 && nz > 0, "tm_cell_build: bad sizes");
+  hipStream_t s = (hipStream_t)stream;
+  long ncell = (long)nx * ny * nz;
+  if (N == 0) return TM_OK;
+  // counters are cleared with a fill kernel-free memset only outside capture;
+  // callers that capture pass zeroed buffers and set cell_count/cell_fill themselves
+  TM_HIP_CHECK(hipMemsetAsync(cell_count, 0, sizeof(int) * ncell, s));
+  TM_HIP_CHECK(hipMemsetAsync(cell_fill, 0, sizeof(int) * ncell, s));
+  cell_assign_kernel<<<tm_div_up(N, 256), 256, 0, s>>>(N, pos, ox, oy, oz, cx, cy, cz, nx, ny, nz,
+                                                       periodic, cell_of, cell_count, flags);
+  TM_LAUNCH_CHECK();
+  scan_kernel<<<1, 1024, 0, s>>>((int)ncell, cell_count, cell_start);
+  TM_LAUNCH_CHECK();
+  cell_fill_kernel<<<tm_div_up(N, 256), 256, 0, s>>>(N, cell_of, cell_start, cell_fill, cell_atoms);
+  TM_LAUNCH_CHECK();
+  cell_sort_kernel<<<tm_div_up(ncell, 256), 256, 0, s>>>((int)ncell, cell_start, cell_atoms);
+  TM_LAUNCH_CHECK();
+  return TM_OK;
+}
+
+int tm_nl_build(int N, const double* pos, const int* elem, const int* mol, const int* mol_start,
+                const double* box, const int* cell_start, const int* cell_atoms,
+                const int* cell_of, int nx, int ny, int nz, int periodic, int nele, double s_r,
+                double s_a, int ldr, int lda, int* nbr, int* ncnt, int* rseg, int* ang, int* acnt,
+                int* aseg, int* rev, int* flags, void* stream) {
+  TM_REQUIRE(N >= 0 && nele > 0 && nele <= TM_MAX_ELE, "tm_nl_build: bad nele %d", nele);
+  TM_REQUIRE(ldr > 0 && ldr <= 256 && lda > 0 && lda <= ldr, "tm_nl_build: bad strides %d %d", ldr,
+             lda);
+  TM_REQUIRE((long)N * ldr < (1l << 31), "tm_nl_build: N*ldr overflows int32 slots");
+  if (N == 0) return TM_OK;
+  hipStream_t s = (hipStream_t)stream;
+  CellGrid g;
+  g.cell_start = cell_start;
+  g.cell_atoms = cell_atoms;
+  g.cell_of = cell_of;
+  g.nc[0] = nx; g.nc[1] = ny; g.nc[2] = nz;
+  g.periodic = periodic;
+  if (cell_atoms != nullptr) TM_REQUIRE(cell_start && cell_of, "tm_nl_build: incomplete cell grid");
+  nl_rows_kernel<<<tm_div_up(N, NL_WAVES), NL_WAVES * 64, 0, s>>>(
+      N, pos, elem, mol, mol_start, box, g, nele, s_r, s_a, ldr, lda, nbr, ncnt, rseg, ang, acnt,
+      aseg, flags);
+  TM_LAUNCH_CHECK();
+  long total = (long)N * ldr;
+  nl_rev_kernel<<<tm_div_up(total, 256), 256, 0, s>>>(N, ldr, nele, elem, nbr, ncnt, rseg, rev,
+                                                      flags);
+  TM_LAUNCH_CHECK();
+  return TM_OK;
+}
+
+int tm_triples(int N, int ldr, int lda, int nele, const int* nbr, const int* ang, const int* acnt,
+               const int* aseg, const long* trip_start, int* out_jkpr, void* stream) {
+  TM_REQUIRE(N >= 0 && nele > 0 && nele <= TM_MAX_ELE, "tm_triples: bad sizes");
+  if (N == 0) return TM_OK;
+  hipStream_t s = (hipStream_t)stream;
+  triples_kernel<<<tm_div_up(N, 128), 128, 0, s>>>(N, ldr, lda, nele, nbr, ang, acnt, aseg,
+                                                   trip_start, out_jkpr);
+  TM_LAUNCH_CHECK();
+  return TM_OK;
+}
+
+}  // extern "C"

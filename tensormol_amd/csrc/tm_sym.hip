@@ -1,0 +1,463 @@
+// ANI-style radial and angular symmetry functions and their adjoint.
+//
+// Forward restates TFSymRSet_Linear_WithEle (TensorMol/TFDescriptors/
+// RawSymFunc.py:1696-1755), TFSymASet_Linear_WithEle (:868-962) and the
+// concatenation of TFSymSet_Scattered_Linear_WithEle (:2223-2265):
+//   G[i, l*NR + s]          = sum_j exp(-eta (r_ij - Rs_s)^2) fc_r(r_ij)
+//   G[i, nele*NR + p*NA*NT + t*NA + s]
+//        = sum_{j<k} 2^(1-zeta) (1 + cos(theta_ijk - theta_t))^zeta
+//                     exp(-eta ((r_ij + r_ik)/2 - Rs_s)^2) fc_a(r_ij) fc_a(r_ik)
+// with fc(r) = 0.5 (cos(3.14159265359 r / Rc) + 1) and the neighbour list as
+// the only cutoff mask.  cos(theta - theta_t) is evaluated as
+// c cos(theta_t) + sqrt(1 - c^2) sin(theta_t) with c = cos(theta), which is the
+// same function as the reference's acos-then-cos.
+//
+// One wavefront per centre atom.  Neighbour geometry is staged in LDS; the
+// angular triples of a centre are enumerated on the fly from its short
+// sub-list, grouped by element pair, so nothing per-triple is stored in HBM.
+// Every output element is written exactly once; no atomics.
+//
+// The adjoint takes dE/dG per atom and returns, for every neighbour slot
+// (i -> j), the vector w_ij = dE/d(x_j - x_i) of centre i's descriptor; a
+// gather over mirror slots turns those into per-atom gradients without
+// atomics, so forces are bitwise reproducible.
+#include "tm_common.h"
+
+namespace {
+
+constexpr int SYM_WAVES = 4;
+constexpr int MAX_ROW = 160;      // radial row entries staged per wave
+constexpr int MAX_ANG = 48;       // angular sub-list entries per centre
+constexpr int NR_MAX = 64;        // radial shells
+constexpr int NA = 8;             // angular shells (fixed by the lane mapping)
+constexpr int NT = 8;             // angular theta points
+
+struct SymArgs {
+  int nele, nelep, nr;            // elements, element pairs, radial shells
+  int D;                          // descriptor length
+  float eta, zeta, rr, ra;        // AN1 parameters and cutoffs (A)
+  float two_1mz;                  // 2^(1-zeta)
+  int zeta_int;                   // zeta when it is a small positive integer, else 0
+  float rs_r[NR_MAX];
+  float rs_a[NA];
+  float cos_t[NT], sin_t[NT];
+};
+
+constexpr float PI_LIT = 3.14159265359f;
+
+__device__ __forceinline__ float powz(float b, const SymArgs& a) {
+  if (a.zeta_int > 0) {
+    float r = 1.f, x = b;
+    int e = a.zeta_int;
+    while (e) {
+      if (e & 1) r *= x;
+      x *= x;
+      e >>= 1;
+    }
+    return r;
+  }
+  return b > 0.f ? __expf(a.zeta * __logf(b)) : 0.f;
+}
+
+struct RowLds {
+  float4 d[SYM_WAVES][MAX_ROW];      // (x_j - x_i, r)
+  float fc[SYM_WAVES][MAX_ROW];      // radial cutoff function
+  float fa[SYM_WAVES][MAX_ROW];      // angular cutoff function
+  int el[SYM_WAVES][MAX_ROW];
+};
+
+__device__ __forceinline__ void load_row(int i, int w, int lane, const double* __restrict__ pos,
+                                         const double* __restrict__ box, const int* __restrict__ mol,
+                                         const int* __restrict__ elem, const int* __restrict__ nbr,
+                                         int n, int ldr, const SymArgs& a, RowLds& L) {
+  double Lx = 0, Ly = 0, Lz = 0;
+  if (box != nullptr) {
+    int m = mol[i];
+    Lx = box[3 * m]; Ly = box[3 * m + 1]; Lz = box[3 * m + 2];
+  }
+  const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+  for (int q = lane; q < n; q += 64) {
+    int j = nbr[(long)i * ldr + q];
+    float dx = (float)tm_min_image(pos[3 * j] - xi, Lx);
+    float dy = (float)tm_min_image(pos[3 * j + 1] - yi, Ly);
+    float dz = (float)tm_min_image(pos[3 * j + 2] - zi, Lz);
+    float r = sqrtf(dx * dx + dy * dy + dz * dz + 1e-27f);
+    L.d[w][q] = make_float4(dx, dy, dz, r);
+    L.fc[w][q] = 0.5f * (__cosf(PI_LIT * r / a.rr) + 1.0f);
+    L.fa[w][q] = 0.5f * (__cosf(PI_LIT * r / a.ra) + 1.0f);
+    L.el[w][q] = elem[j];
+  }
+}
+
+// decode triple index t (in the centre's enumeration) into sub-list
+// positions (ja, kb) and the element pair p; blocks are ordered by
+// (ea <= eb) and inside a block by (k, j)
+__device__ __forceinline__ void decode_triple(int t, const int* seg, int nele, int& ja, int& kb,
+                                              int& p) {
+  p = 0;
+  for (int ea = 0; ea < nele; ++ea) {
+    int la = seg[ea + 1] - seg[ea];
+    for (int eb = ea; eb < nele; ++eb, ++p) {
+      int lb = seg[eb + 1] - seg[eb];
+      int nb = (ea == eb) ? la * (la - 1) / 2 : la * lb;
+      if (t < nb) {
+        if (ea != eb) {
+          kb = seg[eb] + t / la;
+          ja = seg[ea] + t % la;
+        } else {
+          // k-major triangle: k = 1.., j < k; t = k(k-1)/2 + j
+          int k = (int)((1.0f + sqrtf(1.0f + 8.0f * (float)t)) * 0.5f);
+          while (k * (k - 1) / 2 > t) --k;
+          while ((k + 1) * k / 2 <= t) ++k;
+          kb = seg[ea] + k;
+          ja = seg[ea] + (t - k * (k - 1) / 2);
+        }
+        return;
+      }
+      t -= nb;
+    }
+  }
+  ja = kb = 0;
+  p = -1;
+}
+
+__global__ void __launch_bounds__(SYM_WAVES * 64)
+sym_fwd_kernel(int N, const double* __restrict__ pos, const double* __restrict__ box,
+               const int* __restrict__ mol, const int* __restrict__ elem,
+               const int* __restrict__ nbr, const int* __restrict__ ncnt,
+               const int* __restrict__ rseg, const int* __restrict__ ang,
+               const int* __restrict__ acnt, const int* __restrict__ aseg, int ldr, int lda,
+               SymArgs a, float* __restrict__ G, int ldg) {
+  __shared__ RowLds L;
+  __shared__ float tc[SYM_WAVES][64], ts[SYM_WAVES][64], tf[SYM_WAVES][64];
+  __shared__ float tR[SYM_WAVES][NA][64];
+  __shared__ int tp[SYM_WAVES][64];
+  __shared__ int segs[SYM_WAVES][2][TM_MAX_ELE + 1];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = blockIdx.x * SYM_WAVES + w;
+  if (i >= N) return;   // only wave-level synchronisation below
+  const int n = min(ncnt[i], MAX_ROW);
+  if (lane <= a.nele) {
+    segs[w][0][lane] = min(rseg[(long)i * (a.nele + 1) + lane], n);
+    segs[w][1][lane] = aseg[(long)i * (a.nele + 1) + lane];
+  }
+  load_row(i, w, lane, pos, box, mol, elem, nbr, n, ldr, a, L);
+  __syncwarp();
+  float* Gi = G + (long)i * ldg;
+  // ---------------- radial: lane = (half h, shell s)
+  {
+    const int h = lane >> 5, s = lane & 31;
+    for (int e = 0; e < a.nele; ++e) {
+      int lo = segs[w][0][e], hi = segs[w][0][e + 1];
+      for (int s0 = 0; s0 < a.nr; s0 += 32) {
+        int ss = s0 + s;
+        float rs = ss < a.nr ? a.rs_r[ss] : 0.f;
+        float acc = 0.f;
+        for (int q = lo + h; q < hi; q += 2) {
+          float r = L.d[w][q].w;
+          float x = r - rs;
+          acc += __expf(-a.eta * x * x) * L.fc[w][q];
+        }
+        acc += __shfl_xor(acc, 32, 64);
+        if (h == 0 && ss < a.nr) Gi[e * a.nr + ss] = acc;
+      }
+    }
+  }
+  // ---------------- angular: phase A lane = triple, phase B lane = (t, s)
+  const int m = min(acnt[i], MAX_ANG);
+  const int* arow = ang + (long)i * lda;
+  const int ntrip = m * (m - 1) / 2;
+  const int off = a.nele * a.nr;
+  const int ot = lane >> 3, os = lane & 7;
+  const float ct = a.cos_t[ot], st = a.sin_t[ot];
+  int next_p = 0, cur_p = -1;
+  float acc = 0.f;
+  for (int base = 0; base < ntrip; base += 64) {
+    int t = base + lane;
+    if (t < ntrip) {
+      int ja, kb, p;
+      decode_triple(t, segs[w][1], a.nele, ja, kb, p);
+      int qj = arow[ja], qk = arow[kb];
+      float4 dj = L.d[w][qj], dk = L.d[w][qk];
+      float c = (dj.x * dk.x + dj.y * dk.y + dj.z * dk.z) / (dj.w * dk.w);
+      c = fminf(fmaxf(c, -1.0f), 1.0f);
+      float sn = sqrtf(fmaxf(1.0f - c * c, 0.0f));
+      float ravg = 0.5f * (dj.w + dk.w);
+      tc[w][lane] = c;
+      ts[w][lane] = sn;
+      tf[w][lane] = L.fa[w][qj] * L.fa[w][qk];
+      tp[w][lane] = p;
+#pragma unroll
+      for (int s = 0; s < NA; ++s) {
+        float x = ravg - a.rs_a[s];
+        tR[w][s][lane] = __expf(-a.eta * x * x);
+      }
+    }
+    __syncwarp();
+    int cnt = min(64, ntrip - base);
+    for (int u = 0; u < cnt; ++u) {
+      int p = tp[w][u];
+      if (p != cur_p) {
+        if (cur_p >= 0) {
+          for (; next_p < cur_p; ++next_p) Gi[off + next_p * 64 + lane] = 0.f;
+          Gi[off + cur_p * 64 + lane] = acc;
+          next_p = cur_p + 1;
+        }
+        cur_p = p;
+        acc = 0.f;
+      }
+      float v = tc[w][u] * ct + ts[w][u] * st;
+      acc += a.two_1mz * powz(1.0f + v, a) * tR[w][os][u] * tf[w][u];
+    }
+    __syncwarp();
+  }
+  if (cur_p >= 0) {
+    for (; next_p < cur_p; ++next_p) Gi[off + next_p * 64 + lane] = 0.f;
+    Gi[off + cur_p * 64 + lane] = acc;
+    next_p = cur_p + 1;
+  }
+  for (; next_p < a.nelep; ++next_p) Gi[off + next_p * 64 + lane] = 0.f;
+}
+
+// dE/dG (two contributions combined: gE + u_i * gQ) -> w slots
+__global__ void __launch_bounds__(SYM_WAVES * 64)
+sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__ box,
+               const int* __restrict__ mol, const int* __restrict__ elem,
+               const int* __restrict__ nbr, const int* __restrict__ ncnt,
+               const int* __restrict__ rseg, const int* __restrict__ ang,
+               const int* __restrict__ acnt, const int* __restrict__ aseg, int ldr, int lda,
+               SymArgs a, const float* __restrict__ gE, const float* __restrict__ gQ,
+               const float* __restrict__ u, int ldg, float4* __restrict__ wslot) {
+  __shared__ RowLds L;
+  __shared__ float g[SYM_WAVES][1024];
+  __shared__ float4 wacc[SYM_WAVES][MAX_ROW];
+  __shared__ float4 ca[SYM_WAVES][64], cb[SYM_WAVES][64];
+  __shared__ int tja[SYM_WAVES][64], tkb[SYM_WAVES][64];
+  __shared__ int segs[SYM_WAVES][2][TM_MAX_ELE + 1];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = blockIdx.x * SYM_WAVES + w;
+  if (i >= N) return;
+  const int n = min(ncnt[i], MAX_ROW);
+  if (lane <= a.nele) {
+    segs[w][0][lane] = min(rseg[(long)i * (a.nele + 1) + lane], n);
+    segs[w][1][lane] = aseg[(long)i * (a.nele + 1) + lane];
+  }
+  const float ui = (gQ != nullptr) ? u[i] : 0.f;
+  for (int f = lane; f < a.D; f += 64) {
+    float v = gE[(long)i * ldg + f];
+    if (gQ != nullptr) v += ui * gQ[(long)i * ldg + f];
+    g[w][f] = v;
+  }
+  load_row(i, w, lane, pos, box, mol, elem, nbr, n, ldr, a, L);
+  __syncwarp();
+  // ---------------- radial: lane = neighbour
+  const float kr = PI_LIT / a.rr;
+  for (int q = lane; q < n; q += 64) {
+    float4 d = L.d[w][q];
+    float r = d.w;
+    int e = L.el[w][q];
+    float fc = L.fc[w][q];
+    float dfc = -0.5f * kr * __sinf(kr * r);
+    float dEdr = 0.f;
+    for (int s = 0; s < a.nr; ++s) {
+      float x = r - a.rs_r[s];
+      float ex = __expf(-a.eta * x * x);
+      dEdr += g[w][e * a.nr + s] * ex * (dfc - 2.0f * a.eta * x * fc);
+    }
+    float sc = dEdr / r;
+    wacc[w][q] = make_float4(sc * d.x, sc * d.y, sc * d.z, 0.f);
+  }
+  __syncwarp();
+  // ---------------- angular: lane = triple, contributions staged per chunk
+  const int m = min(acnt[i], MAX_ANG);
+  const int* arow = ang + (long)i * lda;
+  const int ntrip = m * (m - 1) / 2;
+  const int off = a.nele * a.nr;
+  const float ka = PI_LIT / a.ra;
+  float4 wang = make_float4(0.f, 0.f, 0.f, 0.f);   // lane = sub-list member
+  for (int base = 0; base < ntrip; base += 64) {
+    int t = base + lane;
+    if (t < ntrip) {
+      int ja, kb, p;
+      decode_triple(t, segs[w][1], a.nele, ja, kb, p);
+      int qj = arow[ja], qk = arow[kb];
+      float4 dj = L.d[w][qj], dk = L.d[w][qk];
+      float rj = dj.w, rk = dk.w;
+      float inv = 1.0f / (rj * rk);
+      float c = (dj.x * dk.x + dj.y * dk.y + dj.z * dk.z) * inv;
+      c = fminf(fmaxf(c, -1.0f), 1.0f);
+      float s2 = fmaxf(1.0f - c * c, 0.0f);
+      float sn = sqrtf(s2);
+      float cos_over_sin = sn > 1e-6f ? c / sn : 0.f;
+      float ravg = 0.5f * (rj + rk);
+      float fj = L.fa[w][qj], fk = L.fa[w][qk];
+      float dfj = -0.5f * ka * __sinf(ka * rj), dfk = -0.5f * ka * __sinf(ka * rk);
+      float F = fj * fk;
+      float R[NA], Rp[NA];
+#pragma unroll
+      for (int s = 0; s < NA; ++s) {
+        float x = ravg - a.rs_a[s];
+        R[s] = __expf(-a.eta * x * x);
+        Rp[s] = -2.0f * a.eta * x * R[s];
+      }
+      const float* gp = &g[w][off + p * 64];
+      float dEdc = 0.f, dEdrav = 0.f, dEdF = 0.f;
+#pragma unroll
+      for (int tt = 0; tt < NT; ++tt) {
+        float S = 0.f, Sp = 0.f;
+#pragma unroll
+        for (int s = 0; s < NA; ++s) {
+          float gv = gp[tt * NA + s];
+          S += gv * R[s];
+          Sp += gv * Rp[s];
+        }
+        float v = c * a.cos_t[tt] + sn * a.sin_t[tt];
+        float b = 1.0f + v;
+        float Az = powz(b, a);                     // (1+v)^zeta
+        float dAz = b > 0.f ? a.zeta * Az / b : 0.f;  // zeta (1+v)^(zeta-1)
+        float dv = a.cos_t[tt] - cos_over_sin * a.sin_t[tt];
+        dEdc += a.two_1mz * dAz * dv * S;
+        dEdrav += a.two_1mz * Az * Sp;
+        dEdF += a.two_1mz * Az * S;
+      }
+      dEdc *= F;
+      dEdrav *= F;
+      // chain rule into d_j, d_k
+      float cj = dEdrav * 0.5f / rj + dEdF * dfj * fk / rj - dEdc * c / (rj * rj);
+      float ck = dEdrav * 0.5f / rk + dEdF * fj * dfk / rk - dEdc * c / (rk * rk);
+      float x = dEdc * inv;
+      ca[w][lane] = make_float4(cj * dj.x + x * dk.x, cj * dj.y + x * dk.y, cj * dj.z + x * dk.z, 0.f);
+      cb[w][lane] = make_float4(ck * dk.x + x * dj.x, ck * dk.y + x * dj.y, ck * dk.z + x * dj.z, 0.f);
+      tja[w][lane] = ja;
+      tkb[w][lane] = kb;
+    }
+    __syncwarp();
+    int cnt = min(64, ntrip - base);
+    for (int v = 0; v < cnt; ++v) {
+      if (tja[w][v] == lane) {
+        float4 c4 = ca[w][v];
+        wang.x += c4.x; wang.y += c4.y; wang.z += c4.z;
+      }
+      if (tkb[w][v] == lane) {
+        float4 c4 = cb[w][v];
+        wang.x += c4.x; wang.y += c4.y; wang.z += c4.z;
+      }
+    }
+    __syncwarp();
+  }
+  if (lane < m) {
+    int q = arow[lane];
+    float4 t = wacc[w][q];
+    wacc[w][q] = make_float4(t.x + wang.x, t.y + wang.y, t.z + wang.z, 0.f);
+  }
+  __syncwarp();
+  for (int q = lane; q < n; q += 64) wslot[(long)i * ldr + q] = wacc[w][q];
+}
+
+// grad_a = sum_n (w[rev(a,n)] - w[(a,n)]) + extra_a ; optional force output
+// force = -scale * grad (J/mol/A when scale = JOULEPERHARTREE)
+__global__ void force_gather_kernel(int N, int ldr, const int* __restrict__ ncnt,
+                                    const int* __restrict__ rev, const float4* __restrict__ wslot,
+                                    const double* __restrict__ extra, double scale,
+                                    double* __restrict__ grad, double* __restrict__ force) {
+  int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= N) return;
+  double gx = 0, gy = 0, gz = 0;
+  int n = ncnt[a];
+  for (int q = 0; q < n; ++q) {
+    long sl = (long)a * ldr + q;
+    float4 wo = wslot[sl];
+    float4 wi = wslot[rev[sl]];
+    gx += (double)wi.x - (double)wo.x;
+    gy += (double)wi.y - (double)wo.y;
+    gz += (double)wi.z - (double)wo.z;
+  }
+  if (extra != nullptr) {
+    gx += extra[3 * a]; gy += extra[3 * a + 1]; gz += extra[3 * a + 2];
+  }
+  if (grad != nullptr) {
+    grad[3 * a] = gx; grad[3 * a + 1] = gy; grad[3 * a + 2] = gz;
+  }
+  if (force != nullptr) {
+    force[3 * a] = -scale * gx; force[3 * a + 1] = -scale * gy; force[3 * a + 2] = -scale * gz;
+  }
+}
+
+int fill_args(SymArgs& a, int nele, int nr, float eta, float zeta, float rr, float ra,
+              const float* rs_r, const float* rs_a, const float* thetas) {
+  TM_REQUIRE(nele > 0 && nele <= TM_MAX_ELE, "sym: bad nele %d", nele);
+  TM_REQUIRE(nr > 0 && nr <= NR_MAX, "sym: radial shells %d outside 1..%d", nr, NR_MAX);
+  a.nele = nele;
+  a.nelep = nele * (nele + 1) / 2;
+  a.nr = nr;
+  a.D = nele * nr + a.nelep * NA * NT;
+  TM_REQUIRE(a.D <= 1024, "sym: descriptor length %d above 1024", a.D);
+  a.eta = eta; a.zeta = zeta; a.rr = rr; a.ra = ra;
+  a.two_1mz = powf(2.0f, 1.0f - zeta);
+  a.zeta_int = (zeta == floorf(zeta) && zeta >= 1.f && zeta <= 64.f) ? (int)zeta : 0;
+  for (int s = 0; s < nr; ++s) a.rs_r[s] = rs_r[s];
+  for (int s = 0; s < NA; ++s) a.rs_a[s] = rs_a[s];
+  for (int t = 0; t < NT; ++t) {
+    a.cos_t[t] = cosf(thetas[t]);
+    a.sin_t[t] = sinf(thetas[t]);
+  }
+  return TM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Host-side parameter arrays (rs_r[nr], rs_a[8], thetas[8]) are read on the
+// host and passed to the kernels by value.
+int tm_sym_forward(int N, const double* pos, const double* box, const int* mol, const int* elem,
+                   const int* nbr, const int* ncnt, const int* rseg, const int* ang,
+                   const int* acnt, const int* aseg, int ldr, int lda, int nele, int nr,
+                   float eta, float zeta, float rr, float ra, const float* rs_r,
+                   const float* rs_a, const float* thetas, float* G, int ldg, void* stream) {
+  SymArgs a;
+  int st = fill_args(a, nele, nr, eta, zeta, rr, ra, rs_r, rs_a, thetas);
+  if (st) return st;
+  TM_REQUIRE(ldg >= a.D, "tm_sym_forward: ldg %d < D %d", ldg, a.D);
+  TM_REQUIRE(ldr <= MAX_ROW + 96 && lda <= MAX_ANG, "tm_sym_forward: strides %d/%d above %d/%d",
+             ldr, lda, MAX_ROW, MAX_ANG);
+  if (N == 0) return TM_OK;
+  hipStream_t s = (hipStream_t)stream;
+  sym_fwd_kernel<<<tm_div_up(N, SYM_WAVES), SYM_WAVES * 64, 0, s>>>(
+      N, pos, box, mol, elem, nbr, ncnt, rseg, ang, acnt, aseg, ldr, lda, a, G, ldg);
+  TM_LAUNCH_CHECK();
+  return TM_OK;
+}
+
+int tm_sym_adjoint(int N, const double* pos, const double* box, const int* mol, const int* elem,
+                   const int* nbr, const int* ncnt, const int* rseg, const int* ang,
+                   const int* acnt, const int* aseg, int ldr, int lda, int nele, int nr,
+                   float eta, float zeta, float rr, float ra, const float* rs_r,
+                   const float* rs_a, const float* thetas, const float* gE, const float* gQ,
+                   const float* u, int ldg, float* wslot, void* stream) {
+  SymArgs a;
+  int st = fill_args(a, nele, nr, eta, zeta, rr, ra, rs_r, rs_a, thetas);
+  if (st) return st;
+  TM_REQUIRE(ldg >= a.D, "tm_sym_adjoint: ldg %d < D %d", ldg, a.D);
+  TM_REQUIRE(ldr <= MAX_ROW + 96 && lda <= MAX_ANG, "tm_sym_adjoint: strides above limits");
+  if (N == 0) return TM_OK;
+  hipStream_t s = (hipStream_t)stream;
+  sym_adj_kernel<<<tm_div_up(N, SYM_WAVES), SYM_WAVES * 64, 0, s>>>(
+      N, pos, box, mol, elem, nbr, ncnt, rseg, ang, acnt, aseg, ldr, lda, a, gE, gQ, u, ldg,
+      (float4*)wslot);
+  TM_LAUNCH_CHECK();
+  return TM_OK;
+}
+
+int tm_force_gather(int N, int ldr, const int* ncnt, const int* rev, const float* wslot,
+                    const double* extra, double scale, double* grad, double* force,
+                    void* stream) {
+  if (N == 0) return TM_OK;
+  hipStream_t s = (hipStream_t)stream;
+  force_gather_kernel<<<tm_div_up(N, 256), 256, 0, s>>>(N, ldr, ncnt, rev, (const float4*)wslot,
+                                                        extra, scale, grad, force);
+  TM_LAUNCH_CHECK();
+  return TM_OK;
+}
+
+}  // extern "C"

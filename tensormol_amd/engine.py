@@ -1,0 +1,349 @@
+"""Device-resident BP+EE energy/force engine over packed atoms.
+
+One evaluation is a fixed sequence of HIP launches on the caller's current
+stream (no host synchronisation except the optional overflow check):
+
+  cell grid (large systems) -> neighbour rows (4.6 A + 3.1 A sub-list, mirror
+  slots) -> descriptors G -> per-element energy and charge MLPs with their input
+  gradients -> neutral charges -> DSF-elu Coulomb + vdW (phi_i, direct
+  gradient) -> charge upstream u_i = phi_i - mean(phi) -> descriptor adjoint
+  with dE/dG = dE_i/dG + u_i dq_i/dG -> mirror-slot force gather ->
+  per-molecule FP64 reductions.
+
+This is the data flow of MolInstance_DirectBP_EE_ChargeEncode_Update_vdw_DSF_
+elu_Normalize_Dropout.evaluate (TensorMol/TFNetworks/TFMolInstanceDirect.py:
+5684-5711) with tf.gradients (:5761) replaced by the hand-written adjoint.
+Memory layout in HBM: positions float64 [N,3]; descriptors float32 [N,D];
+neighbour rows int32 ELL [N, ldr]; per-atom scalars float32/float64 [N].
+"""
+import math
+import struct
+
+import numpy as np
+import torch
+
+from . import _lib
+from .params import PARAMS, BOHRPERA, JOULEPERHARTREE, TWO_OVER_SQRTPI
+from .model import BPEEModel
+
+_P = _lib.ptr
+
+
+def nl_threshold(rc):
+    """First float64 s for which sqrt(s) + 1e-13 < rc is false (MolEmb.cpp:1214-1218).
+
+    sqrt and + are correctly rounded and monotone, so the reference predicate
+    equals s < threshold for s = (dx*dx + dy*dy) + dz*dz."""
+    def as_d(k):
+        return struct.unpack("<d", struct.pack("<q", k))[0]
+
+    def pred(s):
+        return math.sqrt(s) + 0.0000000000001 < rc
+
+    lo = 0
+    hi = struct.unpack("<q", struct.pack("<d", 4.0 * rc * rc + 1.0))[0]
+    while hi - lo > 1:
+        mid = (lo + hi) // 2
+        if pred(as_d(mid)):
+            lo = mid
+        else:
+            hi = mid
+    return as_d(hi)
+
+
+def dsf_constants(dsf_alpha, elu_width, ee_off):
+    """[alpha', R_sr, R_lr, ZZ, YY, elu_shift, elu_alpha] in bohr units.
+
+    elu_shift = DSF(R_sr), elu_alpha = DSF_Gradient(R_sr) (Util.py:172-192 as
+    used at TFMolInstanceDirect.py:4371-4372); ZZ and YY as in
+    TFCoulombEluSRDSFLR (RawSymFunc.py:1338-1340)."""
+    a = dsf_alpha / BOHRPERA
+    R = elu_width * BOHRPERA
+    Rc = ee_off * BOHRPERA
+    XX = a * Rc
+    ZZ = math.erfc(XX) / Rc
+    YY = TWO_OVER_SQRTPI * a * math.exp(-XX * XX) / Rc
+    shift = math.erfc(a * R) / R - ZZ + (R - Rc) * (ZZ / Rc + YY)
+    grad = -((math.erfc(a * R) / R / R + TWO_OVER_SQRTPI * a * math.exp(-a * R * a * R) / R)
+             - (ZZ / Rc + YY))
+    return np.array([a, R, Rc, ZZ, YY, shift, grad], dtype=np.float64)
+
+
+class Workspace:
+    """Device buffers and static topology of one packed batch."""
+    pass
+
+
+class BPEEEngine:
+    """Energies, charges and forces of a BPEEModel on one GPU.
+
+    precision: "fp32" (plain FP32 VALU GEMMs, every intermediate in HBM),
+    "bf16" / "fp16" (fused MFMA kernel, FP32 accumulation) or "bf16x3"
+    (MFMA with hi/lo operand splitting, near-FP32 accuracy).
+    """
+
+    def __init__(self, model: BPEEModel, device="cuda", precision=None, ldr=128, lda=48,
+                 check_overflow=True):
+        self.model = model
+        self.device = torch.device(device)
+        self.precision = precision or PARAMS["MLPPrecision"]
+        self.ldr, self.lda = int(ldr), int(lda)
+        self.check_overflow = check_overflow
+        self.lib = _lib.lib()
+        sym = model.sym
+        self.nele = sym.nele
+        self.D = sym.D
+        self.s_r = nl_threshold(sym.rr)
+        self.s_a = nl_threshold(sym.ra)
+        self.sym_rs_r = np.ascontiguousarray(sym.rs_r, dtype=np.float32)
+        self.sym_rs_a = np.ascontiguousarray(sym.rs_a, dtype=np.float32)
+        self.sym_th = np.ascontiguousarray(sym.thetas, dtype=np.float32)
+        self.dsf = dsf_constants(model.dsf_alpha, model.elu_width, model.ee_off)
+        c6, r0 = model.vdw_tables()
+        self.c6 = np.ascontiguousarray(c6, dtype=np.float64)
+        self.r0 = np.ascontiguousarray(r0, dtype=np.float64)
+        self.vdw_width = model.poly_width * BOHRPERA
+        self.hid = np.array(model.hidden, dtype=np.int32)
+        self._pack_f32()
+        if self.precision != "fp32":
+            from . import mlp_mfma
+            self.mfma = mlp_mfma.MFMAWeights(self, self.precision)
+
+    # ------------------------------------------------------------ weights
+    def _pack_f32(self):
+        self.w_f32 = {}
+        for tag, nets in (("E", self.model.energy_nets), ("Q", self.model.charge_nets)):
+            for a, e in enumerate(self.model.eles):
+                parts = []
+                for (W, b) in nets[e]:
+                    parts.append(np.asarray(W, np.float32).ravel())
+                    parts.append(np.asarray(b, np.float32).ravel())
+                flat = np.concatenate(parts)
+                self.w_f32[(tag, a)] = torch.tensor(flat, device=self.device)
+
+    # ---------------------------------------------------------- topology
+    def prepare(self, Z, mol_start, boxes=None, margin=4.0, pos0=None):
+        """Allocate device buffers for a packed batch (Z (N,), mol_start (B+1,),
+        optional boxes (B,3) orthorhombic edges in A; 0 or None = open)."""
+        dev = self.device
+        ws = Workspace()
+        Z = np.asarray(Z)
+        ms = np.asarray(mol_start, dtype=np.int64)
+        N, B = len(Z), len(ms) - 1
+        ws.N, ws.B = N, B
+        elem = self.model.element_index(Z)
+        ws.elem_np = elem
+        ws.mol_np = np.repeat(np.arange(B, dtype=np.int32), np.diff(ms))
+        ws.ms_np = ms
+        ws.elem = torch.tensor(elem, dtype=torch.int32, device=dev)
+        ws.mol = torch.tensor(ws.mol_np, dtype=torch.int32, device=dev)
+        ws.mol_start = torch.tensor(ms, dtype=torch.int32, device=dev)
+        ws.natom = torch.tensor(np.diff(ms), dtype=torch.float64, device=dev)
+        ws.box = None
+        ws.periodic = 0
+        if boxes is not None:
+            bx = np.asarray(boxes, dtype=np.float64).reshape(B, 3)
+            if np.any(bx > 0):
+                if np.any(bx <= 0):
+                    raise ValueError("mixed open and periodic molecules in one batch")
+                if np.any(bx < 2.0 * max(self.model.sym.rr, self.model.ee_off)):
+                    raise ValueError("box edge below twice the largest cutoff: minimum image is ambiguous")
+                ws.box = torch.tensor(bx, device=dev)
+                ws.periodic = 1
+        ws.boxes_np = None if ws.box is None else np.asarray(boxes, dtype=np.float64).reshape(B, 3)
+        # element-sorted permutation (rows of each per-element GEMM are contiguous)
+        perm = np.argsort(elem, kind="stable").astype(np.int32)
+        counts = np.bincount(elem, minlength=self.nele)
+        ws.eoff = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+        ws.perm = torch.tensor(perm, device=dev)
+        # cell grid for one large system
+        ws.use_cells = (B == 1 and N > 256)
+        if ws.use_cells:
+            self._setup_grid(ws, pos0, margin)
+        i32 = dict(dtype=torch.int32, device=dev)
+        f32 = dict(dtype=torch.float32, device=dev)
+        f64 = dict(dtype=torch.float64, device=dev)
+        ws.nbr = torch.zeros(N * self.ldr, **i32)
+        ws.ncnt = torch.zeros(N, **i32)
+        ws.rseg = torch.zeros(N * (self.nele + 1), **i32)
+        ws.ang = torch.zeros(N * self.lda, **i32)
+        ws.acnt = torch.zeros(N, **i32)
+        ws.aseg = torch.zeros(N * (self.nele + 1), **i32)
+        ws.rev = torch.zeros(N * self.ldr, **i32)
+        ws.flags = torch.zeros(8, **i32)
+        ws.G = torch.zeros(N, self.D, **f32)
+        ws.JE = torch.zeros(N, self.D, **f32)
+        ws.JQ = torch.zeros(N, self.D, **f32)
+        ws.eps = torch.zeros(N, **f32)
+        ws.qraw = torch.zeros(N, **f32)
+        ws.qmean = torch.zeros(B, **f64)
+        ws.phimean = torch.zeros(B, **f64)
+        ws.xq = torch.zeros(N, 4, **f32)
+        ws.q = torch.zeros(N, **f64)
+        ws.qx = torch.zeros(N, 3, **f64)
+        ws.phi = torch.zeros(N, **f64)
+        ws.gradc = torch.zeros(N, 3, **f64)
+        ws.eatom = torch.zeros(N, 2, **f64)
+        ws.u = torch.zeros(N, **f32)
+        ws.wslot = torch.zeros(N * self.ldr * 4, **f32)
+        ws.grad = torch.zeros(N, 3, **f64)
+        ws.force = torch.zeros(N, 3, **f64)
+        ws.Ebp = torch.zeros(B, **f64)
+        ws.Eext = torch.zeros(B, 2, **f64)
+        ws.dipole = torch.zeros(B, 3, **f64)
+        if self.precision == "fp32":
+            hmax = int(self.hid.max())
+            maxrows = int(counts.max()) if N else 0
+            nh = len(self.hid)
+            ws.mlp_ws = torch.zeros(max(1, (2 * nh + 2) * maxrows * hmax), **f32)
+        else:
+            ws.mlp_ws = self.mfma.workspace(ws)
+        return ws
+
+    def _setup_grid(self, ws, pos0, margin):
+        rc = self.model.sym.rr
+        if ws.periodic:
+            L = ws.boxes_np[0]
+            nc = np.maximum(1, np.floor(L / rc)).astype(int)
+            cs = L / nc
+            origin = np.zeros(3)
+        else:
+            if pos0 is None:
+                raise ValueError("open large systems need pos0 to size the cell grid")
+            p = np.asarray(pos0)
+            lo = p.min(0) - margin
+            hi = p.max(0) + margin
+            nc = np.maximum(1, np.floor((hi - lo) / rc)).astype(int)
+            cs = (hi - lo) / nc
+            origin = lo
+        ws.grid_nc = [int(v) for v in nc]
+        ws.grid_cs = [float(v) for v in cs]
+        ws.grid_origin = [float(v) for v in origin]
+        ws.coul_reach = int(math.ceil(self.model.ee_off / float(min(cs))))
+        ncell = int(np.prod(nc))
+        i32 = dict(dtype=torch.int32, device=self.device)
+        ws.cell_of = torch.zeros(ws.N, **i32)
+        ws.cell_count = torch.zeros(ncell, **i32)
+        ws.cell_start = torch.zeros(ncell + 1, **i32)
+        ws.cell_fill = torch.zeros(ncell, **i32)
+        ws.cell_atoms = torch.zeros(ws.N, **i32)
+
+    # ---------------------------------------------------------- stages
+    def _stream(self):
+        return ctypes_stream(torch.cuda.current_stream(self.device))
+
+    def build_lists(self, ws, pos):
+        L, st = self.lib, self._stream()
+        L.tm_zero_i32(8, _P(ws.flags), st)
+        grid = (None, None, None, 1, 1, 1)
+        if ws.use_cells:
+            o, c, n = ws.grid_origin, ws.grid_cs, ws.grid_nc
+            L.tm_cell_build(ws.N, _P(pos), o[0], o[1], o[2], c[0], c[1], c[2], n[0], n[1], n[2],
+                            ws.periodic, _P(ws.cell_of), _P(ws.cell_count), _P(ws.cell_start),
+                            _P(ws.cell_fill), _P(ws.cell_atoms), _P(ws.flags), st)
+            grid = (_P(ws.cell_start), _P(ws.cell_atoms), _P(ws.cell_of), n[0], n[1], n[2])
+        L.tm_nl_build(ws.N, _P(pos), _P(ws.elem), _P(ws.mol), _P(ws.mol_start), _P(ws.box),
+                      grid[0], grid[1], grid[2], grid[3], grid[4], grid[5], ws.periodic,
+                      self.nele, self.s_r, self.s_a, self.ldr, self.lda, _P(ws.nbr), _P(ws.ncnt),
+                      _P(ws.rseg), _P(ws.ang), _P(ws.acnt), _P(ws.aseg), _P(ws.rev),
+                      _P(ws.flags), st)
+
+    def check_flags(self, ws):
+        f = ws.flags.cpu().numpy()
+        if f[0] or f[1]:
+            raise RuntimeError(f"neighbour row overflow (flags {f[:4]}); raise ldr/lda")
+        if f[3]:
+            raise FloatingPointError("non-finite coordinates")
+        return f
+
+    def descriptors(self, ws, pos):
+        sym = self.model.sym
+        self.lib.tm_sym_forward(ws.N, _P(pos), _P(ws.box), _P(ws.mol), _P(ws.elem), _P(ws.nbr),
+                                _P(ws.ncnt), _P(ws.rseg), _P(ws.ang), _P(ws.acnt), _P(ws.aseg),
+                                self.ldr, self.lda, self.nele, sym.nr_r, sym.eta, sym.zeta, sym.rr,
+                                sym.ra, _lib.fptr(self.sym_rs_r), _lib.fptr(self.sym_rs_a),
+                                _lib.fptr(self.sym_th), _P(ws.G), self.D, self._stream())
+
+    def networks(self, ws):
+        if self.precision == "fp32":
+            st = self._stream()
+            hid = _lib.fptr(self.hid)
+            for tag, y, J in (("E", ws.eps, ws.JE), ("Q", ws.qraw, ws.JQ)):
+                for a in range(self.nele):
+                    M = int(ws.eoff[a + 1] - ws.eoff[a])
+                    if M == 0:
+                        continue
+                    perm = ws.perm[int(ws.eoff[a]):]
+                    self.lib.tm_mlp_f32(M, self.D, len(self.hid), hid, _P(ws.G), self.D, _P(perm),
+                                        _P(self.w_f32[(tag, a)]), float(self.model.alpha), _P(y),
+                                        _P(J), _P(ws.mlp_ws), st)
+        else:
+            self.mfma.run(ws)
+
+    def electrostatics(self, ws, pos):
+        L, st = self.lib, self._stream()
+        L.tm_mol_sum_f32(ws.B, _P(ws.mol_start), 1, _P(ws.qraw), 1, _P(ws.qmean), 1, st)
+        L.tm_charges(ws.N, _P(ws.mol), _P(ws.qmean), _P(ws.qraw), _P(pos), _P(ws.xq), _P(ws.q),
+                     _P(ws.qx), st)
+        grid = (None, None, None, 1, 1, 1, 0)
+        if ws.use_cells:
+            n = ws.grid_nc
+            grid = (_P(ws.cell_start), _P(ws.cell_atoms), _P(ws.cell_of), n[0], n[1], n[2],
+                    ws.coul_reach)
+        L.tm_coulomb(ws.N, _P(ws.xq), _P(ws.elem), _P(ws.mol), _P(ws.mol_start), _P(ws.box),
+                     grid[0], grid[1], grid[2], grid[3], grid[4], grid[5], ws.periodic, grid[6],
+                     self.dsf.ctypes.data_as(_lib.P), self.model.ee_off, self.vdw_width,
+                     int(bool(PARAMS["AddEcc"])), int(self.model.has_vdw), self.nele,
+                     self.c6.ctypes.data_as(_lib.P), self.r0.ctypes.data_as(_lib.P),
+                     _P(ws.phi), _P(ws.gradc), _P(ws.eatom), st)
+        L.tm_mol_sum_f64(ws.B, _P(ws.mol_start), 1, _P(ws.phi), 1, _P(ws.phimean), 1, st)
+        L.tm_charge_upstream(ws.N, _P(ws.mol), _P(ws.phi), _P(ws.phimean), _P(ws.u), st)
+
+    def adjoint(self, ws, pos):
+        L, st = self.lib, self._stream()
+        sym = self.model.sym
+        L.tm_sym_adjoint(ws.N, _P(pos), _P(ws.box), _P(ws.mol), _P(ws.elem), _P(ws.nbr),
+                         _P(ws.ncnt), _P(ws.rseg), _P(ws.ang), _P(ws.acnt), _P(ws.aseg), self.ldr,
+                         self.lda, self.nele, sym.nr_r, sym.eta, sym.zeta, sym.rr, sym.ra,
+                         _lib.fptr(self.sym_rs_r), _lib.fptr(self.sym_rs_a), _lib.fptr(self.sym_th),
+                         _P(ws.JE), _P(ws.JQ), _P(ws.u), self.D, _P(ws.wslot), st)
+        L.tm_force_gather(ws.N, self.ldr, _P(ws.ncnt), _P(ws.rev), _P(ws.wslot), _P(ws.gradc),
+                          JOULEPERHARTREE, _P(ws.grad), _P(ws.force), st)
+
+    def reductions(self, ws):
+        L, st = self.lib, self._stream()
+        L.tm_mol_sum_f32(ws.B, _P(ws.mol_start), 1, _P(ws.eps), 1, _P(ws.Ebp), 0, st)
+        L.tm_mol_sum_f64(ws.B, _P(ws.mol_start), 2, _P(ws.eatom), 2, _P(ws.Eext), 0, st)
+        L.tm_mol_sum_f64(ws.B, _P(ws.mol_start), 3, _P(ws.qx), 3, _P(ws.dipole), 0, st)
+
+    def run(self, ws, pos, forces=True, check=None):
+        """Enqueue one full evaluation for positions pos (cuda float64 [N,3])."""
+        assert pos.dtype == torch.float64 and pos.is_contiguous() and pos.shape == (ws.N, 3)
+        self.build_lists(ws, pos)
+        if (self.check_overflow if check is None else check):
+            self.check_flags(ws)
+        self.descriptors(ws, pos)
+        self.networks(ws)
+        self.electrostatics(ws, pos)
+        if forces:
+            self.adjoint(ws, pos)
+        self.reductions(ws)
+
+    def results(self, ws):
+        """Host copies of the outputs of the last run (float64 numpy)."""
+        Ebp = ws.Ebp.cpu().numpy()
+        Eext = ws.Eext.cpu().numpy()
+        return dict(Etotal=Ebp + Eext[:, 0] + Eext[:, 1], Ebp=Ebp, Ecc=Eext[:, 0],
+                    Evdw=Eext[:, 1], Ebp_atom=ws.eps.double().cpu().numpy(),
+                    charge=ws.q.cpu().numpy(), dipole=ws.dipole.cpu().numpy(),
+                    gradient=ws.grad.cpu().numpy(), force=ws.force.cpu().numpy(),
+                    G=ws.G.cpu().numpy())
+
+    def energy_tensor(self, ws):
+        """Per-molecule total energy as a device tensor (no host copy)."""
+        return ws.Ebp + ws.Eext[:, 0] + ws.Eext[:, 1]
+
+
+def ctypes_stream(s):
+    import ctypes
+    return ctypes.c_void_p(s.cuda_stream)
