@@ -103,14 +103,18 @@ int tm_force_gather(int N, int ldr, const int* ncnt, const int* rev, const float
 int tm_mlp_f32(int M, int D, int nh, const int* hid, const float* G, int ldg, const int* perm,
                const float* W, float alpha, float* y, float* dG, float* ws, void* stream);
 
-/* tm_mlp_mfma: fused BF16/FP16 MFMA path, FP32 accumulation, all (net,
- * element) segments in one launch.  seg (device int32 [nseg*4]) =
- * (net, element, first row, rows); wpack: weights packed by
- * tensormol_amd.mlp_mfma; mode 0 = bf16, 1 = fp16, 2 = bf16 hi/lo split.
- * y[net*N + atom], dG[net][atom*ldg + f]. */
-int tm_mlp_mfma(int N, int D, int H, int nseg, const int* seg, int ldg, const float* G,
-                const int* perm, const void* wpack, const float* bias, float alpha, int mode,
-                float* y, float* dG, int* flags, void* ws, void* stream);
+/* tm_mlp_mfma: fused BF16/FP16 MFMA path, FP32 accumulation, both nets and all
+ * elements in one launch.  tiles (device int32 [ntiles][2]) = (segment, first
+ * row of the tile inside the segment); seg (device int32 [nseg][4]) = (net,
+ * blob index, first row in perm, rows); blobs: device array of weight-blob
+ * pointers packed as tm_mlp_mfma_pack_info describes; mode 0 = bf16, 1 = fp16,
+ * 2 = bf16 hi/lo split.  Writes y[net*N + atom] and dG[(net*N + atom)*ldg + f]. */
+int tm_mlp_mfma(int N, int D, int H, int ntiles, const int* tiles, const int* seg, int ldg,
+                const float* G, const int* perm, const void* blobs, float alpha, int mode,
+                float* y, float* dG, void* stream);
+/* out[0] blob bytes, out[1] byte offset of the float section (b1, b2, b3, wo,
+ * bo), out[2..7] element offsets of the forward (layers 1-3) and backward
+ * (layers 3, 2, 1) fragment arrays, out[8] planes (2 for the split mode). */
 int tm_mlp_mfma_pack_info(int D, int H, int nh, int mode, long* out);
 
 /* ---- electrostatics ----------------------------------------------------

@@ -1,0 +1,516 @@
+// Fused per-element MLPs on MFMA (gfx950): forward and input gradient of
+// both the energy and the charge network in one launch.
+//
+// Math restates energy_inference / dipole_inference
+// (TensorMol/TFNetworks/TFMolInstanceDirect.py:5164-5212, 5219-5272), three
+// hidden layers h_k = act(h_{k-1} W_k + b_k), act(z) = log(1 + e^{a z})/a
+// (Util.py:200-201), output y = h_3 w_o + b_o; and the adjoint that
+// tf.gradients builds (:5761) for dy/dG:
+//   dZ3 = w_o * act'(Z3), dH_{k-1} = dZ_k W_k^T, dZ_{k-1} = dH_{k-1} * act'(Z_{k-1}),
+//   dG = dZ1 W_1^T, act'(z) = sigmoid(a z).
+//
+// Work decomposition: one 512-thread workgroup per tile of MT atoms of one
+// (net, element) segment (atoms are element-sorted so a tile never straddles
+// an element).  Every GEMM is computed transposed, out^T = W^T in^T, with
+// v_mfma_f32_16x16x32_{bf16,f16}: A = weights (pre-packed in fragment order,
+// one 1 KiB coalesced load per fragment), B = activations from LDS
+// ([atom][feature], 16-byte reads), C = 16 features x 16 atoms.  Wave w owns
+// output features [64w, 64w+64) of every hidden layer, so the activation
+// derivative of its features stays in its registers from the forward to the
+// backward pass; only activations / upstream gradients cross waves, through
+// two LDS ping-pong buffers.  The descriptor block streams through LDS in
+// 64-feature chunks, converted from FP32 on the fly.
+//
+// Modes: 0 bf16, 1 fp16 (operands rounded once, FP32 accumulation),
+// 2 bf16 hi/lo split: x = hi + lo for weights and activations and
+// acc += Whi*xhi + Wlo*xhi + Whi*xlo (near-FP32 products).
+#include "tm_common.h"
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int NW = 8;       // waves per workgroup
+constexpr int FT = 4;       // 16-feature tiles per wave (64 features)
+constexpr int KC = 64;      // descriptor chunk staged per step
+constexpr int HMAX = 512;   // widest hidden layer
+
+struct MlpArgs {
+  int N, D, H, ldg;
+  int ntiles;
+  const int* tiles;        // [ntiles][2] = (segment, first row in segment)
+  const int* seg;          // [nseg][4] = (net, blob index, first row in perm, rows)
+  const float* G;
+  const int* perm;
+  const unsigned short* const* blobs;   // device array of per-(net,element) weight blobs
+  float alpha;
+  float* y;                // [2][N]
+  float* dG;               // [2][N][ldg]
+  int* flags;
+};
+
+// ---- packed blob layout (elements of 2 bytes), identical on the host ----
+__host__ __device__ inline long frag_elems(int rows16, int cols32) {
+  return (long)rows16 * cols32 * 512;
+}
+struct BlobLayout {
+  long f1, f2, f3, b3, b2, b1, fend;   // fragment planes (offsets in 2-byte elements)
+  long bias_bytes;                     // byte offset of the float section
+};
+__host__ __device__ inline BlobLayout blob_layout(int D, int H, int planes) {
+  BlobLayout L;
+  long o = 0;
+  L.f1 = o; o += planes * frag_elems(H / 16, D / 32);
+  L.f2 = o; o += planes * frag_elems(H / 16, H / 32);
+  L.f3 = o; o += planes * frag_elems(H / 16, H / 32);
+  L.b3 = o; o += planes * frag_elems(H / 16, H / 32);
+  L.b2 = o; o += planes * frag_elems(H / 16, H / 32);
+  L.b1 = o; o += planes * frag_elems(D / 16, H / 32);
+  L.fend = o;
+  L.bias_bytes = ((o * 2 + 255) / 256) * 256;
+  return L;
+}
+// float section: b1[H], b2[H], b3[H], wo[H], bo
+
+template <int MODE>
+struct Ty;
+template <>
+struct Ty<0> {
+  typedef bf16x8 vec;
+  static __device__ __forceinline__ unsigned short cvt(float x) {
+    return __builtin_bit_cast(unsigned short, (__bf16)x);
+  }
+  static __device__ __forceinline__ float back(unsigned short b) {
+    return (float)__builtin_bit_cast(__bf16, b);
+  }
+  static __device__ __forceinline__ floatx4 mma(const uint4& a, const uint4& b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                   __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+  }
+};
+template <>
+struct Ty<1> {
+  static __device__ __forceinline__ unsigned short cvt(float x) {
+    return __builtin_bit_cast(unsigned short, (_Float16)x);
+  }
+  static __device__ __forceinline__ float back(unsigned short b) {
+    return (float)__builtin_bit_cast(_Float16, b);
+  }
+  static __device__ __forceinline__ floatx4 mma(const uint4& a, const uint4& b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a),
+                                                  __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
+  }
+};
+template <>
+struct Ty<2> : Ty<0> {};
+
+__device__ __forceinline__ float softplus_a(float z, float alpha) {
+  float x = alpha * z;
+  return (fmaxf(x, 0.f) + __logf(1.0f + __expf(-fabsf(x)))) / alpha;
+}
+__device__ __forceinline__ float sigmoid_a(float z, float alpha) {
+  return 1.0f / (1.0f + __expf(-alpha * z));
+}
+
+template <int MODE, int MT>
+struct Smem {
+  static constexpr int PL = MODE == 2 ? 2 : 1;
+  static constexpr int SH = HMAX + 8;                // activation row stride (elements)
+  static constexpr int SC = KC + 8;                  // chunk row stride
+  unsigned short X[PL][MT][SH];
+  unsigned short Y[PL][MT][SH];                      // also the descriptor chunk buffer
+  float epart[NW][MT];
+  int atoms[MT];
+};
+
+// write 4 consecutive features (values v) of one atom into an activation buffer
+template <int MODE, int MT>
+__device__ __forceinline__ void put4(unsigned short (*buf)[MT][HMAX + 8], int atom, int f,
+                                     const float v[4]) {
+  unsigned short h[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) h[i] = Ty<MODE>::cvt(v[i]);
+  uint2 w;
+  w.x = (unsigned)h[0] | ((unsigned)h[1] << 16);
+  w.y = (unsigned)h[2] | ((unsigned)h[3] << 16);
+  *reinterpret_cast<uint2*>(&buf[0][atom][f]) = w;
+  if (MODE == 2) {
+    unsigned short l[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) l[i] = Ty<MODE>::cvt(v[i] - Ty<MODE>::back(h[i]));
+    uint2 u;
+    u.x = (unsigned)l[0] | ((unsigned)l[1] << 16);
+    u.y = (unsigned)l[2] | ((unsigned)l[3] << 16);
+    *reinterpret_cast<uint2*>(&buf[1][atom][f]) = u;
+  }
+}
+
+// acc[ft][at] += sum over K of W-fragments (rows tiles rt0 + ft) x B from LDS buffer
+template <int MODE, int MT, int NFT, int STRIDE>
+__device__ __forceinline__ void gemm_tiles(floatx4 (&acc)[NFT][MT / 16],
+                                           const unsigned short* __restrict__ wf, long plane_elems,
+                                           int nks, const int* rtile, const unsigned short* bbase,
+                                           long bplane, int lane) {
+  constexpr int AT = MT / 16;
+  const int brow = lane & 15;
+  const int bk = (lane >> 4) * 8;
+  for (int ks = 0; ks < nks; ++ks) {
+    uint4 a[NFT], b[AT];
+#pragma unroll
+    for (int ft = 0; ft < NFT; ++ft)
+      a[ft] = *reinterpret_cast<const uint4*>(wf + ((long)(rtile[ft] * nks + ks) * 64 + lane) * 8);
+#pragma unroll
+    for (int at = 0; at < AT; ++at)
+      b[at] = *reinterpret_cast<const uint4*>(bbase + (long)(at * 16 + brow) * STRIDE + ks * 32 + bk);
+#pragma unroll
+    for (int ft = 0; ft < NFT; ++ft)
+#pragma unroll
+      for (int at = 0; at < AT; ++at) acc[ft][at] = Ty<MODE>::mma(a[ft], b[at], acc[ft][at]);
+    if (MODE == 2) {
+      uint4 al[NFT], bl[AT];
+#pragma unroll
+      for (int ft = 0; ft < NFT; ++ft)
+        al[ft] = *reinterpret_cast<const uint4*>(wf + plane_elems +
+                                                 ((long)(rtile[ft] * nks + ks) * 64 + lane) * 8);
+#pragma unroll
+      for (int at = 0; at < AT; ++at)
+        bl[at] = *reinterpret_cast<const uint4*>(bbase + bplane + (long)(at * 16 + brow) * STRIDE +
+                                                 ks * 32 + bk);
+#pragma unroll
+      for (int ft = 0; ft < NFT; ++ft)
+#pragma unroll
+        for (int at = 0; at < AT; ++at) {
+          acc[ft][at] = Ty<MODE>::mma(al[ft], b[at], acc[ft][at]);
+          acc[ft][at] = Ty<MODE>::mma(a[ft], bl[at], acc[ft][at]);
+        }
+    }
+  }
+}
+
+template <int MODE, int MT>
+__global__ void __launch_bounds__(NW * 64)
+mlp_mfma_kernel(MlpArgs a) {
+  constexpr int AT = MT / 16;
+  constexpr int PL = MODE == 2 ? 2 : 1;
+  constexpr int SH = HMAX + 8;
+  constexpr int SC = KC + 8;
+  typedef Smem<MODE, MT> S;
+  __shared__ S sm;
+  const int tid = threadIdx.x;
+  const int w = tid >> 6, lane = tid & 63;
+  const int tile = blockIdx.x;
+  const int s = a.tiles[2 * tile], row0 = a.tiles[2 * tile + 1];
+  const int net = a.seg[4 * s], blob = a.seg[4 * s + 1], first = a.seg[4 * s + 2];
+  const int rows = a.seg[4 * s + 3];
+  const int D = a.D, H = a.H;
+  const BlobLayout BL = blob_layout(D, H, PL);
+  const unsigned short* wb = a.blobs[blob];
+  const float* fsec = reinterpret_cast<const float*>(reinterpret_cast<const char*>(wb) + BL.bias_bytes);
+  const float* b1 = fsec;
+  const float* b2 = fsec + H;
+  const float* b3 = fsec + 2 * H;
+  const float* wo = fsec + 3 * H;
+  const float bo = fsec[4 * H];
+  if (tid < MT) {
+    int r = row0 + tid;
+    sm.atoms[tid] = r < rows ? a.perm[first + r] : -1;
+  }
+  const bool wact = (w * 64) < H;   // this wave owns hidden features
+  int rt[FT];
+#pragma unroll
+  for (int ft = 0; ft < FT; ++ft) rt[ft] = 4 * w + ft;
+  const int fq = (lane >> 4) * 4;   // first of the 4 features this lane holds in a C tile
+  const int ca = lane & 15;         // atom column of this lane in a C tile
+  // activation derivatives of layers 1 and 2 for this wave's features
+  float s1[FT][AT][4], s2[FT][AT][4];
+  floatx4 acc[FT][AT];
+
+  // ------------------------------------------------------------ layer 1
+#pragma unroll
+  for (int ft = 0; ft < FT; ++ft)
+#pragma unroll
+    for (int at = 0; at < AT; ++at) acc[ft][at] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const long p1 = frag_elems(H / 16, D / 32);
+  for (int kc = 0; kc < D; kc += KC) {
+    __syncthreads();
+    // stage G[atom][kc .. kc+KC) as bf16 (hi/lo) into Y (row stride SC)
+    for (int e = tid; e < MT * KC / 4; e += NW * 64) {
+      int r = e / (KC / 4), c4 = (e % (KC / 4)) * 4;
+      int atom = sm.atoms[r];
+      float4 g = atom >= 0 ? *reinterpret_cast<const float4*>(a.G + (long)atom * a.ldg + kc + c4)
+                           : make_float4(0.f, 0.f, 0.f, 0.f);
+      float v[4] = {g.x, g.y, g.z, g.w};
+      unsigned short* dst = &sm.Y[0][0][0] + (long)r * SC + c4;
+      unsigned short h[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) h[i] = Ty<MODE>::cvt(v[i]);
+      uint2 ww;
+      ww.x = (unsigned)h[0] | ((unsigned)h[1] << 16);
+      ww.y = (unsigned)h[2] | ((unsigned)h[3] << 16);
+      *reinterpret_cast<uint2*>(dst) = ww;
+      if (MODE == 2) {
+        unsigned short l[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) l[i] = Ty<MODE>::cvt(v[i] - Ty<MODE>::back(h[i]));
+        uint2 u;
+        u.x = (unsigned)l[0] | ((unsigned)l[1] << 16);
+        u.y = (unsigned)l[2] | ((unsigned)l[3] << 16);
+        *reinterpret_cast<uint2*>(dst + (long)MT * SC) = u;
+      }
+    }
+    __syncthreads();
+    if (wact) {
+      // fragments of W1 for k-steps kc/32 .. : offset the fragment base by ks0
+      const int ks0 = kc / 32;
+      const int nks_all = D / 32;
+      const unsigned short* bb = &sm.Y[0][0][0];
+      const int brow = lane & 15, bk = (lane >> 4) * 8;
+      for (int ks = 0; ks < KC / 32; ++ks) {
+        uint4 af[FT], bf[AT];
+#pragma unroll
+        for (int ft = 0; ft < FT; ++ft)
+          af[ft] = *reinterpret_cast<const uint4*>(
+              wb + BL.f1 + ((long)(rt[ft] * nks_all + ks0 + ks) * 64 + lane) * 8);
+#pragma unroll
+        for (int at = 0; at < AT; ++at)
+          bf[at] = *reinterpret_cast<const uint4*>(bb + (long)(at * 16 + brow) * SC + ks * 32 + bk);
+#pragma unroll
+        for (int ft = 0; ft < FT; ++ft)
+#pragma unroll
+          for (int at = 0; at < AT; ++at) acc[ft][at] = Ty<MODE>::mma(af[ft], bf[at], acc[ft][at]);
+        if (MODE == 2) {
+          uint4 al[FT], bl[AT];
+#pragma unroll
+          for (int ft = 0; ft < FT; ++ft)
+            al[ft] = *reinterpret_cast<const uint4*>(
+                wb + BL.f1 + p1 + ((long)(rt[ft] * nks_all + ks0 + ks) * 64 + lane) * 8);
+#pragma unroll
+          for (int at = 0; at < AT; ++at)
+            bl[at] = *reinterpret_cast<const uint4*>(bb + (long)MT * SC + (long)(at * 16 + brow) * SC +
+                                                     ks * 32 + bk);
+#pragma unroll
+          for (int ft = 0; ft < FT; ++ft)
+#pragma unroll
+            for (int at = 0; at < AT; ++at) {
+              acc[ft][at] = Ty<MODE>::mma(al[ft], bf[at], acc[ft][at]);
+              acc[ft][at] = Ty<MODE>::mma(af[ft], bl[at], acc[ft][at]);
+            }
+        }
+      }
+    }
+  }
+  // epilogue 1 -> X, s1
+  if (wact) {
+#pragma unroll
+    for (int ft = 0; ft < FT; ++ft)
+#pragma unroll
+      for (int at = 0; at < AT; ++at) {
+        int f = 64 * w + 16 * ft + fq;
+        float v[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          float z = acc[ft][at][i] + b1[f + i];
+          v[i] = softplus_a(z, a.alpha);
+          s1[ft][at][i] = sigmoid_a(z, a.alpha);
+        }
+        put4<MODE, MT>(sm.X, at * 16 + ca, f, v);
+      }
+  }
+  __syncthreads();
+  const long phh = frag_elems(H / 16, H / 32);
+  // ------------------------------------------------------------ layer 2
+  if (wact) {
+#pragma unroll
+    for (int ft = 0; ft < FT; ++ft)
+#pragma unroll
+      for (int at = 0; at < AT; ++at) acc[ft][at] = floatx4{0.f, 0.f, 0.f, 0.f};
+    gemm_tiles<MODE, MT, FT, SH>(acc, wb + BL.f2, phh, H / 32, rt, &sm.X[0][0][0], (long)MT * SH,
+                                 lane);
+#pragma unroll
+    for (int ft = 0; ft < FT; ++ft)
+#pragma unroll
+      for (int at = 0; at < AT; ++at) {
+        int f = 64 * w + 16 * ft + fq;
+        float v[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          float z = acc[ft][at][i] + b2[f + i];
+          v[i] = softplus_a(z, a.alpha);
+          s2[ft][at][i] = sigmoid_a(z, a.alpha);
+        }
+        put4<MODE, MT>(sm.Y, at * 16 + ca, f, v);
+      }
+  }
+  __syncthreads();
+  // ------------------------------------------------------------ layer 3 + head
+  if (wact) {
+#pragma unroll
+    for (int ft = 0; ft < FT; ++ft)
+#pragma unroll
+      for (int at = 0; at < AT; ++at) acc[ft][at] = floatx4{0.f, 0.f, 0.f, 0.f};
+    gemm_tiles<MODE, MT, FT, SH>(acc, wb + BL.f3, phh, H / 32, rt, &sm.Y[0][0][0], (long)MT * SH,
+                                 lane);
+    float ep[AT];
+#pragma unroll
+    for (int at = 0; at < AT; ++at) ep[at] = 0.f;
+#pragma unroll
+    for (int ft = 0; ft < FT; ++ft)
+#pragma unroll
+      for (int at = 0; at < AT; ++at) {
+        int f = 64 * w + 16 * ft + fq;
+        float v[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          float z = acc[ft][at][i] + b3[f + i];
+          float wv = wo[f + i];
+          ep[at] += softplus_a(z, a.alpha) * wv;
+          v[i] = wv * sigmoid_a(z, a.alpha);   // dZ3 with upstream 1
+        }
+        put4<MODE, MT>(sm.X, at * 16 + ca, f, v);
+      }
+#pragma unroll
+    for (int at = 0; at < AT; ++at) {
+      float p = ep[at];
+      p += __shfl_xor(p, 16, 64);
+      p += __shfl_xor(p, 32, 64);
+      if (lane < 16) sm.epart[w][at * 16 + lane] = p;
+    }
+  } else {
+    for (int t = lane; t < MT; t += 64) sm.epart[w][t] = 0.f;
+  }
+  __syncthreads();
+  if (tid < MT) {
+    int atom = sm.atoms[tid];
+    if (atom >= 0) {
+      float yv = bo;
+#pragma unroll
+      for (int q = 0; q < NW; ++q) yv += sm.epart[q][tid];
+      a.y[(long)net * a.N + atom] = yv;
+    }
+  }
+  // ------------------------------------------------------------ backward 3: X (dZ3) -> Y (dZ2)
+  if (wact) {
+#pragma unroll
+    for (int ft = 0; ft < FT; ++ft)
+#pragma unroll
+      for (int at = 0; at < AT; ++at) acc[ft][at] = floatx4{0.f, 0.f, 0.f, 0.f};
+    gemm_tiles<MODE, MT, FT, SH>(acc, wb + BL.b3, phh, H / 32, rt, &sm.X[0][0][0], (long)MT * SH,
+                                 lane);
+#pragma unroll
+    for (int ft = 0; ft < FT; ++ft)
+#pragma unroll
+      for (int at = 0; at < AT; ++at) {
+        int f = 64 * w + 16 * ft + fq;
+        float v[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] = acc[ft][at][i] * s2[ft][at][i];
+        put4<MODE, MT>(sm.Y, at * 16 + ca, f, v);
+      }
+  }
+  __syncthreads();
+  // ------------------------------------------------------------ backward 2: Y (dZ2) -> X (dZ1)
+  if (wact) {
+#pragma unroll
+    for (int ft = 0; ft < FT; ++ft)
+#pragma unroll
+      for (int at = 0; at < AT; ++at) acc[ft][at] = floatx4{0.f, 0.f, 0.f, 0.f};
+    gemm_tiles<MODE, MT, FT, SH>(acc, wb + BL.b2, phh, H / 32, rt, &sm.Y[0][0][0], (long)MT * SH,
+                                 lane);
+#pragma unroll
+    for (int ft = 0; ft < FT; ++ft)
+#pragma unroll
+      for (int at = 0; at < AT; ++at) {
+        int f = 64 * w + 16 * ft + fq;
+        float v[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] = acc[ft][at][i] * s1[ft][at][i];
+        put4<MODE, MT>(sm.X, at * 16 + ca, f, v);
+      }
+  }
+  __syncthreads();
+  // ------------------------------------------------------------ backward 1: X (dZ1) -> dG
+  {
+    const long pd = frag_elems(D / 16, H / 32);
+    float* dGn = a.dG + (long)net * a.N * a.ldg;
+    const int ntile = D / 16;
+    for (int t0 = w; t0 < ntile; t0 += 2 * NW) {
+      int rt2[2] = {t0, t0 + NW < ntile ? t0 + NW : t0};
+      floatx4 acc2[2][AT];
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int at = 0; at < AT; ++at) acc2[u][at] = floatx4{0.f, 0.f, 0.f, 0.f};
+      gemm_tiles<MODE, MT, 2, SH>(acc2, wb + BL.b1, pd, H / 32, rt2, &sm.X[0][0][0], (long)MT * SH,
+                                  lane);
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        if (u == 1 && rt2[1] == rt2[0]) continue;
+#pragma unroll
+        for (int at = 0; at < AT; ++at) {
+          int atom = sm.atoms[at * 16 + ca];
+          if (atom < 0) continue;
+          int f = 16 * rt2[u] + fq;
+          *reinterpret_cast<float4*>(dGn + (long)atom * a.ldg + f) =
+              make_float4(acc2[u][at][0], acc2[u][at][1], acc2[u][at][2], acc2[u][at][3]);
+        }
+      }
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+// out[0] = blob bytes, out[1] = byte offset of the float section,
+// out[2..7] = element offsets of f1, f2, f3, b3, b2, b1 fragment arrays,
+// out[8] = planes
+int tm_mlp_mfma_pack_info(int D, int H, int nh, int mode, long* out) {
+  TM_REQUIRE(nh == 3, "tm_mlp_mfma: exactly 3 hidden layers supported (got %d)", nh);
+  TM_REQUIRE(H % 64 == 0 && H <= HMAX && H >= 64, "tm_mlp_mfma: hidden width %d not a multiple of 64 in 64..512", H);
+  TM_REQUIRE(D % KC == 0 && D > 0, "tm_mlp_mfma: descriptor length %d not a multiple of %d", D, KC);
+  TM_REQUIRE(mode >= 0 && mode <= 2, "tm_mlp_mfma: bad mode %d", mode);
+  int planes = mode == 2 ? 2 : 1;
+  BlobLayout L = blob_layout(D, H, planes);
+  out[0] = L.bias_bytes + (4l * H + 4) * 4;
+  out[1] = L.bias_bytes;
+  out[2] = L.f1; out[3] = L.f2; out[4] = L.f3; out[5] = L.b3; out[6] = L.b2; out[7] = L.b1;
+  out[8] = planes;
+  return TM_OK;
+}
+
+// tiles: device int32 [ntiles][2] = (segment, first row); seg: device int32
+// [nseg][4] = (net, blob index, first row in perm, rows); blobs: device array
+// of per-(net, element) weight blob pointers (layout of tm_mlp_mfma_pack_info).
+int tm_mlp_mfma(int N, int D, int H, int ntiles, const int* tiles, const int* seg, int ldg,
+                const float* G, const int* perm, const void* blobs, float alpha, int mode,
+                float* y, float* dG, void* stream) {
+  long info[9];
+  int st = tm_mlp_mfma_pack_info(D, H, 3, mode, info);
+  if (st) return st;
+  TM_REQUIRE(ldg % 4 == 0 && ldg >= D, "tm_mlp_mfma: ldg %d", ldg);
+  TM_REQUIRE(ntiles >= 0, "tm_mlp_mfma: bad tile count");
+  if (ntiles == 0) return TM_OK;
+  MlpArgs a;
+  a.N = N; a.D = D; a.H = H; a.ldg = ldg;
+  a.ntiles = ntiles;
+  a.tiles = tiles;
+  a.seg = seg;
+  a.G = G;
+  a.perm = perm;
+  a.blobs = (const unsigned short* const*)blobs;
+  a.alpha = alpha;
+  a.y = y;
+  a.dG = dG;
+  a.flags = nullptr;
+  hipStream_t s = (hipStream_t)stream;
+  if (mode == 0) mlp_mfma_kernel<0, 64><<<ntiles, NW * 64, 0, s>>>(a);
+  else if (mode == 1) mlp_mfma_kernel<1, 64><<<ntiles, NW * 64, 0, s>>>(a);
+  else mlp_mfma_kernel<2, 32><<<ntiles, NW * 64, 0, s>>>(a);
+  TM_LAUNCH_CHECK();
+  return TM_OK;
+}
+
+}  // extern "C"

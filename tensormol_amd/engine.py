@@ -172,10 +172,11 @@ class BPEEEngine:
         ws.rev = torch.zeros(N * self.ldr, **i32)
         ws.flags = torch.zeros(8, **i32)
         ws.G = torch.zeros(N, self.D, **f32)
-        ws.JE = torch.zeros(N, self.D, **f32)
-        ws.JQ = torch.zeros(N, self.D, **f32)
-        ws.eps = torch.zeros(N, **f32)
-        ws.qraw = torch.zeros(N, **f32)
+        # outputs of the two networks: y[net, atom], dy/dG[net, atom, :]
+        ws.Y2 = torch.zeros(2, N, **f32)
+        ws.J2 = torch.zeros(2, N, self.D, **f32)
+        ws.eps, ws.qraw = ws.Y2[0], ws.Y2[1]
+        ws.JE, ws.JQ = ws.J2[0], ws.J2[1]
         ws.qmean = torch.zeros(B, **f64)
         ws.phimean = torch.zeros(B, **f64)
         ws.xq = torch.zeros(N, 4, **f32)

@@ -1,0 +1,108 @@
+"""Host side of the fused MFMA MLP kernel: weight packing and tile lists.
+
+Weights of every (net, element) pair are packed once into a blob whose layout
+tm_mlp_mfma_pack_info reports: six fragment arrays (forward W1^T, W2^T, W3^T;
+backward W3, W2, W1) in the operand order of v_mfma_f32_16x16x32 (A operand:
+lane l holds row 16*t + l%16, columns 32*s + 8*(l//16) .. +7), followed by the
+float32 biases and output layer.  The split mode stores a hi and a lo plane.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+
+MODES = {"bf16": 0, "fp16": 1, "bf16x3": 2}
+TILE_ROWS = {0: 64, 1: 64, 2: 32}
+
+
+def frag_pack(M):
+    """[R, C] matrix -> A-operand fragments [R/16][C/32][64 lanes][8]."""
+    R, C = M.shape
+    return M.reshape(R // 16, 16, C // 32, 4, 8).transpose(0, 2, 3, 1, 4).reshape(-1)
+
+
+def _to16(x, mode):
+    t = torch.tensor(np.ascontiguousarray(x), dtype=torch.float32)
+    if mode == 1:
+        h = t.to(torch.float16)
+        return h.view(torch.int16).numpy(), h.to(torch.float32).numpy()
+    h = t.to(torch.bfloat16)
+    return h.view(torch.int16).numpy(), h.to(torch.float32).numpy()
+
+
+class MFMAWeights:
+    def __init__(self, engine, precision):
+        if precision not in MODES:
+            raise ValueError(f"unknown MLP precision {precision!r}; use fp32, bf16, fp16 or bf16x3")
+        self.engine = engine
+        self.mode = MODES[precision]
+        self.mt = TILE_ROWS[self.mode]
+        model = engine.model
+        self.D = model.sym.D
+        if len(model.hidden) != 3 or len(set(model.hidden)) != 1:
+            raise ValueError("the MFMA path needs three equal hidden layers")
+        self.H = int(model.hidden[0])
+        info = (ctypes.c_long * 9)()
+        engine.lib.tm_mlp_mfma_pack_info(self.D, self.H, 3, self.mode, info)
+        self.info = list(info)
+        self.blobs = []
+        for tag, nets in (("E", model.energy_nets), ("Q", model.charge_nets)):
+            for e in model.eles:
+                self.blobs.append(self._blob(nets[e]))
+        dev = engine.device
+        self.blob_t = [torch.tensor(b, device=dev) for b in self.blobs]
+        self.blob_ptrs = torch.tensor([t.data_ptr() for t in self.blob_t], dtype=torch.int64,
+                                      device=dev)
+
+    def _blob(self, layers):
+        (W1, b1), (W2, b2), (W3, b3), (Wo, bo) = layers
+        nbytes, fsec = self.info[0], self.info[1]
+        offs = self.info[2:8]
+        planes = self.info[8]
+        blob = np.zeros(nbytes, dtype=np.uint8)
+        el = blob[:fsec].view(np.int16)
+        for off, M in zip(offs, (W1.T, W2.T, W3.T, W3, W2, W1)):
+            hi_bits, hi_val = _to16(M, self.mode)
+            packed = frag_pack(hi_bits.reshape(M.shape))
+            n = packed.size
+            el[off:off + n] = packed
+            if planes == 2:
+                lo_bits, _ = _to16(np.asarray(M, np.float64) - hi_val, self.mode)
+                el[off + n:off + 2 * n] = frag_pack(lo_bits.reshape(M.shape))
+        H = self.H
+        fl = blob[fsec:].view(np.float32)
+        fl[0:H] = b1
+        fl[H:2 * H] = b2
+        fl[2 * H:3 * H] = b3
+        fl[3 * H:4 * H] = np.asarray(Wo).reshape(-1)
+        fl[4 * H] = np.asarray(bo).reshape(-1)[0]
+        return blob
+
+    def workspace(self, ws):
+        """Segment and tile tables for one packed batch (rows are element-sorted)."""
+        nele = self.engine.nele
+        segs, tiles = [], []
+        for net in range(2):
+            for a in range(nele):
+                M = int(ws.eoff[a + 1] - ws.eoff[a])
+                if M == 0:
+                    continue
+                s = len(segs)
+                segs.append((net, net * nele + a, int(ws.eoff[a]), M))
+                for r0 in range(0, M, self.mt):
+                    tiles.append((s, r0))
+        dev = self.engine.device
+        t = dict(seg=torch.tensor(np.array(segs, dtype=np.int32).reshape(-1, 4), device=dev),
+                 tiles=torch.tensor(np.array(tiles, dtype=np.int32).reshape(-1, 2), device=dev),
+                 ntiles=len(tiles))
+        return t
+
+    def run(self, ws):
+        eng = self.engine
+        m = ws.mlp_ws
+        eng.lib.tm_mlp_mfma(ws.N, self.D, self.H, m["ntiles"], _lib.ptr(m["tiles"]),
+                            _lib.ptr(m["seg"]), self.D, _lib.ptr(ws.G), _lib.ptr(ws.perm),
+                            _lib.ptr(self.blob_ptrs), float(eng.model.alpha), self.mode,
+                            _lib.ptr(ws.Y2), _lib.ptr(ws.J2), eng._stream())

@@ -1,0 +1,59 @@
+"""C-ABI library checks that need no GPU: it loads, exports every symbol the
+header declares, and reports argument errors through tm_last_error."""
+import ctypes
+import glob
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        txt = open(h).read()
+        names |= set(re.findall(r"^(?:int|const char\*)\s+(tm_\w+)\s*\(", txt, re.M))
+    return names
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from tensormol_amd import build, _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        build.build()
+    return _lib.lib()
+
+
+def test_header_declares_entry_points():
+    names = _declared()
+    for n in ("tm_nl_build", "tm_sym_forward", "tm_sym_adjoint", "tm_mlp_mfma", "tm_coulomb",
+              "tm_vv_kick", "tm_nh_kick", "tm_fire_update"):
+        assert n in names
+
+
+def test_every_declared_symbol_is_exported(lib):
+    for n in _declared():
+        assert hasattr(lib.dll, n), f"{n} declared in include/ but not exported"
+
+
+def test_python_binding_covers_header(lib):
+    from tensormol_amd._lib import SIGNATURES
+    missing = {n for n in _declared() if n not in SIGNATURES and n != "tm_last_error"}
+    assert not missing, missing
+    for n in SIGNATURES:
+        assert n in _declared(), f"{n} bound in _lib but not declared in the header"
+
+
+def test_version_and_argument_errors(lib):
+    assert lib.tm_version() == 1
+    with pytest.raises(RuntimeError, match="bad nele"):
+        lib.tm_nl_build(10, None, None, None, None, None, None, None, None, 1, 1, 1, 0, 0, 1.0, 1.0,
+                        128, 48, None, None, None, None, None, None, None, None, None)
+    with pytest.raises(RuntimeError, match="3 hidden layers"):
+        out = (ctypes.c_long * 9)()
+        lib.tm_mlp_mfma_pack_info(256, 512, 2, 0, out)
+    out = (ctypes.c_long * 9)()
+    lib.tm_mlp_mfma_pack_info(256, 512, 3, 0, out)
+    assert out[8] == 1 and out[0] > 0
