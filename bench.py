@@ -1,0 +1,249 @@
+"""Benchmark: NVE molecular dynamics of the periodic 4096-water box (12,288 atoms)
+on one MI355X, BP+EE energy and forces every step.
+
+Workload: rigid waters on a jittered 16^3 lattice in a 49.7 A cube (seed 1),
+minimum-image cell lists, 2-element descriptor of length 256, energy and
+charge nets 256-512-512-512-1 softplus with seeded Glorot weights, DSF alpha
+0.2/A with a 15 A cutoff, Maxwell-Boltzmann velocities at 300 K, Velocity
+Verlet at dt = 0.5 fs.  Descriptors FP32, MLP GEMMs BF16 on MFMA with FP32
+accumulation, forces by the hand-written adjoint, FP64 positions, velocities
+and energy reductions.
+
+A step is one Velocity-Verlet step: drift, neighbour rebuild, full E+F, kick.
+value = atoms * timed steps / wall time of the timed steps (inputs resident in
+HBM).  The JSON line also carries the force RMS error of frame 0 against the
+FP64 CPU oracle, the plain-FP32 path's rate on the same box, the roofline of
+the MLP kernel and the CPU baseline (the oracle, timed on this host).
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+MLP_PEAK_TFLOPS = {"bf16": 2500.0, "fp16": 2500.0, "bf16x3": 2500.0, "fp32": 157.3}
+
+
+def mlp_flops_per_atom(D, H):
+    """Two nets x (forward + input gradient): 4 * 2 * (D*H + 2*H*H + H)."""
+    return 4 * 2 * (D * H + 2 * H * H + H)
+
+
+def build_system(args):
+    from tensormol_amd import workloads as W
+    from tensormol_amd.model import BPEEModel
+    Z, x, box = W.water_box(nside=args.nside, edge=args.edge, seed=1)
+    model = BPEEModel([1, 8], dsf_alpha=0.2, seed=0)
+    return Z, x, box, model
+
+
+def run_md(eng, ws, Z, x, box, steps, warmup, dt, graph=True, mlp_samples=20):
+    """NVE Velocity Verlet on the device.  Returns wall time of the timed steps,
+    per-step times (HIP events between steps), live MLP-kernel launch times
+    (HIP events around the launch, on the stream it runs on) and the total
+    energy before and after the timed steps."""
+    import torch
+    from tensormol_amd import _lib, params
+    from tensormol_amd import workloads as W
+    dev = eng.device
+    m = torch.tensor(params.atomic_masses(Z), device=dev)
+    v0 = W.maxwell_boltzmann(Z, 300.0, 1, params.atomic_masses(Z))
+    pos = torch.tensor(x, dtype=torch.float64, device=dev)
+    v = torch.tensor(v0, device=dev)
+    a = torch.zeros_like(v)
+    ke = torch.zeros(ws.N, dtype=torch.float64, device=dev)
+    L = eng.lib
+    P = _lib.ptr
+    side = torch.cuda.Stream(device=dev)
+    side.wait_stream(torch.cuda.current_stream(dev))
+    mlp_ms = []
+    with torch.cuda.stream(side):
+        st = eng._stream()
+        # initial forces -> accelerations (a = F/m before the first drift)
+        eng.run(ws, pos)
+        L.tm_vv_kick(ws.N, 0.0, P(ws.force), P(m), P(v), P(a), P(ke), st)
+
+        def step(record=False):
+            L.tm_vv_drift(ws.N, dt, P(v), P(a), P(pos), st)
+            eng.build_lists(ws, pos)
+            eng.descriptors(ws, pos)
+            if record:
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(side)
+            eng.networks(ws)
+            if record:
+                e1.record(side)
+                mlp_ms.append((e0, e1))
+            eng.electrostatics(ws, pos)
+            eng.adjoint(ws, pos)
+            eng.reductions(ws)
+            L.tm_vv_kick(ws.N, dt, P(ws.force), P(m), P(v), P(a), P(ke), st)
+
+        for _ in range(warmup):
+            step()
+        # live launch time of the MLP kernel, on the stream it is launched on
+        for _ in range(mlp_samples):
+            step(record=True)
+        eng.check_flags(ws)
+        g = None
+        if graph:
+            g = torch.cuda.CUDAGraph()
+            side.synchronize()
+    if graph:
+        with torch.cuda.graph(g, stream=side):
+            step()
+        replay = g.replay
+    else:
+        replay = step
+    side.synchronize()
+    with torch.cuda.stream(side):
+        e_start = (eng.energy_tensor(ws).item(), ke.sum().item())
+        step_ev = []
+        first = torch.cuda.Event(enable_timing=True)
+        side.synchronize()
+        t0 = time.perf_counter()
+        first.record(side)
+        for _ in range(steps):
+            replay()
+            ev = torch.cuda.Event(enable_timing=True)
+            ev.record(side)
+            step_ev.append(ev)
+        side.synchronize()
+        t1 = time.perf_counter()
+        eng.check_flags(ws)
+        e_end = (eng.energy_tensor(ws).item(), ke.sum().item())
+    per_step = []
+    prev = first
+    for ev in step_ev:
+        per_step.append(prev.elapsed_time(ev))
+        prev = ev
+    mlp = np.array([a_.elapsed_time(b_) for a_, b_ in mlp_ms])
+    return dict(wall=t1 - t0, per_step_ms=np.array(per_step), mlp_ms=mlp, e_start=e_start,
+                e_end=e_end, graph=graph)
+
+
+def cpu_baseline(Z, x, box, model):
+    """The FP64 oracle on this host: one full E+F of the box (neighbour lists
+    included), which is also the frame-0 reference for the force error."""
+    import torch
+    from oracle import oracle as O
+    torch.set_num_threads(os.cpu_count() or 1)
+    sp = O.SymParams(model.eles)
+    t0 = time.perf_counter()
+    r = O.evaluate(x, Z, np.array([0, len(Z)]), sp, model.energy_nets, model.charge_nets,
+                   alpha=model.alpha, dsf_alpha=model.dsf_alpha, has_vdw=model.has_vdw,
+                   boxes=box[None])
+    dt = time.perf_counter() - t0
+    return r, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--precision", default="bf16")
+    ap.add_argument("--nside", type=int, default=16)
+    ap.add_argument("--edge", type=float, default=49.7)
+    ap.add_argument("--dt", type=float, default=0.5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus}); "
+              "the single-GPU value is the figure of merit", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from tensormol_amd.engine import BPEEEngine
+
+    Z, x, box, model = build_system(args)
+    N = len(Z)
+    eng = BPEEEngine(model, precision=args.precision, check_overflow=False)
+    ws = eng.prepare(Z, np.array([0, N]), box[None])
+    print(f"[bench] {N} atoms, precision {args.precision}, warmup {args.warmup}, steps {args.steps}",
+          file=sys.stderr, flush=True)
+    res = run_md(eng, ws, Z, x, box, args.steps, args.warmup, args.dt, graph=not args.no_graph)
+    value = N * args.steps / res["wall"]
+    ms_per_step = 1e3 * res["wall"] / args.steps
+    print(f"[bench] {value:.4g} atom-steps/s, {ms_per_step:.3f} ms/step", file=sys.stderr, flush=True)
+
+    # frame-0 force error of this precision vs the FP64 oracle (+ CPU baseline timing)
+    pos0 = torch.tensor(x, dtype=torch.float64, device=eng.device)
+    eng.run(ws, pos0, check=True)
+    torch.cuda.synchronize()
+    r_gpu = eng.results(ws)
+    cpu = None
+    force_rms_err = None
+    energy_err = None
+    if not args.no_cpu_baseline:
+        ref, cpu_s = cpu_baseline(Z, x, box, model)
+        force_rms_err = float(np.sqrt(((r_gpu["gradient"] - ref["gradient"]) ** 2).mean()))
+        energy_err = float(abs(r_gpu["Etotal"][0] - ref["Etotal"][0]))
+        cpu = {"value": N / cpu_s, "unit": "atom-steps/s", "cores": torch.get_num_threads(),
+               "kind": "port",
+               "sample": f"one FP64 E+F of the same {N}-atom box (neighbour lists included) by the "
+                         f"numpy/torch-CPU oracle, {cpu_s:.1f} s"}
+        print(f"[bench] oracle {cpu_s:.1f} s; force rms err {force_rms_err:.3e} Ha/A; |dE| "
+              f"{energy_err:.3e} Ha", file=sys.stderr, flush=True)
+
+    # plain FP32 path (VALU GEMMs, every intermediate in HBM) on the same box
+    fp32_value = None
+    if not args.no_fp32 and args.precision != "fp32":
+        eng32 = BPEEEngine(model, precision="fp32", check_overflow=False)
+        ws32 = eng32.prepare(Z, np.array([0, N]), box[None])
+        r32 = run_md(eng32, ws32, Z, x, box, 10, 2, args.dt, graph=False, mlp_samples=0)
+        fp32_value = N * 10 / r32["wall"]
+        print(f"[bench] plain fp32 path {fp32_value:.4g} atom-steps/s", file=sys.stderr, flush=True)
+
+    mlp_avg_ms = float(res["mlp_ms"].mean())
+    flops = mlp_flops_per_atom(eng.D, int(model.hidden[0])) * N
+    achieved = flops / (mlp_avg_ms * 1e-3) / 1e12
+    peak = MLP_PEAK_TFLOPS[args.precision]
+    roofline = {"bound": "mfma" if args.precision != "fp32" else "valu", "kernel": "mlp_mfma_kernel",
+                "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
+                "frac": round(achieved / peak, 4), "traffic": None,
+                "flops_per_launch": flops, "avg_launch_ms": round(mlp_avg_ms, 4)}
+    # total energy (Ha) at the start and end of the timed steps; KE sums are
+    # m v^2 / 2 in kg/mol A^2/fs^2 = 1e10 J/mol
+    etot = lambda e: e[0] + e[1] * 1e10 / 2625499.638
+    drift = etot(res["e_end"]) - etot(res["e_start"])
+    out = {
+        "metric": "atom-steps/s (E+F) at force RMS<1e-3 Ha/A, BF16-MFMA vs plain FP32; p99 ms",
+        "value": round(value, 1),
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "p50_ms": round(float(np.percentile(res["per_step_ms"], 50)), 4),
+        "p99_ms": round(float(np.percentile(res["per_step_ms"], 99)), 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.precision,
+        "data": "synthetic",
+        "config": {"workload": f"periodic {N // 3}-water box NVE MD ({N} atoms, {args.edge} A cube), "
+                               "BP+EE 256-512-512-512-1 x2 nets, DSF 15 A",
+                   "atoms": N, "dt_fs": args.dt, "parallelism": "single GPU",
+                   "hip_graph": not args.no_graph},
+        "nve_energy_drift_Ha": round(float(drift), 6),
+        "force_rms_err_vs_fp64": force_rms_err,
+        "energy_err_vs_fp64_Ha": energy_err,
+        "plain_fp32_value": None if fp32_value is None else round(fp32_value, 1),
+        "speedup_vs_plain_fp32": None if fp32_value is None else round(value / fp32_value, 2),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

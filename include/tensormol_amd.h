@@ -3,9 +3,9 @@
  * Conventions
  *   - Every pointer argument is a device pointer unless documented as a host
  *     array; sizes are element counts; `stream` is a hipStream_t.
- *   - Calls only enqueue work on `stream` (no host synchronisation), so they
- *     can be captured into a HIP graph.  The exception is tm_cell_build, which
- *     clears its counters with hipMemsetAsync.
+ *   - Calls only enqueue kernels on `stream` (no host synchronisation, no
+ *     memset or copy nodes), so a sequence of them can be captured into a HIP
+ *     graph and replayed.
  *   - Return value: 0 on success, negative on error; tm_last_error() then
  *     holds a message (thread-local).
  *   - Packed atoms: molecule m owns atoms [mol_start[m], mol_start[m+1]);
@@ -36,7 +36,8 @@ int tm_zero_i32(int n, int* p, void* stream);
  *
  * tm_cell_build bins atoms into an nx*ny*nz grid of cells with origin o and
  * edge c (periodic: positions wrapped; open: clamped, flag 2 raised) and
- * sorts each cell's atoms by index.
+ * sorts each cell's atoms by index.  Buffers: cell_of[N], cell_count[ncell],
+ * cell_start[ncell+1], cell_fill[max(ncell, N)], cell_atoms[N].
  */
 int tm_cell_build(int N, const double* pos, double ox, double oy, double oz, double cx, double cy,
                   double cz, int nx, int ny, int nz, int periodic, int* cell_of, int* cell_count,
@@ -62,6 +63,16 @@ int tm_nl_build(int N, const double* pos, const int* elem, const int* mol, const
  * acnt[i]*(acnt[i]-1)/2 (int64). */
 int tm_triples(int N, int ldr, int lda, int nele, const int* nbr, const int* ang, const int* acnt,
                const int* aseg, const long* trip_start, int* out_jkpr, void* stream);
+
+/* Pair list for any cutoff (replaces Make_NListNaive with DoPerms 0/1 and the
+ * EE list NeighborListSet(..., DoPerms=False).buildPairs, Neighbors.py:262-282):
+ * call with start == NULL to get count[i], then with start = exclusive prefix
+ * (int64) to fill out[start[i] ..].  nreal (per molecule, may be NULL) limits
+ * the centres; half = 1 keeps j > i only; reach = cell stencil radius. */
+int tm_pairs(int N, const double* pos, const int* mol, const int* mol_start, const int* nreal,
+             const double* box, const int* cell_start, const int* cell_atoms, const int* cell_of,
+             int nx, int ny, int nz, int periodic, int reach, double s_c, int half, int* count,
+             const long* start, int* out, void* stream);
 
 /* ---- descriptors -------------------------------------------------------
  * TFSymRSet_Linear_WithEle + TFSymASet_Linear_WithEle concatenated as in
@@ -128,12 +139,14 @@ int tm_charges(int N, const int* mol, const double* qmean, const float* qraw, co
  * cell grid (or the molecule).  consts (host f64[7]) = alpha', R_sr, R_lr, ZZ,
  * YY, elu_shift, elu_alpha in bohr units; c6/r0 host f64[nele].  Outputs
  * phi[i] = sum_j K(r_ij) q_j, grad (dE/dx at fixed q, Ha/A), eatom[2i] =
- * q_i phi_i / 2, eatom[2i+1] = vdW energy share. */
+ * q_i phi_i / 2, eatom[2i+1] = vdW energy share.  With a cell grid xqc
+ * (float4 [N]) and elc (int [N]) are scratch for the cell-ordered copies. */
 int tm_coulomb(int N, const float* xq, const int* elem, const int* mol, const int* mol_start,
                const double* box, const int* cell_start, const int* cell_atoms, const int* cell_of,
                int nx, int ny, int nz, int periodic, int reach, const double* consts,
                double rc_ee, double width, int add_ecc, int has_vdw, int nele, const double* c6,
-               const double* r0, double* phi, double* grad, double* eatom, void* stream);
+               const double* r0, float* xqc, int* elc, double* phi, double* grad, double* eatom,
+               void* stream);
 
 /* u_i = phi_i - mean_m(phi): dE_cc/dqraw_i through the neutrality step. */
 int tm_charge_upstream(int N, const int* mol, const double* phi, const double* phimean, float* u,
@@ -141,11 +154,13 @@ int tm_charge_upstream(int N, const int* mol, const double* phi, const double* p
 
 /* Per-molecule column sums (or means, mean != 0) of an [N, C] array:
  * the sparse_reduce_sum / reduce_sum per molecule of RawSymFunc.py:1354-1359
- * and TFMolInstanceDirect.py:5212, in a fixed FP64 order. */
+ * and TFMolInstanceDirect.py:5212, in a fixed FP64 order.  scratch (may be
+ * NULL; ceil(N/1024)*C doubles) lets one large molecule (B == 1, n_total =
+ * N > 2048) be summed by many blocks. */
 int tm_mol_sum_f32(int B, const int* mol_start, int C, const float* in, int ldin, double* out,
-                   int mean, void* stream);
+                   int mean, double* scratch, int n_total, void* stream);
 int tm_mol_sum_f64(int B, const int* mol_start, int C, const double* in, int ldin, double* out,
-                   int mean, void* stream);
+                   int mean, double* scratch, int n_total, void* stream);
 
 /* ---- integrators (TensorMol/Simulations/SimpleMD.py) --------------------
  * VelocityVerletStep (SimpleMD.py:14-38) split into drift and kick;

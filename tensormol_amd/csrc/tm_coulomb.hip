@@ -92,14 +92,88 @@ __device__ __forceinline__ float min_image_f(float d, float L) {
   return L > 0.f ? d - L * rintf(d / L) : d;
 }
 
+struct PairAcc {
+  double phi, gx, gy, gz, ev;
+};
+
+// energy terms of one pair (i, j) seen from centre i; d = x_j - x_i (A)
+__device__ __forceinline__ void pair_terms(const CoulArgs& a, float qi, float qj, float c6i, float r0i,
+                                           float c6j, float r0j, float dx, float dy, float dz,
+                                           float s, PairAcc& acc) {
+  const float B1 = (float)BOHRPERA_D, B2 = (float)(BOHRPERA_D * BOHRPERA_D);
+  float rA = sqrtf(s + 1e-27f);   // the reference's regulariser (RawSymFunc.py:1330)
+  float inv_rA = 1.0f / rA;
+  float dEdr = 0.f;               // dE_pair/dr_A
+  if (a.add_ecc) {
+    float r = rA * B1;
+    float K, dK;
+    if (r > a.r_sr) {
+      if (r > a.r_lr) {
+        K = 0.f; dK = 0.f;
+      } else {
+        float ar = a.alpha * r;
+        float ec = erfcf(ar);
+        float inv = 1.0f / r;
+        K = ec * inv - a.zz + (r - a.r_lr) * (a.zz / a.r_lr + a.yy);
+        dK = -ec * inv * inv - 1.1283791671f * a.alpha * __expf(-ar * ar) * inv + a.zz / a.r_lr +
+             a.yy;
+      }
+    } else {
+      float ex = __expf(r - a.r_sr);
+      K = a.ea * (ex - 1.0f) + a.shift;
+      dK = a.ea * ex;
+    }
+    acc.phi += (double)(K * qj);
+    dEdr += qi * qj * dK * B1;
+  }
+  if (a.has_vdw) {
+    float r = rA * B2;
+    float t = r / a.width;
+    float S, dS;
+    if (t > 1.f) { S = 1.f; dS = 0.f; }
+    else if (t > 0.f) { S = -t * t * (2.f * t - 3.f); dS = 6.f * t * (1.f - t) / a.width; }
+    else { S = 0.f; dS = 0.f; }
+    // C r^-6 / (1 + 6 (r/R0)^-12) written as C r^6 / (r^12 + 6 R0^12): the same
+    // function, finite for every r >= 0 in FP32
+    float C = sqrtf(c6i * c6j);
+    float R0 = r0i + r0j;
+    float r2 = r * r, r6 = r2 * r2 * r2, r12 = r6 * r6;
+    float q2 = R0 * R0, q6 = q2 * q2 * q2, q12 = 6.0f * q6 * q6;
+    float den = r12 + q12;
+    float f = C * r6 / den;
+    float df = 6.0f * C * ((r6 / r) / den) * ((q12 - r12) / den);
+    float V = -S * f;
+    float dV = -(dS * f + S * df);
+    acc.ev += 0.5 * (double)V;
+    dEdr += dV * B2;
+  }
+  // dE/dx_i = dE/dr_A * (x_i - x_j)/r_A = -dEdr * d / r_A
+  float sc = -dEdr * inv_rA;
+  acc.gx += (double)(sc * dx);
+  acc.gy += (double)(sc * dy);
+  acc.gz += (double)(sc * dz);
+}
+
+constexpr int QCAP = 128;   // compacted pair queue per wave
+
+// One wavefront per centre.  Candidates (contiguous in cell order, or the
+// molecule's atoms) are tested 64 at a time; those inside the cutoff are
+// compacted into an LDS queue and evaluated 64 at a time with every lane
+// busy.  Queue order = candidate order, so the per-lane sums and the final
+// wave reduction are formed in a fixed order.
 __global__ void __launch_bounds__(C_WAVES * 64)
 coulomb_kernel(int N, const float4* __restrict__ xq, const int* __restrict__ elem,
+               const float4* __restrict__ xqc, const int* __restrict__ elc,
                const int* __restrict__ mol, const int* __restrict__ mol_start,
                const double* __restrict__ box, Grid grid, CoulArgs a, double* __restrict__ phi,
                double* __restrict__ grad, double* __restrict__ eatom) {
+  __shared__ float4 qx[C_WAVES][QCAP];
+  __shared__ int qe[C_WAVES][QCAP];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int i = blockIdx.x * C_WAVES + w;
-  if (i >= N) return;
+  const int wi = blockIdx.x * C_WAVES + w;
+  if (wi >= N) return;   // wave-level synchronisation only below
+  // centres are visited in cell order so neighbouring waves share candidates in L2
+  const int i = grid.cell_atoms != nullptr ? grid.cell_atoms[wi] : wi;
   const int mi = mol[i];
   float Lx = 0.f, Ly = 0.f, Lz = 0.f;
   if (box != nullptr) {
@@ -108,87 +182,83 @@ coulomb_kernel(int N, const float4* __restrict__ xq, const int* __restrict__ ele
   const float4 pi = xq[i];
   const int ei = elem[i];
   const float c6i = a.c6[ei], r0i = a.r0[ei];
-  const float B1 = (float)BOHRPERA_D, B2 = (float)(BOHRPERA_D * BOHRPERA_D);
-  double sphi = 0.0, gx = 0.0, gy = 0.0, gz = 0.0, ev = 0.0;
-  auto visit = [&](int lo, int hi, int mapped) {
-    for (int base = lo; base < hi; base += 64) {
-      int p = base + lane;
-      if (p >= hi) continue;
-      int j = mapped ? grid.cell_atoms[p] : p;
-      if (j == i || (mapped && mol[j] != mi)) continue;
-      float4 pj = xq[j];
+  PairAcc acc = {0.0, 0.0, 0.0, 0.0, 0.0};
+  int qn = 0;   // wave-uniform queue length
+  auto drain = [&](int n) {   // evaluate the first n (<= 64) queued pairs
+    if (lane < n) {
+      float4 pj = qx[w][lane];
+      int ej = qe[w][lane];
       float dx = min_image_f(pj.x - pi.x, Lx);
       float dy = min_image_f(pj.y - pi.y, Ly);
       float dz = min_image_f(pj.z - pi.z, Lz);
       float s = dx * dx + dy * dy + dz * dz;
-      if (!(s < a.rc2)) continue;
-      float rA = sqrtf(s);
-      float inv_rA = 1.0f / rA;
-      float dEdr = 0.f;   // dE_pair/dr_A
-      if (a.add_ecc) {
-        float r = rA * B1;
-        float K, dK;
-        if (r > a.r_sr) {
-          if (r > a.r_lr) {
-            K = 0.f; dK = 0.f;
-          } else {
-            float ar = a.alpha * r;
-            float ec = erfcf(ar);
-            float inv = 1.0f / r;
-            K = ec * inv - a.zz + (r - a.r_lr) * (a.zz / a.r_lr + a.yy);
-            dK = -ec * inv * inv - 1.1283791671f * a.alpha * __expf(-ar * ar) * inv +
-                 a.zz / a.r_lr + a.yy;
-          }
-        } else {
-          float ex = __expf(r - a.r_sr);
-          K = a.ea * (ex - 1.0f) + a.shift;
-          dK = a.ea * ex;
+      pair_terms(a, pi.w, pj.w, c6i, r0i, a.c6[ej], a.r0[ej], dx, dy, dz, s, acc);
+    }
+  };
+  auto visit = [&](int lo, int hi, int mapped) {
+    const float4* src = mapped ? xqc : xq;
+    const int* esrc = mapped ? elc : elem;
+    for (int base = lo; base < hi; base += 64) {
+      int p = base + lane;
+      bool ok = false;
+      float4 pj;
+      if (p < hi) {
+        pj = src[p];
+        int j = mapped ? grid.cell_atoms[p] : p;
+        if (j != i && (!mapped || mol[j] == mi)) {
+          float dx = min_image_f(pj.x - pi.x, Lx);
+          float dy = min_image_f(pj.y - pi.y, Ly);
+          float dz = min_image_f(pj.z - pi.z, Lz);
+          ok = dx * dx + dy * dy + dz * dz < a.rc2;
         }
-        sphi += (double)(K * pj.w);
-        dEdr += pi.w * pj.w * dK * B1;
       }
-      if (a.has_vdw) {
-        int ej = elem[j];
-        float r = rA * B2;
-        float t = r / a.width;
-        float S, dS;
-        if (t > 1.f) { S = 1.f; dS = 0.f; }
-        else if (t > 0.f) { S = -t * t * (2.f * t - 3.f); dS = 6.f * t * (1.f - t) / a.width; }
-        else { S = 0.f; dS = 0.f; }
-        float C = sqrtf(c6i * a.c6[ej]);
-        float R0 = r0i + a.r0[ej];
-        float ur = R0 / r;
-        float u2 = ur * ur, u4 = u2 * u2, u12 = u4 * u4 * u4;   // (r/R0)^-12
-        float den = 1.0f + 6.0f * u12;
-        float ir = 1.0f / r;
-        float ir2 = ir * ir;
-        float ir6 = ir2 * ir2 * ir2;
-        float f = C * ir6 / den;
-        float df = C * ir6 * ir * (-6.0f + 36.0f * u12) / (den * den);
-        float V = -S * f;
-        float dV = -(dS * f + S * df);
-        ev += 0.5 * (double)V;
-        dEdr += dV * B2;
+      unsigned long long bal = __ballot(ok);
+      int before = __popcll(bal & ((1ull << lane) - 1ull));
+      if (ok) {
+        qx[w][qn + before] = pj;
+        qe[w][qn + before] = esrc[p];
       }
-      // dE/dx_i = dE/dr_A * (x_i - x_j)/r_A = -dEdr * d / r_A
-      float sc = -dEdr * inv_rA;
-      gx += (double)(sc * dx);
-      gy += (double)(sc * dy);
-      gz += (double)(sc * dz);
+      qn += __popcll(bal);
+      __syncwarp();
+      if (qn >= 64) {
+        drain(64);
+        __syncwarp();
+        // move the remainder (< 64 entries) to the front
+        float4 t4;
+        int te = 0;
+        bool mv = lane < qn - 64;
+        if (mv) { t4 = qx[w][64 + lane]; te = qe[w][64 + lane]; }
+        __syncwarp();
+        if (mv) { qx[w][lane] = t4; qe[w][lane] = te; }
+        qn -= 64;
+        __syncwarp();
+      }
     }
   };
   candidate_ranges(i, mol, mol_start, grid, visit);
-  sphi = tm_wave_sum_d(sphi);
-  gx = tm_wave_sum_d(gx);
-  gy = tm_wave_sum_d(gy);
-  gz = tm_wave_sum_d(gz);
-  ev = tm_wave_sum_d(ev);
+  drain(qn);
+  double sphi = tm_wave_sum_d(acc.phi);
+  double gx = tm_wave_sum_d(acc.gx);
+  double gy = tm_wave_sum_d(acc.gy);
+  double gz = tm_wave_sum_d(acc.gz);
+  double ev = tm_wave_sum_d(acc.ev);
   if (lane == 0) {
     phi[i] = sphi;
     grad[3 * i] = gx; grad[3 * i + 1] = gy; grad[3 * i + 2] = gz;
     eatom[2 * i] = 0.5 * (double)pi.w * sphi;
     eatom[2 * i + 1] = ev;
   }
+}
+
+// positions and charges in cell order for the candidate sweeps
+__global__ void cell_gather_kernel(int N, const int* __restrict__ cell_atoms,
+                                   const float4* __restrict__ xq, const int* __restrict__ elem,
+                                   float4* __restrict__ xqc, int* __restrict__ elc) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= N) return;
+  int i = cell_atoms[p];
+  xqc[p] = xq[i];
+  elc[p] = elem[i];
 }
 
 // Per-molecule column sums of an [N, C] array (float or double input) into
@@ -218,6 +288,54 @@ mol_sum_kernel(const int* __restrict__ mol_start, int C, const T* __restrict__ i
   }
 }
 
+// One large molecule: stage 1 sums fixed 1024-atom chunks into partial[k][c],
+// stage 2 sums the partials in chunk order.  Deterministic.
+template <class T>
+__global__ void __launch_bounds__(256)
+chunk_sum_kernel(int n, int C, const T* __restrict__ in, int ldin, double* __restrict__ partial) {
+  __shared__ double part[256];
+  const int k = blockIdx.x;
+  const int lo = k * 1024, hi = min(n, lo + 1024);
+  for (int c = 0; c < C; ++c) {
+    double acc = 0.0;
+    for (int a = lo + threadIdx.x; a < hi; a += 256) acc += (double)in[(long)a * ldin + c];
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if (threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(long)k * C + c] = part[0];
+    __syncthreads();
+  }
+}
+
+__global__ void chunk_final_kernel(int nchunk, int C, int n, const double* __restrict__ partial,
+                                   double* __restrict__ out, int scale_by_count) {
+  int c = threadIdx.x;
+  if (c >= C) return;
+  double v = 0.0;
+  for (int k = 0; k < nchunk; ++k) v += partial[(long)k * C + c];
+  if (scale_by_count) v = n > 0 ? v / (double)n : 0.0;
+  out[c] = v;
+}
+
+template <class T>
+int mol_sum(int B, const int* mol_start, int C, const T* in, int ldin, double* out, int mean,
+            double* scratch, int n_total, hipStream_t s) {
+  if (B == 1 && scratch != nullptr && n_total > 2048) {
+    int nchunk = (n_total + 1023) / 1024;
+    chunk_sum_kernel<T><<<nchunk, 256, 0, s>>>(n_total, C, in, ldin, scratch);
+    TM_LAUNCH_CHECK();
+    chunk_final_kernel<<<1, 64, 0, s>>>(nchunk, C, n_total, scratch, out, mean);
+    TM_LAUNCH_CHECK();
+    return TM_OK;
+  }
+  mol_sum_kernel<T><<<B, 256, 0, s>>>(mol_start, C, in, ldin, out, mean);
+  TM_LAUNCH_CHECK();
+  return TM_OK;
+}
+
 __global__ void charges_kernel(int N, const int* __restrict__ mol, const double* __restrict__ qmean,
                                const float* __restrict__ qraw, const double* __restrict__ pos,
                                float4* __restrict__ xq, double* __restrict__ q,
@@ -245,22 +363,21 @@ __global__ void charge_upstream_kernel(int N, const int* __restrict__ mol,
 
 extern "C" {
 
+// scratch (optional, >= ceil(N/1024)*C doubles) enables the multi-block path
+// for a single large molecule; n_total = N.
 int tm_mol_sum_f32(int B, const int* mol_start, int C, const float* in, int ldin, double* out,
-                   int mean, void* stream) {
-  TM_REQUIRE(B >= 0 && C > 0, "tm_mol_sum_f32: bad sizes");
+                   int mean, double* scratch, int n_total, void* stream) {
+  TM_REQUIRE(B >= 0 && C > 0 && C <= 64, "tm_mol_sum_f32: bad sizes");
   if (B == 0) return TM_OK;
-  mol_sum_kernel<float><<<B, 256, 0, (hipStream_t)stream>>>(mol_start, C, in, ldin, out, mean);
-  TM_LAUNCH_CHECK();
-  return TM_OK;
+  return mol_sum<float>(B, mol_start, C, in, ldin, out, mean, scratch, n_total, (hipStream_t)stream);
 }
 
 int tm_mol_sum_f64(int B, const int* mol_start, int C, const double* in, int ldin, double* out,
-                   int mean, void* stream) {
-  TM_REQUIRE(B >= 0 && C > 0, "tm_mol_sum_f64: bad sizes");
+                   int mean, double* scratch, int n_total, void* stream) {
+  TM_REQUIRE(B >= 0 && C > 0 && C <= 64, "tm_mol_sum_f64: bad sizes");
   if (B == 0) return TM_OK;
-  mol_sum_kernel<double><<<B, 256, 0, (hipStream_t)stream>>>(mol_start, C, in, ldin, out, mean);
-  TM_LAUNCH_CHECK();
-  return TM_OK;
+  return mol_sum<double>(B, mol_start, C, in, ldin, out, mean, scratch, n_total,
+                         (hipStream_t)stream);
 }
 
 int tm_charges(int N, const int* mol, const double* qmean, const float* qraw, const double* pos,
@@ -283,12 +400,17 @@ int tm_charge_upstream(int N, const int* mol, const double* phi, const double* p
 
 // consts: [alpha', R_sr, R_lr, ZZ, YY, elu_shift, elu_alpha] (bohr), rc_ee (A),
 // poly width (bohr*BOHRPERA), c6[nele], r0[nele] (host arrays, atomic units).
+// With a cell grid, xqc (float4 [N]) and elc (int [N]) receive positions,
+// charges and elements in cell order.
 int tm_coulomb(int N, const float* xq, const int* elem, const int* mol, const int* mol_start,
                const double* box, const int* cell_start, const int* cell_atoms, const int* cell_of,
                int nx, int ny, int nz, int periodic, int reach, const double* consts,
                double rc_ee, double width, int add_ecc, int has_vdw, int nele, const double* c6,
-               const double* r0, double* phi, double* grad, double* eatom, void* stream) {
+               const double* r0, float* xqc, int* elc, double* phi, double* grad, double* eatom,
+               void* stream) {
   TM_REQUIRE(nele > 0 && nele <= TM_MAX_ELE, "tm_coulomb: bad nele");
+  TM_REQUIRE(cell_atoms == nullptr || (xqc != nullptr && elc != nullptr),
+             "tm_coulomb: cell sweeps need xqc/elc buffers");
   if (N == 0) return TM_OK;
   CoulArgs a;
   a.alpha = (float)consts[0]; a.r_sr = (float)consts[1]; a.r_lr = (float)consts[2];
@@ -307,8 +429,15 @@ int tm_coulomb(int N, const float* xq, const int* elem, const int* mol, const in
   g.nc[0] = nx; g.nc[1] = ny; g.nc[2] = nz;
   g.periodic = periodic;
   g.reach = reach;
-  coulomb_kernel<<<tm_div_up(N, C_WAVES), C_WAVES * 64, 0, (hipStream_t)stream>>>(
-      N, (const float4*)xq, elem, mol, mol_start, box, g, a, phi, grad, eatom);
+  hipStream_t s = (hipStream_t)stream;
+  if (cell_atoms != nullptr) {
+    cell_gather_kernel<<<tm_div_up(N, 256), 256, 0, s>>>(N, cell_atoms, (const float4*)xq, elem,
+                                                         (float4*)xqc, elc);
+    TM_LAUNCH_CHECK();
+  }
+  coulomb_kernel<<<tm_div_up(N, C_WAVES), C_WAVES * 64, 0, s>>>(
+      N, (const float4*)xq, elem, (const float4*)xqc, elc, mol, mol_start, box, g, a, phi, grad,
+      eatom);
   TM_LAUNCH_CHECK();
   return TM_OK;
 }

@@ -264,6 +264,16 @@ __global__ void cell_assign_kernel(int N, const double* __restrict__ pos, double
   atomicAdd(&cell_count[cell], 1);
 }
 
+__global__ void copy_i32_kernel(int n, const int* __restrict__ src, int* __restrict__ dst) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) dst[t] = src[t];
+}
+
+__global__ void zero2_kernel(long n, int* __restrict__ a, int* __restrict__ b) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) { a[t] = 0; b[t] = 0; }
+}
+
 // exclusive scan of cell_count into cell_start (single block, any length)
 __global__ void __launch_bounds__(1024) scan_kernel(int n, const int* __restrict__ in,
                                                     int* __restrict__ out) {
@@ -300,20 +310,20 @@ __global__ void cell_fill_kernel(int N, const int* __restrict__ cell_of,
   cell_atoms[slot] = i;
 }
 
-// order atoms inside each cell by index, so every later sweep is deterministic
+// order atoms inside each cell by index, so every later sweep is
+// deterministic: one wavefront per cell ranks its entries (cells hold a few
+// dozen atoms; larger cells are ranked in 64-entry rounds)
 __global__ void cell_sort_kernel(int ncell, const int* __restrict__ cell_start,
-                                 int* __restrict__ cell_atoms) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
+                                 const int* __restrict__ unsorted, int* __restrict__ cell_atoms) {
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (c >= ncell) return;
-  int lo = cell_start[c], hi = cell_start[c + 1];
-  for (int a = lo + 1; a < hi; ++a) {
-    int v = cell_atoms[a];
-    int b = a - 1;
-    while (b >= lo && cell_atoms[b] > v) {
-      cell_atoms[b + 1] = cell_atoms[b];
-      --b;
-    }
-    cell_atoms[b + 1] = v;
+  const int lo = cell_start[c], hi = cell_start[c + 1];
+  for (int a = lo + lane; a < hi; a += 64) {
+    int v = unsorted[a];
+    int rank = 0;
+    for (int b = lo; b < hi; ++b) rank += unsorted[b] < v;
+    cell_atoms[lo + rank] = v;
   }
 }
 
@@ -352,10 +362,57 @@ __global__ void triples_kernel(int N, int ldr, int lda, int nele, const int* __r
   }
 }
 
+// Pair list for an arbitrary cutoff (the EE list of Neighbors.py:262-282 and
+// Make_NListNaive with DoPerms 0/1): pass 1 counts, pass 2 writes each row in
+// candidate order (ascending j for molecule sweeps, cell order otherwise).
+// Centres i with local index >= nreal[m] get empty rows; with half = 1 only
+// j > i is kept (pairs stored once under min(i, j)).
+template <bool FILL>
+__global__ void __launch_bounds__(NL_WAVES * 64)
+pairs_kernel(int N, const double* __restrict__ pos, const int* __restrict__ mol,
+             const int* __restrict__ mol_start, const int* __restrict__ nreal,
+             const double* __restrict__ box, CellGrid grid, int reach, double s_c, int half,
+             int* __restrict__ count, const long* __restrict__ start, int* __restrict__ out) {
+  const int w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * NL_WAVES + w;
+  if (i >= N) return;
+  const int mi = mol[i];
+  if (nreal != nullptr && i - mol_start[mi] >= nreal[mi]) {
+    if (!FILL && lane == 0) count[i] = 0;
+    return;
+  }
+  double Lx = 0, Ly = 0, Lz = 0;
+  if (box != nullptr) {
+    Lx = box[3 * mi]; Ly = box[3 * mi + 1]; Lz = box[3 * mi + 2];
+  }
+  long o = FILL ? start[i] : 0;
+  int cnt = 0;
+  auto visit = [&](int lo, int hi, int mapped) {
+    for (int base = lo; base < hi; base += 64) {
+      int p = base + lane;
+      bool ok = false;
+      int j = -1;
+      if (p < hi) {
+        j = mapped ? grid.cell_atoms[p] : p;
+        if (j != i && (!half || j > i) && (!mapped || mol[j] == mi))
+          ok = sq_dist_nocontract(pos, i, j, Lx, Ly, Lz) < s_c;
+      }
+      unsigned long long bal = __ballot(ok);
+      if (FILL && ok) out[o + cnt + __popcll(bal & ((1ull << lane) - 1ull))] = j;
+      cnt += __popcll(bal);
+    }
+  };
+  for_candidate_ranges(i, mol, mol_start, grid, reach, visit);
+  if (!FILL && lane == 0) count[i] = cnt;
+}
+
 }  // namespace
 
 extern "C" {
 
+// cell_fill must hold max(ncell, N) ints: per-cell counters, then the
+// unsorted staging copy of cell_atoms.
 int tm_cell_build(int N, const double* pos, double ox, double oy, double oz, double cx, double cy,
                   double cz, int nx, int ny, int nz, int periodic, int* cell_of, int* cell_count,
                   int* cell_start, int* cell_fill, int* cell_atoms, int* flags, void* stream) {
@@ -363,18 +420,20 @@ int tm_cell_build(int N, const double* pos, double ox, double oy, double oz, dou
   hipStream_t s = (hipStream_t)stream;
   long ncell = (long)nx * ny * nz;
   if (N == 0) return TM_OK;
-  // counters are cleared with a fill kernel-free memset only outside capture;
-  // callers that capture pass zeroed buffers and set cell_count/cell_fill themselves
-  TM_HIP_CHECK(hipMemsetAsync(cell_count, 0, sizeof(int) * ncell, s));
-  TM_HIP_CHECK(hipMemsetAsync(cell_fill, 0, sizeof(int) * ncell, s));
+  // counters are cleared by a kernel (no memset nodes when the caller captures a graph)
+  zero2_kernel<<<tm_div_up(ncell, 256), 256, 0, s>>>(ncell, cell_count, cell_fill);
+  TM_LAUNCH_CHECK();
   cell_assign_kernel<<<tm_div_up(N, 256), 256, 0, s>>>(N, pos, ox, oy, oz, cx, cy, cz, nx, ny, nz,
                                                        periodic, cell_of, cell_count, flags);
   TM_LAUNCH_CHECK();
   scan_kernel<<<1, 1024, 0, s>>>((int)ncell, cell_count, cell_start);
   TM_LAUNCH_CHECK();
+  // cell_fill doubles as the unsorted staging array once its counters are spent
   cell_fill_kernel<<<tm_div_up(N, 256), 256, 0, s>>>(N, cell_of, cell_start, cell_fill, cell_atoms);
   TM_LAUNCH_CHECK();
-  cell_sort_kernel<<<tm_div_up(ncell, 256), 256, 0, s>>>((int)ncell, cell_start, cell_atoms);
+  copy_i32_kernel<<<tm_div_up(N, 256), 256, 0, s>>>(N, cell_atoms, cell_fill);
+  TM_LAUNCH_CHECK();
+  cell_sort_kernel<<<tm_div_up(ncell, 4), 256, 0, s>>>((int)ncell, cell_start, cell_fill, cell_atoms);
   TM_LAUNCH_CHECK();
   return TM_OK;
 }
@@ -415,6 +474,29 @@ int tm_triples(int N, int ldr, int lda, int nele, const int* nbr, const int* ang
   hipStream_t s = (hipStream_t)stream;
   triples_kernel<<<tm_div_up(N, 128), 128, 0, s>>>(N, ldr, lda, nele, nbr, ang, acnt, aseg,
                                                    trip_start, out_jkpr);
+  TM_LAUNCH_CHECK();
+  return TM_OK;
+}
+
+
+int tm_pairs(int N, const double* pos, const int* mol, const int* mol_start, const int* nreal,
+             const double* box, const int* cell_start, const int* cell_atoms, const int* cell_of,
+             int nx, int ny, int nz, int periodic, int reach, double s_c, int half, int* count,
+             const long* start, int* out, void* stream) {
+  TM_REQUIRE(N >= 0 && reach >= 1, "tm_pairs: bad sizes");
+  TM_REQUIRE(start == nullptr || out != nullptr, "tm_pairs: fill pass needs an output array");
+  if (N == 0) return TM_OK;
+  hipStream_t s = (hipStream_t)stream;
+  CellGrid g;
+  g.cell_start = cell_start; g.cell_atoms = cell_atoms; g.cell_of = cell_of;
+  g.nc[0] = nx; g.nc[1] = ny; g.nc[2] = nz;
+  g.periodic = periodic;
+  if (start == nullptr)
+    pairs_kernel<false><<<tm_div_up(N, NL_WAVES), NL_WAVES * 64, 0, s>>>(
+        N, pos, mol, mol_start, nreal, box, g, reach, s_c, half, count, nullptr, nullptr);
+  else
+    pairs_kernel<true><<<tm_div_up(N, NL_WAVES), NL_WAVES * 64, 0, s>>>(
+        N, pos, mol, mol_start, nreal, box, g, reach, s_c, half, count, start, out);
   TM_LAUNCH_CHECK();
   return TM_OK;
 }

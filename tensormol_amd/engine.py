@@ -171,6 +171,7 @@ class BPEEEngine:
         ws.aseg = torch.zeros(N * (self.nele + 1), **i32)
         ws.rev = torch.zeros(N * self.ldr, **i32)
         ws.flags = torch.zeros(8, **i32)
+        ws.red = torch.zeros(((N + 1023) // 1024) * 8 + 8, **f64)
         ws.G = torch.zeros(N, self.D, **f32)
         # outputs of the two networks: y[net, atom], dy/dG[net, atom, :]
         ws.Y2 = torch.zeros(2, N, **f32)
@@ -180,6 +181,8 @@ class BPEEEngine:
         ws.qmean = torch.zeros(B, **f64)
         ws.phimean = torch.zeros(B, **f64)
         ws.xq = torch.zeros(N, 4, **f32)
+        ws.xqc = torch.zeros(N, 4, **f32) if ws.use_cells else None
+        ws.elc = torch.zeros(N, **i32) if ws.use_cells else None
         ws.q = torch.zeros(N, **f64)
         ws.qx = torch.zeros(N, 3, **f64)
         ws.phi = torch.zeros(N, **f64)
@@ -226,7 +229,7 @@ class BPEEEngine:
         ws.cell_of = torch.zeros(ws.N, **i32)
         ws.cell_count = torch.zeros(ncell, **i32)
         ws.cell_start = torch.zeros(ncell + 1, **i32)
-        ws.cell_fill = torch.zeros(ncell, **i32)
+        ws.cell_fill = torch.zeros(max(ncell, ws.N), **i32)   # counters, then sort staging
         ws.cell_atoms = torch.zeros(ws.N, **i32)
 
     # ---------------------------------------------------------- stages
@@ -283,7 +286,7 @@ class BPEEEngine:
 
     def electrostatics(self, ws, pos):
         L, st = self.lib, self._stream()
-        L.tm_mol_sum_f32(ws.B, _P(ws.mol_start), 1, _P(ws.qraw), 1, _P(ws.qmean), 1, st)
+        L.tm_mol_sum_f32(ws.B, _P(ws.mol_start), 1, _P(ws.qraw), 1, _P(ws.qmean), 1, _P(ws.red), ws.N, st)
         L.tm_charges(ws.N, _P(ws.mol), _P(ws.qmean), _P(ws.qraw), _P(pos), _P(ws.xq), _P(ws.q),
                      _P(ws.qx), st)
         grid = (None, None, None, 1, 1, 1, 0)
@@ -296,8 +299,8 @@ class BPEEEngine:
                      self.dsf.ctypes.data_as(_lib.P), self.model.ee_off, self.vdw_width,
                      int(bool(PARAMS["AddEcc"])), int(self.model.has_vdw), self.nele,
                      self.c6.ctypes.data_as(_lib.P), self.r0.ctypes.data_as(_lib.P),
-                     _P(ws.phi), _P(ws.gradc), _P(ws.eatom), st)
-        L.tm_mol_sum_f64(ws.B, _P(ws.mol_start), 1, _P(ws.phi), 1, _P(ws.phimean), 1, st)
+                     _P(ws.xqc), _P(ws.elc), _P(ws.phi), _P(ws.gradc), _P(ws.eatom), st)
+        L.tm_mol_sum_f64(ws.B, _P(ws.mol_start), 1, _P(ws.phi), 1, _P(ws.phimean), 1, _P(ws.red), ws.N, st)
         L.tm_charge_upstream(ws.N, _P(ws.mol), _P(ws.phi), _P(ws.phimean), _P(ws.u), st)
 
     def adjoint(self, ws, pos):
@@ -313,9 +316,9 @@ class BPEEEngine:
 
     def reductions(self, ws):
         L, st = self.lib, self._stream()
-        L.tm_mol_sum_f32(ws.B, _P(ws.mol_start), 1, _P(ws.eps), 1, _P(ws.Ebp), 0, st)
-        L.tm_mol_sum_f64(ws.B, _P(ws.mol_start), 2, _P(ws.eatom), 2, _P(ws.Eext), 0, st)
-        L.tm_mol_sum_f64(ws.B, _P(ws.mol_start), 3, _P(ws.qx), 3, _P(ws.dipole), 0, st)
+        L.tm_mol_sum_f32(ws.B, _P(ws.mol_start), 1, _P(ws.eps), 1, _P(ws.Ebp), 0, _P(ws.red), ws.N, st)
+        L.tm_mol_sum_f64(ws.B, _P(ws.mol_start), 2, _P(ws.eatom), 2, _P(ws.Eext), 0, _P(ws.red), ws.N, st)
+        L.tm_mol_sum_f64(ws.B, _P(ws.mol_start), 3, _P(ws.qx), 3, _P(ws.dipole), 0, _P(ws.red), ws.N, st)
 
     def run(self, ws, pos, forces=True, check=None):
         """Enqueue one full evaluation for positions pos (cuda float64 [N,3])."""

@@ -2,7 +2,7 @@
 
 Values restate TensorMol's PhysicalData.py:24-54 and the TMParams.py defaults
 that the BP+EE evaluation, MD and optimiser drivers read
-(TMParams.py:26-38,57-58,66,79,98-105,140-154).  Drop-in scripts mutate PARAMS
+(TMParams.py:26-38,57-58,66,79,97-120,140-154).  Drop-in scripts mutate PARAMS
 before constructing objects, as with the reference; the engine copies what it
 needs into a frozen config at construction and reads nothing global inside a
 launch.
@@ -52,22 +52,27 @@ class _Params(dict):
         self["EECutoffOff"] = 15.0
         self["DSFAlpha"] = 0.18
         self["KeepProb"] = [1.0, 1.0, 1.0, 1.0]
-        self["MDdt"] = 0.5
+        self["MDdt"] = 0.2
         self["MDTemp"] = 300.0
-        self["MDMaxStep"] = 100
+        self["MDMaxStep"] = 20000
         self["MDThermostat"] = None
         self["MDV0"] = "Random"
         self["MDLogTrajectory"] = False
-        self["OptMaxCycles"] = 500
+        self["OptMaxCycles"] = 50
         self["OptThresh"] = 0.0001
         self["OptMaxStep"] = 0.1
         self["OptStepSize"] = 0.1
         self["OptMomentum"] = 0.0
         self["OptMomentumDecay"] = 0.8
-        self["GSSearchAlpha"] = 0.1
+        self["GSSearchAlpha"] = 0.05
+        self["RemoveInvariant"] = True
         self["ConjGrad_Restart"] = 10
         self["FIREMaxStep"] = 0.1
-        # precision of the MLP GEMMs on the GPU: "bf16", "fp16" or "fp32"
+        self["tm_root"] = "."
+        self["sets_dir"] = "./datasets/"
+        self["networks_directory"] = "./networks/"
+        self["results_dir"] = "./results/"
+        # precision of the MLP GEMMs on the GPU: "bf16", "fp16", "bf16x3" or "fp32"
         self["MLPPrecision"] = "bf16"
 
 
