@@ -34,28 +34,36 @@ int tm_zero_i32(int n, int* p, void* stream);
  * NeighborListSet.buildPairsAndTriplesWithEleIndex
  * (TensorMol/ForceModifiers/Neighbors.py:117-201, 344-423).
  *
- * tm_cell_build bins atoms into an nx*ny*nz grid of cells with origin o and
- * edge c (periodic: positions wrapped; open: clamped, flag 2 raised) and
- * sorts each cell's atoms by index.  Buffers: cell_of[N], cell_count[ncell],
- * cell_start[ncell+1], cell_fill[max(ncell, N)], cell_atoms[N].
+ * Cell grid (one large open or periodic system, or a batch): tm_cell_build
+ * bins atoms into an nx*ny*nz grid of cells with origin o and edges c
+ * (periodic: positions wrapped; open: clamped, flag 2 raised) and sorts each
+ * cell's atoms by index.  Buffers: cell_of[N], cell_count[ncell],
+ * cell_start[ncell+1], cell_fill[max(ncell, N)], cell_atoms[N] (atoms in cell
+ * order), cell_pos[N] (cell-order position of each atom), posc[3N] (positions
+ * in cell order), molc[N] (molecules in cell order; NULL for one system).
+ * Sweeps over the grid cover the cutoff sphere with pruned stencil rows; cells
+ * of about half the short cutoff are the intended size.
  */
-int tm_cell_build(int N, const double* pos, double ox, double oy, double oz, double cx, double cy,
-                  double cz, int nx, int ny, int nz, int periodic, int* cell_of, int* cell_count,
-                  int* cell_start, int* cell_fill, int* cell_atoms, int* flags, void* stream);
+int tm_cell_build(int N, const double* pos, const int* mol, double ox, double oy, double oz,
+                  double cx, double cy, double cz, int nx, int ny, int nz, int periodic,
+                  int* cell_of, int* cell_count, int* cell_start, int* cell_fill, int* cell_atoms,
+                  int* cell_pos, double* posc, int* molc, int* flags, void* stream);
 
 /* Builds, for every atom i, the full (DoPerms=1) list of neighbours j with
  * squared distance s < s_r and marks those with s < s_a.  s_r/s_a are the
  * first doubles failing sqrt(s) + 1e-13 < rc (MolEmb.cpp:1214-1218), so the
- * sets are bit-exact.  Rows are sorted by (element of j, j).  Candidates come
- * from the molecule (cell_atoms == NULL) or from the 27-cell stencil.
+ * sets are bit-exact; rc (>= the pair cutoff) sizes the cell stencil.  Rows
+ * are sorted by (element of j, j).  Candidates come from the molecule
+ * (cell_atoms == NULL) or from the grid of tm_cell_build.
  * Outputs: nbr[i*ldr+n], ncnt[i], rseg[i*(nele+1)+e], ang[i*lda+a] (position
  * in row i), acnt[i], aseg[i*(nele+1)+e], rev[i*ldr+n] (mirror slot), flags
  * (0: row overflow, 1: sub-list overflow, 3: non-finite input). */
 int tm_nl_build(int N, const double* pos, const int* elem, const int* mol, const int* mol_start,
                 const double* box, const int* cell_start, const int* cell_atoms,
-                const int* cell_of, int nx, int ny, int nz, int periodic, int nele, double s_r,
-                double s_a, int ldr, int lda, int* nbr, int* ncnt, int* rseg, int* ang, int* acnt,
-                int* aseg, int* rev, int* flags, void* stream);
+                const int* cell_of, const int* cell_pos, const double* posc, const int* molc,
+                int nx, int ny, int nz, double csx, double csy, double csz, int periodic, int nele,
+                double rc, double s_r, double s_a, int ldr, int lda, int* nbr, int* ncnt,
+                int* rseg, int* ang, int* acnt, int* aseg, int* rev, int* flags, void* stream);
 
 /* Materialises the sorted triples of every centre (Neighbors.py:369-420):
  * out_jkpr[4*t] = (j, k, element-pair index, rank in its (i, pair) group),
@@ -68,11 +76,12 @@ int tm_triples(int N, int ldr, int lda, int nele, const int* nbr, const int* ang
  * EE list NeighborListSet(..., DoPerms=False).buildPairs, Neighbors.py:262-282):
  * call with start == NULL to get count[i], then with start = exclusive prefix
  * (int64) to fill out[start[i] ..].  nreal (per molecule, may be NULL) limits
- * the centres; half = 1 keeps j > i only; reach = cell stencil radius. */
+ * the centres; half = 1 keeps j > i only. */
 int tm_pairs(int N, const double* pos, const int* mol, const int* mol_start, const int* nreal,
              const double* box, const int* cell_start, const int* cell_atoms, const int* cell_of,
-             int nx, int ny, int nz, int periodic, int reach, double s_c, int half, int* count,
-             const long* start, int* out, void* stream);
+             const int* cell_pos, const double* posc, const int* molc, int nx, int ny, int nz,
+             double csx, double csy, double csz, int periodic, double rc, double s_c, int half,
+             int* count, const long* start, int* out, int* flags, void* stream);
 
 /* ---- descriptors -------------------------------------------------------
  * TFSymRSet_Linear_WithEle + TFSymASet_Linear_WithEle concatenated as in
@@ -137,16 +146,17 @@ int tm_charges(int N, const int* mol, const double* qmean, const float* qraw, co
 /* DSF-elu Coulomb (RawSymFunc.py:1307-1359) and Grimme-C6 vdW
  * (RawSymFunc.py:1361-1414) over all pairs within rc_ee, evaluated from the
  * cell grid (or the molecule).  consts (host f64[7]) = alpha', R_sr, R_lr, ZZ,
- * YY, elu_shift, elu_alpha in bohr units; c6/r0 host f64[nele].  Outputs
+ * YY, elu_shift, elu_alpha in bohr units; c6/r0 host f64[nele].  The grid
+ * arguments are those of tm_cell_build (NULL cell_atoms: molecule sweeps).  Outputs
  * phi[i] = sum_j K(r_ij) q_j, grad (dE/dx at fixed q, Ha/A), eatom[2i] =
  * q_i phi_i / 2, eatom[2i+1] = vdW energy share.  With a cell grid xqc
  * (float4 [N]) and elc (int [N]) are scratch for the cell-ordered copies. */
 int tm_coulomb(int N, const float* xq, const int* elem, const int* mol, const int* mol_start,
                const double* box, const int* cell_start, const int* cell_atoms, const int* cell_of,
-               int nx, int ny, int nz, int periodic, int reach, const double* consts,
-               double rc_ee, double width, int add_ecc, int has_vdw, int nele, const double* c6,
-               const double* r0, float* xqc, int* elc, double* phi, double* grad, double* eatom,
-               void* stream);
+               const int* cell_pos, const int* molc, int nx, int ny, int nz, double csx, double csy,
+               double csz, int periodic, const double* consts, double rc_ee, double width,
+               int add_ecc, int has_vdw, int nele, const double* c6, const double* r0, float* xqc,
+               int* elc, double* phi, double* grad, double* eatom, int* flags, void* stream);
 
 /* u_i = phi_i - mean_m(phi): dE_cc/dqraw_i through the neutrality step. */
 int tm_charge_upstream(int N, const int* mol, const double* phi, const double* phimean, float* u,

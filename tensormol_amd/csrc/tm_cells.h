@@ -1,0 +1,167 @@
+// Candidate generation over a cell grid, shared by the neighbour-row,
+// pair-list and Coulomb sweeps.
+//
+// The stencil of a centre is the set of x-rows (dy, dz) whose minimum
+// cell-to-cell distance is below the cutoff, each trimmed in x to the cells
+// that can still hold a neighbour; with cells of about half the short cutoff
+// this is a close cover of the cutoff sphere.  Every row is one contiguous
+// range of the cell-ordered atom array (two when it wraps a periodic edge).
+// The ranges of one centre are collected in LDS with a prefix sum of their
+// lengths, so a wavefront sweeps the concatenated candidates 64 at a time
+// whatever the cell occupancy.
+//
+// Minimum distances are computed from cell indices; for an open grid atoms
+// outside it are clamped to edge cells, which only lowers index differences,
+// so the bound stays a lower bound of the true distance.
+#pragma once
+#include "tm_common.h"
+
+struct CellGrid {
+  const int* cell_start;   // ncell+1, offsets into cell_atoms
+  const int* cell_atoms;   // atoms sorted by cell (index order inside a cell)
+  const int* cell_of;      // cell of each atom
+  const int* cell_pos;     // position of each atom in cell_atoms
+  const double* posc;      // positions in cell order (float64 [N,3])
+  const int* molc;         // molecule of each cell-ordered atom (NULL: one system)
+  int nc[3];               // cells per axis
+  float cs[3];             // cell edges (A)
+  int periodic;            // 1: wrap, 0: clamp
+};
+
+// Ranges per centre (at most two per stencil row).  A sweep of cutoff rc over
+// cells of edge cs has at most (2*ceil(rc/cs)+3)^2 rows.
+template <int CAP>
+struct RangeLds {
+  static constexpr int cap = CAP;
+  int lo[CAP];
+  int pre[CAP + 1];
+};
+
+// Builds the candidate ranges of atom i for cutoff rc into R (one wavefront,
+// all lanes participate).  Returns the number of candidates; *nrange gets the
+// number of ranges (capped at CAP, overflow flagged).
+template <int CAP>
+__device__ inline int tm_build_ranges(int i, const CellGrid& g, float rc, RangeLds<CAP>& R, int lane,
+                                      int* flags, int* nrange) {
+  const int c = g.cell_of[i];
+  const int cx = c % g.nc[0], cy = (c / g.nc[0]) % g.nc[1], cz = c / (g.nc[0] * g.nc[1]);
+  int k[3];
+  bool full[3];
+  for (int d = 0; d < 3; ++d) {
+    k[d] = (int)ceilf(rc / g.cs[d]) + 1;
+    full[d] = g.periodic && (2 * k[d] + 1 >= g.nc[d]);
+  }
+  const int ny = full[1] ? g.nc[1] : 2 * k[1] + 1;
+  const int nz = full[2] ? g.nc[2] : 2 * k[2] + 1;
+  const float rc2 = rc * rc * 1.0001f;
+  int nr = 0;   // wave-uniform
+  for (int base = 0; base < ny * nz; base += 64) {
+    int t = base + lane;
+    int lo0 = 0, hi0 = 0, lo1 = 0, hi1 = 0, cnt = 0;
+    if (t < ny * nz) {
+      int oy = t % ny, oz = t / ny;
+      int dy, dz, y, z;
+      bool ok = true;
+      if (full[1]) { y = oy; dy = y - cy; if (g.periodic) { if (dy > g.nc[1] / 2) dy -= g.nc[1]; if (dy < -g.nc[1] / 2) dy += g.nc[1]; } }
+      else { dy = oy - k[1]; y = cy + dy; if (g.periodic) y = ((y % g.nc[1]) + g.nc[1]) % g.nc[1]; else ok = ok && y >= 0 && y < g.nc[1]; }
+      if (full[2]) { z = oz; dz = z - cz; if (g.periodic) { if (dz > g.nc[2] / 2) dz -= g.nc[2]; if (dz < -g.nc[2] / 2) dz += g.nc[2]; } }
+      else { dz = oz - k[2]; z = cz + dz; if (g.periodic) z = ((z % g.nc[2]) + g.nc[2]) % g.nc[2]; else ok = ok && z >= 0 && z < g.nc[2]; }
+      if (ok) {
+        float my = fmaxf(0.f, (float)(abs(dy) - 1)) * g.cs[1];
+        float mz = fmaxf(0.f, (float)(abs(dz) - 1)) * g.cs[2];
+        float rem = rc2 - my * my - mz * mz;
+        if (rem > 0.f) {
+          int kx = (int)floorf(sqrtf(rem) / g.cs[0]) + 1;
+          int row = g.nc[0] * (y + g.nc[1] * z);
+          if (g.periodic && 2 * kx + 1 >= g.nc[0]) {
+            lo0 = g.cell_start[row]; hi0 = g.cell_start[row + g.nc[0]]; cnt = 1;
+          } else if (!g.periodic) {
+            int xlo = max(cx - kx, 0), xhi = min(cx + kx, g.nc[0] - 1);
+            lo0 = g.cell_start[row + xlo]; hi0 = g.cell_start[row + xhi + 1]; cnt = 1;
+          } else {
+            int xlo = cx - kx, xhi = cx + kx;
+            if (xlo < 0) {
+              lo0 = g.cell_start[row + xlo + g.nc[0]]; hi0 = g.cell_start[row + g.nc[0]];
+              lo1 = g.cell_start[row]; hi1 = g.cell_start[row + xhi + 1]; cnt = 2;
+            } else if (xhi >= g.nc[0]) {
+              lo0 = g.cell_start[row + xlo]; hi0 = g.cell_start[row + g.nc[0]];
+              lo1 = g.cell_start[row]; hi1 = g.cell_start[row + xhi - g.nc[0] + 1]; cnt = 2;
+            } else {
+              lo0 = g.cell_start[row + xlo]; hi0 = g.cell_start[row + xhi + 1]; cnt = 1;
+            }
+          }
+          if (cnt >= 1 && hi0 <= lo0) { lo0 = lo1; hi0 = hi1; cnt -= 1; }
+          if (cnt == 2 && hi1 <= lo1) cnt = 1;
+        }
+      }
+    }
+    // compact the non-empty ranges of this round into R (lane order = row order)
+    unsigned long long b1 = __ballot(cnt >= 1);
+    unsigned long long b2 = __ballot(cnt >= 2);
+    int before = __popcll(b1 & ((1ull << lane) - 1ull)) + __popcll(b2 & ((1ull << lane) - 1ull));
+    int slot = nr + before;
+    if (cnt >= 1 && slot < CAP) { R.lo[slot] = lo0; R.pre[slot + 1] = hi0 - lo0; }
+    if (cnt >= 2 && slot + 1 < CAP) { R.lo[slot + 1] = lo1; R.pre[slot + 2] = hi1 - lo1; }
+    nr += __popcll(b1) + __popcll(b2);
+  }
+  if (nr > CAP) {
+    if (lane == 0 && flags) atomicOr(&flags[TM_FLAG_NB_OVERFLOW], 4);
+    nr = CAP;
+  }
+  __syncwarp();
+  // inclusive scan of lengths into pre[1..nr] (sequential over 64-wide blocks)
+  if (lane == 0) R.pre[0] = 0;
+  int carry = 0;
+  for (int base = 0; base < nr; base += 64) {
+    int t = base + lane;
+    int v = t < nr ? R.pre[t + 1] : 0;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      int u = __shfl_up(v, o, 64);
+      if (lane >= o) v += u;
+    }
+    if (t < nr) R.pre[t + 1] = carry + v;
+    carry += __shfl(v, 63, 64);
+  }
+  __syncwarp();
+  *nrange = nr;
+  return carry;
+}
+
+// Position (in cell order) of the t-th candidate: binary search in pre.
+template <int CAP>
+__device__ __forceinline__ int tm_candidate(const RangeLds<CAP>& R, int nr, int t) {
+  int lo = 0, hi = nr;   // find r with pre[r] <= t < pre[r+1]
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (R.pre[mid] <= t) lo = mid; else hi = mid;
+  }
+  return R.lo[lo] + (t - R.pre[lo]);
+}
+
+// Sweep the candidates of centre i: calls f(p, valid, mapped) with all 64
+// lanes in lockstep (so f may use ballots); p indexes the cell-ordered arrays
+// when mapped, the atom arrays otherwise (molecule range, no grid).
+template <int CAP, class F>
+__device__ __forceinline__ void tm_sweep(int i, const int* __restrict__ mol,
+                                         const int* __restrict__ mol_start, const CellGrid& g,
+                                         float rc, RangeLds<CAP>& R, int lane, int* flags, F&& f) {
+  if (g.cell_atoms == nullptr) {
+    const int m = mol[i];
+    const int lo = mol_start[m], hi = mol_start[m + 1];
+    for (int base = lo; base < hi; base += 64) {
+      int p = base + lane;
+      f(p, p < hi, 0);
+    }
+    return;
+  }
+  int nr = 0;
+  const int total = tm_build_ranges(i, g, rc, R, lane, flags, &nr);
+  for (int base = 0; base < total; base += 64) {
+    int t = base + lane;
+    bool valid = t < total;
+    int p = valid ? tm_candidate(R, nr, t) : 0;
+    f(p, valid, 1);
+  }
+}
+

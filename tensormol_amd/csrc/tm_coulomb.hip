@@ -18,75 +18,21 @@
 // Neighbors.py:262-282), evaluated on the fly from the cell grid; no pair
 // list is stored.  One wavefront per centre gathers its own terms, so every
 // per-atom sum is formed in a fixed order (no atomics) and accumulated in FP64.
-#include "tm_common.h"
+#include "tm_cells.h"
 
 namespace {
 
 constexpr int C_WAVES = 4;
 constexpr double BOHRPERA_D = 1.889725989;
+constexpr int C_RANGES = 1024;   // 15 A stencil over ~2.3 A cells: <= ~500 ranges
 
 struct CoulArgs {
   float alpha, r_sr, r_lr, zz, yy, shift, ea;     // bohr units
-  float rc2;                                     // EE cutoff squared (A^2)
+  float rc, rc2;                                 // EE cutoff (A) and its square
   float width;                                   // vdW poly width (bohr after scaling)
   int add_ecc, has_vdw;
   float c6[TM_MAX_ELE], r0[TM_MAX_ELE];
 };
-
-struct Grid {
-  const int* cell_start;
-  const int* cell_atoms;
-  const int* cell_of;
-  int nc[3];
-  int periodic;
-  int reach;
-};
-
-template <class F>
-__device__ __forceinline__ void candidate_ranges(int i, const int* __restrict__ mol,
-                                                 const int* __restrict__ mol_start, const Grid& g,
-                                                 F&& f) {
-  if (g.cell_atoms == nullptr) {
-    int m = mol[i];
-    f(mol_start[m], mol_start[m + 1], 0);
-    return;
-  }
-  int c = g.cell_of[i];
-  int cx = c % g.nc[0], cy = (c / g.nc[0]) % g.nc[1], cz = c / (g.nc[0] * g.nc[1]);
-  int k = g.reach;
-  bool full[3];
-  for (int d = 0; d < 3; ++d) full[d] = g.periodic ? (2 * k + 1 >= g.nc[d]) : false;
-  int zlo = full[2] ? 0 : cz - k, zhi = full[2] ? g.nc[2] - 1 : cz + k;
-  int ylo = full[1] ? 0 : cy - k, yhi = full[1] ? g.nc[1] - 1 : cy + k;
-  if (!g.periodic) {
-    zlo = max(zlo, 0); zhi = min(zhi, g.nc[2] - 1);
-    ylo = max(ylo, 0); yhi = min(yhi, g.nc[1] - 1);
-  }
-  for (int zz = zlo; zz <= zhi; ++zz) {
-    int z = g.periodic ? ((zz % g.nc[2]) + g.nc[2]) % g.nc[2] : zz;
-    for (int yy = ylo; yy <= yhi; ++yy) {
-      int y = g.periodic ? ((yy % g.nc[1]) + g.nc[1]) % g.nc[1] : yy;
-      int base = g.nc[0] * (y + g.nc[1] * z);
-      if (full[0]) {
-        f(g.cell_start[base], g.cell_start[base + g.nc[0]], 1);
-      } else if (!g.periodic) {
-        int xlo = max(cx - k, 0), xhi = min(cx + k, g.nc[0] - 1);
-        f(g.cell_start[base + xlo], g.cell_start[base + xhi + 1], 1);
-      } else {
-        int xlo = cx - k, xhi = cx + k;
-        if (xlo < 0) {
-          f(g.cell_start[base + xlo + g.nc[0]], g.cell_start[base + g.nc[0]], 1);
-          f(g.cell_start[base], g.cell_start[base + xhi + 1], 1);
-        } else if (xhi >= g.nc[0]) {
-          f(g.cell_start[base + xlo], g.cell_start[base + g.nc[0]], 1);
-          f(g.cell_start[base], g.cell_start[base + xhi - g.nc[0] + 1], 1);
-        } else {
-          f(g.cell_start[base + xlo], g.cell_start[base + xhi + 1], 1);
-        }
-      }
-    }
-  }
-}
 
 __device__ __forceinline__ float min_image_f(float d, float L) {
   return L > 0.f ? d - L * rintf(d / L) : d;
@@ -165,15 +111,17 @@ __global__ void __launch_bounds__(C_WAVES * 64)
 coulomb_kernel(int N, const float4* __restrict__ xq, const int* __restrict__ elem,
                const float4* __restrict__ xqc, const int* __restrict__ elc,
                const int* __restrict__ mol, const int* __restrict__ mol_start,
-               const double* __restrict__ box, Grid grid, CoulArgs a, double* __restrict__ phi,
-               double* __restrict__ grad, double* __restrict__ eatom) {
+               const double* __restrict__ box, CellGrid grid, CoulArgs a, double* __restrict__ phi,
+               double* __restrict__ grad, double* __restrict__ eatom, int* __restrict__ flags) {
   __shared__ float4 qx[C_WAVES][QCAP];
   __shared__ int qe[C_WAVES][QCAP];
+  __shared__ RangeLds<C_RANGES> rng[C_WAVES];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int wi = blockIdx.x * C_WAVES + w;
   if (wi >= N) return;   // wave-level synchronisation only below
   // centres are visited in cell order so neighbouring waves share candidates in L2
   const int i = grid.cell_atoms != nullptr ? grid.cell_atoms[wi] : wi;
+  const int cpi = grid.cell_atoms != nullptr ? wi : -1;
   const int mi = mol[i];
   float Lx = 0.f, Ly = 0.f, Lz = 0.f;
   if (box != nullptr) {
@@ -195,47 +143,40 @@ coulomb_kernel(int N, const float4* __restrict__ xq, const int* __restrict__ ele
       pair_terms(a, pi.w, pj.w, c6i, r0i, a.c6[ej], a.r0[ej], dx, dy, dz, s, acc);
     }
   };
-  auto visit = [&](int lo, int hi, int mapped) {
-    const float4* src = mapped ? xqc : xq;
-    const int* esrc = mapped ? elc : elem;
-    for (int base = lo; base < hi; base += 64) {
-      int p = base + lane;
-      bool ok = false;
-      float4 pj;
-      if (p < hi) {
-        pj = src[p];
-        int j = mapped ? grid.cell_atoms[p] : p;
-        if (j != i && (!mapped || mol[j] == mi)) {
-          float dx = min_image_f(pj.x - pi.x, Lx);
-          float dy = min_image_f(pj.y - pi.y, Ly);
-          float dz = min_image_f(pj.z - pi.z, Lz);
-          ok = dx * dx + dy * dy + dz * dz < a.rc2;
-        }
-      }
-      unsigned long long bal = __ballot(ok);
-      int before = __popcll(bal & ((1ull << lane) - 1ull));
-      if (ok) {
-        qx[w][qn + before] = pj;
-        qe[w][qn + before] = esrc[p];
-      }
-      qn += __popcll(bal);
+  auto visit = [&](int p, bool valid, int mapped) {
+    bool ok = false;
+    float4 pj = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (valid && (mapped ? p != cpi : p != i) &&
+        (!mapped || grid.molc == nullptr || grid.molc[p] == mi)) {
+      pj = mapped ? xqc[p] : xq[p];
+      float dx = min_image_f(pj.x - pi.x, Lx);
+      float dy = min_image_f(pj.y - pi.y, Ly);
+      float dz = min_image_f(pj.z - pi.z, Lz);
+      ok = dx * dx + dy * dy + dz * dz < a.rc2;
+    }
+    unsigned long long bal = __ballot(ok);
+    int before = __popcll(bal & ((1ull << lane) - 1ull));
+    if (ok) {
+      qx[w][qn + before] = pj;
+      qe[w][qn + before] = mapped ? elc[p] : elem[p];
+    }
+    qn += __popcll(bal);
+    __syncwarp();
+    if (qn >= 64) {
+      drain(64);
       __syncwarp();
-      if (qn >= 64) {
-        drain(64);
-        __syncwarp();
-        // move the remainder (< 64 entries) to the front
-        float4 t4;
-        int te = 0;
-        bool mv = lane < qn - 64;
-        if (mv) { t4 = qx[w][64 + lane]; te = qe[w][64 + lane]; }
-        __syncwarp();
-        if (mv) { qx[w][lane] = t4; qe[w][lane] = te; }
-        qn -= 64;
-        __syncwarp();
-      }
+      // move the remainder (< 64 entries) to the front
+      float4 t4;
+      int te = 0;
+      bool mv = lane < qn - 64;
+      if (mv) { t4 = qx[w][64 + lane]; te = qe[w][64 + lane]; }
+      __syncwarp();
+      if (mv) { qx[w][lane] = t4; qe[w][lane] = te; }
+      qn -= 64;
+      __syncwarp();
     }
   };
-  candidate_ranges(i, mol, mol_start, grid, visit);
+  tm_sweep(i, mol, mol_start, grid, a.rc, rng[w], lane, flags, visit);
   drain(qn);
   double sphi = tm_wave_sum_d(acc.phi);
   double gx = tm_wave_sum_d(acc.gx);
@@ -404,18 +345,25 @@ int tm_charge_upstream(int N, const int* mol, const double* phi, const double* p
 // charges and elements in cell order.
 int tm_coulomb(int N, const float* xq, const int* elem, const int* mol, const int* mol_start,
                const double* box, const int* cell_start, const int* cell_atoms, const int* cell_of,
-               int nx, int ny, int nz, int periodic, int reach, const double* consts,
-               double rc_ee, double width, int add_ecc, int has_vdw, int nele, const double* c6,
-               const double* r0, float* xqc, int* elc, double* phi, double* grad, double* eatom,
-               void* stream) {
+               const int* cell_pos, const int* molc, int nx, int ny, int nz, double csx, double csy,
+               double csz, int periodic, const double* consts, double rc_ee, double width,
+               int add_ecc, int has_vdw, int nele, const double* c6, const double* r0, float* xqc,
+               int* elc, double* phi, double* grad, double* eatom, int* flags, void* stream) {
   TM_REQUIRE(nele > 0 && nele <= TM_MAX_ELE, "tm_coulomb: bad nele");
-  TM_REQUIRE(cell_atoms == nullptr || (xqc != nullptr && elc != nullptr),
-             "tm_coulomb: cell sweeps need xqc/elc buffers");
+  TM_REQUIRE(cell_atoms == nullptr || (xqc != nullptr && elc != nullptr && cell_pos != nullptr),
+             "tm_coulomb: cell sweeps need cell_pos and the xqc/elc buffers");
+  if (cell_atoms != nullptr) {
+    int k = (int)ceil(rc_ee / fmin(csx, fmin(csy, csz))) + 1;
+    TM_REQUIRE((2 * k + 3) * (2 * k + 3) * 2 <= 2 * C_RANGES,
+               "tm_coulomb: cells of %.2f A are too small for a %.1f A cutoff",
+               fmin(csx, fmin(csy, csz)), rc_ee);
+  }
   if (N == 0) return TM_OK;
   CoulArgs a;
   a.alpha = (float)consts[0]; a.r_sr = (float)consts[1]; a.r_lr = (float)consts[2];
   a.zz = (float)consts[3]; a.yy = (float)consts[4]; a.shift = (float)consts[5];
   a.ea = (float)consts[6];
+  a.rc = (float)rc_ee;
   a.rc2 = (float)(rc_ee * rc_ee);
   a.width = (float)width;
   a.add_ecc = add_ecc;
@@ -424,11 +372,12 @@ int tm_coulomb(int N, const float* xq, const int* elem, const int* mol, const in
     a.c6[e] = e < nele ? (float)c6[e] : 0.f;
     a.r0[e] = e < nele ? (float)r0[e] : 0.f;
   }
-  Grid g;
+  CellGrid g;
   g.cell_start = cell_start; g.cell_atoms = cell_atoms; g.cell_of = cell_of;
+  g.cell_pos = cell_pos; g.posc = nullptr; g.molc = molc;
   g.nc[0] = nx; g.nc[1] = ny; g.nc[2] = nz;
+  g.cs[0] = (float)csx; g.cs[1] = (float)csy; g.cs[2] = (float)csz;
   g.periodic = periodic;
-  g.reach = reach;
   hipStream_t s = (hipStream_t)stream;
   if (cell_atoms != nullptr) {
     cell_gather_kernel<<<tm_div_up(N, 256), 256, 0, s>>>(N, cell_atoms, (const float4*)xq, elem,
@@ -437,7 +386,7 @@ int tm_coulomb(int N, const float* xq, const int* elem, const int* mol, const in
   }
   coulomb_kernel<<<tm_div_up(N, C_WAVES), C_WAVES * 64, 0, s>>>(
       N, (const float4*)xq, elem, (const float4*)xqc, elc, mol, mol_start, box, g, a, phi, grad,
-      eatom);
+      eatom, flags);
   TM_LAUNCH_CHECK();
   return TM_OK;
 }

@@ -23,96 +23,37 @@
 //
 // Candidate atoms come either from the atom's molecule (batches of small
 // molecules) or from a cell grid (one large open or periodic system).
-#include "tm_common.h"
+#include "tm_cells.h"
 
 namespace {
 
 constexpr int NL_WAVES = 4;           // waves per block, one centre per wave
 constexpr int NL_LDS_CAP = 256;       // candidate buffer per wave
+constexpr int NL_RANGES = 192;        // stencil ranges of a 4.6 A sweep over ~2.3 A cells
 
-struct CellGrid {
-  const int* cell_start;   // ncell+1, offsets into cell_atoms
-  const int* cell_atoms;   // atoms sorted by cell
-  const int* cell_of;      // cell of each atom
-  int nc[3];               // cells per axis
-  int periodic;            // 1: wrap stencil, 0: clamp
-};
-
-__device__ __forceinline__ double sq_dist_nocontract(const double* __restrict__ pos, int i, int j,
-                                                     double Lx, double Ly, double Lz) {
+__device__ __forceinline__ double sq_dist_nocontract(double xi, double yi, double zi, double xj,
+                                                     double yj, double zj, double Lx, double Ly,
+                                                     double Lz) {
 #pragma clang fp contract(off)
-  double dx = tm_min_image(pos[3 * i + 0] - pos[3 * j + 0], Lx);
-  double dy = tm_min_image(pos[3 * i + 1] - pos[3 * j + 1], Ly);
-  double dz = tm_min_image(pos[3 * i + 2] - pos[3 * j + 2], Lz);
+  double dx = tm_min_image(xi - xj, Lx);
+  double dy = tm_min_image(yi - yj, Ly);
+  double dz = tm_min_image(zi - zj, Lz);
   double s = dx * dx + dy * dy;
   s = s + dz * dz;
   return s;
-}
-
-// Visit the candidate ranges of atom i.  Calls f(lo, hi, mapped) for each
-// contiguous range of candidate positions; mapped != 0 means positions index
-// cell_atoms, else they are atom indices.  The loop structure is uniform
-// across the wave (depends only on i).
-template <class F>
-__device__ __forceinline__ void for_candidate_ranges(int i, const int* __restrict__ mol,
-                                                     const int* __restrict__ mol_start,
-                                                     const CellGrid& g, int reach, F&& f) {
-  if (g.cell_atoms == nullptr) {
-    int m = mol[i];
-    f(mol_start[m], mol_start[m + 1], 0);
-    return;
-  }
-  int c = g.cell_of[i];
-  int cx = c % g.nc[0];
-  int cy = (c / g.nc[0]) % g.nc[1];
-  int cz = c / (g.nc[0] * g.nc[1]);
-  int k[3] = {reach, reach, reach};
-  bool full[3];
-  for (int d = 0; d < 3; ++d) full[d] = g.periodic ? (2 * k[d] + 1 >= g.nc[d]) : false;
-  int zlo = full[2] ? 0 : cz - k[2], zhi = full[2] ? g.nc[2] - 1 : cz + k[2];
-  int ylo = full[1] ? 0 : cy - k[1], yhi = full[1] ? g.nc[1] - 1 : cy + k[1];
-  if (!g.periodic) {
-    zlo = max(zlo, 0); zhi = min(zhi, g.nc[2] - 1);
-    ylo = max(ylo, 0); yhi = min(yhi, g.nc[1] - 1);
-  }
-  for (int zz = zlo; zz <= zhi; ++zz) {
-    int z = zz;
-    if (g.periodic) z = ((zz % g.nc[2]) + g.nc[2]) % g.nc[2];
-    for (int yy = ylo; yy <= yhi; ++yy) {
-      int y = yy;
-      if (g.periodic) y = ((yy % g.nc[1]) + g.nc[1]) % g.nc[1];
-      int base = g.nc[0] * (y + g.nc[1] * z);
-      if (full[0] || (!g.periodic && (cx - k[0] <= 0 && cx + k[0] >= g.nc[0] - 1))) {
-        f(g.cell_start[base], g.cell_start[base + g.nc[0]], 1);
-      } else if (!g.periodic) {
-        int xlo = max(cx - k[0], 0), xhi = min(cx + k[0], g.nc[0] - 1);
-        f(g.cell_start[base + xlo], g.cell_start[base + xhi + 1], 1);
-      } else {
-        int xlo = cx - k[0], xhi = cx + k[0];
-        if (xlo < 0) {
-          f(g.cell_start[base + xlo + g.nc[0]], g.cell_start[base + g.nc[0]], 1);
-          f(g.cell_start[base], g.cell_start[base + xhi + 1], 1);
-        } else if (xhi >= g.nc[0]) {
-          f(g.cell_start[base + xlo], g.cell_start[base + g.nc[0]], 1);
-          f(g.cell_start[base], g.cell_start[base + xhi - g.nc[0] + 1], 1);
-        } else {
-          f(g.cell_start[base + xlo], g.cell_start[base + xhi + 1], 1);
-        }
-      }
-    }
-  }
 }
 
 __global__ void __launch_bounds__(NL_WAVES * 64)
 nl_rows_kernel(int N, const double* __restrict__ pos, const int* __restrict__ elem,
                const int* __restrict__ mol, const int* __restrict__ mol_start,
                const double* __restrict__ box, CellGrid grid, int nele,
-               double s_r, double s_a, int ldr, int lda,
+               double s_r, double s_a, float rc, int ldr, int lda,
                int* __restrict__ nbr, int* __restrict__ ncnt, int* __restrict__ rseg,
                int* __restrict__ ang, int* __restrict__ acnt, int* __restrict__ aseg,
                int* __restrict__ flags) {
   __shared__ unsigned long long keys[NL_WAVES][NL_LDS_CAP];
   __shared__ unsigned long long sorted[NL_WAVES][NL_LDS_CAP];
+  __shared__ RangeLds<NL_RANGES> rng[NL_WAVES];
   const int w = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int i = blockIdx.x * NL_WAVES + w;
@@ -124,32 +65,29 @@ nl_rows_kernel(int N, const double* __restrict__ pos, const int* __restrict__ el
     if (box != nullptr) {
       Lx = box[3 * mi]; Ly = box[3 * mi + 1]; Lz = box[3 * mi + 2];
     }
-    auto visit = [&](int lo, int hi, int mapped) {
-      for (int base = lo; base < hi; base += 64) {
-        int p = base + lane;
-        bool ok = false;
-        int j = -1;
-        bool in_a = false;
-        if (p < hi) {
-          j = mapped ? grid.cell_atoms[p] : p;
-          if (j != i && (!mapped || mol[j] == mi)) {
-            double s = sq_dist_nocontract(pos, i, j, Lx, Ly, Lz);
-            ok = s < s_r;
-            in_a = s < s_a;
-          }
-        }
-        unsigned long long bal = __ballot(ok);
-        int before = __popcll(bal & ((1ull << lane) - 1ull));
-        if (ok) {
-          int slot = count + before;
-          if (slot < NL_LDS_CAP)
-            keys[w][slot] = ((unsigned long long)(unsigned)elem[j] << 40) |
-                            ((unsigned long long)(in_a ? 1u : 0u) << 32) | (unsigned)j;
-        }
-        count += __popcll(bal);
+    const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+    const int cpi = grid.cell_pos != nullptr ? grid.cell_pos[i] : -1;
+    auto visit = [&](int p, bool valid, int mapped) {
+      bool ok = false, in_a = false;
+      if (valid && (mapped ? p != cpi : p != i) &&
+          (!mapped || grid.molc == nullptr || grid.molc[p] == mi)) {
+        const double* q = mapped ? grid.posc + 3 * (long)p : pos + 3 * (long)p;
+        double s = sq_dist_nocontract(xi, yi, zi, q[0], q[1], q[2], Lx, Ly, Lz);
+        ok = s < s_r;
+        in_a = s < s_a;
       }
+      unsigned long long bal = __ballot(ok);
+      int before = __popcll(bal & ((1ull << lane) - 1ull));
+      if (ok) {
+        int slot = count + before;
+        int j = mapped ? grid.cell_atoms[p] : p;
+        if (slot < NL_LDS_CAP)
+          keys[w][slot] = ((unsigned long long)(unsigned)elem[j] << 40) |
+                          ((unsigned long long)(in_a ? 1u : 0u) << 32) | (unsigned)j;
+      }
+      count += __popcll(bal);
     };
-    for_candidate_ranges(i, mol, mol_start, grid, 1, visit);
+    tm_sweep(i, mol, mol_start, grid, rc, rng[w], lane, flags, visit);
     if (count > NL_LDS_CAP || count > ldr) {
       if (lane == 0) atomicOr(&flags[TM_FLAG_NB_OVERFLOW], 1);
       count = min(count, min(NL_LDS_CAP, ldr));
@@ -314,7 +252,10 @@ __global__ void cell_fill_kernel(int N, const int* __restrict__ cell_of,
 // deterministic: one wavefront per cell ranks its entries (cells hold a few
 // dozen atoms; larger cells are ranked in 64-entry rounds)
 __global__ void cell_sort_kernel(int ncell, const int* __restrict__ cell_start,
-                                 const int* __restrict__ unsorted, int* __restrict__ cell_atoms) {
+                                 const int* __restrict__ unsorted, int* __restrict__ cell_atoms,
+                                 int* __restrict__ cell_pos, const double* __restrict__ pos,
+                                 double* __restrict__ posc, const int* __restrict__ mol,
+                                 int* __restrict__ molc) {
   const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (c >= ncell) return;
@@ -323,7 +264,13 @@ __global__ void cell_sort_kernel(int ncell, const int* __restrict__ cell_start,
     int v = unsorted[a];
     int rank = 0;
     for (int b = lo; b < hi; ++b) rank += unsorted[b] < v;
-    cell_atoms[lo + rank] = v;
+    int q = lo + rank;
+    cell_atoms[q] = v;
+    cell_pos[v] = q;
+    posc[3 * (long)q] = pos[3 * (long)v];
+    posc[3 * (long)q + 1] = pos[3 * (long)v + 1];
+    posc[3 * (long)q + 2] = pos[3 * (long)v + 2];
+    if (molc != nullptr) molc[q] = mol[v];
   }
 }
 
@@ -371,12 +318,14 @@ template <bool FILL>
 __global__ void __launch_bounds__(NL_WAVES * 64)
 pairs_kernel(int N, const double* __restrict__ pos, const int* __restrict__ mol,
              const int* __restrict__ mol_start, const int* __restrict__ nreal,
-             const double* __restrict__ box, CellGrid grid, int reach, double s_c, int half,
-             int* __restrict__ count, const long* __restrict__ start, int* __restrict__ out) {
+             const double* __restrict__ box, CellGrid grid, float rc, double s_c, int half,
+             int* __restrict__ count, const long* __restrict__ start, int* __restrict__ out,
+             int* __restrict__ flags) {
+  __shared__ RangeLds<1024> rng[NL_WAVES];
   const int w = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const int i = blockIdx.x * NL_WAVES + w;
-  if (i >= N) return;
+  if (i >= N) return;   // wave-level synchronisation only
   const int mi = mol[i];
   if (nreal != nullptr && i - mol_start[mi] >= nreal[mi]) {
     if (!FILL && lane == 0) count[i] = 0;
@@ -386,24 +335,24 @@ pairs_kernel(int N, const double* __restrict__ pos, const int* __restrict__ mol,
   if (box != nullptr) {
     Lx = box[3 * mi]; Ly = box[3 * mi + 1]; Lz = box[3 * mi + 2];
   }
+  const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+  const int cpi = grid.cell_pos != nullptr ? grid.cell_pos[i] : -1;
   long o = FILL ? start[i] : 0;
   int cnt = 0;
-  auto visit = [&](int lo, int hi, int mapped) {
-    for (int base = lo; base < hi; base += 64) {
-      int p = base + lane;
-      bool ok = false;
-      int j = -1;
-      if (p < hi) {
-        j = mapped ? grid.cell_atoms[p] : p;
-        if (j != i && (!half || j > i) && (!mapped || mol[j] == mi))
-          ok = sq_dist_nocontract(pos, i, j, Lx, Ly, Lz) < s_c;
-      }
-      unsigned long long bal = __ballot(ok);
-      if (FILL && ok) out[o + cnt + __popcll(bal & ((1ull << lane) - 1ull))] = j;
-      cnt += __popcll(bal);
+  auto visit = [&](int p, bool valid, int mapped) {
+    bool ok = false;
+    int j = -1;
+    if (valid && (mapped ? p != cpi : p != i) &&
+        (!mapped || grid.molc == nullptr || grid.molc[p] == mi)) {
+      j = mapped ? grid.cell_atoms[p] : p;
+      const double* q = mapped ? grid.posc + 3 * (long)p : pos + 3 * (long)p;
+      ok = (!half || j > i) && sq_dist_nocontract(xi, yi, zi, q[0], q[1], q[2], Lx, Ly, Lz) < s_c;
     }
+    unsigned long long bal = __ballot(ok);
+    if (FILL && ok) out[o + cnt + __popcll(bal & ((1ull << lane) - 1ull))] = j;
+    cnt += __popcll(bal);
   };
-  for_candidate_ranges(i, mol, mol_start, grid, reach, visit);
+  tm_sweep(i, mol, mol_start, grid, rc, rng[w], lane, flags, visit);
   if (!FILL && lane == 0) count[i] = cnt;
 }
 
@@ -411,12 +360,28 @@ pairs_kernel(int N, const double* __restrict__ pos, const int* __restrict__ mol,
 
 extern "C" {
 
+static CellGrid make_grid(const int* cell_start, const int* cell_atoms, const int* cell_of,
+                          const int* cell_pos, const double* posc, const int* molc, int nx, int ny,
+                          int nz, double csx, double csy, double csz, int periodic) {
+  CellGrid g;
+  g.cell_start = cell_start; g.cell_atoms = cell_atoms; g.cell_of = cell_of;
+  g.cell_pos = cell_pos; g.posc = posc; g.molc = molc;
+  g.nc[0] = nx; g.nc[1] = ny; g.nc[2] = nz;
+  g.cs[0] = (float)csx; g.cs[1] = (float)csy; g.cs[2] = (float)csz;
+  g.periodic = periodic;
+  return g;
+}
+
 // cell_fill must hold max(ncell, N) ints: per-cell counters, then the
-// unsorted staging copy of cell_atoms.
-int tm_cell_build(int N, const double* pos, double ox, double oy, double oz, double cx, double cy,
-                  double cz, int nx, int ny, int nz, int periodic, int* cell_of, int* cell_count,
-                  int* cell_start, int* cell_fill, int* cell_atoms, int* flags, void* stream) {
+// unsorted staging copy of cell_atoms.  Outputs cell_atoms (index order inside
+// each cell), cell_pos (position of every atom in cell order), posc (positions
+// in cell order) and molc (molecules in cell order; may be NULL for one system).
+int tm_cell_build(int N, const double* pos, const int* mol, double ox, double oy, double oz,
+                  double cx, double cy, double cz, int nx, int ny, int nz, int periodic,
+                  int* cell_of, int* cell_count, int* cell_start, int* cell_fill, int* cell_atoms,
+                  int* cell_pos, double* posc, int* molc, int* flags, void* stream) {
   TM_REQUIRE(N >= 0 && nx > 0 && ny > 0 && nz > 0, "tm_cell_build: bad sizes");
+  TM_REQUIRE((long)nx * ny * nz < (1l << 26), "tm_cell_build: %d x %d x %d cells is too many", nx, ny, nz);
   hipStream_t s = (hipStream_t)stream;
   long ncell = (long)nx * ny * nz;
   if (N == 0) return TM_OK;
@@ -428,37 +393,40 @@ int tm_cell_build(int N, const double* pos, double ox, double oy, double oz, dou
   TM_LAUNCH_CHECK();
   scan_kernel<<<1, 1024, 0, s>>>((int)ncell, cell_count, cell_start);
   TM_LAUNCH_CHECK();
-  // cell_fill doubles as the unsorted staging array once its counters are spent
   cell_fill_kernel<<<tm_div_up(N, 256), 256, 0, s>>>(N, cell_of, cell_start, cell_fill, cell_atoms);
   TM_LAUNCH_CHECK();
+  // cell_fill doubles as the unsorted staging array once its counters are spent
   copy_i32_kernel<<<tm_div_up(N, 256), 256, 0, s>>>(N, cell_atoms, cell_fill);
   TM_LAUNCH_CHECK();
-  cell_sort_kernel<<<tm_div_up(ncell, 4), 256, 0, s>>>((int)ncell, cell_start, cell_fill, cell_atoms);
+  cell_sort_kernel<<<tm_div_up(ncell, 4), 256, 0, s>>>((int)ncell, cell_start, cell_fill,
+                                                       cell_atoms, cell_pos, pos, posc, mol, molc);
   TM_LAUNCH_CHECK();
   return TM_OK;
 }
 
 int tm_nl_build(int N, const double* pos, const int* elem, const int* mol, const int* mol_start,
                 const double* box, const int* cell_start, const int* cell_atoms,
-                const int* cell_of, int nx, int ny, int nz, int periodic, int nele, double s_r,
-                double s_a, int ldr, int lda, int* nbr, int* ncnt, int* rseg, int* ang, int* acnt,
-                int* aseg, int* rev, int* flags, void* stream) {
+                const int* cell_of, const int* cell_pos, const double* posc, const int* molc,
+                int nx, int ny, int nz, double csx, double csy, double csz, int periodic, int nele,
+                double rc, double s_r, double s_a, int ldr, int lda, int* nbr, int* ncnt,
+                int* rseg, int* ang, int* acnt, int* aseg, int* rev, int* flags, void* stream) {
   TM_REQUIRE(N >= 0 && nele > 0 && nele <= TM_MAX_ELE, "tm_nl_build: bad nele %d", nele);
   TM_REQUIRE(ldr > 0 && ldr <= 256 && lda > 0 && lda <= ldr, "tm_nl_build: bad strides %d %d", ldr,
              lda);
   TM_REQUIRE((long)N * ldr < (1l << 31), "tm_nl_build: N*ldr overflows int32 slots");
+  if (cell_atoms != nullptr) {
+    TM_REQUIRE(cell_start && cell_of && cell_pos && posc, "tm_nl_build: incomplete cell grid");
+    int k = (int)ceil(rc / fmin(csx, fmin(csy, csz))) + 1;
+    TM_REQUIRE((2 * k + 3) * (2 * k + 3) * 2 <= NL_RANGES,
+               "tm_nl_build: cells of %.2f A are too small for rc %.2f", fmin(csx, fmin(csy, csz)), rc);
+  }
   if (N == 0) return TM_OK;
   hipStream_t s = (hipStream_t)stream;
-  CellGrid g;
-  g.cell_start = cell_start;
-  g.cell_atoms = cell_atoms;
-  g.cell_of = cell_of;
-  g.nc[0] = nx; g.nc[1] = ny; g.nc[2] = nz;
-  g.periodic = periodic;
-  if (cell_atoms != nullptr) TM_REQUIRE(cell_start && cell_of, "tm_nl_build: incomplete cell grid");
+  CellGrid g = make_grid(cell_start, cell_atoms, cell_of, cell_pos, posc, molc, nx, ny, nz, csx, csy,
+                         csz, periodic);
   nl_rows_kernel<<<tm_div_up(N, NL_WAVES), NL_WAVES * 64, 0, s>>>(
-      N, pos, elem, mol, mol_start, box, g, nele, s_r, s_a, ldr, lda, nbr, ncnt, rseg, ang, acnt,
-      aseg, flags);
+      N, pos, elem, mol, mol_start, box, g, nele, s_r, s_a, (float)rc, ldr, lda, nbr, ncnt, rseg,
+      ang, acnt, aseg, flags);
   TM_LAUNCH_CHECK();
   long total = (long)N * ldr;
   nl_rev_kernel<<<tm_div_up(total, 256), 256, 0, s>>>(N, ldr, nele, elem, nbr, ncnt, rseg, rev,
@@ -481,22 +449,26 @@ int tm_triples(int N, int ldr, int lda, int nele, const int* nbr, const int* ang
 
 int tm_pairs(int N, const double* pos, const int* mol, const int* mol_start, const int* nreal,
              const double* box, const int* cell_start, const int* cell_atoms, const int* cell_of,
-             int nx, int ny, int nz, int periodic, int reach, double s_c, int half, int* count,
-             const long* start, int* out, void* stream) {
-  TM_REQUIRE(N >= 0 && reach >= 1, "tm_pairs: bad sizes");
+             const int* cell_pos, const double* posc, const int* molc, int nx, int ny, int nz,
+             double csx, double csy, double csz, int periodic, double rc, double s_c, int half,
+             int* count, const long* start, int* out, int* flags, void* stream) {
+  TM_REQUIRE(N >= 0, "tm_pairs: bad sizes");
   TM_REQUIRE(start == nullptr || out != nullptr, "tm_pairs: fill pass needs an output array");
+  if (cell_atoms != nullptr) {
+    int k = (int)ceil(rc / fmin(csx, fmin(csy, csz))) + 1;
+    TM_REQUIRE((2 * k + 3) * (2 * k + 3) * 2 <= 1024,
+               "tm_pairs: cells of %.2f A are too small for rc %.2f", fmin(csx, fmin(csy, csz)), rc);
+  }
   if (N == 0) return TM_OK;
   hipStream_t s = (hipStream_t)stream;
-  CellGrid g;
-  g.cell_start = cell_start; g.cell_atoms = cell_atoms; g.cell_of = cell_of;
-  g.nc[0] = nx; g.nc[1] = ny; g.nc[2] = nz;
-  g.periodic = periodic;
+  CellGrid g = make_grid(cell_start, cell_atoms, cell_of, cell_pos, posc, molc, nx, ny, nz, csx, csy,
+                         csz, periodic);
   if (start == nullptr)
     pairs_kernel<false><<<tm_div_up(N, NL_WAVES), NL_WAVES * 64, 0, s>>>(
-        N, pos, mol, mol_start, nreal, box, g, reach, s_c, half, count, nullptr, nullptr);
+        N, pos, mol, mol_start, nreal, box, g, (float)rc, s_c, half, count, nullptr, nullptr, flags);
   else
     pairs_kernel<true><<<tm_div_up(N, NL_WAVES), NL_WAVES * 64, 0, s>>>(
-        N, pos, mol, mol_start, nreal, box, g, reach, s_c, half, count, start, out);
+        N, pos, mol, mol_start, nreal, box, g, (float)rc, s_c, half, count, start, out, flags);
   TM_LAUNCH_CHECK();
   return TM_OK;
 }

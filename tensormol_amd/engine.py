@@ -205,10 +205,11 @@ class BPEEEngine:
         return ws
 
     def _setup_grid(self, ws, pos0, margin):
-        rc = self.model.sym.rr
+        """Cell grid with edges of about half the short cutoff (>= rr/2)."""
+        target = 0.5 * self.model.sym.rr
         if ws.periodic:
             L = ws.boxes_np[0]
-            nc = np.maximum(1, np.floor(L / rc)).astype(int)
+            nc = np.maximum(1, np.floor(L / target)).astype(int)
             cs = L / nc
             origin = np.zeros(3)
         else:
@@ -217,13 +218,12 @@ class BPEEEngine:
             p = np.asarray(pos0)
             lo = p.min(0) - margin
             hi = p.max(0) + margin
-            nc = np.maximum(1, np.floor((hi - lo) / rc)).astype(int)
+            nc = np.maximum(1, np.floor((hi - lo) / target)).astype(int)
             cs = (hi - lo) / nc
             origin = lo
         ws.grid_nc = [int(v) for v in nc]
         ws.grid_cs = [float(v) for v in cs]
         ws.grid_origin = [float(v) for v in origin]
-        ws.coul_reach = int(math.ceil(self.model.ee_off / float(min(cs))))
         ncell = int(np.prod(nc))
         i32 = dict(dtype=torch.int32, device=self.device)
         ws.cell_of = torch.zeros(ws.N, **i32)
@@ -231,6 +231,17 @@ class BPEEEngine:
         ws.cell_start = torch.zeros(ncell + 1, **i32)
         ws.cell_fill = torch.zeros(max(ncell, ws.N), **i32)   # counters, then sort staging
         ws.cell_atoms = torch.zeros(ws.N, **i32)
+        ws.cell_pos = torch.zeros(ws.N, **i32)
+        ws.posc = torch.zeros(ws.N, 3, dtype=torch.float64, device=self.device)
+        ws.molc = torch.zeros(ws.N, **i32) if ws.B > 1 else None
+
+    def _grid_args(self, ws):
+        """(cell_start, cell_atoms, cell_of, cell_pos, posc, molc, nx, ny, nz, cx, cy, cz)."""
+        if not ws.use_cells:
+            return (None, None, None, None, None, None, 1, 1, 1, 1.0, 1.0, 1.0)
+        n, c = ws.grid_nc, ws.grid_cs
+        return (_P(ws.cell_start), _P(ws.cell_atoms), _P(ws.cell_of), _P(ws.cell_pos), _P(ws.posc),
+                _P(ws.molc), n[0], n[1], n[2], c[0], c[1], c[2])
 
     # ---------------------------------------------------------- stages
     def _stream(self):
@@ -239,18 +250,17 @@ class BPEEEngine:
     def build_lists(self, ws, pos):
         L, st = self.lib, self._stream()
         L.tm_zero_i32(8, _P(ws.flags), st)
-        grid = (None, None, None, 1, 1, 1)
         if ws.use_cells:
             o, c, n = ws.grid_origin, ws.grid_cs, ws.grid_nc
-            L.tm_cell_build(ws.N, _P(pos), o[0], o[1], o[2], c[0], c[1], c[2], n[0], n[1], n[2],
-                            ws.periodic, _P(ws.cell_of), _P(ws.cell_count), _P(ws.cell_start),
-                            _P(ws.cell_fill), _P(ws.cell_atoms), _P(ws.flags), st)
-            grid = (_P(ws.cell_start), _P(ws.cell_atoms), _P(ws.cell_of), n[0], n[1], n[2])
-        L.tm_nl_build(ws.N, _P(pos), _P(ws.elem), _P(ws.mol), _P(ws.mol_start), _P(ws.box),
-                      grid[0], grid[1], grid[2], grid[3], grid[4], grid[5], ws.periodic,
-                      self.nele, self.s_r, self.s_a, self.ldr, self.lda, _P(ws.nbr), _P(ws.ncnt),
-                      _P(ws.rseg), _P(ws.ang), _P(ws.acnt), _P(ws.aseg), _P(ws.rev),
-                      _P(ws.flags), st)
+            L.tm_cell_build(ws.N, _P(pos), _P(ws.mol), o[0], o[1], o[2], c[0], c[1], c[2], n[0], n[1],
+                            n[2], ws.periodic, _P(ws.cell_of), _P(ws.cell_count), _P(ws.cell_start),
+                            _P(ws.cell_fill), _P(ws.cell_atoms), _P(ws.cell_pos), _P(ws.posc),
+                            _P(ws.molc), _P(ws.flags), st)
+        g = self._grid_args(ws)
+        L.tm_nl_build(ws.N, _P(pos), _P(ws.elem), _P(ws.mol), _P(ws.mol_start), _P(ws.box), *g,
+                      ws.periodic, self.nele, self.model.sym.rr, self.s_r, self.s_a, self.ldr,
+                      self.lda, _P(ws.nbr), _P(ws.ncnt), _P(ws.rseg), _P(ws.ang), _P(ws.acnt),
+                      _P(ws.aseg), _P(ws.rev), _P(ws.flags), st)
 
     def check_flags(self, ws):
         f = ws.flags.cpu().numpy()
@@ -289,17 +299,14 @@ class BPEEEngine:
         L.tm_mol_sum_f32(ws.B, _P(ws.mol_start), 1, _P(ws.qraw), 1, _P(ws.qmean), 1, _P(ws.red), ws.N, st)
         L.tm_charges(ws.N, _P(ws.mol), _P(ws.qmean), _P(ws.qraw), _P(pos), _P(ws.xq), _P(ws.q),
                      _P(ws.qx), st)
-        grid = (None, None, None, 1, 1, 1, 0)
-        if ws.use_cells:
-            n = ws.grid_nc
-            grid = (_P(ws.cell_start), _P(ws.cell_atoms), _P(ws.cell_of), n[0], n[1], n[2],
-                    ws.coul_reach)
+        g = self._grid_args(ws)
         L.tm_coulomb(ws.N, _P(ws.xq), _P(ws.elem), _P(ws.mol), _P(ws.mol_start), _P(ws.box),
-                     grid[0], grid[1], grid[2], grid[3], grid[4], grid[5], ws.periodic, grid[6],
-                     self.dsf.ctypes.data_as(_lib.P), self.model.ee_off, self.vdw_width,
+                     g[0], g[1], g[2], g[3], g[5], g[6], g[7], g[8], g[9], g[10], g[11],
+                     ws.periodic, self.dsf.ctypes.data_as(_lib.P), self.model.ee_off, self.vdw_width,
                      int(bool(PARAMS["AddEcc"])), int(self.model.has_vdw), self.nele,
                      self.c6.ctypes.data_as(_lib.P), self.r0.ctypes.data_as(_lib.P),
-                     _P(ws.xqc), _P(ws.elc), _P(ws.phi), _P(ws.gradc), _P(ws.eatom), st)
+                     _P(ws.xqc), _P(ws.elc), _P(ws.phi), _P(ws.gradc), _P(ws.eatom), _P(ws.flags),
+                     st)
         L.tm_mol_sum_f64(ws.B, _P(ws.mol_start), 1, _P(ws.phi), 1, _P(ws.phimean), 1, _P(ws.red), ws.N, st)
         L.tm_charge_upstream(ws.N, _P(ws.mol), _P(ws.phi), _P(ws.phimean), _P(ws.u), st)
 

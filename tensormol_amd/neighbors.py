@@ -74,14 +74,15 @@ def _pairs(pk, rc, half):
     st = _stream()
     thr = nl_threshold(rc)
     count = torch.zeros(max(1, pk.N), dtype=torch.int32, device=pk.dev)
+    flags = torch.zeros(8, dtype=torch.int32, device=pk.dev)
     args = (pk.N, _P(pk.pos), _P(pk.mol), _P(pk.mol_start), _P(pk.nreal), None, None, None, None,
-            1, 1, 1, 0, 1, thr, int(half))
-    L.tm_pairs(*args, _P(count), None, None, st)
+            None, None, None, 1, 1, 1, 1.0, 1.0, 1.0, 0, float(rc), thr, int(half))
+    L.tm_pairs(*args, _P(count), None, None, _P(flags), st)
     cnt = count[:pk.N].cpu().numpy().astype(np.int64)
     start = np.concatenate([[0], np.cumsum(cnt)]).astype(np.int64)
     out = torch.zeros(max(1, int(start[-1])), dtype=torch.int32, device=pk.dev)
     st_t = torch.tensor(start[:-1] if pk.N else np.zeros(1, np.int64), device=pk.dev)
-    L.tm_pairs(*args, _P(count), _P(st_t), _P(out), st)
+    L.tm_pairs(*args, _P(count), _P(st_t), _P(out), _P(flags), st)
     j = out.cpu().numpy()[: start[-1]].astype(np.int64)
     i = np.repeat(np.arange(pk.N), cnt)
     m = pk.mol_np[i].astype(np.int64)
@@ -231,8 +232,9 @@ class NeighborListSet:
         rev = torch.zeros(max(1, N * ldr), **i32)
         flags = torch.zeros(8, **i32)
         L.tm_nl_build(N, _P(pk.pos), _P(el), _P(pk.mol), _P(pk.mol_start), None, None, None, None,
-                      1, 1, 1, 0, nele, nl_threshold(rr), nl_threshold(ra), ldr, lda, _P(nbr),
-                      _P(ncnt), _P(rseg), _P(ang), _P(acnt), _P(aseg), _P(rev), _P(flags), st)
+                      None, None, None, 1, 1, 1, 1.0, 1.0, 1.0, 0, nele, float(rr), nl_threshold(rr),
+                      nl_threshold(ra), ldr, lda, _P(nbr), _P(ncnt), _P(rseg), _P(ang), _P(acnt),
+                      _P(aseg), _P(rev), _P(flags), st)
         f = flags.cpu().numpy()
         if f[0] or f[1]:
             raise RuntimeError(f"neighbour rows exceed {ldr} (pairs) or {lda} (triples) entries")

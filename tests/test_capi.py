@@ -49,11 +49,24 @@ def test_python_binding_covers_header(lib):
 def test_version_and_argument_errors(lib):
     assert lib.tm_version() == 1
     with pytest.raises(RuntimeError, match="bad nele"):
-        lib.tm_nl_build(10, None, None, None, None, None, None, None, None, 1, 1, 1, 0, 0, 1.0, 1.0,
-                        128, 48, None, None, None, None, None, None, None, None, None)
+        lib.tm_nl_build(10, None, None, None, None, None, None, None, None, None, None, None, 1, 1, 1,
+                        1.0, 1.0, 1.0, 0, 0, 4.6, 1.0, 1.0, 128, 48, None, None, None, None, None,
+                        None, None, None, None)
     with pytest.raises(RuntimeError, match="3 hidden layers"):
         out = (ctypes.c_long * 9)()
         lib.tm_mlp_mfma_pack_info(256, 512, 2, 0, out)
     out = (ctypes.c_long * 9)()
     lib.tm_mlp_mfma_pack_info(256, 512, 3, 0, out)
     assert out[8] == 1 and out[0] > 0
+
+
+def test_binding_argument_counts_match_header():
+    """Each ctypes signature has as many arguments as the C declaration."""
+    from tensormol_amd._lib import SIGNATURES
+    txt = "\n".join(open(h).read() for h in glob.glob(os.path.join(ROOT, "include", "*.h")))
+    for name, args in SIGNATURES.items():
+        m = re.search(r"(?:int|const char\*)\s+" + name + r"\s*\(([^;]*?)\);", txt, re.S)
+        assert m, name
+        params = m.group(1).strip()
+        n = 0 if params in ("", "void") else params.count(",") + 1
+        assert n == len(args), f"{name}: header has {n} parameters, binding {len(args)}"
