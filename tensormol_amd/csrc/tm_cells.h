@@ -157,11 +157,28 @@ __device__ __forceinline__ void tm_sweep(int i, const int* __restrict__ mol,
   }
   int nr = 0;
   const int total = tm_build_ranges(i, g, rc, R, lane, flags, &nr);
+  // Walk the concatenated ranges 64 candidates at a time.  The ranges that a
+  // chunk touches start at r0 (wave-uniform); lane k holds range r0 + k, and
+  // each lane finds its own range by counting, with shuffles, the range
+  // starts at or below its candidate index.
+  int r0 = 0;
   for (int base = 0; base < total; base += 64) {
-    int t = base + lane;
-    bool valid = t < total;
-    int p = valid ? tm_candidate(R, nr, t) : 0;
-    f(p, valid, 1);
+    const int t = base + lane;
+    const int rk = r0 + lane;
+    const int pre_k = rk < nr ? R.pre[rk] : 0x7fffffff;
+    const int lo_k = rk < nr ? R.lo[rk] : 0;
+    // ranges starting inside this chunk: those with pre <= base + 63
+    const unsigned long long inside = __ballot(rk < nr && pre_k <= base + 63);
+    const int span = __popcll(inside);   // >= 1: range r0 contains base
+    int idx = 0;
+    for (int k = 1; k < span; ++k) idx += (__shfl(pre_k, k, 64) <= t) ? 1 : 0;
+    const int lo_r = __shfl(lo_k, idx, 64);
+    const int pre_r = __shfl(pre_k, idx, 64);
+    const bool valid = t < total;
+    f(valid ? lo_r + (t - pre_r) : 0, valid, 1);
+    // range holding candidate base + 64: ranges r0 .. r0+span-1 start at or
+    // below base + 63; range r0+span (if any) starts after base + 63
+    r0 += span - 1;
+    if (r0 + 1 < nr && R.pre[r0 + 1] <= base + 64) r0 += 1;
   }
 }
-

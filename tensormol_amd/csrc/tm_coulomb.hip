@@ -24,7 +24,7 @@ namespace {
 
 constexpr int C_WAVES = 4;
 constexpr double BOHRPERA_D = 1.889725989;
-constexpr int C_RANGES = 1024;   // 15 A stencil over ~2.3 A cells: <= ~500 ranges
+constexpr int C_RANGES = 512;    // 15 A stencil over ~2.3 A cells: ~230 rows (+ wraps)
 
 struct CoulArgs {
   float alpha, r_sr, r_lr, zz, yy, shift, ea;     // bohr units
@@ -354,7 +354,7 @@ int tm_coulomb(int N, const float* xq, const int* elem, const int* mol, const in
              "tm_coulomb: cell sweeps need cell_pos and the xqc/elc buffers");
   if (cell_atoms != nullptr) {
     int k = (int)ceil(rc_ee / fmin(csx, fmin(csy, csz))) + 1;
-    TM_REQUIRE((2 * k + 3) * (2 * k + 3) * 2 <= 2 * C_RANGES,
+    TM_REQUIRE((2 * k + 1) * (2 * k + 1) <= 2 * C_RANGES,
                "tm_coulomb: cells of %.2f A are too small for a %.1f A cutoff",
                fmin(csx, fmin(csy, csz)), rc_ee);
   }

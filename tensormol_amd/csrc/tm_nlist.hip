@@ -212,30 +212,30 @@ __global__ void zero2_kernel(long n, int* __restrict__ a, int* __restrict__ b) {
   if (t < n) { a[t] = 0; b[t] = 0; }
 }
 
-// exclusive scan of cell_count into cell_start (single block, any length)
+// exclusive scan of cell_count into cell_start (single block): each thread
+// sums a contiguous chunk, one block scan of the 1024 partials, then each
+// thread writes its chunk
 __global__ void __launch_bounds__(1024) scan_kernel(int n, const int* __restrict__ in,
                                                     int* __restrict__ out) {
   __shared__ int part[1024];
-  __shared__ int carry;
-  if (threadIdx.x == 0) carry = 0;
+  const int per = (n + 1023) / 1024;
+  const int lo = threadIdx.x * per, hi = min(n, lo + per);
+  int sum = 0;
+  for (int a = lo; a < hi; ++a) sum += in[a];
+  part[threadIdx.x] = sum;
   __syncthreads();
-  for (int base = 0; base < n; base += 1024) {
-    int idx = base + threadIdx.x;
-    int v = idx < n ? in[idx] : 0;
-    part[threadIdx.x] = v;
+  for (int o = 1; o < 1024; o <<= 1) {
+    int v = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
     __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-      int t = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
-      __syncthreads();
-      part[threadIdx.x] += t;
-      __syncthreads();
-    }
-    if (idx < n) out[idx] = carry + part[threadIdx.x] - v;
-    __syncthreads();
-    if (threadIdx.x == 1023) carry += part[1023];
+    part[threadIdx.x] += v;
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[n] = carry;
+  int run = part[threadIdx.x] - sum;   // exclusive prefix of this chunk
+  for (int a = lo; a < hi; ++a) {
+    out[a] = run;
+    run += in[a];
+  }
+  if (threadIdx.x == 1023) out[n] = part[1023];
 }
 
 __global__ void cell_fill_kernel(int N, const int* __restrict__ cell_of,

@@ -26,7 +26,7 @@
 namespace {
 
 constexpr int SYM_WAVES = 4;
-constexpr int MAX_ROW = 160;      // radial row entries staged per wave
+constexpr int MAX_ROW = 128;      // radial row entries staged per wave (= max row stride)
 constexpr int MAX_ANG = 48;       // angular sub-list entries per centre
 constexpr int NR_MAX = 64;        // radial shells
 constexpr int NA = 8;             // angular shells (fixed by the lane mapping)
@@ -46,13 +46,12 @@ struct SymArgs {
 constexpr float PI_LIT = 3.14159265359f;
 
 __device__ __forceinline__ float powz(float b, const SymArgs& a) {
-  if (a.zeta_int > 0) {
+  if (a.zeta_int > 0) {   // square-and-multiply over the 7 bits of zeta <= 64
     float r = 1.f, x = b;
-    int e = a.zeta_int;
-    while (e) {
-      if (e & 1) r *= x;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      r = ((a.zeta_int >> k) & 1) ? r * x : r;
       x *= x;
-      e >>= 1;
     }
     return r;
   }
@@ -229,7 +228,8 @@ sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__
                SymArgs a, const float* __restrict__ gE, const float* __restrict__ gQ,
                const float* __restrict__ u, int ldg, float4* __restrict__ wslot) {
   __shared__ RowLds L;
-  __shared__ float g[SYM_WAVES][1024];
+  extern __shared__ float gdyn[];   // SYM_WAVES * D floats: dE/dG of each wave's centre
+  float* gw = gdyn + (threadIdx.x >> 6) * a.D;
   __shared__ float4 wacc[SYM_WAVES][MAX_ROW];
   __shared__ float4 ca[SYM_WAVES][64], cb[SYM_WAVES][64];
   __shared__ int tja[SYM_WAVES][64], tkb[SYM_WAVES][64];
@@ -246,7 +246,7 @@ sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__
   for (int f = lane; f < a.D; f += 64) {
     float v = gE[(long)i * ldg + f];
     if (gQ != nullptr) v += ui * gQ[(long)i * ldg + f];
-    g[w][f] = v;
+    gw[f] = v;
   }
   load_row(i, w, lane, pos, box, mol, elem, nbr, n, ldr, a, L);
   __syncwarp();
@@ -262,7 +262,7 @@ sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__
     for (int s = 0; s < a.nr; ++s) {
       float x = r - a.rs_r[s];
       float ex = __expf(-a.eta * x * x);
-      dEdr += g[w][e * a.nr + s] * ex * (dfc - 2.0f * a.eta * x * fc);
+      dEdr += gw[e * a.nr + s] * ex * (dfc - 2.0f * a.eta * x * fc);
     }
     float sc = dEdr / r;
     wacc[w][q] = make_float4(sc * d.x, sc * d.y, sc * d.z, 0.f);
@@ -300,7 +300,7 @@ sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__
         R[s] = __expf(-a.eta * x * x);
         Rp[s] = -2.0f * a.eta * x * R[s];
       }
-      const float* gp = &g[w][off + p * 64];
+      const float* gp = gw + off + p * 64;
       float dEdc = 0.f, dEdrav = 0.f, dEdF = 0.f;
 #pragma unroll
       for (int tt = 0; tt < NT; ++tt) {
@@ -355,23 +355,34 @@ sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__
 }
 
 // grad_a = sum_n (w[rev(a,n)] - w[(a,n)]) + extra_a ; optional force output
-// force = -scale * grad (J/mol/A when scale = JOULEPERHARTREE)
+// force = -scale * grad (J/mol/A when scale = JOULEPERHARTREE).  16 lanes per
+// atom, each summing every 16th slot, then a fixed butterfly over the 16.
 __global__ void force_gather_kernel(int N, int ldr, const int* __restrict__ ncnt,
                                     const int* __restrict__ rev, const float4* __restrict__ wslot,
                                     const double* __restrict__ extra, double scale,
                                     double* __restrict__ grad, double* __restrict__ force) {
-  int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= N) return;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int a = (int)(t >> 4);
+  const int sub = threadIdx.x & 15;
   double gx = 0, gy = 0, gz = 0;
-  int n = ncnt[a];
-  for (int q = 0; q < n; ++q) {
-    long sl = (long)a * ldr + q;
-    float4 wo = wslot[sl];
-    float4 wi = wslot[rev[sl]];
-    gx += (double)wi.x - (double)wo.x;
-    gy += (double)wi.y - (double)wo.y;
-    gz += (double)wi.z - (double)wo.z;
+  if (a < N) {
+    const int n = ncnt[a];
+    for (int q = sub; q < n; q += 16) {
+      long sl = (long)a * ldr + q;
+      float4 wo = wslot[sl];
+      float4 wi = wslot[rev[sl]];
+      gx += (double)wi.x - (double)wo.x;
+      gy += (double)wi.y - (double)wo.y;
+      gz += (double)wi.z - (double)wo.z;
+    }
   }
+#pragma unroll
+  for (int o = 8; o > 0; o >>= 1) {
+    gx += __shfl_xor(gx, o, 16);
+    gy += __shfl_xor(gy, o, 16);
+    gz += __shfl_xor(gz, o, 16);
+  }
+  if (a >= N || sub != 0) return;
   if (extra != nullptr) {
     gx += extra[3 * a]; gy += extra[3 * a + 1]; gz += extra[3 * a + 2];
   }
@@ -419,7 +430,7 @@ int tm_sym_forward(int N, const double* pos, const double* box, const int* mol, 
   int st = fill_args(a, nele, nr, eta, zeta, rr, ra, rs_r, rs_a, thetas);
   if (st) return st;
   TM_REQUIRE(ldg >= a.D, "tm_sym_forward: ldg %d < D %d", ldg, a.D);
-  TM_REQUIRE(ldr <= MAX_ROW + 96 && lda <= MAX_ANG, "tm_sym_forward: strides %d/%d above %d/%d",
+  TM_REQUIRE(ldr <= MAX_ROW && lda <= MAX_ANG, "tm_sym_forward: strides %d/%d above %d/%d",
              ldr, lda, MAX_ROW, MAX_ANG);
   if (N == 0) return TM_OK;
   hipStream_t s = (hipStream_t)stream;
@@ -439,10 +450,11 @@ int tm_sym_adjoint(int N, const double* pos, const double* box, const int* mol, 
   int st = fill_args(a, nele, nr, eta, zeta, rr, ra, rs_r, rs_a, thetas);
   if (st) return st;
   TM_REQUIRE(ldg >= a.D, "tm_sym_adjoint: ldg %d < D %d", ldg, a.D);
-  TM_REQUIRE(ldr <= MAX_ROW + 96 && lda <= MAX_ANG, "tm_sym_adjoint: strides above limits");
+  TM_REQUIRE(ldr <= MAX_ROW && lda <= MAX_ANG, "tm_sym_adjoint: strides %d/%d above %d/%d", ldr,
+             lda, MAX_ROW, MAX_ANG);
   if (N == 0) return TM_OK;
   hipStream_t s = (hipStream_t)stream;
-  sym_adj_kernel<<<tm_div_up(N, SYM_WAVES), SYM_WAVES * 64, 0, s>>>(
+  sym_adj_kernel<<<tm_div_up(N, SYM_WAVES), SYM_WAVES * 64, sizeof(float) * SYM_WAVES * a.D, s>>>(
       N, pos, box, mol, elem, nbr, ncnt, rseg, ang, acnt, aseg, ldr, lda, a, gE, gQ, u, ldg,
       (float4*)wslot);
   TM_LAUNCH_CHECK();
@@ -454,8 +466,9 @@ int tm_force_gather(int N, int ldr, const int* ncnt, const int* rev, const float
                     void* stream) {
   if (N == 0) return TM_OK;
   hipStream_t s = (hipStream_t)stream;
-  force_gather_kernel<<<tm_div_up(N, 256), 256, 0, s>>>(N, ldr, ncnt, rev, (const float4*)wslot,
-                                                        extra, scale, grad, force);
+  force_gather_kernel<<<tm_div_up(16l * N, 256), 256, 0, s>>>(N, ldr, ncnt, rev,
+                                                              (const float4*)wslot, extra, scale,
+                                                              grad, force);
   TM_LAUNCH_CHECK();
   return TM_OK;
 }
