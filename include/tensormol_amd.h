@@ -176,7 +176,9 @@ int tm_mol_sum_f64(int B, const int* mol_start, int C, const double* in, int ldi
  * VelocityVerletStep (SimpleMD.py:14-38) split into drift and kick;
  * NoseThermostat.step (SimpleMD.py:104-129) with one eta per molecule;
  * FIRE (Bitzek et al. 2006; no reference counterpart, the optimiser of the
- * serving workload). */
+ * serving workload): tm_fire_update works on atoms [0, N) of molecules
+ * mol0 .. mol0+B-1; state[4b] = (dt, alpha, steps with P > 0, active),
+ * mix[3b] scratch; dots_sum[3b] = per-molecule (F.v, F.F, v.v). */
 int tm_vv_drift(int N, double dt, const double* v, const double* a, double* x, void* stream);
 int tm_vv_kick(int N, double dt, const double* f, const double* m, double* v, double* a,
                double* ke, void* stream);
@@ -187,7 +189,7 @@ int tm_nh_eta(int B, const int* mol_start, double dt, const double* kT, const do
 int tm_nh_kick(int N, double dt, const int* mol, const double* eta, const double* f,
                const double* m, const double* vh, double* v, double* a, double* ke, void* stream);
 int tm_fire_dots(int N, const double* f, const double* v, double* dots, void* stream);
-int tm_fire_update(int N, int B, const int* mol, const double* dots_sum, double* state,
+int tm_fire_update(int N, int B, const int* mol, int mol0, const double* dots_sum, double* state,
                    double* mix, const double* f, const double* m, double* v, double* x,
                    double dtmax, int nmin, double finc, double fdec, double alpha0,
                    double falpha, double maxstep, void* stream);

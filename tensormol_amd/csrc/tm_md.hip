@@ -138,13 +138,13 @@ __global__ void fire_state_kernel(int B, const double* __restrict__ dots_sum,
 
 // v <- (1-alpha) v + alpha |v| F/|F| (or 0), then semi-implicit Euler:
 // v += dt F / m_eff ; x += dt v  (unit mass scaled to A and fs as in MD)
-__global__ void fire_move_kernel(int N, const int* __restrict__ mol, const double* __restrict__ mix,
-                                 const double* __restrict__ state, const double* __restrict__ f,
-                                 const double* __restrict__ m, double* __restrict__ v,
-                                 double* __restrict__ x, double maxstep) {
+__global__ void fire_move_kernel(int N, const int* __restrict__ mol, int mol0,
+                                 const double* __restrict__ mix, const double* __restrict__ state,
+                                 const double* __restrict__ f, const double* __restrict__ m,
+                                 double* __restrict__ v, double* __restrict__ x, double maxstep) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  int b = mol[i];
+  int b = mol[i] - mol0;
   double c_v = mix[3 * b], c_f = mix[3 * b + 1], zero_v = mix[3 * b + 2];
   double dt = state[4 * b];
   double dx[3], n2 = 0.0;
@@ -214,7 +214,9 @@ int tm_fire_dots(int N, const double* f, const double* v, double* dots, void* st
   return TM_OK;
 }
 
-int tm_fire_update(int N, int B, const int* mol, const double* dots_sum, double* state,
+// Atoms [0, N) of the pointers belong to molecules mol0 .. mol0+B-1 (mol[] holds
+// global molecule ids); dots_sum, state and mix are indexed by mol - mol0.
+int tm_fire_update(int N, int B, const int* mol, int mol0, const double* dots_sum, double* state,
                    double* mix, const double* f, const double* m, double* v, double* x,
                    double dtmax, int nmin, double finc, double fdec, double alpha0,
                    double falpha, double maxstep, void* stream) {
@@ -223,7 +225,7 @@ int tm_fire_update(int N, int B, const int* mol, const double* dots_sum, double*
   fire_state_kernel<<<tm_div_up(B, 128), 128, 0, s>>>(B, dots_sum, state, mix, dtmax, nmin, finc,
                                                       fdec, alpha0, falpha);
   TM_LAUNCH_CHECK();
-  fire_move_kernel<<<tm_div_up(N, 256), 256, 0, s>>>(N, mol, mix, state, f, m, v, x, maxstep);
+  fire_move_kernel<<<tm_div_up(N, 256), 256, 0, s>>>(N, mol, mol0, mix, state, f, m, v, x, maxstep);
   TM_LAUNCH_CHECK();
   return TM_OK;
 }

@@ -125,44 +125,24 @@ class BPEEEngine:
     def prepare(self, Z, mol_start, boxes=None, margin=4.0, pos0=None):
         """Allocate device buffers for a packed batch (Z (N,), mol_start (B+1,),
         optional boxes (B,3) orthorhombic edges in A; 0 or None = open)."""
+        ms = np.asarray(mol_start, dtype=np.int64)
+        ws = self.alloc(len(Z), len(ms) - 1)
+        self.set_topology(ws, Z, ms, boxes, margin=margin, pos0=pos0)
+        return ws
+
+    def alloc(self, ncap, bcap):
+        """Device buffers for up to ncap atoms in up to bcap molecules."""
         dev = self.device
         ws = Workspace()
-        Z = np.asarray(Z)
-        ms = np.asarray(mol_start, dtype=np.int64)
-        N, B = len(Z), len(ms) - 1
-        ws.N, ws.B = N, B
-        elem = self.model.element_index(Z)
-        ws.elem_np = elem
-        ws.mol_np = np.repeat(np.arange(B, dtype=np.int32), np.diff(ms))
-        ws.ms_np = ms
-        ws.elem = torch.tensor(elem, dtype=torch.int32, device=dev)
-        ws.mol = torch.tensor(ws.mol_np, dtype=torch.int32, device=dev)
-        ws.mol_start = torch.tensor(ms, dtype=torch.int32, device=dev)
-        ws.natom = torch.tensor(np.diff(ms), dtype=torch.float64, device=dev)
-        ws.box = None
-        ws.periodic = 0
-        if boxes is not None:
-            bx = np.asarray(boxes, dtype=np.float64).reshape(B, 3)
-            if np.any(bx > 0):
-                if np.any(bx <= 0):
-                    raise ValueError("mixed open and periodic molecules in one batch")
-                if np.any(bx < 2.0 * max(self.model.sym.rr, self.model.ee_off)):
-                    raise ValueError("box edge below twice the largest cutoff: minimum image is ambiguous")
-                ws.box = torch.tensor(bx, device=dev)
-                ws.periodic = 1
-        ws.boxes_np = None if ws.box is None else np.asarray(boxes, dtype=np.float64).reshape(B, 3)
-        # element-sorted permutation (rows of each per-element GEMM are contiguous)
-        perm = np.argsort(elem, kind="stable").astype(np.int32)
-        counts = np.bincount(elem, minlength=self.nele)
-        ws.eoff = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
-        ws.perm = torch.tensor(perm, device=dev)
-        # cell grid for one large system
-        ws.use_cells = (B == 1 and N > 256)
-        if ws.use_cells:
-            self._setup_grid(ws, pos0, margin)
+        ws.ncap, ws.bcap = int(ncap), int(bcap)
+        N, B = max(1, ws.ncap), max(1, ws.bcap)
         i32 = dict(dtype=torch.int32, device=dev)
         f32 = dict(dtype=torch.float32, device=dev)
         f64 = dict(dtype=torch.float64, device=dev)
+        ws.elem = torch.zeros(N, **i32)
+        ws.mol = torch.zeros(N, **i32)
+        ws.mol_start = torch.zeros(B + 1, **i32)
+        ws.perm = torch.zeros(N, **i32)
         ws.nbr = torch.zeros(N * self.ldr, **i32)
         ws.ncnt = torch.zeros(N, **i32)
         ws.rseg = torch.zeros(N * (self.nele + 1), **i32)
@@ -173,16 +153,13 @@ class BPEEEngine:
         ws.flags = torch.zeros(8, **i32)
         ws.red = torch.zeros(((N + 1023) // 1024) * 8 + 8, **f64)
         ws.G = torch.zeros(N, self.D, **f32)
-        # outputs of the two networks: y[net, atom], dy/dG[net, atom, :]
-        ws.Y2 = torch.zeros(2, N, **f32)
-        ws.J2 = torch.zeros(2, N, self.D, **f32)
-        ws.eps, ws.qraw = ws.Y2[0], ws.Y2[1]
-        ws.JE, ws.JQ = ws.J2[0], ws.J2[1]
+        ws.Y2 = torch.zeros(2 * N, **f32)
+        ws.J2 = torch.zeros(2 * N * self.D, **f32)
         ws.qmean = torch.zeros(B, **f64)
         ws.phimean = torch.zeros(B, **f64)
         ws.xq = torch.zeros(N, 4, **f32)
-        ws.xqc = torch.zeros(N, 4, **f32) if ws.use_cells else None
-        ws.elc = torch.zeros(N, **i32) if ws.use_cells else None
+        ws.xqc = None
+        ws.elc = None
         ws.q = torch.zeros(N, **f64)
         ws.qx = torch.zeros(N, 3, **f64)
         ws.phi = torch.zeros(N, **f64)
@@ -195,11 +172,64 @@ class BPEEEngine:
         ws.Ebp = torch.zeros(B, **f64)
         ws.Eext = torch.zeros(B, 2, **f64)
         ws.dipole = torch.zeros(B, 3, **f64)
+        ws.mlp_ws = None
+        ws.use_cells = False
+        return ws
+
+    def set_topology(self, ws, Z, mol_start, boxes=None, margin=4.0, pos0=None):
+        """Point a workspace at a new packed batch (N <= ncap, B <= bcap).
+        Only small host-built index arrays are copied; no buffer is reallocated
+        unless a cell grid is (re)built for a single large system."""
+        dev = self.device
+        Z = np.asarray(Z)
+        ms = np.asarray(mol_start, dtype=np.int64)
+        N, B = len(Z), len(ms) - 1
+        if N > ws.ncap or B > ws.bcap:
+            raise ValueError(f"batch of {N} atoms / {B} molecules exceeds the workspace "
+                             f"({ws.ncap} / {ws.bcap})")
+        ws.N, ws.B = N, B
+        elem = self.model.element_index(Z)
+        ws.elem_np = elem
+        ws.mol_np = np.repeat(np.arange(B, dtype=np.int32), np.diff(ms))
+        ws.ms_np = ms
+        perm = np.argsort(elem, kind="stable").astype(np.int32)
+        counts = np.bincount(elem, minlength=self.nele)
+        ws.eoff = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+        if N:
+            ws.elem[:N].copy_(torch.from_numpy(elem))
+            ws.mol[:N].copy_(torch.from_numpy(ws.mol_np))
+            ws.perm[:N].copy_(torch.from_numpy(perm))
+        ws.mol_start[:B + 1].copy_(torch.from_numpy(ms.astype(np.int32)))
+        ws.box = None
+        ws.periodic = 0
+        ws.boxes_np = None
+        if boxes is not None:
+            bx = np.asarray(boxes, dtype=np.float64).reshape(B, 3)
+            if np.any(bx > 0):
+                if np.any(bx <= 0):
+                    raise ValueError("mixed open and periodic molecules in one batch")
+                if np.any(bx < 2.0 * max(self.model.sym.rr, self.model.ee_off)):
+                    raise ValueError("box edge below twice the largest cutoff: minimum image is ambiguous")
+                ws.box = torch.tensor(bx, device=dev)
+                ws.periodic = 1
+                ws.boxes_np = bx
+        # network outputs laid out for this N: y[net*N + atom], dG[(net*N + atom)*D + f]
+        ws.eps, ws.qraw = ws.Y2[:N], ws.Y2[N:2 * N]
+        ws.JE = ws.J2[:N * self.D].view(N, self.D)
+        ws.JQ = ws.J2[N * self.D:2 * N * self.D].view(N, self.D)
+        # cell grid for one large system
+        ws.use_cells = (B == 1 and N > 256)
+        if ws.use_cells:
+            self._setup_grid(ws, pos0, margin)
+            ws.xqc = torch.zeros(max(1, N), 4, dtype=torch.float32, device=dev)
+            ws.elc = torch.zeros(max(1, N), dtype=torch.int32, device=dev)
         if self.precision == "fp32":
             hmax = int(self.hid.max())
             maxrows = int(counts.max()) if N else 0
             nh = len(self.hid)
-            ws.mlp_ws = torch.zeros(max(1, (2 * nh + 2) * maxrows * hmax), **f32)
+            need = max(1, (2 * nh + 2) * maxrows * hmax)
+            if ws.mlp_ws is None or ws.mlp_ws.numel() < need:
+                ws.mlp_ws = torch.zeros(need, dtype=torch.float32, device=dev)
         else:
             ws.mlp_ws = self.mfma.workspace(ws)
         return ws
@@ -329,7 +359,7 @@ class BPEEEngine:
 
     def run(self, ws, pos, forces=True, check=None):
         """Enqueue one full evaluation for positions pos (cuda float64 [N,3])."""
-        assert pos.dtype == torch.float64 and pos.is_contiguous() and pos.shape == (ws.N, 3)
+        assert pos.dtype == torch.float64 and pos.is_contiguous() and pos.shape[0] >= ws.N
         self.build_lists(ws, pos)
         if (self.check_overflow if check is None else check):
             self.check_flags(ws)
@@ -342,17 +372,17 @@ class BPEEEngine:
 
     def results(self, ws):
         """Host copies of the outputs of the last run (float64 numpy)."""
-        Ebp = ws.Ebp.cpu().numpy()
-        Eext = ws.Eext.cpu().numpy()
+        Ebp = ws.Ebp[:ws.B].cpu().numpy()
+        Eext = ws.Eext[:ws.B].cpu().numpy()
         return dict(Etotal=Ebp + Eext[:, 0] + Eext[:, 1], Ebp=Ebp, Ecc=Eext[:, 0],
                     Evdw=Eext[:, 1], Ebp_atom=ws.eps.double().cpu().numpy(),
-                    charge=ws.q.cpu().numpy(), dipole=ws.dipole.cpu().numpy(),
-                    gradient=ws.grad.cpu().numpy(), force=ws.force.cpu().numpy(),
-                    G=ws.G.cpu().numpy())
+                    charge=ws.q[:ws.N].cpu().numpy(), dipole=ws.dipole[:ws.B].cpu().numpy(),
+                    gradient=ws.grad[:ws.N].cpu().numpy(), force=ws.force[:ws.N].cpu().numpy(),
+                    G=ws.G[:ws.N].cpu().numpy())
 
     def energy_tensor(self, ws):
         """Per-molecule total energy as a device tensor (no host copy)."""
-        return ws.Ebp + ws.Eext[:, 0] + ws.Eext[:, 1]
+        return ws.Ebp[:ws.B] + ws.Eext[:ws.B, 0] + ws.Eext[:ws.B, 1]
 
 
 def ctypes_stream(s):

@@ -183,7 +183,7 @@ class DeviceFIRE:
         L.tm_fire_dots(ws.N, _P(ws.force), _P(self.v), _P(self.dots), st)
         L.tm_mol_sum_f64(ws.B, _P(ws.mol_start), 3, _P(self.dots), 3, _P(self.dots_sum), 0,
                          _P(ws.red), ws.N, st)
-        L.tm_fire_update(ws.N, ws.B, _P(ws.mol), _P(self.dots_sum), _P(self.state), _P(self.mix),
+        L.tm_fire_update(ws.N, ws.B, _P(ws.mol), 0, _P(self.dots_sum), _P(self.state), _P(self.mix),
                          _P(ws.force), _P(self.m), _P(self.v), _P(self.pos), 10.0 * self.dt0, 5, 1.1,
                          0.5, 0.1, 0.99, self.maxstep, st)
 

@@ -107,74 +107,89 @@ def _unit(rng):
     return v / np.linalg.norm(v)
 
 
-def _bond_dir(rng, prev, prevprev):
-    """Unit vector for the next bond: bond angle U(105, 125) degrees to the
-    previous bond and a uniform dihedral (uniform direction for the first bond)."""
+def _cross(a, b):
+    return (a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0])
+
+
+def _unitv(a):
+    n = math.sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2])
+    return (a[0] / n, a[1] / n, a[2] / n), n
+
+
+def _bond_dirs(rng, prev, prevprev, k):
+    """k unit vectors for the next bond: bond angle U(105, 125) degrees to the
+    previous bond and a uniform dihedral (uniform directions for the first bond)."""
     if prev is None:
-        return _unit(rng)
-    b1 = prev / np.linalg.norm(prev)
-    ref = prevprev if prevprev is not None else _unit(rng)
-    n = np.cross(b1, ref)
-    if np.linalg.norm(n) < 1e-8:
-        n = np.cross(b1, _unit(rng))
-    n /= np.linalg.norm(n)
-    m = np.cross(n, b1)
-    th = math.radians(rng.uniform(105.0, 125.0))
-    ph = rng.uniform(0.0, 2.0 * math.pi)
-    # angle between the new bond and -b1 is th
-    return -math.cos(th) * b1 + math.sin(th) * (math.cos(ph) * n + math.sin(ph) * m)
+        v = rng.normal(size=(k, 3))
+        return v / np.linalg.norm(v, axis=1, keepdims=True)
+    b1, _ = _unitv(tuple(prev))
+    ref = tuple(prevprev) if prevprev is not None else tuple(rng.normal(size=3))
+    n, nn = _unitv(_cross(b1, ref)) if np.linalg.norm(_cross(b1, ref)) > 1e-8 else \
+        _unitv(_cross(b1, tuple(rng.normal(size=3))))
+    m = _cross(n, b1)
+    th = np.radians(rng.uniform(105.0, 125.0, k))[:, None]
+    ph = rng.uniform(0.0, 2.0 * math.pi, k)[:, None]
+    B1, Nn, M = np.array(b1), np.array(n), np.array(m)
+    return -np.cos(th) * B1 + np.sin(th) * (np.cos(ph) * Nn + np.sin(ph) * M)
 
 
-def chain_molecule(natom, rng, heavy=((6, 0.7), (7, 0.15), (8, 0.15)), min_nb=1.8, min_13=1.5):
+def chain_molecule(natom, rng, heavy=((6, 0.7), (7, 0.15), (8, 0.15)), min_nb=1.8, min_13=1.5,
+                   tries=4):
     """Heavy-atom chain (bonds U(1.1, 1.5) A, bond angles U(105, 125) deg,
     random dihedrals) with pendant H at 1.09 A.  Non-bonded pairs (neither 1-2
-    nor 1-3 neighbours) stay at least min_nb apart, 1-3 pairs at least min_13.  Half the atoms (rounded up)
-    are heavy; H are spread over the heavy atoms round robin."""
+    nor 1-3 neighbours) stay at least min_nb apart, 1-3 pairs at least min_13.
+    Half the atoms (rounded up) are heavy; H are spread over the heavy atoms
+    round robin.  Each placement tests `tries` candidates at once and keeps the
+    first acceptable one."""
     nheavy = max(1, (natom + 1) // 2)
     nh = natom - nheavy
-    cap = natom
     for _attempt in range(200):
-        pos = np.zeros((cap, 3))
+        pos = np.zeros((natom, 3))
         n = 1
         nbrs = [set()]
 
-        def place(c, host):
+        def place(cands, host):
             nonlocal n
-            d = pos[:n] - c
-            d2 = np.einsum("ij,ij->i", d, d)
-            bad = np.nonzero(d2 < min_nb * min_nb)[0]
-            for q in bad:
-                if q == host:
-                    continue
-                if q in nbrs[host] and d2[q] >= min_13 * min_13:
-                    continue
+            d = cands[:, None, :] - pos[None, :n, :]
+            d2 = (d * d).sum(-1)
+            lim = np.full(n, min_nb * min_nb)
+            lim[host] = 0.0
+            lim[list(nbrs[host])] = min_13 * min_13
+            ok = (d2 >= lim).all(axis=1)
+            good = np.flatnonzero(ok)
+            if len(good) == 0:
                 return False
-            pos[n] = c
+            pos[n] = cands[good[0]]
             nbrs.append({host})
             nbrs[host].add(n)
             n += 1
             return True
 
         failed = False
-        i = 1
-        while i < nheavy:
+        for i in range(1, nheavy):
             prev = pos[i - 1] - pos[i - 2] if i >= 2 else None
             pp = pos[i - 2] - pos[i - 3] if i >= 3 else None
-            for _ in range(100):
-                if place(pos[i - 1] + rng.uniform(1.1, 1.5) * _bond_dir(rng, prev, pp), i - 1):
+            placed = False
+            for _ in range(32):
+                c = pos[i - 1] + rng.uniform(1.1, 1.5, (tries, 1)) * _bond_dirs(rng, prev, pp, tries)
+                if place(c, i - 1):
+                    placed = True
                     break
-            else:
+            if not placed:
                 failed = True
                 break
-            i += 1
         if failed:
             continue
         for h in range(nh):
             host = h % nheavy
-            for _ in range(400):
-                if place(pos[host] + 1.09 * _unit(rng), host):
+            placed = False
+            for _ in range(64):
+                v = rng.normal(size=(tries, 3))
+                c = pos[host] + 1.09 * v / np.linalg.norm(v, axis=1, keepdims=True)
+                if place(c, host):
+                    placed = True
                     break
-            else:
+            if not placed:
                 failed = True
                 break
         if failed:
