@@ -14,6 +14,15 @@ value = atoms * timed steps / wall time of the timed steps (inputs resident in
 HBM).  The JSON line also carries the force RMS error of frame 0 against the
 FP64 CPU oracle, the plain-FP32 path's rate on the same box, the roofline of
 the MLP kernel and the CPU baseline (the oracle, timed on this host).
+
+Serving leg ("serving", and the top-level p50_ms / p99_ms request latencies):
+a seeded stream of 4096 C/H/N/O chain molecules (8-256 atoms, log-uniform;
+50% single points, 30% 50-step Nose-Hoover NVT, 20% 100-step FIRE) arriving
+as a Poisson process at 2000 requests/s, served by the continuous-batching
+server with the CHNO model (768-512-512-512-1 nets).  Reported: latency
+percentiles (arrival to result), atom-steps/s of the realtime stream and of
+the saturated server (all requests queued at t = 0), and the plain FP32 path
+serving the first requests of the stream one molecule per launch.
 """
 import argparse
 import ctypes
@@ -144,6 +153,79 @@ def cpu_baseline(Z, x, box, model):
     return r, dt
 
 
+def run_serving(args, stream):
+    """Serve the request stream: realtime arrivals (latency), all requests at
+    t = 0 (saturated throughput), and the plain-FP32 path one molecule per
+    launch on the first requests of the stream (baseline)."""
+    import torch
+    from oracle import oracle as O
+    from tensormol_amd.engine import BPEEEngine
+    from tensormol_amd.model import BPEEModel
+    from tensormol_amd.serving import Request, Server, atom_steps, latency_stats
+
+    def mk(k=None):
+        sel = stream if k is None else stream[:k]
+        return [Request(r, Z, x, kind, st, t, seed=sd) for r, (Z, x, kind, st, t, sd) in enumerate(sel)]
+
+    log = lambda m: print(f"[serve] {m}", file=sys.stderr, flush=True)
+    model = BPEEModel([1, 6, 7, 8], dsf_alpha=0.2, seed=0)
+    eng = BPEEEngine(model, precision=args.precision, check_overflow=False)
+    srv = Server(eng, max_atoms=args.serve_atoms, max_mols=8192,
+                 hbm_budget_bytes=args.serve_hbm_gib * 2 ** 30)
+    srv.run(mk(64), realtime=False)   # warm-up
+    torch.cuda.synchronize()
+    done = srv.run(mk(), realtime=True)
+    lat = latency_stats(done)
+    rt_rate = atom_steps(done) / srv.wall
+    rt_iters, rt_wall = srv.iters, srv.wall
+    log(f"realtime: {len(done)} requests in {rt_wall:.2f} s, {rt_iters} iterations, "
+        f"p50 {lat['p50_ms']:.2f} ms p99 {lat['p99_ms']:.2f} ms, {rt_rate:.4g} atom-steps/s")
+    done_sat = srv.run(mk(), realtime=False)
+    sat_rate = atom_steps(done_sat) / srv.wall
+    log(f"saturated: {srv.wall:.2f} s, {srv.iters} iterations, {sat_rate:.4g} atom-steps/s")
+    # force error of the single points against the FP64 oracle
+    sp = sorted((r for r in done if r.kind == "sp"), key=lambda r: r.rid)[:args.serve_check]
+    errs = []
+    for r in sp:
+        ref = O.evaluate(r.xyz, r.Z, np.array([0, len(r.Z)]), O.SymParams(model.eles),
+                         model.energy_nets, model.charge_nets, alpha=model.alpha,
+                         dsf_alpha=model.dsf_alpha, has_vdw=model.has_vdw)
+        errs.append(float(np.sqrt(((r.forces / 2625499.638 + ref["gradient"]) ** 2).mean())))
+    # plain FP32 path, one molecule per launch
+    eng32 = BPEEEngine(model, precision="fp32", check_overflow=False)
+    one = Server(eng32, max_atoms=max(len(z) for z, *_ in stream), max_mols=1)
+    one.run(mk(4), realtime=False)
+    torch.cuda.synchronize()
+    k = args.serve_baseline_requests
+    d32 = one.run(mk(k), realtime=False)
+    base_rate = atom_steps(d32) / one.wall
+    log(f"launch-per-molecule fp32: {k} requests, {atom_steps(d32)} atom-steps in {one.wall:.2f} s, "
+        f"{base_rate:.4g} atom-steps/s")
+    natoms = np.array([len(z) for z, *_ in stream])
+    failed = {}
+    for r in done:
+        if r.error is not None:
+            failed[r.kind] = failed.get(r.kind, 0) + 1
+    return {
+        "workload": f"{len(stream)} C/H/N/O chain requests, {natoms.min()}-{natoms.max()} atoms "
+                    f"(mean {natoms.mean():.1f}), 50% single point / 30% 50-step NVT / 20% "
+                    f"100-step FIRE, Poisson arrivals at {args.serve_rate:g}/s",
+        "p50_ms": round(lat["p50_ms"], 3), "p99_ms": round(lat["p99_ms"], 3),
+        "mean_ms": round(lat["mean_ms"], 3),
+        "stream_atom_steps_per_s": round(rt_rate, 1), "stream_wall_s": round(rt_wall, 3),
+        "stream_iterations": rt_iters,
+        "saturated_atom_steps_per_s": round(sat_rate, 1),
+        "launch_per_molecule_fp32_atom_steps_per_s": round(base_rate, 1),
+        "baseline_sample": f"first {k} requests of the stream, served one molecule per launch "
+                           "by the plain FP32 path",
+        "speedup_vs_launch_per_molecule_fp32": round(sat_rate / base_rate, 2),
+        "capacity_atoms": srv.max_atoms,
+        "failed_requests_by_kind": failed,
+        "force_rms_err_vs_fp64_max": max(errs) if errs else None,
+        "force_check_sample": f"first {len(errs)} single-point requests",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,13 +238,31 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=4096, help="0 disables the serving leg")
+    ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--serve-atoms", type=int, default=65536)
+    ap.add_argument("--serve-hbm-gib", type=float, default=16.0)
+    ap.add_argument("--serve-baseline-requests", type=int, default=128)
+    ap.add_argument("--serve-check", type=int, default=8)
+    ap.add_argument("--serve-only", action="store_true", help="print the serving record only")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus}); "
               "the single-GPU value is the figure of merit", file=sys.stderr)
         sys.exit(2)
+    stream = None
+    if args.serve_requests > 0:
+        # generated by spawned worker processes before this process touches the GPU
+        from tensormol_amd.workloads import request_stream
+        t0 = time.perf_counter()
+        stream = request_stream(args.serve_requests, rate=args.serve_rate)
+        print(f"[serve] stream of {len(stream)} requests generated in {time.perf_counter() - t0:.1f} s",
+              file=sys.stderr, flush=True)
     import torch
     from tensormol_amd.engine import BPEEEngine
+    if args.serve_only:
+        print(json.dumps(run_serving(args, stream)), flush=True)
+        return
 
     Z, x, box, model = build_system(args)
     N = len(Z)
@@ -203,6 +303,8 @@ def main():
         fp32_value = N * 10 / r32["wall"]
         print(f"[bench] plain fp32 path {fp32_value:.4g} atom-steps/s", file=sys.stderr, flush=True)
 
+    serving = run_serving(args, stream) if stream is not None else None
+
     mlp_avg_ms = float(res["mlp_ms"].mean())
     flops = mlp_flops_per_atom(eng.D, int(model.hidden[0])) * N
     achieved = flops / (mlp_avg_ms * 1e-3) / 1e12
@@ -223,8 +325,10 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms_per_step, 4),
-        "p50_ms": round(float(np.percentile(res["per_step_ms"], 50)), 4),
-        "p99_ms": round(float(np.percentile(res["per_step_ms"], 99)), 4),
+        "p50_ms": None if serving is None else serving["p50_ms"],
+        "p99_ms": None if serving is None else serving["p99_ms"],
+        "md_step_p50_ms": round(float(np.percentile(res["per_step_ms"], 50)), 4),
+        "md_step_p99_ms": round(float(np.percentile(res["per_step_ms"], 99)), 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -239,6 +343,7 @@ def main():
         "energy_err_vs_fp64_Ha": energy_err,
         "plain_fp32_value": None if fp32_value is None else round(fp32_value, 1),
         "speedup_vs_plain_fp32": None if fp32_value is None else round(value / fp32_value, 2),
+        "serving": serving,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -153,13 +153,14 @@ class DeviceFIRE:
     """Batched FIRE on the device over an engine workspace.
 
     Standard parameters (Bitzek et al. 2006): N_min 5, f_inc 1.1, f_dec 0.5,
-    alpha_start 0.1, f_alpha 0.99; dt_max = 10 dt0.  Moves are capped at
+    alpha_start 0.1, f_alpha 0.99; dt0 = 1 fs and dt_max = 10 dt0 (about
+    ASE's defaults of 0.1 and 1.0 in its time unit of 10.18 fs).  Moves are capped at
     `maxstep` A per atom per iteration.  A molecule is converged when its
     maximum atomic force falls below fmax (Ha/A); converged molecules stop
     moving (their state is frozen on the device).
     """
 
-    def __init__(self, engine, ws, Z, x0, dt0=0.1, fmax=1e-3, maxstep=0.1):
+    def __init__(self, engine, ws, Z, x0, dt0=1.0, fmax=1e-3, maxstep=0.1):
         self.eng = engine
         self.ws = ws
         dev = engine.device

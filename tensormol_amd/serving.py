@@ -15,7 +15,8 @@ Every iteration of Server.run:
   3. runs one E+F launch sequence over all atoms, then the NVT integrator
      kernels on the nvt range and the FIRE kernels on the opt range;
   4. retires finished requests (single points after one evaluation, NVT after
-     their step count, FIRE after convergence or the step limit), copying
+     their step count, FIRE once the molecule's RMS atomic force is below
+     fire_fmax Ha/A, or at the step limit), copying
      their results out and recording the latency.
 Per-request state never leaves the device while the request is active, and
 every per-molecule quantity (descriptors, networks, Coulomb, reductions,
@@ -30,7 +31,7 @@ import torch
 
 from . import _lib
 from .params import IDEALGASR, JOULEPERHARTREE, atomic_masses
-from .workloads import chain_molecule, maxwell_boltzmann
+from .workloads import STREAM_KINDS, maxwell_boltzmann, request_stream
 
 _P = _lib.ptr
 KIND_ORDER = {"nvt": 0, "opt": 1, "sp": 2}
@@ -53,51 +54,24 @@ class Request:
     t_admit: float = None
     t_done: float = None
     converged: bool = None
+    error: str = None       # set when the request failed (its molecule only)
     # server bookkeeping: atom offset and molecule slot in the current packed batch
     _off: int = field(default=-1, repr=False)
     _mol: int = field(default=-1, repr=False)
 
 
-def _gen(args):
-    size, seed = args
-    return chain_molecule(int(size), np.random.default_rng(seed))
-
-
-def make_stream(n=4096, rate=2000.0, lo=8, hi=256, seed=3, kinds=((0.5, "sp", 1), (0.3, "nvt", 50),
-                                                                  (0.2, "opt", 100)), workers=None):
-    """Seeded request stream: log-uniform sizes in [lo, hi], random-walk C/H/N/O
-    chains (each from its own seed, so the stream does not depend on how it is
-    generated), kinds drawn with the given probabilities, Poisson arrivals.
-    Geometries are generated by `workers` spawned processes (default: the CPU
-    count, at most 16)."""
-    rng = np.random.default_rng(seed)
-    sizes = np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi + 1), n))).astype(int).clip(lo, hi)
-    gaps = rng.exponential(1.0 / rate, n)
-    t = np.cumsum(gaps) - gaps[0]
-    probs = np.array([k[0] for k in kinds])
-    choice = rng.choice(len(kinds), size=n, p=probs / probs.sum())
-    jobs = [(int(sizes[r]), [seed, 1000, r]) for r in range(n)]
-    import os
-    workers = workers if workers is not None else min(16, os.cpu_count() or 1)
-    if workers > 1 and n >= 64:
-        import multiprocessing as mp
-        with mp.get_context("spawn").Pool(workers) as pool:
-            geos = pool.map(_gen, jobs, chunksize=max(1, n // (8 * workers)))
-    else:
-        geos = [_gen(j) for j in jobs]
-    out = []
-    for r in range(n):
-        Z, x = geos[r]
-        _, kind, steps = kinds[choice[r]]
-        out.append(Request(r, Z, x, kind, steps, float(t[r]), seed=seed * 100000 + r))
-    return out
+def make_stream(n=4096, rate=2000.0, lo=8, hi=256, seed=3, kinds=STREAM_KINDS, workers=None):
+    """The seeded request stream of workloads.request_stream as Request objects."""
+    return [Request(r, Z, x, kind, steps, t, seed=sd)
+            for r, (Z, x, kind, steps, t, sd) in enumerate(
+                request_stream(n, rate, lo, hi, seed, kinds, workers))]
 
 
 class Server:
     """Continuous batching over one engine workspace of fixed capacity."""
 
     def __init__(self, engine, max_atoms=65536, max_mols=8192, hbm_budget_bytes=None, T=300.0,
-                 dt=0.5, fire_dt0=0.1, fire_fmax=5e-3, fire_maxstep=0.1):
+                 dt=0.5, fire_dt0=1.0, fire_fmax=5e-3, fire_maxstep=0.1):
         self.eng = engine
         per_atom = self.bytes_per_atom()
         if hbm_budget_bytes is not None:
@@ -249,6 +223,12 @@ class Server:
         """Serve `requests` (sorted by arrival).  With realtime, arrivals follow
         the wall clock; otherwise every request is available at once."""
         pending = sorted(requests, key=lambda r: r.t_arrival)
+        for r in pending:
+            if len(r.Z) > self.max_atoms:
+                raise ValueError(f"request {r.rid}: {len(r.Z)} atoms exceed the server capacity "
+                                 f"of {self.max_atoms} atoms")
+            if r.kind not in KIND_ORDER:
+                raise ValueError(f"request {r.rid}: unknown kind {r.kind!r}")
         pi = 0
         done = []
         t0 = time.perf_counter()
@@ -284,21 +264,34 @@ class Server:
             self._step()
             it += 1
             ws = self.ws
-            flags = ws.flags.cpu().numpy()   # synchronises
-            if flags[0] or flags[1] or flags[3]:
-                raise RuntimeError(f"server batch flags {flags[:4]} (row overflow or non-finite input)")
-            self.atom_steps += ws.N
-            # retire
-            E = None
             bn, bo, _ = self.groups
-            keep = []
-            fin = []
-            conv = None
+            B = len(self.active)
+            # one synchronising copy: batch flags, per-molecule energies, FIRE |F|^2
+            parts = [ws.flags[:4].double(), self.eng.energy_tensor(ws)]
             if bo:
-                conv = (self.dsum[bn:bn + bo, 1].cpu().numpy())
+                parts.append(self.dsum[bn:bn + bo, 1])
+            stat = torch.cat(parts).cpu().numpy()
+            flags, E, conv = stat[:4], stat[4:4 + B], stat[4 + B:]
+            # failures stay with their molecule: a row overflow (a collapsed
+            # geometry) or a non-finite energy fails that request only, since
+            # rows, mirror slots and reductions never cross molecules
+            bad = ~np.isfinite(E)
+            if bo:
+                bad[bn:bn + bo] |= ~np.isfinite(conv)
+            if flags[0] or flags[1]:
+                nc = ws.ncnt[:ws.N].cpu().numpy()
+                ac = ws.acnt[:ws.N].cpu().numpy()
+                full = (nc >= self.eng.ldr) | (ac >= self.eng.lda)
+                bad[np.unique(ws.mol_np[full])] = True
+            self.atom_steps += ws.N
+            keep, fin = [], []
             for k, r in enumerate(self.active):
                 r.steps_done += 1
-                finished = False
+                if bad[k]:
+                    r.error = ("neighbour row overflow (collapsed geometry)" if flags[0] or flags[1]
+                               else "non-finite energy or forces")
+                    fin.append((k, r))
+                    continue
                 if r.kind == "sp":
                     finished = True
                 elif r.kind == "nvt":
@@ -309,15 +302,22 @@ class Server:
                     r.converged = bool(c)
                 (fin if finished else keep).append((k, r))
             if fin:
-                E = self.eng.energy_tensor(ws).cpu().numpy()
+                # sp results sit at the end of the batch, nvt/opt at the front
+                n_dyn = int(self.ms[bn + bo])
+                F = ws.force[n_dyn:ws.N].cpu().numpy() if any(
+                    r.kind == "sp" and r.error is None for _, r in fin) else None
+                X = self.pos[:n_dyn].cpu().numpy() if any(
+                    r.kind != "sp" and r.error is None for _, r in fin) else None
+                t_done = time.perf_counter() - t0
                 for k, r in fin:
                     a, b_ = int(self.ms[k]), int(self.ms[k + 1])
-                    r.energy = float(E[k])
-                    if r.kind == "sp":
-                        r.forces = ws.force[a:b_].cpu().numpy()
-                    else:
-                        r.positions = self.pos[a:b_].cpu().numpy()
-                    r.t_done = time.perf_counter() - t0
+                    if r.error is None:
+                        r.energy = float(E[k])
+                        if r.kind == "sp":
+                            r.forces = F[a - n_dyn:b_ - n_dyn].copy()
+                        else:
+                            r.positions = X[a:b_].copy()
+                    r.t_done = t_done
                     r._off = -1
                     done.append(r)
             # finished requests leave: survivors keep their device offsets and
@@ -333,10 +333,12 @@ class Server:
 
 
 def latency_stats(done):
-    lat = np.array([r.t_done - r.t_arrival for r in done]) * 1e3
+    """Arrival-to-result latency percentiles of the completed requests."""
+    lat = np.array([r.t_done - r.t_arrival for r in done if r.error is None]) * 1e3
     return {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "mean_ms": float(lat.mean())}
 
 
 def atom_steps(done):
-    return int(sum(len(r.Z) * r.steps_done for r in done))
+    """Atom-steps of the completed requests (failed requests do not count)."""
+    return int(sum(len(r.Z) * r.steps_done for r in done if r.error is None))

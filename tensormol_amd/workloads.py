@@ -213,6 +213,45 @@ def chain_set(sizes, seed=3):
     return [chain_molecule(int(n), rng) for n in sizes]
 
 
+def _chain_job(job):
+    size, seed = job
+    return chain_molecule(int(size), np.random.default_rng(seed))
+
+
+def chain_molecules(jobs, workers=1):
+    """chain_molecule for each (size, seed) job, each from its own generator,
+    so the result does not depend on `workers` (spawned processes that import
+    numpy and this module only)."""
+    if workers > 1 and len(jobs) >= 64:
+        import multiprocessing as mp
+        with mp.get_context("spawn").Pool(workers) as pool:
+            return pool.map(_chain_job, jobs, chunksize=max(1, len(jobs) // (8 * workers)))
+    return [_chain_job(j) for j in jobs]
+
+
+STREAM_KINDS = ((0.5, "sp", 1), (0.3, "nvt", 50), (0.2, "opt", 100))
+
+
+def request_stream(n=4096, rate=2000.0, lo=8, hi=256, seed=3, kinds=STREAM_KINDS, workers=None):
+    """Seeded serving stream: n requests, atom counts log-uniform in [lo, hi],
+    random-walk C/H/N/O chains (each from its own seed), kinds (probability,
+    name, steps) drawn with the given probabilities, Poisson arrivals at `rate`
+    per second.  Returns a list of (Z, xyz, kind, steps, t_arrival, seed).
+    Geometries are generated by `workers` spawned processes (default: the CPU
+    count, at most 16); call this before the process initialises a GPU."""
+    import os
+    rng = np.random.default_rng(seed)
+    sizes = np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi + 1), n))).astype(int).clip(lo, hi)
+    gaps = rng.exponential(1.0 / rate, n)
+    t = np.cumsum(gaps) - gaps[0]
+    probs = np.array([k[0] for k in kinds])
+    choice = rng.choice(len(kinds), size=n, p=probs / probs.sum())
+    workers = workers if workers is not None else min(16, os.cpu_count() or 1)
+    geos = chain_molecules([(int(sizes[r]), [seed, 1000, r]) for r in range(n)], workers)
+    return [(geos[r][0], geos[r][1], kinds[choice[r]][1], kinds[choice[r]][2], float(t[r]),
+             seed * 100000 + r) for r in range(n)]
+
+
 def pack(mols):
     """Concatenate molecules into packed arrays: Z (N,), xyz (N,3), mol_start (B+1,)."""
     Z = np.concatenate([m[0] for m in mols]).astype(np.int32)
