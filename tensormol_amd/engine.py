@@ -176,10 +176,12 @@ class BPEEEngine:
         ws.use_cells = False
         return ws
 
-    def set_topology(self, ws, Z, mol_start, boxes=None, margin=4.0, pos0=None):
+    def set_topology(self, ws, Z, mol_start, boxes=None, margin=4.0, pos0=None, stager=None):
         """Point a workspace at a new packed batch (N <= ncap, B <= bcap).
         Only small host-built index arrays are copied; no buffer is reallocated
-        unless a cell grid is (re)built for a single large system."""
+        unless a cell grid is (re)built for a single large system.  With an
+        open PinnedStager the copies are queued without blocking the host."""
+        up = (lambda a, out=None: stager.put(a, out)) if stager is not None else _plain_upload(self.device)
         dev = self.device
         Z = np.asarray(Z)
         ms = np.asarray(mol_start, dtype=np.int64)
@@ -196,10 +198,10 @@ class BPEEEngine:
         counts = np.bincount(elem, minlength=self.nele)
         ws.eoff = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
         if N:
-            ws.elem[:N].copy_(torch.from_numpy(elem))
-            ws.mol[:N].copy_(torch.from_numpy(ws.mol_np))
-            ws.perm[:N].copy_(torch.from_numpy(perm))
-        ws.mol_start[:B + 1].copy_(torch.from_numpy(ms.astype(np.int32)))
+            up(elem, ws.elem[:N])
+            up(ws.mol_np, ws.mol[:N])
+            up(perm, ws.perm[:N])
+        up(ms.astype(np.int32), ws.mol_start[:B + 1])
         ws.box = None
         ws.periodic = 0
         ws.boxes_np = None
@@ -231,7 +233,7 @@ class BPEEEngine:
             if ws.mlp_ws is None or ws.mlp_ws.numel() < need:
                 ws.mlp_ws = torch.zeros(need, dtype=torch.float32, device=dev)
         else:
-            ws.mlp_ws = self.mfma.workspace(ws)
+            ws.mlp_ws = self.mfma.workspace(ws, up)
         return ws
 
     def _setup_grid(self, ws, pos0, margin):
@@ -383,6 +385,16 @@ class BPEEEngine:
     def energy_tensor(self, ws):
         """Per-molecule total energy as a device tensor (no host copy)."""
         return ws.Ebp[:ws.B] + ws.Eext[:ws.B, 0] + ws.Eext[:ws.B, 1]
+
+
+def _plain_upload(device):
+    def up(a, out=None):
+        t = torch.from_numpy(np.ascontiguousarray(a))
+        if out is None:
+            return t.to(device)
+        out.copy_(t)
+        return out
+    return up
 
 
 def ctypes_stream(s):

@@ -80,24 +80,27 @@ class MFMAWeights:
         fl[4 * H] = np.asarray(bo).reshape(-1)[0]
         return blob
 
-    def workspace(self, ws):
-        """Segment and tile tables for one packed batch (rows are element-sorted)."""
+    def workspace(self, ws, upload=None):
+        """Segment and tile tables for one packed batch (rows are element-sorted):
+        seg[s] = (net, net * nele + element, first row, rows), tiles[t] = (seg, first row)."""
         nele = self.engine.nele
+        counts = np.diff(ws.eoff).astype(np.int64)
         segs, tiles = [], []
         for net in range(2):
             for a in range(nele):
-                M = int(ws.eoff[a + 1] - ws.eoff[a])
+                M = int(counts[a])
                 if M == 0:
                     continue
                 s = len(segs)
                 segs.append((net, net * nele + a, int(ws.eoff[a]), M))
-                for r0 in range(0, M, self.mt):
-                    tiles.append((s, r0))
-        dev = self.engine.device
-        t = dict(seg=torch.tensor(np.array(segs, dtype=np.int32).reshape(-1, 4), device=dev),
-                 tiles=torch.tensor(np.array(tiles, dtype=np.int32).reshape(-1, 2), device=dev),
-                 ntiles=len(tiles))
-        return t
+                r0 = np.arange(0, M, self.mt, dtype=np.int32)
+                tiles.append(np.stack([np.full_like(r0, s), r0], axis=1))
+        seg = np.array(segs, dtype=np.int32).reshape(-1, 4)
+        til = np.concatenate(tiles).astype(np.int32) if tiles else np.zeros((0, 2), np.int32)
+        if upload is None:
+            dev = self.engine.device
+            upload = lambda a, out=None: torch.from_numpy(a).to(dev)
+        return dict(seg=upload(seg), tiles=upload(til), ntiles=len(til))
 
     def run(self, ws):
         eng = self.engine

@@ -100,11 +100,16 @@ class BPEEModel:
         return c6, r0
 
     def element_index(self, Z):
-        lut = {e: a for a, e in enumerate(self.eles)}
-        try:
-            return np.array([lut[int(z)] for z in Z], dtype=np.int32)
-        except KeyError as exc:
-            raise ValueError(f"element {exc} not in model elements {self.eles}") from None
+        """Index of each atom's element in self.eles (int32)."""
+        Z = np.asarray(Z, dtype=np.int64)
+        lut = np.full(max(max(self.eles), int(Z.max()) if Z.size else 0) + 1, -1, dtype=np.int32)
+        lut[np.array(self.eles)] = np.arange(len(self.eles), dtype=np.int32)
+        if Z.size and Z.min() < 0:
+            raise ValueError(f"element {int(Z.min())} not in model elements {self.eles}")
+        idx = lut[Z]
+        if Z.size and idx.min() < 0:
+            raise ValueError(f"element {int(Z[np.argmin(idx)])} not in model elements {self.eles}")
+        return idx
 
     def save(self, path):
         arrs = {"eles": np.array(self.eles), "hidden": np.array(self.hidden),

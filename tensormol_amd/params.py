@@ -27,9 +27,18 @@ ATOMIC_VDW_RADIUS = {1: 1.001, 2: 1.012, 3: 0.825, 4: 1.408, 5: 1.485, 6: 1.452,
                      8: 1.342, 9: 1.287, 10: 1.243}
 
 
+_MASS_LUT = np.full(max(ATOMICMASSESAMU) + 1, np.nan)
+for _z, _m in ATOMICMASSESAMU.items():
+    _MASS_LUT[_z] = 0.000999977 * _m
+
+
 def atomic_masses(Z):
     """kg/mol per atom: 0.000999977 * amu (PhysicalData.py:54)."""
-    return 0.000999977 * np.array([ATOMICMASSESAMU[int(z)] for z in Z], dtype=np.float64)
+    Z = np.asarray(Z, dtype=np.int64)
+    if Z.size and (Z.min() < 0 or Z.max() >= len(_MASS_LUT) or np.isnan(_MASS_LUT[Z]).any()):
+        bad = sorted(set(int(z) for z in Z if z < 0 or z >= len(_MASS_LUT) or np.isnan(_MASS_LUT[z])))
+        raise KeyError(bad[0])
+    return _MASS_LUT[Z].astype(np.float64)
 
 
 class _Params(dict):

@@ -7,17 +7,26 @@ is the one-at-a-time i-PI client (TMIPIinterface.py:19-61), and a new molecule
 size rebuilds its TensorFlow graph (TFMolInstanceDirect.py:5694-5699).
 
 Every iteration of Server.run:
-  1. admits requests whose arrival time has passed, while the packed batch
-     stays within the atom capacity derived from an HBM budget;
-  2. re-packs the active requests into kind-contiguous groups [nvt | opt | sp]
-     (state of continuing requests moves with on-device gathers; new requests'
-     coordinates are uploaded once);
-  3. runs one E+F launch sequence over all atoms, then the NVT integrator
-     kernels on the nvt range and the FIRE kernels on the opt range;
-  4. retires finished requests (single points after one evaluation, NVT after
-     their step count, FIRE once the molecule's RMS atomic force is below
-     fire_fmax Ha/A, or at the step limit), copying
-     their results out and recording the latency.
+  1. forms the next batch from the requests of the batch on the GPU that will
+     still be running after it (single points leave after one step, NVT and
+     FIRE at their step limits) plus the requests that have arrived, while
+     the packed batch stays within the atom capacity derived from an HBM
+     budget;
+  2. re-packs it into kind-contiguous groups [nvt | opt | sp] (state of
+     continuing requests moves with on-device gathers queued behind the
+     running step; new requests' coordinates are uploaded once, through
+     pinned staging, so the host never waits for the GPU here);
+  3. queues one E+F launch sequence over all atoms, the NVT integrator
+     kernels on the nvt range and the FIRE kernels on the opt range, and
+     copies of the step's status and results into pinned memory;
+  4. waits for the previous step's status and retires its finished requests
+     (single points after one evaluation, NVT after their step count, FIRE
+     once the molecule's RMS atomic force is below fire_fmax Ha/A, or at the
+     step limit), copying their results out and recording the latency.
+A request found finished in 4 that step 1 did not foresee (a converged FIRE
+run, a failure) is already in the next batch: it rides along once more and
+that step's output for it is discarded, so its result is the state at the
+step that finished it.
 Per-request state never leaves the device while the request is active, and
 every per-molecule quantity (descriptors, networks, Coulomb, reductions,
 integrators) depends only on the molecule's own atoms, so a request gives the
@@ -58,6 +67,7 @@ class Request:
     # server bookkeeping: atom offset and molecule slot in the current packed batch
     _off: int = field(default=-1, repr=False)
     _mol: int = field(default=-1, repr=False)
+    _retired: bool = field(default=False, repr=False)
 
 
 def make_stream(n=4096, rate=2000.0, lo=8, hi=256, seed=3, kinds=STREAM_KINDS, workers=None):
@@ -100,6 +110,14 @@ class Server:
         self.mix = torch.zeros(b, 3, **f64)
         self.dsum = torch.zeros(b, 3, **f64)
         self.active = []
+        self.natoms = 0
+        from .staging import PinnedOutputs, PinnedStager
+        self.stager = PinnedStager(dev, slot_bytes=96 * n + 96 * b + (1 << 16))
+        # two sets of pinned result buffers: the step on the GPU and the one being read
+        spec = {"stat": ((4 + 2 * b,), torch.float64), "force": ((n, 3), torch.float64),
+                "pos": ((n, 3), torch.float64), "ncnt": ((n,), torch.int32),
+                "acnt": ((n,), torch.int32)}
+        self.outs = [PinnedOutputs(spec), PinnedOutputs(spec)]
 
     def bytes_per_atom(self):
         """Device bytes of one atom's share of the workspace and server state."""
@@ -108,81 +126,84 @@ class Server:
         return ws + 8 * (3 * 4 + 2 + 2 + 3)
 
     # ---------------------------------------------------------------- packing
-    def _repack(self, new):
-        """Order active + new requests as [nvt | opt | sp], move device state."""
-        old = self.active
-        reqs = sorted(old + new, key=lambda r: (KIND_ORDER[r.kind], r.rid))
-        sizes = np.array([len(r.Z) for r in reqs], dtype=np.int64)
-        ms = np.concatenate([[0], np.cumsum(sizes)])
-        N, B = int(ms[-1]), len(reqs)
+    @staticmethod
+    def _ranges(starts, lens):
+        """Concatenation of arange(starts[k], starts[k] + lens[k]) over k."""
+        tot = int(lens.sum())
+        if tot == 0:
+            return np.zeros(0, dtype=np.int64)
+        rel = np.arange(tot, dtype=np.int64) - np.repeat(np.cumsum(lens) - lens, lens)
+        return np.repeat(starts, lens) + rel
+
+    def _repack(self, keep, new):
+        """Order continuing + new requests as [nvt | opt | sp], move device state
+        (every copy queued behind the running step)."""
+        reqs = sorted(keep + new, key=lambda r: (KIND_ORDER[r.kind], r.rid))
+        up = self.stager.put
+        self.stager.begin()
+        B = len(reqs)
         dev = self.eng.device
-        # gather index for continuing requests (old offset -> new offset)
-        src, dst = [], []
-        for k, r in enumerate(reqs):
-            if r._off >= 0:
-                src.append(np.arange(r._off, r._off + len(r.Z)))
-                dst.append(np.arange(ms[k], ms[k + 1]))
-        old_eta = self.eta.clone() if old else None
-        old_fs = self.fstate.clone() if old else None
-        old_mol = {id(r): r._mol for r in old}
-        if src:
-            s_idx = torch.from_numpy(np.concatenate(src)).to(dev)
-            d_idx = torch.from_numpy(np.concatenate(dst)).to(dev)
+        sizes = np.fromiter((len(r.Z) for r in reqs), dtype=np.int64, count=B)
+        ms = np.zeros(B + 1, dtype=np.int64)
+        np.cumsum(sizes, out=ms[1:])
+        N = int(ms[-1])
+        kinds = np.fromiter((KIND_ORDER[r.kind] for r in reqs), dtype=np.int64, count=B)
+        old_off = np.fromiter((r._off for r in reqs), dtype=np.int64, count=B)
+        old_slot = np.fromiter((r._mol for r in reqs), dtype=np.int64, count=B)
+        cont = old_off >= 0
+        eta_c = fs_c = None
+        if cont.any():
+            # continuing requests: atom state and per-molecule state gathered on the device
+            s_idx = up(self._ranges(old_off[cont], sizes[cont]))
+            d_idx = up(self._ranges(ms[:-1][cont], sizes[cont]))
             for t in (self.pos, self.v, self.a):
-                vals = t.index_select(0, s_idx)
-                t.index_copy_(0, d_idx, vals)
-        # new requests: upload positions, masses, velocities
-        if new:
-            xs = np.concatenate([r.xyz for r in new]).astype(np.float64)
+                t.index_copy_(0, d_idx, t.index_select(0, s_idx))
+            so = up(old_slot[cont])
+            sn = up(np.nonzero(cont)[0])
+            eta_c = self.eta.index_select(0, so)
+            fs_c = self.fstate.index_select(0, so)
+        newk = np.nonzero(~cont)[0]
+        if len(newk):
+            xs = np.concatenate([reqs[k].xyz for k in newk]).astype(np.float64)
             vs = np.zeros_like(xs)
-            idx = []
             o = 0
-            slot = {id(r): k for k, r in enumerate(reqs)}
-            for r in new:
-                k = slot[id(r)]
-                n_ = len(r.Z)
+            for k in newk:
+                r = reqs[k]
                 if r.kind == "nvt":
-                    vs[o:o + n_] = maxwell_boltzmann(r.Z, self.T, r.seed, atomic_masses(r.Z))
-                idx.append(np.arange(ms[k], ms[k + 1]))
-                o += n_
-            d_idx = torch.from_numpy(np.concatenate(idx)).to(dev)
-            self.pos.index_copy_(0, d_idx, torch.from_numpy(xs).to(dev))
-            self.v.index_copy_(0, d_idx, torch.from_numpy(vs).to(dev))
-            self.a.index_copy_(0, d_idx, torch.zeros_like(torch.from_numpy(xs)).to(dev))
+                    vs[o:o + len(r.Z)] = maxwell_boltzmann(r.Z, self.T, r.seed, atomic_masses(r.Z))
+                o += len(r.Z)
+            d_idx = up(self._ranges(ms[:-1][newk], sizes[newk]))
+            self.pos.index_copy_(0, d_idx, up(xs))
+            self.v.index_copy_(0, d_idx, up(vs))
+            self.a.index_fill_(0, d_idx, 0.0)
         Zall = np.concatenate([r.Z for r in reqs]) if reqs else np.zeros(0, np.int32)
         if N:
-            self.m[:N].copy_(torch.from_numpy(atomic_masses(Zall)))
-        # per-molecule state
-        eta = np.zeros(B)
-        kT = np.zeros(B)
-        Q = np.ones(B)
+            up(atomic_masses(Zall), self.m[:N])
+        # per-molecule state: thermostat constants from kind and size (SimpleMD.py:109-125),
+        # fresh FIRE state for new optimisations, carried state for continuing requests
+        kT = np.where(kinds == 0, IDEALGASR * 1e-10 * self.T, 0.0)
+        tau = 20.0 * self.dt * sizes
+        Q = np.where(kinds == 0, kT * tau * tau, 1.0)
         fs = np.zeros((B, 4))
-        eta_old = old_eta.cpu().numpy() if old_eta is not None else None
-        fs_old = old_fs.cpu().numpy() if old_fs is not None else None
-        fdt0 = self.fire[0]
-        for k, r in enumerate(reqs):
-            n_ = len(r.Z)
-            if r.kind == "nvt":
-                kT[k] = IDEALGASR * 1e-10 * self.T
-                tau = 20.0 * self.dt * n_
-                Q[k] = kT[k] * tau * tau
-                if id(r) in old_mol:
-                    eta[k] = eta_old[old_mol[id(r)]]
-            elif r.kind == "opt":
-                fs[k] = fs_old[old_mol[id(r)]] if id(r) in old_mol else (fdt0, 0.1, 0.0, 1.0)
+        fs[kinds == 1] = (self.fire[0], 0.1, 0.0, 1.0)
         if B:
-            self.eta[:B].copy_(torch.from_numpy(eta))
-            self.kT[:B].copy_(torch.from_numpy(kT))
-            self.Q[:B].copy_(torch.from_numpy(Q))
-            self.fstate[:B].copy_(torch.from_numpy(fs))
+            self.eta[:B].zero_()
+            up(kT, self.kT[:B])
+            up(Q, self.Q[:B])
+            up(fs, self.fstate[:B])
+            if eta_c is not None:
+                self.eta.index_copy_(0, sn, eta_c)
+                self.fstate.index_copy_(0, sn, fs_c)
         for k, r in enumerate(reqs):
             r._off = int(ms[k])
             r._mol = k
         self.active = reqs
         self.ms = ms
-        self.groups = [sum(1 for r in reqs if r.kind == g) for g in ("nvt", "opt", "sp")]
+        self.natoms = N
+        self.groups = [int(c) for c in np.bincount(kinds, minlength=3)]
         if reqs:
-            self.eng.set_topology(self.ws, Zall, ms)
+            self.eng.set_topology(self.ws, Zall, ms, stager=self.stager)
+        self.stager.end()
 
     # ----------------------------------------------------------------- step
     def _step(self):
@@ -219,9 +240,84 @@ class Server:
                              off3(self.pos), 10.0 * fdt0, 5, 1.1, 0.5, 0.1, 0.99, fmaxstep, st)
 
     # ----------------------------------------------------------------- loop
+    def _launch(self, outs):
+        """Queue one step of the current batch and the copies of its status
+        (flags, per-molecule energies, FIRE |F|^2), the single points' forces,
+        the nvt/opt positions and the row counts into pinned memory."""
+        ws = self.ws
+        bn, bo, _ = self.groups
+        self._step()
+        parts = [ws.flags[:4].double(), self.eng.energy_tensor(ws)]
+        if bo:
+            parts.append(self.dsum[bn:bn + bo, 1])
+        outs.fetch("stat", torch.cat(parts))
+        n_dyn = int(self.ms[bn + bo])
+        outs.fetch("force", ws.force[n_dyn:ws.N])
+        outs.fetch("pos", self.pos[:n_dyn])
+        outs.fetch("ncnt", ws.ncnt[:ws.N])
+        outs.fetch("acnt", ws.acnt[:ws.N])
+        outs.mark(self.eng.device)
+        return dict(reqs=self.active, ms=self.ms, groups=tuple(self.groups), N=ws.N,
+                    mol_np=ws.mol_np, outs=outs)
+
+    @staticmethod
+    def _leaves_after(r):
+        """Whether r leaves after its next step whatever that step computes."""
+        return r.kind == "sp" or r.steps_done + 1 >= r.nsteps
+
+    def _finish(self, batch, t0, fthr2, done):
+        """Wait for a launched step and retire its finished requests."""
+        outs = batch["outs"]
+        outs.wait()
+        reqs, ms, (bn, bo, _) = batch["reqs"], batch["ms"], batch["groups"]
+        B, N = len(reqs), batch["N"]
+        stat = outs.host("stat", 4 + B + bo)
+        flags, E, conv = stat[:4], stat[4:4 + B], stat[4 + B:]
+        # failures stay with their molecule, since rows, mirror slots and
+        # reductions never cross molecules.  A molecule fails when one of its
+        # rows reaches capacity (ldr - 1 and lda - 1 neighbours are the most a
+        # request may have: a collapsed geometry), judged per molecule so the
+        # verdict does not depend on the batch, or when its energy is not finite
+        full = (outs.host("ncnt", N) >= self.eng.ldr) | (outs.host("acnt", N) >= self.eng.lda)
+        row_full = np.zeros(B, dtype=bool)
+        row_full[np.unique(batch["mol_np"][full])] = True
+        bad = row_full | ~np.isfinite(E)
+        if bo:
+            bad[bn:bn + bo] |= ~np.isfinite(conv)
+        n_dyn = int(ms[bn + bo])
+        t_done = time.perf_counter() - t0
+        for k, r in enumerate(reqs):
+            if r._retired:      # finished at the previous step; this output is discarded
+                continue
+            r.steps_done += 1
+            if bad[k]:
+                r.error = ("neighbour row at capacity (collapsed geometry)" if row_full[k]
+                           else "non-finite energy or forces")
+                finished = True
+            elif r.kind == "sp":
+                finished = True
+            elif r.kind == "nvt":
+                finished = r.steps_done >= r.nsteps
+            else:
+                c = conv[k - bn] / len(r.Z) < fthr2
+                finished = c or r.steps_done >= r.nsteps
+                r.converged = bool(c)
+            if not finished:
+                continue
+            a, b_ = int(ms[k]), int(ms[k + 1])
+            if r.error is None:
+                r.energy = float(E[k])
+                if r.kind == "sp":
+                    r.forces = outs.host("force", N - n_dyn)[a - n_dyn:b_ - n_dyn].copy()
+                else:
+                    r.positions = outs.host("pos", n_dyn)[a:b_].copy()
+            r.t_done = t_done
+            r._retired = True
+            done.append(r)
+
     def run(self, requests, realtime=True, max_iters=10 ** 7, on_iter=None):
-        """Serve `requests` (sorted by arrival).  With realtime, arrivals follow
-        the wall clock; otherwise every request is available at once."""
+        """Serve `requests`.  With realtime, arrivals follow the wall clock;
+        otherwise every request is available at once."""
         pending = sorted(requests, key=lambda r: r.t_arrival)
         for r in pending:
             if len(r.Z) > self.max_atoms:
@@ -233,100 +329,55 @@ class Server:
         done = []
         t0 = time.perf_counter()
         it = 0
-        fdt0, fmax, _ = self.fire
+        _, fmax, _ = self.fire
         fthr2 = (fmax * JOULEPERHARTREE) ** 2
         self.active = []
-        self._dirty = False
+        self.natoms = 0
         self.atom_steps = 0
-        while (pi < len(pending) or self.active) and it < max_iters:
+        inflight = None
+        while it < max_iters:
+            # continuing requests: those of the launched step that outlive it
+            if inflight is not None:
+                keep = [r for r in inflight["reqs"] if not r._retired and not self._leaves_after(r)]
+            else:
+                keep = []
+            natoms = sum(len(r.Z) for r in keep)
+            nmol = len(keep)
             now = time.perf_counter() - t0
             new = []
-            natoms = sum(len(r.Z) for r in self.active)
-            nmol = len(self.active)
             while pi < len(pending) and (not realtime or pending[pi].t_arrival <= now):
                 r = pending[pi]
                 if natoms + len(r.Z) > self.max_atoms or nmol + 1 > self.max_mols:
-                    break   # admission by capacity (HBM budget); retry next iteration
+                    break   # admission by capacity (HBM budget); retried next iteration
                 r.t_admit = now
+                r._off = r._mol = -1
+                r._retired = False
                 new.append(r)
                 natoms += len(r.Z)
                 nmol += 1
                 pi += 1
-            if not self.active and not new:
-                if pi < len(pending):
-                    wait = pending[pi].t_arrival - (time.perf_counter() - t0)
-                    if wait > 0:
-                        time.sleep(min(wait, 0.001))
-                continue
-            if new or self._dirty:
-                self._repack(new)
-                self._dirty = False
-            self._step()
-            it += 1
-            ws = self.ws
-            bn, bo, _ = self.groups
-            B = len(self.active)
-            # one synchronising copy: batch flags, per-molecule energies, FIRE |F|^2
-            parts = [ws.flags[:4].double(), self.eng.energy_tensor(ws)]
-            if bo:
-                parts.append(self.dsum[bn:bn + bo, 1])
-            stat = torch.cat(parts).cpu().numpy()
-            flags, E, conv = stat[:4], stat[4:4 + B], stat[4 + B:]
-            # failures stay with their molecule: a row overflow (a collapsed
-            # geometry) or a non-finite energy fails that request only, since
-            # rows, mirror slots and reductions never cross molecules
-            bad = ~np.isfinite(E)
-            if bo:
-                bad[bn:bn + bo] |= ~np.isfinite(conv)
-            if flags[0] or flags[1]:
-                nc = ws.ncnt[:ws.N].cpu().numpy()
-                ac = ws.acnt[:ws.N].cpu().numpy()
-                full = (nc >= self.eng.ldr) | (ac >= self.eng.lda)
-                bad[np.unique(ws.mol_np[full])] = True
-            self.atom_steps += ws.N
-            keep, fin = [], []
-            for k, r in enumerate(self.active):
-                r.steps_done += 1
-                if bad[k]:
-                    r.error = ("neighbour row overflow (collapsed geometry)" if flags[0] or flags[1]
-                               else "non-finite energy or forces")
-                    fin.append((k, r))
-                    continue
-                if r.kind == "sp":
-                    finished = True
-                elif r.kind == "nvt":
-                    finished = r.steps_done >= r.nsteps
-                else:
-                    c = conv[k - bn] / len(r.Z) < fthr2
-                    finished = c or r.steps_done >= r.nsteps
-                    r.converged = bool(c)
-                (fin if finished else keep).append((k, r))
-            if fin:
-                # sp results sit at the end of the batch, nvt/opt at the front
-                n_dyn = int(self.ms[bn + bo])
-                F = ws.force[n_dyn:ws.N].cpu().numpy() if any(
-                    r.kind == "sp" and r.error is None for _, r in fin) else None
-                X = self.pos[:n_dyn].cpu().numpy() if any(
-                    r.kind != "sp" and r.error is None for _, r in fin) else None
-                t_done = time.perf_counter() - t0
-                for k, r in fin:
-                    a, b_ = int(self.ms[k]), int(self.ms[k + 1])
-                    if r.error is None:
-                        r.energy = float(E[k])
-                        if r.kind == "sp":
-                            r.forces = F[a - n_dyn:b_ - n_dyn].copy()
-                        else:
-                            r.positions = X[a:b_].copy()
-                    r.t_done = t_done
-                    r._off = -1
-                    done.append(r)
-            # finished requests leave: survivors keep their device offsets and
-            # slots until the next re-pack
-            if fin:
-                self._dirty = True
-            self.active = [r for _, r in keep]
-            if on_iter is not None:
-                on_iter(self, it)
+            nxt = None
+            if keep or new:
+                if new or inflight is None or len(keep) != len(inflight["reqs"]):
+                    self._repack(keep, new)
+                self.outs.reverse()
+                nxt = self._launch(self.outs[0])
+                it += 1
+                self.atom_steps += nxt["N"]
+            if inflight is not None:
+                self._finish(inflight, t0, fthr2, done)
+                if on_iter is not None:
+                    on_iter(self, it)
+            inflight = nxt
+            if inflight is None:
+                if pi >= len(pending):
+                    break
+                wait = pending[pi].t_arrival - (time.perf_counter() - t0)
+                if realtime and wait > 0:
+                    time.sleep(min(wait, 0.001))
+        if inflight is not None:
+            self._finish(inflight, t0, fthr2, done)
+        self.active = []
         self.wall = time.perf_counter() - t0
         self.iters = it
         return done
