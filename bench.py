@@ -138,6 +138,129 @@ def run_md(eng, ws, Z, x, box, steps, warmup, dt, graph=True, mlp_samples=20):
                 e_end=e_end, graph=graph)
 
 
+STAGES = ("build_lists", "descriptors", "networks", "electrostatics", "adjoint", "reductions")
+
+
+def stage_times(eng, ws, pos, reps, stream):
+    """Mean GPU time (ms) of each engine stage, from HIP events recorded on
+    the stream the stages run on, over `reps` eager evaluations."""
+    import torch
+    recs = []
+    with torch.cuda.stream(stream):
+        for _ in range(reps):
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(STAGES) + 1)]
+            ev[0].record(stream)
+            for k, name in enumerate(STAGES):
+                fn = getattr(eng, name)
+                fn(ws) if name in ("networks", "reductions") else fn(ws, pos)
+                ev[k + 1].record(stream)
+            recs.append(ev)
+    stream.synchronize()
+    out = {}
+    for k, name in enumerate(STAGES):
+        out[name] = float(np.mean([e[k].elapsed_time(e[k + 1]) for e in recs]))
+    return out
+
+
+def descriptor_bytes(N, D, P_r, P_r3):
+    """Algorithmic HBM bytes of the descriptor forward pass: positions (12N),
+    row offsets (4(N+1)), the 4.6 A and 3.1 A neighbour indices (4 P_r,
+    4 P_r3; triples are enumerated on the fly) and the descriptor write (4 D N)."""
+    return 12 * N + 4 * (N + 1) + 4 * P_r + 4 * P_r3 + 4 * D * N
+
+
+def run_sphere(args):
+    """Single-point E+F of the open-boundary C/H/N/O sphere (24,000 atoms)."""
+    import torch
+    from oracle import oracle as O
+    from tensormol_amd import workloads as W
+    from tensormol_amd.engine import BPEEEngine
+    from tensormol_amd.model import BPEEModel
+    log = lambda m: print(f"[sphere] {m}", file=sys.stderr, flush=True)
+    Z, x = W.chno_sphere(natom=args.sphere_atoms, seed=2)
+    N = len(Z)
+    model = BPEEModel([1, 6, 7, 8], dsf_alpha=0.2, seed=0)
+    eng = BPEEEngine(model, precision=args.sphere_precision, check_overflow=False)
+    ms = np.array([0, N])
+    ws = eng.prepare(Z, ms, None, pos0=x)
+    dev = eng.device
+    pos = torch.tensor(x, dtype=torch.float64, device=dev)
+    side = torch.cuda.Stream(device=dev)
+    side.wait_stream(torch.cuda.current_stream(dev))
+    with torch.cuda.stream(side):
+        for _ in range(3):
+            eng.run(ws, pos)
+        eng.check_flags(ws)
+    side.synchronize()
+    stages = stage_times(eng, ws, pos, 10, side)
+    P_r = int(ws.ncnt[:N].sum())
+    P_r3 = int(ws.acnt[:N].sum())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=side):
+        eng.run(ws, pos, check=False)
+    side.synchronize()
+    with torch.cuda.stream(side):   # replay() launches on the current stream
+        t0 = time.perf_counter()
+        for _ in range(args.sphere_evals):
+            g.replay()
+        side.synchronize()
+        wall = time.perf_counter() - t0
+        eng.check_flags(ws)
+        r = eng.results(ws)
+    value = N * args.sphere_evals / wall
+    log(f"{N} atoms {args.sphere_precision}: {1e3 * wall / args.sphere_evals:.3f} ms per E+F, "
+        f"{value:.4g} atom-steps/s; stages {stages}")
+    t0 = time.perf_counter()
+    ref = O.evaluate(x, Z, ms, O.SymParams(model.eles), model.energy_nets, model.charge_nets,
+                     alpha=model.alpha, dsf_alpha=model.dsf_alpha, has_vdw=model.has_vdw)
+    cpu_s = time.perf_counter() - t0
+    frms = float(np.sqrt(((r["gradient"] - ref["gradient"]) ** 2).mean()))
+    dE = float(abs(r["Etotal"][0] - ref["Etotal"][0]))
+    log(f"oracle {cpu_s:.1f} s: force rms err {frms:.3e} Ha/A, |dE| {dE:.3e} Ha")
+    eng32 = BPEEEngine(model, precision="fp32", check_overflow=False)
+    ws32 = eng32.prepare(Z, ms, None, pos0=x)
+    with torch.cuda.stream(side):
+        eng32.run(ws32, pos)
+        side.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            eng32.run(ws32, pos, check=False)
+        side.synchronize()
+    v32 = N * 5 / (time.perf_counter() - t0)
+    D, H = eng.D, int(model.hidden[0])
+    flops = mlp_flops_per_atom(D, H) * N
+    bdesc = descriptor_bytes(N, D, P_r, P_r3)
+    peak = MLP_PEAK_TFLOPS[args.sphere_precision]
+    mlp_tf = flops / (stages["networks"] * 1e-3) / 1e12
+    desc_gbs = bdesc / (stages["descriptors"] * 1e-3) / 1e9
+    adj_gbs = (bdesc + 12 * N) / (stages["adjoint"] * 1e-3) / 1e9
+    import torch as _t
+    return {
+        "metric": "atom-steps/s (E+F) at force RMS<1e-3 Ha/A, single point",
+        "value": round(value, 1), "unit": "atom-steps/s", "n_gpus": 1,
+        "evals": args.sphere_evals, "ms_per_eval": round(1e3 * wall / args.sphere_evals, 4),
+        "higher_is_better": True, "dtype": args.sphere_precision, "data": "synthetic",
+        "config": {"workload": f"open {N}-atom C/H/N/O sphere (H .5 C .3 N .1 O .1, 0.10/A^3, "
+                               "min distance 1.0 A) single-point E+F, D 768, nets 768-512-512-512-1 x2",
+                   "atoms": N, "hip_graph": True},
+        "pairs_4.6A": P_r, "pairs_3.1A": P_r3,
+        "force_rms_err_vs_fp64": frms, "energy_err_vs_fp64_Ha": dE,
+        "plain_fp32_value": round(v32, 1), "speedup_vs_plain_fp32": round(value / v32, 2),
+        "stage_ms": {k: round(v, 4) for k, v in stages.items()},
+        "roofline": {
+            "mlp": {"bound": "mfma", "achieved": round(mlp_tf, 2), "peak": peak, "unit": "TFLOP/s",
+                    "frac": round(mlp_tf / peak, 4), "flops_per_launch": flops},
+            "descriptors": {"bound": "hbm", "achieved": round(desc_gbs, 1), "peak": 8000.0,
+                            "unit": "GB/s", "frac": round(desc_gbs / 8000.0, 4),
+                            "bytes_per_launch": bdesc},
+            "adjoint": {"bound": "hbm", "achieved": round(adj_gbs, 1), "peak": 8000.0, "unit": "GB/s",
+                        "frac": round(adj_gbs / 8000.0, 4), "bytes_per_launch": bdesc + 12 * N}},
+        "cpu_baseline": {"value": round(N / cpu_s, 1), "unit": "atom-steps/s",
+                         "cores": _t.get_num_threads(), "kind": "port",
+                         "sample": f"one FP64 E+F of the same sphere by the oracle, {cpu_s:.1f} s"},
+    }
+
+
 def cpu_baseline(Z, x, box, model):
     """The FP64 oracle on this host: one full E+F of the box (neighbour lists
     included), which is also the frame-0 reference for the force error."""
@@ -253,11 +376,21 @@ def main():
     ap.add_argument("--serve-baseline-requests", type=int, default=128)
     ap.add_argument("--serve-check", type=int, default=8)
     ap.add_argument("--serve-only", action="store_true", help="print the serving record only")
+    ap.add_argument("--calib-copy-mib", type=int, default=256,
+                    help="size of the calibration copy run before the MD (0: none)")
+    ap.add_argument("--sphere", action="store_true",
+                    help="single-point E+F of the 24,000-atom C/H/N/O sphere instead (own JSON line)")
+    ap.add_argument("--sphere-atoms", type=int, default=24000)
+    ap.add_argument("--sphere-precision", default="fp16")
+    ap.add_argument("--sphere-evals", type=int, default=50)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus}); "
               "the single-GPU value is the figure of merit", file=sys.stderr)
         sys.exit(2)
+    if args.sphere:
+        print(json.dumps(run_sphere(args)), flush=True)
+        return
     stream = None
     if args.serve_requests > 0:
         # generated by spawned worker processes before this process touches the GPU
@@ -274,6 +407,15 @@ def main():
 
     Z, x, box, model = build_system(args)
     N = len(Z)
+    if args.calib_copy_mib > 0:
+        # a plain device copy of known size: the calibration kernel for the
+        # HBM byte counters (FETCH_SIZE / WRITE_SIZE) of profiled runs
+        nb = args.calib_copy_mib << 20
+        src = torch.ones(nb // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     eng = BPEEEngine(model, precision=args.precision, check_overflow=False)
     ws = eng.prepare(Z, np.array([0, N]), box[None])
     print(f"[bench] {N} atoms, precision {args.precision}, warmup {args.warmup}, steps {args.steps}",
@@ -288,6 +430,11 @@ def main():
     eng.run(ws, pos0, check=True)
     torch.cuda.synchronize()
     r_gpu = eng.results(ws)
+    side = torch.cuda.Stream(device=eng.device)
+    side.wait_stream(torch.cuda.current_stream(eng.device))
+    stages = stage_times(eng, ws, pos0, 10, side)
+    P_r, P_r3 = int(ws.ncnt[:N].sum()), int(ws.acnt[:N].sum())
+    print(f"[bench] stages (ms, eager) {stages}", file=sys.stderr, flush=True)
     cpu = None
     force_rms_err = None
     energy_err = None
@@ -321,6 +468,16 @@ def main():
                 "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
                 "frac": round(achieved / peak, 4), "traffic": None,
                 "flops_per_launch": flops, "avg_launch_ms": round(mlp_avg_ms, 4)}
+    bdesc = descriptor_bytes(N, eng.D, P_r, P_r3)
+    by_stage = {
+        "mlp": dict(roofline, achieved=round(achieved, 2)),
+        "descriptors": {"bound": "hbm", "unit": "GB/s", "peak": 8000.0, "bytes_per_launch": bdesc,
+                        "achieved": round(bdesc / (stages["descriptors"] * 1e-3) / 1e9, 1)},
+        "adjoint": {"bound": "hbm", "unit": "GB/s", "peak": 8000.0, "bytes_per_launch": bdesc + 12 * N,
+                    "achieved": round((bdesc + 12 * N) / (stages["adjoint"] * 1e-3) / 1e9, 1)},
+    }
+    for v in by_stage.values():
+        v["frac"] = round(v["achieved"] / v["peak"], 4)
     # total energy (Ha) at the start and end of the timed steps; KE sums are
     # m v^2 / 2 in kg/mol A^2/fs^2 = 1e10 J/mol
     etot = lambda e: e[0] + e[1] * 1e10 / 2625499.638
@@ -352,6 +509,9 @@ def main():
         "plain_fp32_value": None if fp32_value is None else round(fp32_value, 1),
         "speedup_vs_plain_fp32": None if fp32_value is None else round(value / fp32_value, 2),
         "serving": serving,
+        "stage_ms_eager": {k: round(v, 4) for k, v in stages.items()},
+        "pairs_4.6A": P_r, "pairs_3.1A": P_r3,
+        "roofline_by_stage": by_stage,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -124,8 +124,10 @@ int tm_mlp_f32(int M, int D, int nh, const int* hid, const float* G, int ldg, co
                const float* W, float alpha, float* y, float* dG, float* ws, void* stream);
 
 /* tm_mlp_mfma: fused BF16/FP16 MFMA path, FP32 accumulation, both nets and all
- * elements in one launch.  tiles (device int32 [ntiles][2]) = (segment, first
- * row of the tile inside the segment); seg (device int32 [nseg][4]) = (net,
+ * elements in one launch, one workgroup per entry of tiles (device int32
+ * [ntiles][2]) = (segment, first row of the tile inside the segment); entries
+ * with segment -1 are skipped (padding of an XCD-ordered table, so each XCD's
+ * L2 serves few weight blobs); seg (device int32 [nseg][4]) = (net,
  * blob index, first row in perm, rows); blobs: device array of weight-blob
  * pointers packed as tm_mlp_mfma_pack_info describes; mode 0 = bf16, 1 = fp16,
  * 2 = bf16 hi/lo split.  Writes y[net*N + atom] and dG[(net*N + atom)*ldg + f]. */

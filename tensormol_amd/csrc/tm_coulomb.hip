@@ -28,44 +28,64 @@ constexpr int C_RANGES = 512;    // 15 A stencil over ~2.3 A cells: ~230 rows (+
 
 struct CoulArgs {
   float alpha, r_sr, r_lr, zz, yy, shift, ea;     // bohr units
+  float lin;                                     // ZZ / R_lr + YY (slope of the shift)
   float rc, rc2;                                 // EE cutoff (A) and its square
-  float width;                                   // vdW poly width (bohr after scaling)
+  float inv_width;                               // 1 / vdW poly width (bohr after scaling)
   int add_ecc, has_vdw;
   float c6[TM_MAX_ELE], r0[TM_MAX_ELE];
 };
 
-__device__ __forceinline__ float min_image_f(float d, float L) {
-  return L > 0.f ? d - L * rintf(d / L) : d;
+// erfc(z) for z >= 0 given e2 = exp(-z^2): the Chebyshev fit "erfcc" of
+// Numerical Recipes (Press, Teukolsky, Vetterling, Flannery, 2nd ed., sec. 6.2),
+// erfc(z) = t exp(-z^2 + P(t)), t = 1/(1 + z/2), fractional error < 1.2e-7.
+__device__ __forceinline__ float erfc_cheb(float z, float e2) {
+  const float t = __builtin_amdgcn_rcpf(1.0f + 0.5f * z);
+  float P = 0.17087277f;
+  P = fmaf(P, t, -0.82215223f);
+  P = fmaf(P, t, 1.48851587f);
+  P = fmaf(P, t, -1.13520398f);
+  P = fmaf(P, t, 0.27886807f);
+  P = fmaf(P, t, -0.18628806f);
+  P = fmaf(P, t, 0.09678418f);
+  P = fmaf(P, t, 0.37409196f);
+  P = fmaf(P, t, 1.00002368f);
+  P = fmaf(P, t, -1.26551223f);
+  return t * e2 * __expf(P);
+}
+
+// minimum image with the reciprocal edge (inv_L = 0 for open boundaries)
+__device__ __forceinline__ float min_image_f(float d, float L, float inv_L) {
+  return d - L * rintf(d * inv_L);
 }
 
 struct PairAcc {
   double phi, gx, gy, gz, ev;
 };
 
-// energy terms of one pair (i, j) seen from centre i; d = x_j - x_i (A)
-__device__ __forceinline__ void pair_terms(const CoulArgs& a, float qi, float qj, float c6i, float r0i,
-                                           float c6j, float r0j, float dx, float dy, float dz,
-                                           float s, PairAcc& acc) {
+// energy terms of one pair (i, j) seen from centre i; d = x_j - x_i (A).
+// vc = sqrt(C6_i C6_j), vq = 6 (R0_i + R0_j)^12 (per element pair).
+__device__ __forceinline__ void pair_terms(const CoulArgs& a, float qi, float qj, float vc, float vq,
+                                           float dx, float dy, float dz, PairAcc& acc) {
   const float B1 = (float)BOHRPERA_D, B2 = (float)(BOHRPERA_D * BOHRPERA_D);
-  float rA = sqrtf(s + 1e-27f);   // the reference's regulariser (RawSymFunc.py:1330)
-  float inv_rA = 1.0f / rA;
+  const float iB1 = (float)(1.0 / BOHRPERA_D), iB2 = (float)(1.0 / (BOHRPERA_D * BOHRPERA_D));
+  // r_A = sqrt(s + 1e-27), the reference's regulariser (RawSymFunc.py:1330)
+  const float s = fmaf(dx, dx, fmaf(dy, dy, dz * dz)) + 1e-27f;
+  const float inv_rA = __builtin_amdgcn_rsqf(s);
+  const float rA = s * inv_rA;
   float dEdr = 0.f;               // dE_pair/dr_A
   if (a.add_ecc) {
-    float r = rA * B1;
-    float K, dK;
+    const float r = rA * B1, inv_r = inv_rA * iB1;
+    float K = 0.f, dK = 0.f;
     if (r > a.r_sr) {
-      if (r > a.r_lr) {
-        K = 0.f; dK = 0.f;
-      } else {
-        float ar = a.alpha * r;
-        float ec = erfcf(ar);
-        float inv = 1.0f / r;
-        K = ec * inv - a.zz + (r - a.r_lr) * (a.zz / a.r_lr + a.yy);
-        dK = -ec * inv * inv - 1.1283791671f * a.alpha * __expf(-ar * ar) * inv + a.zz / a.r_lr +
-             a.yy;
+      if (r <= a.r_lr) {
+        const float z = a.alpha * r;
+        const float e2 = __expf(-z * z);
+        const float ec = erfc_cheb(z, e2);
+        K = ec * inv_r - a.zz + (r - a.r_lr) * a.lin;
+        dK = -ec * inv_r * inv_r - 1.1283791671f * a.alpha * e2 * inv_r + a.lin;
       }
     } else {
-      float ex = __expf(r - a.r_sr);
+      const float ex = __expf(r - a.r_sr);
       K = a.ea * (ex - 1.0f) + a.shift;
       dK = a.ea * ex;
     }
@@ -73,28 +93,25 @@ __device__ __forceinline__ void pair_terms(const CoulArgs& a, float qi, float qj
     dEdr += qi * qj * dK * B1;
   }
   if (a.has_vdw) {
-    float r = rA * B2;
-    float t = r / a.width;
+    const float r = rA * B2, inv_r = inv_rA * iB2;
+    const float t = r * a.inv_width;
     float S, dS;
     if (t > 1.f) { S = 1.f; dS = 0.f; }
-    else if (t > 0.f) { S = -t * t * (2.f * t - 3.f); dS = 6.f * t * (1.f - t) / a.width; }
+    else if (t > 0.f) { S = -t * t * (2.f * t - 3.f); dS = 6.f * t * (1.f - t) * a.inv_width; }
     else { S = 0.f; dS = 0.f; }
     // C r^-6 / (1 + 6 (r/R0)^-12) written as C r^6 / (r^12 + 6 R0^12): the same
-    // function, finite for every r >= 0 in FP32
-    float C = sqrtf(c6i * c6j);
-    float R0 = r0i + r0j;
-    float r2 = r * r, r6 = r2 * r2 * r2, r12 = r6 * r6;
-    float q2 = R0 * R0, q6 = q2 * q2 * q2, q12 = 6.0f * q6 * q6;
-    float den = r12 + q12;
-    float f = C * r6 / den;
-    float df = 6.0f * C * ((r6 / r) / den) * ((q12 - r12) / den);
-    float V = -S * f;
-    float dV = -(dS * f + S * df);
+    // function, finite for every r >= 0 in FP32 (r <= 15 A BOHRPERA^2 here)
+    const float r2 = r * r, r6 = r2 * r2 * r2, r12 = r6 * r6;
+    const float iden = __builtin_amdgcn_rcpf(r12 + vq);
+    const float f = vc * r6 * iden;
+    const float df = 6.0f * vc * (r6 * inv_r) * iden * ((vq - r12) * iden);
+    const float V = -S * f;
+    const float dV = -(dS * f + S * df);
     acc.ev += 0.5 * (double)V;
     dEdr += dV * B2;
   }
   // dE/dx_i = dE/dr_A * (x_i - x_j)/r_A = -dEdr * d / r_A
-  float sc = -dEdr * inv_rA;
+  const float sc = -dEdr * inv_rA;
   acc.gx += (double)(sc * dx);
   acc.gy += (double)(sc * dy);
   acc.gz += (double)(sc * dz);
@@ -104,9 +121,10 @@ constexpr int QCAP = 128;   // compacted pair queue per wave
 
 // One wavefront per centre.  Candidates (contiguous in cell order, or the
 // molecule's atoms) are tested 64 at a time; those inside the cutoff are
-// compacted into an LDS queue and evaluated 64 at a time with every lane
-// busy.  Queue order = candidate order, so the per-lane sums and the final
-// wave reduction are formed in a fixed order.
+// compacted into an LDS queue (minimum-image separation and charge) and
+// evaluated 64 at a time with every lane busy.  Queue order = candidate
+// order, so the per-lane sums and the final wave reduction are formed in a
+// fixed order.
 __global__ void __launch_bounds__(C_WAVES * 64)
 coulomb_kernel(int N, const float4* __restrict__ xq, const int* __restrict__ elem,
                const float4* __restrict__ xqc, const int* __restrict__ elc,
@@ -116,6 +134,15 @@ coulomb_kernel(int N, const float4* __restrict__ xq, const int* __restrict__ ele
   __shared__ float4 qx[C_WAVES][QCAP];
   __shared__ int qe[C_WAVES][QCAP];
   __shared__ RangeLds<C_RANGES> rng[C_WAVES];
+  __shared__ float vdw_c[TM_MAX_ELE * TM_MAX_ELE], vdw_q[TM_MAX_ELE * TM_MAX_ELE];
+  if (threadIdx.x < TM_MAX_ELE * TM_MAX_ELE) {
+    const int ea = threadIdx.x / TM_MAX_ELE, eb = threadIdx.x % TM_MAX_ELE;
+    const float R0 = a.r0[ea] + a.r0[eb];
+    const float q2 = R0 * R0, q6 = q2 * q2 * q2;
+    vdw_c[threadIdx.x] = sqrtf(a.c6[ea] * a.c6[eb]);
+    vdw_q[threadIdx.x] = 6.0f * q6 * q6;
+  }
+  __syncthreads();
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int wi = blockIdx.x * C_WAVES + w;
   if (wi >= N) return;   // wave-level synchronisation only below
@@ -123,41 +150,40 @@ coulomb_kernel(int N, const float4* __restrict__ xq, const int* __restrict__ ele
   const int i = grid.cell_atoms != nullptr ? grid.cell_atoms[wi] : wi;
   const int cpi = grid.cell_atoms != nullptr ? wi : -1;
   const int mi = mol[i];
-  float Lx = 0.f, Ly = 0.f, Lz = 0.f;
+  float Lx = 0.f, Ly = 0.f, Lz = 0.f, iLx = 0.f, iLy = 0.f, iLz = 0.f;
   if (box != nullptr) {
     Lx = (float)box[3 * mi]; Ly = (float)box[3 * mi + 1]; Lz = (float)box[3 * mi + 2];
+    iLx = 1.0f / Lx; iLy = 1.0f / Ly; iLz = 1.0f / Lz;
   }
   const float4 pi = xq[i];
   const int ei = elem[i];
-  const float c6i = a.c6[ei], r0i = a.r0[ei];
+  const float* vci = vdw_c + ei * TM_MAX_ELE;
+  const float* vqi = vdw_q + ei * TM_MAX_ELE;
   PairAcc acc = {0.0, 0.0, 0.0, 0.0, 0.0};
   int qn = 0;   // wave-uniform queue length
   auto drain = [&](int n) {   // evaluate the first n (<= 64) queued pairs
     if (lane < n) {
-      float4 pj = qx[w][lane];
-      int ej = qe[w][lane];
-      float dx = min_image_f(pj.x - pi.x, Lx);
-      float dy = min_image_f(pj.y - pi.y, Ly);
-      float dz = min_image_f(pj.z - pi.z, Lz);
-      float s = dx * dx + dy * dy + dz * dz;
-      pair_terms(a, pi.w, pj.w, c6i, r0i, a.c6[ej], a.r0[ej], dx, dy, dz, s, acc);
+      const float4 d = qx[w][lane];
+      const int ej = qe[w][lane];
+      pair_terms(a, pi.w, d.w, vci[ej], vqi[ej], d.x, d.y, d.z, acc);
     }
   };
   auto visit = [&](int p, bool valid, int mapped) {
     bool ok = false;
-    float4 pj = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 d = make_float4(0.f, 0.f, 0.f, 0.f);
     if (valid && (mapped ? p != cpi : p != i) &&
         (!mapped || grid.molc == nullptr || grid.molc[p] == mi)) {
-      pj = mapped ? xqc[p] : xq[p];
-      float dx = min_image_f(pj.x - pi.x, Lx);
-      float dy = min_image_f(pj.y - pi.y, Ly);
-      float dz = min_image_f(pj.z - pi.z, Lz);
-      ok = dx * dx + dy * dy + dz * dz < a.rc2;
+      const float4 pj = mapped ? xqc[p] : xq[p];
+      d.x = min_image_f(pj.x - pi.x, Lx, iLx);
+      d.y = min_image_f(pj.y - pi.y, Ly, iLy);
+      d.z = min_image_f(pj.z - pi.z, Lz, iLz);
+      d.w = pj.w;
+      ok = d.x * d.x + d.y * d.y + d.z * d.z < a.rc2;
     }
     unsigned long long bal = __ballot(ok);
     int before = __popcll(bal & ((1ull << lane) - 1ull));
     if (ok) {
-      qx[w][qn + before] = pj;
+      qx[w][qn + before] = d;
       qe[w][qn + before] = mapped ? elc[p] : elem[p];
     }
     qn += __popcll(bal);
@@ -363,9 +389,10 @@ int tm_coulomb(int N, const float* xq, const int* elem, const int* mol, const in
   a.alpha = (float)consts[0]; a.r_sr = (float)consts[1]; a.r_lr = (float)consts[2];
   a.zz = (float)consts[3]; a.yy = (float)consts[4]; a.shift = (float)consts[5];
   a.ea = (float)consts[6];
+  a.lin = (float)(consts[3] / consts[2] + consts[4]);
   a.rc = (float)rc_ee;
   a.rc2 = (float)(rc_ee * rc_ee);
-  a.width = (float)width;
+  a.inv_width = (float)(1.0 / width);
   a.add_ecc = add_ecc;
   a.has_vdw = has_vdw;
   for (int e = 0; e < TM_MAX_ELE; ++e) {

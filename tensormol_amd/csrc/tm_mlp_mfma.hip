@@ -202,6 +202,7 @@ mlp_mfma_kernel(MlpArgs a) {
   const int w = tid >> 6, lane = tid & 63;
   const int tile = blockIdx.x;
   const int s = a.tiles[2 * tile], row0 = a.tiles[2 * tile + 1];
+  if (s < 0) return;   // padding of the XCD-ordered tile table (whole workgroup)
   const int net = a.seg[4 * s], blob = a.seg[4 * s + 1], first = a.seg[4 * s + 2];
   const int rows = a.seg[4 * s + 3];
   const int D = a.D, H = a.H;
@@ -481,7 +482,8 @@ int tm_mlp_mfma_pack_info(int D, int H, int nh, int mode, long* out) {
   return TM_OK;
 }
 
-// tiles: device int32 [ntiles][2] = (segment, first row); seg: device int32
+// tiles: device int32 [ntiles][2] = (segment, first row), one workgroup per
+// entry (entries with segment -1 are skipped); seg: device int32
 // [nseg][4] = (net, blob index, first row in perm, rows); blobs: device array
 // of per-(net, element) weight blob pointers (layout of tm_mlp_mfma_pack_info).
 int tm_mlp_mfma(int N, int D, int H, int ntiles, const int* tiles, const int* seg, int ldg,

@@ -32,6 +32,29 @@ def _to16(x, mode):
     return h.view(torch.int16).numpy(), h.to(torch.float32).numpy()
 
 
+NXCD = 8   # MI355X: workgroups are dispatched round-robin over 8 XCDs, each with its own L2
+
+
+def xcd_order(til):
+    """Tile table in launch order: XCD x (workgroups x, x + 8, ...) takes the
+    x-th contiguous run of the weight-blob-sorted tiles, so each XCD's L2
+    holds the weights of one or two (net, element) blobs instead of all of
+    them.  Padding entries (segment -1) make the grid a multiple of 8; their
+    workgroups exit at once."""
+    n = len(til)
+    if n == 0:
+        return til
+    bounds = (np.arange(NXCD + 1) * n) // NXCD      # near-equal contiguous runs
+    c = int(np.diff(bounds).max())
+    out = np.full((c * NXCD, 2), -1, dtype=np.int32)
+    out[:, 1] = 0
+    j = np.arange(n)
+    x = np.searchsorted(bounds, j, side="right") - 1
+    t = j - bounds[x]
+    out[t * NXCD + x] = til
+    return out
+
+
 class MFMAWeights:
     def __init__(self, engine, precision):
         if precision not in MODES:
@@ -97,6 +120,7 @@ class MFMAWeights:
                 tiles.append(np.stack([np.full_like(r0, s), r0], axis=1))
         seg = np.array(segs, dtype=np.int32).reshape(-1, 4)
         til = np.concatenate(tiles).astype(np.int32) if tiles else np.zeros((0, 2), np.int32)
+        til = xcd_order(til)
         if upload is None:
             dev = self.engine.device
             upload = lambda a, out=None: torch.from_numpy(a).to(dev)
