@@ -468,6 +468,13 @@ def main():
                 "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
                 "frac": round(achieved / peak, 4), "traffic": None,
                 "flops_per_launch": flops, "avg_launch_ms": round(mlp_avg_ms, 4)}
+    # HBM bytes per launch come from two rocprofv3 counter runs (FETCH_SIZE,
+    # WRITE_SIZE) calibrated on a plain copy; they cannot be counted live here
+    tpath = os.path.join(ROOT, "profiles", "water_box_12288_bf16_hbm_traffic.json")
+    if (os.path.exists(tpath) and args.precision == "bf16" and N == 12288):
+        tr = json.load(open(tpath))
+        roofline["traffic"] = tr["kernels"]["mlp_mfma_kernel"]["bytes"]
+        roofline["traffic_source"] = tr["source"]
     bdesc = descriptor_bytes(N, eng.D, P_r, P_r3)
     by_stage = {
         "mlp": dict(roofline, achieved=round(achieved, 2)),
