@@ -261,6 +261,82 @@ def run_sphere(args):
     }
 
 
+def run_train(args):
+    """Force-matching training steps on 256 C/H/N/O chain molecules of 20-60
+    atoms: labels from an FP64 teacher BP network (the oracle), student of the
+    same 768-512-512-512-1 shape, loss MSE(E) + 0.5 MSE(F), Adam."""
+    import torch
+    from oracle import oracle as O
+    from tensormol_amd import workloads as W
+    from tensormol_amd.model import BPEEModel
+    from tensormol_amd.train import ForceMatchingTrainer
+    log = lambda m: print(f"[train] {m}", file=sys.stderr, flush=True)
+    eles = [1, 6, 7, 8]
+    rng = np.random.default_rng(4)
+    sizes = rng.integers(20, 61, size=args.train_mols)
+    Z, x, ms = W.pack(W.chain_set(sizes, seed=4))
+    N, B = len(Z), len(ms) - 1
+    sp = O.SymParams(eles)
+    t0 = time.perf_counter()
+    lists = O.build_lists(x, Z, ms, sp)
+    teacher = BPEEModel(eles, dsf_alpha=0.2, seed=11)
+    student = BPEEModel(eles, dsf_alpha=0.2, seed=12)
+    xt = torch.tensor(x, requires_grad=True)
+    Et = O.bp_energies(xt, Z, ms, sp, O.layers_torch(teacher.energy_nets), teacher.alpha, lists)
+    (g,) = torch.autograd.grad(Et.sum(), xt)
+    E_ref, F_ref = Et.detach().numpy(), -g.numpy()
+    log(f"{B} molecules, {N} atoms; FP64 teacher labels in {time.perf_counter() - t0:.1f} s")
+    out = {}
+    dev = torch.device("cuda")
+    pos = torch.tensor(x, dtype=torch.float64, device=dev)
+    for dtype in ("bf16", "fp32"):
+        tr = ForceMatchingTrainer(student, mlp_dtype=dtype, lr=1e-4)
+        tb = tr.batch(Z, ms, E_ref, F_ref)
+        for _ in range(args.warmup):
+            tr.step(tb, pos)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        losses = [tr.step(tb, pos) for _ in range(args.train_steps)]
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        losses = [float(v.detach()) for v in losses]
+        out[dtype] = dict(ms_per_step=1e3 * wall / args.train_steps,
+                          atoms_per_s=N * args.train_steps / wall, loss_first=losses[0],
+                          loss_last=losses[-1])
+        log(f"{dtype}: {out[dtype]['ms_per_step']:.2f} ms/step, {out[dtype]['atoms_per_s']:.4g} "
+            f"atoms/s, loss {losses[0]:.4e} -> {losses[-1]:.4e}")
+    # gradient check on the first 16 molecules against the FP64 oracle's double backward
+    k = 16
+    Zs, xs, mss = Z[:ms[k]], x[:ms[k]], ms[:k + 1]
+    ls = O.build_lists(xs, Zs, mss, sp)
+    w_e, w_f = np.full(k, 1.0 / k), np.full(k, 0.5 / (3 * len(Zs)))
+    lt = O.layers_torch(student.energy_nets, requires_grad=True)
+    L, _, _ = O.force_matching_loss(xs, Zs, mss, sp, lt, student.alpha, E_ref[:k], F_ref[:ms[k]],
+                                    w_e, w_f, ls)
+    gref = torch.autograd.grad(L, [t for e in eles for (Wt, bt) in lt[e] for t in (Wt, bt)])
+    rv = np.concatenate([t.numpy().ravel() for t in gref])
+    errs = {}
+    for dtype in ("bf16", "fp32"):
+        tr = ForceMatchingTrainer(student, mlp_dtype=dtype)
+        tb = tr.batch(Zs, mss, E_ref[:k], F_ref[:ms[k]], w_e, w_f)
+        _, gg = tr.gradients(tb, torch.tensor(xs, dtype=torch.float64, device=dev))
+        gv = np.concatenate([t.double().cpu().numpy().ravel() for t in gg])
+        errs[dtype] = float(np.linalg.norm(gv - rv) / np.linalg.norm(rv))
+    log(f"weight-gradient rel err vs FP64 oracle (16 molecules): {errs}")
+    return {
+        "metric": "force-matching training atoms/s (E+F loss, weight gradient by double backward)",
+        "value": round(out["bf16"]["atoms_per_s"], 1), "unit": "atoms/s", "n_gpus": 1,
+        "steps": args.train_steps, "warmup": args.warmup,
+        "ms_per_step": round(out["bf16"]["ms_per_step"], 3), "higher_is_better": True,
+        "dtype": "bf16 GEMM operands (rounded), fp32 products and accumulation", "data": "synthetic",
+        "config": {"workload": f"{B} C/H/N/O chain molecules of 20-60 atoms ({N} atoms), "
+                               "BP nets 768-512-512-512-1, loss MSE(E) + 0.5 MSE(F), Adam"},
+        "fp32_mlp": {k_: round(v, 4) for k_, v in out["fp32"].items()},
+        "loss_first_last_bf16": [out["bf16"]["loss_first"], out["bf16"]["loss_last"]],
+        "weight_grad_rel_err_vs_fp64": errs,
+    }
+
+
 def cpu_baseline(Z, x, box, model):
     """The FP64 oracle on this host: one full E+F of the box (neighbour lists
     included), which is also the frame-0 reference for the force error."""
@@ -378,6 +454,10 @@ def main():
     ap.add_argument("--serve-only", action="store_true", help="print the serving record only")
     ap.add_argument("--calib-copy-mib", type=int, default=256,
                     help="size of the calibration copy run before the MD (0: none)")
+    ap.add_argument("--train", action="store_true",
+                    help="force-matching training steps on 256 chain molecules instead (own JSON line)")
+    ap.add_argument("--train-mols", type=int, default=256)
+    ap.add_argument("--train-steps", type=int, default=20)
     ap.add_argument("--sphere", action="store_true",
                     help="single-point E+F of the 24,000-atom C/H/N/O sphere instead (own JSON line)")
     ap.add_argument("--sphere-atoms", type=int, default=24000)
@@ -390,6 +470,9 @@ def main():
         sys.exit(2)
     if args.sphere:
         print(json.dumps(run_sphere(args)), flush=True)
+        return
+    if args.train:
+        print(json.dumps(run_train(args)), flush=True)
         return
     stream = None
     if args.serve_requests > 0:

@@ -108,6 +108,19 @@ int tm_sym_adjoint(int N, const double* pos, const double* box, const int* mol, 
 /* grad_a = sum_n (w[rev(a,n)] - w[(a,n)]) + extra_a (Ha/A);
  * force = -scale * grad (scale = JOULEPERHARTREE gives J/mol/A as returned by
  * TFMolManage.EvalBPDirectEEUpdateSingle, TFMolManage.py:1320). */
+/* Descriptor tangent V[i, f] = sum_j dG[i, f]/d(x_j - x_i) . (du_j - du_i) for
+ * per-atom displacements du (device double [N,3]); V float [N, ldg].  The
+ * transpose of tm_sym_adjoint.  Replaces the second-order part of the
+ * tf.gradients graph that the force term of the training loss differentiates
+ * through (loss_op_EandG, TFMolInstanceDirect.py:4889-4903, with the energy
+ * gradient of :5761): dLoss/dW = -d/dW sum_i (dE_i/dG_i)(W) . V_i for
+ * du = dLoss/dF. */
+int tm_sym_jvp(int N, const double* pos, const double* box, const int* mol, const int* elem,
+               const int* nbr, const int* ncnt, const int* rseg, const int* ang, const int* acnt,
+               const int* aseg, int ldr, int lda, int nele, int nr, float eta, float zeta,
+               float rr, float ra, const float* rs_r, const float* rs_a, const float* thetas,
+               const double* du, float* V, int ldg, void* stream);
+
 int tm_force_gather(int N, int ldr, const int* ncnt, const int* rev, const float* wslot,
                     const double* extra, double scale, double* grad, double* force,
                     void* stream);

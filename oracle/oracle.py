@@ -22,6 +22,9 @@ swapped index fails):
   vdw kernel                  closed form on one pair
   forces                      central finite differences; sum F = 0; torque 0
   md steps                    energy conservation, time reversibility
+  descriptor_jvp              central finite differences of the descriptors
+  force_matching_loss         central finite differences in the weights;
+                              zero at the teacher's own weights
 TensorFlow's own summation order and last-ulp acos/erfc cannot be pinned
 without TensorFlow (absent here); they sit far below the stated tolerances.
 """
@@ -431,6 +434,70 @@ def evaluate(x_np, z, mol_start, sp, energy_nets, charge_nets, alpha=1.0, dsf_al
 def _t(layers):
     return [(torch.as_tensor(W, dtype=torch.float64), torch.as_tensor(b, dtype=torch.float64))
             for (W, b) in layers]
+
+
+# -------------------------------------------------------------- training
+def descriptors(x, z, sp, lists, box=None):
+    """G = concat(radial, angular) for packed atoms (RawSymFunc.py:2223-2265);
+    x torch f64 (N, 3), differentiable."""
+    N = len(z)
+    return torch.cat([sym_radial(x, lists.pi, lists.pj, lists.pl, N, sp, box),
+                      sym_angular(x, lists.ti, lists.tj, lists.tk, lists.tp, N, sp, box)], dim=1)
+
+
+def bp_energies(x, z, mol_start, sp, layers, alpha, lists, box=None):
+    """Per-molecule BP energy sum_i eps_i (energy_inference,
+    TFMolInstanceDirect.py:5164-5212, without the electrostatic terms);
+    layers: element -> [(W, b), ...] torch f64 tensors (may require grad)."""
+    z = np.asarray(z)
+    B = len(mol_start) - 1
+    G = descriptors(x, z, sp, lists, box)
+    eps = torch.zeros(len(z), dtype=torch.float64)
+    for e in sp.eles:
+        sel = torch.as_tensor(np.nonzero(z == e)[0])
+        if len(sel):
+            eps = eps.index_put((sel,), mlp(G[sel], layers[e], alpha))
+    molt = torch.as_tensor(np.repeat(np.arange(B), np.diff(mol_start)))
+    return torch.zeros(B, dtype=torch.float64).index_add(0, molt, eps)
+
+
+def descriptor_jvp(x_np, z, sp, du, lists, box=None):
+    """dG/dx . du (N, D) in float64: the forward-mode derivative of the
+    descriptors along per-atom displacements du (N, 3)."""
+    x = torch.tensor(x_np, dtype=torch.float64)
+    v = torch.as_tensor(np.asarray(du, dtype=np.float64))
+    _, jv = torch.autograd.functional.jvp(lambda y: descriptors(y, z, sp, lists, box), (x,), (v,))
+    return jv.numpy()
+
+
+def force_matching_loss(x_np, z, mol_start, sp, layers, alpha, E_ref, F_ref, w_e, w_f, lists):
+    """L = sum_b w_e[b] (E_b - E*_b)^2 + sum_i w_f[b(i)] |F_i - F*_i|^2 with
+    F = -dE/dx (Ha/A), built with create_graph so dL/dW exists.
+
+    loss_op_EandG (TFMolInstanceDirect.py:4889-4903) is this loss with
+    w_e = EnergyScalar/2 (maxatom/natom_b)^2 and w_f = GradScalar/2
+    (maxatom/natom_b)^2 (its residuals are energy gradients, -(F - F*));
+    mean squared errors, L = MSE(E) + 0.5 MSE(F), take w_e = 1/B and
+    w_f = 0.5/(3N).  Returns (L, E, F) as torch f64."""
+    x = torch.tensor(x_np, dtype=torch.float64, requires_grad=True)
+    E = bp_energies(x, z, mol_start, sp, layers, alpha, lists)
+    (g,) = torch.autograd.grad(E.sum(), x, create_graph=True)
+    F = -g
+    B = len(mol_start) - 1
+    molt = torch.as_tensor(np.repeat(np.arange(B), np.diff(mol_start)))
+    w_e = torch.as_tensor(np.asarray(w_e, dtype=np.float64))
+    w_f = torch.as_tensor(np.asarray(w_f, dtype=np.float64))
+    E_ref = torch.as_tensor(np.asarray(E_ref, dtype=np.float64))
+    F_ref = torch.as_tensor(np.asarray(F_ref, dtype=np.float64))
+    L = (w_e * (E - E_ref) ** 2).sum() + (w_f[molt][:, None] * (F - F_ref) ** 2).sum()
+    return L, E, F
+
+
+def layers_torch(layers, requires_grad=False):
+    """element -> [(W, b)] numpy -> torch f64 leaves."""
+    return {e: [(torch.tensor(np.asarray(W, dtype=np.float64), requires_grad=requires_grad),
+                 torch.tensor(np.asarray(b, dtype=np.float64), requires_grad=requires_grad))
+                for (W, b) in ls] for e, ls in layers.items()}
 
 
 # ------------------------------------------------------------------- MD

@@ -29,6 +29,7 @@ SIGNATURES = {
     "tm_sym_forward": [I, P, P, P, P, P, P, P, P, P, P, I, I, I, I, F, F, F, F, P, P, P, P, I, P],
     "tm_sym_adjoint": [I, P, P, P, P, P, P, P, P, P, P, I, I, I, I, F, F, F, F, P, P, P, P, P, P,
                        I, P, P],
+    "tm_sym_jvp": [I, P, P, P, P, P, P, P, P, P, P, I, I, I, I, F, F, F, F, P, P, P, P, P, I, P],
     "tm_force_gather": [I, I, P, P, P, P, D, P, P, P],
     "tm_mlp_f32": [I, I, I, P, P, I, P, P, F, P, P, P, P],
     "tm_mol_sum_f32": [I, P, I, P, I, P, I, P, I, P],

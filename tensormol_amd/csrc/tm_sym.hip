@@ -394,6 +394,152 @@ __global__ void force_gather_kernel(int N, int ldr, const int* __restrict__ ncnt
   }
 }
 
+// Tangent (forward-mode derivative) of the descriptor along per-atom
+// displacements du: V[i, f] = sum_j dG[i, f]/d(x_j - x_i) . (du_j - du_i).
+// It is the transpose of the adjoint above (same partial derivatives, the
+// same guard where sin(theta) vanishes), contracted over neighbours instead
+// of features; the lane layout and the write-once order are the forward's.
+// Used for the weight gradient of a force loss: with u = dLoss/dF,
+// dLoss/dW = -d/dW sum_i (dE_i/dG_i)(W) . V_i.
+struct JvpLds {
+  float4 t[SYM_WAVES][MAX_ROW];     // (du_j - du_i, dr_j = d_j . (du_j - du_i) / r_j)
+  float dfc[SYM_WAVES][MAX_ROW];    // d fc_r / dr
+  float dfa[SYM_WAVES][MAX_ROW];    // d fc_a / dr
+};
+
+__global__ void __launch_bounds__(SYM_WAVES * 64)
+sym_jvp_kernel(int N, const double* __restrict__ pos, const double* __restrict__ box,
+               const int* __restrict__ mol, const int* __restrict__ elem,
+               const int* __restrict__ nbr, const int* __restrict__ ncnt,
+               const int* __restrict__ rseg, const int* __restrict__ ang,
+               const int* __restrict__ acnt, const int* __restrict__ aseg, int ldr, int lda,
+               SymArgs a, const double* __restrict__ du, float* __restrict__ V, int ldg) {
+  __shared__ RowLds L;
+  __shared__ JvpLds T;
+  __shared__ float tc[SYM_WAVES][64], tsn[SYM_WAVES][64], tcs[SYM_WAVES][64];
+  __shared__ float tf[SYM_WAVES][64], tdf[SYM_WAVES][64], tdc[SYM_WAVES][64], tdr[SYM_WAVES][64];
+  __shared__ float tR[SYM_WAVES][NA][64], tRp[SYM_WAVES][NA][64];
+  __shared__ int tp[SYM_WAVES][64];
+  __shared__ int segs[SYM_WAVES][2][TM_MAX_ELE + 1];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = blockIdx.x * SYM_WAVES + w;
+  if (i >= N) return;   // only wave-level synchronisation below
+  const int n = min(ncnt[i], MAX_ROW);
+  if (lane <= a.nele) {
+    segs[w][0][lane] = min(rseg[(long)i * (a.nele + 1) + lane], n);
+    segs[w][1][lane] = aseg[(long)i * (a.nele + 1) + lane];
+  }
+  load_row(i, w, lane, pos, box, mol, elem, nbr, n, ldr, a, L);
+  const float kr = PI_LIT / a.rr, ka = PI_LIT / a.ra;
+  {
+    const double uxi = du[3 * i], uyi = du[3 * i + 1], uzi = du[3 * i + 2];
+    __syncwarp();
+    for (int q = lane; q < n; q += 64) {
+      const int j = nbr[(long)i * ldr + q];
+      const float tx = (float)(du[3 * j] - uxi), ty = (float)(du[3 * j + 1] - uyi);
+      const float tz = (float)(du[3 * j + 2] - uzi);
+      const float4 d = L.d[w][q];
+      T.t[w][q] = make_float4(tx, ty, tz, (d.x * tx + d.y * ty + d.z * tz) / d.w);
+      T.dfc[w][q] = -0.5f * kr * __sinf(kr * d.w);
+      T.dfa[w][q] = -0.5f * ka * __sinf(ka * d.w);
+    }
+  }
+  __syncwarp();
+  float* Vi = V + (long)i * ldg;
+  // ---------------- radial: lane = (half h, shell s)
+  {
+    const int h = lane >> 5, s = lane & 31;
+    for (int e = 0; e < a.nele; ++e) {
+      int lo = segs[w][0][e], hi = segs[w][0][e + 1];
+      for (int s0 = 0; s0 < a.nr; s0 += 32) {
+        int ss = s0 + s;
+        float rs = ss < a.nr ? a.rs_r[ss] : 0.f;
+        float acc = 0.f;
+        for (int q = lo + h; q < hi; q += 2) {
+          float r = L.d[w][q].w;
+          float x = r - rs;
+          acc += __expf(-a.eta * x * x) * (T.dfc[w][q] - 2.0f * a.eta * x * L.fc[w][q]) *
+                 T.t[w][q].w;
+        }
+        acc += __shfl_xor(acc, 32, 64);
+        if (h == 0 && ss < a.nr) Vi[e * a.nr + ss] = acc;
+      }
+    }
+  }
+  // ---------------- angular: phase A lane = triple, phase B lane = (t, s)
+  const int m = min(acnt[i], MAX_ANG);
+  const int* arow = ang + (long)i * lda;
+  const int ntrip = m * (m - 1) / 2;
+  const int off = a.nele * a.nr;
+  const int ot = lane >> 3, os = lane & 7;
+  const float ct = a.cos_t[ot], st = a.sin_t[ot];
+  int next_p = 0, cur_p = -1;
+  float acc = 0.f;
+  for (int base = 0; base < ntrip; base += 64) {
+    int t = base + lane;
+    if (t < ntrip) {
+      int ja, kb, p;
+      decode_triple(t, segs[w][1], a.nele, ja, kb, p);
+      int qj = arow[ja], qk = arow[kb];
+      float4 dj = L.d[w][qj], dk = L.d[w][qk];
+      float4 uj = T.t[w][qj], uk = T.t[w][qk];
+      float rj = dj.w, rk = dk.w;
+      float inv = 1.0f / (rj * rk);
+      float c = (dj.x * dk.x + dj.y * dk.y + dj.z * dk.z) * inv;
+      c = fminf(fmaxf(c, -1.0f), 1.0f);
+      float sn = sqrtf(fmaxf(1.0f - c * c, 0.0f));
+      float dc = (uj.x * dk.x + uj.y * dk.y + uj.z * dk.z + dj.x * uk.x + dj.y * uk.y +
+                  dj.z * uk.z) * inv - c * (uj.w / rj + uk.w / rk);
+      float fj = L.fa[w][qj], fk = L.fa[w][qk];
+      float ravg = 0.5f * (rj + rk);
+      tc[w][lane] = c;
+      tsn[w][lane] = sn;
+      tcs[w][lane] = sn > 1e-6f ? c / sn : 0.f;
+      tf[w][lane] = fj * fk;
+      tdf[w][lane] = T.dfa[w][qj] * uj.w * fk + fj * T.dfa[w][qk] * uk.w;
+      tdc[w][lane] = dc;
+      tdr[w][lane] = 0.5f * (uj.w + uk.w);
+      tp[w][lane] = p;
+#pragma unroll
+      for (int s = 0; s < NA; ++s) {
+        float x = ravg - a.rs_a[s];
+        float R = __expf(-a.eta * x * x);
+        tR[w][s][lane] = R;
+        tRp[w][s][lane] = -2.0f * a.eta * x * R;
+      }
+    }
+    __syncwarp();
+    int cnt = min(64, ntrip - base);
+    for (int u = 0; u < cnt; ++u) {
+      int p = tp[w][u];
+      if (p != cur_p) {
+        if (cur_p >= 0) {
+          for (; next_p < cur_p; ++next_p) Vi[off + next_p * 64 + lane] = 0.f;
+          Vi[off + cur_p * 64 + lane] = acc;
+          next_p = cur_p + 1;
+        }
+        cur_p = p;
+        acc = 0.f;
+      }
+      float v = tc[w][u] * ct + tsn[w][u] * st;
+      float b = 1.0f + v;
+      float Az = powz(b, a);
+      float dAz = b > 0.f ? a.zeta * Az / b : 0.f;
+      float dv = ct - tcs[w][u] * st;
+      float R = tR[w][os][u], F = tf[w][u];
+      acc += a.two_1mz * (dAz * dv * tdc[w][u] * R * F +
+                          Az * (tRp[w][os][u] * tdr[w][u] * F + R * tdf[w][u]));
+    }
+    __syncwarp();
+  }
+  if (cur_p >= 0) {
+    for (; next_p < cur_p; ++next_p) Vi[off + next_p * 64 + lane] = 0.f;
+    Vi[off + cur_p * 64 + lane] = acc;
+    next_p = cur_p + 1;
+  }
+  for (; next_p < a.nelep; ++next_p) Vi[off + next_p * 64 + lane] = 0.f;
+}
+
 int fill_args(SymArgs& a, int nele, int nr, float eta, float zeta, float rr, float ra,
               const float* rs_r, const float* rs_a, const float* thetas) {
   TM_REQUIRE(nele > 0 && nele <= TM_MAX_ELE, "sym: bad nele %d", nele);
@@ -457,6 +603,25 @@ int tm_sym_adjoint(int N, const double* pos, const double* box, const int* mol, 
   sym_adj_kernel<<<tm_div_up(N, SYM_WAVES), SYM_WAVES * 64, sizeof(float) * SYM_WAVES * a.D, s>>>(
       N, pos, box, mol, elem, nbr, ncnt, rseg, ang, acnt, aseg, ldr, lda, a, gE, gQ, u, ldg,
       (float4*)wslot);
+  TM_LAUNCH_CHECK();
+  return TM_OK;
+}
+
+int tm_sym_jvp(int N, const double* pos, const double* box, const int* mol, const int* elem,
+               const int* nbr, const int* ncnt, const int* rseg, const int* ang, const int* acnt,
+               const int* aseg, int ldr, int lda, int nele, int nr, float eta, float zeta,
+               float rr, float ra, const float* rs_r, const float* rs_a, const float* thetas,
+               const double* du, float* V, int ldg, void* stream) {
+  SymArgs a;
+  int st = fill_args(a, nele, nr, eta, zeta, rr, ra, rs_r, rs_a, thetas);
+  if (st) return st;
+  TM_REQUIRE(ldg >= a.D, "tm_sym_jvp: ldg %d < D %d", ldg, a.D);
+  TM_REQUIRE(ldr <= MAX_ROW && lda <= MAX_ANG, "tm_sym_jvp: strides %d/%d above %d/%d", ldr, lda,
+             MAX_ROW, MAX_ANG);
+  if (N == 0) return TM_OK;
+  hipStream_t s = (hipStream_t)stream;
+  sym_jvp_kernel<<<tm_div_up(N, SYM_WAVES), SYM_WAVES * 64, 0, s>>>(
+      N, pos, box, mol, elem, nbr, ncnt, rseg, ang, acnt, aseg, ldr, lda, a, du, V, ldg);
   TM_LAUNCH_CHECK();
   return TM_OK;
 }

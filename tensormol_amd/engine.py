@@ -353,6 +353,32 @@ class BPEEEngine:
         L.tm_force_gather(ws.N, self.ldr, _P(ws.ncnt), _P(ws.rev), _P(ws.wslot), _P(ws.gradc),
                           JOULEPERHARTREE, _P(ws.grad), _P(ws.force), st)
 
+    def descriptor_jvp(self, ws, pos, du, out):
+        """out[:N, :D] = dG/dx . du (float32) for per-atom displacements du
+        (cuda float64 [N, 3]); lists and positions of the last build."""
+        sym = self.model.sym
+        assert du.dtype == torch.float64 and du.is_contiguous() and du.shape[0] >= ws.N
+        assert out.dtype == torch.float32 and out.is_contiguous() and out.shape[1] == self.D
+        self.lib.tm_sym_jvp(ws.N, _P(pos), _P(ws.box), _P(ws.mol), _P(ws.elem), _P(ws.nbr),
+                            _P(ws.ncnt), _P(ws.rseg), _P(ws.ang), _P(ws.acnt), _P(ws.aseg),
+                            self.ldr, self.lda, self.nele, sym.nr_r, sym.eta, sym.zeta, sym.rr,
+                            sym.ra, _lib.fptr(self.sym_rs_r), _lib.fptr(self.sym_rs_a),
+                            _lib.fptr(self.sym_th), _P(du), _P(out), self.D, self._stream())
+
+    def descriptor_vjp(self, ws, pos, gE, grad):
+        """grad[:N] = (dG/dx)^T gE: the per-atom gradient of sum_i gE_i . G_i
+        (gE cuda float32 [N, D]); the BP-only adjoint (no charge branch)."""
+        L, st = self.lib, self._stream()
+        sym = self.model.sym
+        assert gE.dtype == torch.float32 and gE.is_contiguous() and gE.shape[1] == self.D
+        L.tm_sym_adjoint(ws.N, _P(pos), _P(ws.box), _P(ws.mol), _P(ws.elem), _P(ws.nbr),
+                         _P(ws.ncnt), _P(ws.rseg), _P(ws.ang), _P(ws.acnt), _P(ws.aseg), self.ldr,
+                         self.lda, self.nele, sym.nr_r, sym.eta, sym.zeta, sym.rr, sym.ra,
+                         _lib.fptr(self.sym_rs_r), _lib.fptr(self.sym_rs_a), _lib.fptr(self.sym_th),
+                         _P(gE), None, None, self.D, _P(ws.wslot), st)
+        L.tm_force_gather(ws.N, self.ldr, _P(ws.ncnt), _P(ws.rev), _P(ws.wslot), None, 1.0,
+                          _P(grad), None, st)
+
     def reductions(self, ws):
         L, st = self.lib, self._stream()
         L.tm_mol_sum_f32(ws.B, _P(ws.mol_start), 1, _P(ws.eps), 1, _P(ws.Ebp), 0, _P(ws.red), ws.N, st)
