@@ -34,7 +34,7 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int NW = 8;       // waves per workgroup
 constexpr int FT = 4;       // 16-feature tiles per wave (64 features)
-constexpr int KC = 64;      // descriptor chunk staged per step
+constexpr int KC = 64;      // the descriptor length must be a multiple of KC
 constexpr int HMAX = 512;   // widest hidden layer
 
 struct MlpArgs {
@@ -106,21 +106,54 @@ struct Ty<1> {
 template <>
 struct Ty<2> : Ty<0> {};
 
-__device__ __forceinline__ float softplus_a(float z, float alpha) {
-  float x = alpha * z;
-  return (fmaxf(x, 0.f) + __logf(1.0f + __expf(-fabsf(x)))) / alpha;
+// act(z) = log(1 + e^{a z}) / a and act'(z) = sigmoid(a z), sharing e = e^{-|a z|}:
+// act = (max(a z, 0) + log(1 + e)) / a, sigmoid = 1/(1+e) for a z >= 0, e/(1+e) below
+__device__ __forceinline__ void act_pair(float z, float alpha, float inv_alpha, float& act,
+                                         float& dact) {
+  const float x = alpha * z;
+  const float e = __expf(-fabsf(x));
+  const float r = __builtin_amdgcn_rcpf(1.0f + e);
+  act = (fmaxf(x, 0.f) + __logf(1.0f + e)) * inv_alpha;
+  dact = x >= 0.f ? r : e * r;
 }
-__device__ __forceinline__ float sigmoid_a(float z, float alpha) {
-  return 1.0f / (1.0f + __expf(-alpha * z));
-}
+
+// Activation derivatives of a wave's features, kept from the forward to the
+// backward pass: packed FP16 pairs in the BF16 / FP16 modes (the backward
+// operands are rounded to 8 / 11 bits anyway), FP32 in the split mode.
+typedef __fp16 half2v __attribute__((ext_vector_type(2)));
+template <int MODE, int NFT, int AT>
+struct DerivRegs {
+  half2v h[NFT][AT][2];   // two derivatives per register
+  __device__ __forceinline__ void set4(int ft, int at, const float d[4]) {
+    h[ft][at][0] = __builtin_amdgcn_cvt_pkrtz(d[0], d[1]);
+    h[ft][at][1] = __builtin_amdgcn_cvt_pkrtz(d[2], d[3]);
+  }
+  __device__ __forceinline__ float get(int ft, int at, int i) const {
+    return (float)h[ft][at][i >> 1][i & 1];
+  }
+};
+template <int NFT, int AT>
+struct DerivRegs<2, NFT, AT> {
+  float h[NFT][AT][4];
+  __device__ __forceinline__ void set4(int ft, int at, const float d[4]) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) h[ft][at][i] = d[i];
+  }
+  __device__ __forceinline__ float get(int ft, int at, int i) const { return h[ft][at][i]; }
+};
 
 template <int MODE, int MT>
 struct Smem {
   static constexpr int PL = MODE == 2 ? 2 : 1;
   static constexpr int SH = HMAX + 8;                // activation row stride (elements)
-  static constexpr int SC = KC + 8;                  // chunk row stride
-  unsigned short X[PL][MT][SH];
-  unsigned short Y[PL][MT][SH];                      // also the descriptor chunk buffer
+  union {
+    struct {
+      unsigned short X[PL][MT][SH];
+      unsigned short Y[PL][MT][SH];
+    };
+    // layer 1: the whole descriptor tile, row stride D + 8 <= 2 SH
+    unsigned short Gt[2 * PL * MT * SH];
+  };
   float epart[NW][MT];
   int atoms[MT];
 };
@@ -147,20 +180,23 @@ __device__ __forceinline__ void put4(unsigned short (*buf)[MT][HMAX + 8], int at
   }
 }
 
-// acc[ft][at] += sum over K of W-fragments (rows tiles rt0 + ft) x B from LDS buffer
-template <int MODE, int MT, int NFT, int STRIDE>
+// acc[ft][at] += sum over K of W-fragments (rows tiles rt0 + ft) x B from LDS buffer.
+// The weight fragments of k-step ks+1 are in flight while the MFMAs of ks run:
+// two register sets alternate (the loop is unrolled by two; nks is even), and
+// scheduling barriers keep the loads ahead of the MFMAs that do not need them.
+template <int MODE, int MT, int NFT>
 __device__ __forceinline__ void gemm_tiles(floatx4 (&acc)[NFT][MT / 16],
                                            const unsigned short* __restrict__ wf, long plane_elems,
                                            int nks, const int* rtile, const unsigned short* bbase,
-                                           long bplane, int lane) {
+                                           long bplane, int lane, int STRIDE) {
   constexpr int AT = MT / 16;
   const int brow = lane & 15;
   const int bk = (lane >> 4) * 8;
-  for (int ks = 0; ks < nks; ++ks) {
-    uint4 a[NFT], b[AT];
-#pragma unroll
-    for (int ft = 0; ft < NFT; ++ft)
-      a[ft] = *reinterpret_cast<const uint4*>(wf + ((long)(rtile[ft] * nks + ks) * 64 + lane) * 8);
+  auto afrag = [&](int ft, int ks, long plane) {
+    return *reinterpret_cast<const uint4*>(wf + plane + ((long)(rtile[ft] * nks + ks) * 64 + lane) * 8);
+  };
+  auto step = [&](const uint4 (&a)[NFT], int ks) {
+    uint4 b[AT];
 #pragma unroll
     for (int at = 0; at < AT; ++at)
       b[at] = *reinterpret_cast<const uint4*>(bbase + (long)(at * 16 + brow) * STRIDE + ks * 32 + bk);
@@ -171,9 +207,7 @@ __device__ __forceinline__ void gemm_tiles(floatx4 (&acc)[NFT][MT / 16],
     if (MODE == 2) {
       uint4 al[NFT], bl[AT];
 #pragma unroll
-      for (int ft = 0; ft < NFT; ++ft)
-        al[ft] = *reinterpret_cast<const uint4*>(wf + plane_elems +
-                                                 ((long)(rtile[ft] * nks + ks) * 64 + lane) * 8);
+      for (int ft = 0; ft < NFT; ++ft) al[ft] = afrag(ft, ks, plane_elems);
 #pragma unroll
       for (int at = 0; at < AT; ++at)
         bl[at] = *reinterpret_cast<const uint4*>(bbase + bplane + (long)(at * 16 + brow) * STRIDE +
@@ -186,6 +220,23 @@ __device__ __forceinline__ void gemm_tiles(floatx4 (&acc)[NFT][MT / 16],
           acc[ft][at] = Ty<MODE>::mma(a[ft], bl[at], acc[ft][at]);
         }
     }
+  };
+  uint4 a0[NFT], a1[NFT];
+#pragma unroll
+  for (int ft = 0; ft < NFT; ++ft) a0[ft] = afrag(ft, 0, 0);
+  for (int ks = 0; ks < nks; ks += 2) {
+#pragma unroll
+    for (int ft = 0; ft < NFT; ++ft) a1[ft] = afrag(ft, ks + 1, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    step(a0, ks);
+    __builtin_amdgcn_sched_barrier(0);
+    if (ks + 2 < nks) {
+#pragma unroll
+      for (int ft = 0; ft < NFT; ++ft) a0[ft] = afrag(ft, ks + 2, 0);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    step(a1, ks + 1);
+    __builtin_amdgcn_sched_barrier(0);
   }
 }
 
@@ -195,7 +246,6 @@ mlp_mfma_kernel(MlpArgs a) {
   constexpr int AT = MT / 16;
   constexpr int PL = MODE == 2 ? 2 : 1;
   constexpr int SH = HMAX + 8;
-  constexpr int SC = KC + 8;
   typedef Smem<MODE, MT> S;
   __shared__ S sm;
   const int tid = threadIdx.x;
@@ -225,7 +275,8 @@ mlp_mfma_kernel(MlpArgs a) {
   const int fq = (lane >> 4) * 4;   // first of the 4 features this lane holds in a C tile
   const int ca = lane & 15;         // atom column of this lane in a C tile
   // activation derivatives of layers 1 and 2 for this wave's features
-  float s1[FT][AT][4], s2[FT][AT][4];
+  DerivRegs<MODE, FT, AT> s1, s2;
+  const float inv_alpha = 1.0f / a.alpha;
   floatx4 acc[FT][AT];
 
   // ------------------------------------------------------------ layer 1
@@ -234,16 +285,17 @@ mlp_mfma_kernel(MlpArgs a) {
 #pragma unroll
     for (int at = 0; at < AT; ++at) acc[ft][at] = floatx4{0.f, 0.f, 0.f, 0.f};
   const long p1 = frag_elems(H / 16, D / 32);
-  for (int kc = 0; kc < D; kc += KC) {
+  {
+    // stage the descriptor tile G[atom][0 .. D) as bf16 (hi/lo) with row stride SG
+    const int SG = D + 8;
     __syncthreads();
-    // stage G[atom][kc .. kc+KC) as bf16 (hi/lo) into Y (row stride SC)
-    for (int e = tid; e < MT * KC / 4; e += NW * 64) {
-      int r = e / (KC / 4), c4 = (e % (KC / 4)) * 4;
+    for (int e = tid; e < MT * D / 4; e += NW * 64) {
+      int r = e / (D / 4), c4 = (e % (D / 4)) * 4;
       int atom = sm.atoms[r];
-      float4 g = atom >= 0 ? *reinterpret_cast<const float4*>(a.G + (long)atom * a.ldg + kc + c4)
+      float4 g = atom >= 0 ? *reinterpret_cast<const float4*>(a.G + (long)atom * a.ldg + c4)
                            : make_float4(0.f, 0.f, 0.f, 0.f);
       float v[4] = {g.x, g.y, g.z, g.w};
-      unsigned short* dst = &sm.Y[0][0][0] + (long)r * SC + c4;
+      unsigned short* dst = sm.Gt + (long)r * SG + c4;
       unsigned short h[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) h[i] = Ty<MODE>::cvt(v[i]);
@@ -258,49 +310,13 @@ mlp_mfma_kernel(MlpArgs a) {
         uint2 u;
         u.x = (unsigned)l[0] | ((unsigned)l[1] << 16);
         u.y = (unsigned)l[2] | ((unsigned)l[3] << 16);
-        *reinterpret_cast<uint2*>(dst + (long)MT * SC) = u;
+        *reinterpret_cast<uint2*>(dst + (long)MT * SG) = u;
       }
     }
     __syncthreads();
-    if (wact) {
-      // fragments of W1 for k-steps kc/32 .. : offset the fragment base by ks0
-      const int ks0 = kc / 32;
-      const int nks_all = D / 32;
-      const unsigned short* bb = &sm.Y[0][0][0];
-      const int brow = lane & 15, bk = (lane >> 4) * 8;
-      for (int ks = 0; ks < KC / 32; ++ks) {
-        uint4 af[FT], bf[AT];
-#pragma unroll
-        for (int ft = 0; ft < FT; ++ft)
-          af[ft] = *reinterpret_cast<const uint4*>(
-              wb + BL.f1 + ((long)(rt[ft] * nks_all + ks0 + ks) * 64 + lane) * 8);
-#pragma unroll
-        for (int at = 0; at < AT; ++at)
-          bf[at] = *reinterpret_cast<const uint4*>(bb + (long)(at * 16 + brow) * SC + ks * 32 + bk);
-#pragma unroll
-        for (int ft = 0; ft < FT; ++ft)
-#pragma unroll
-          for (int at = 0; at < AT; ++at) acc[ft][at] = Ty<MODE>::mma(af[ft], bf[at], acc[ft][at]);
-        if (MODE == 2) {
-          uint4 al[FT], bl[AT];
-#pragma unroll
-          for (int ft = 0; ft < FT; ++ft)
-            al[ft] = *reinterpret_cast<const uint4*>(
-                wb + BL.f1 + p1 + ((long)(rt[ft] * nks_all + ks0 + ks) * 64 + lane) * 8);
-#pragma unroll
-          for (int at = 0; at < AT; ++at)
-            bl[at] = *reinterpret_cast<const uint4*>(bb + (long)MT * SC + (long)(at * 16 + brow) * SC +
-                                                     ks * 32 + bk);
-#pragma unroll
-          for (int ft = 0; ft < FT; ++ft)
-#pragma unroll
-            for (int at = 0; at < AT; ++at) {
-              acc[ft][at] = Ty<MODE>::mma(al[ft], bf[at], acc[ft][at]);
-              acc[ft][at] = Ty<MODE>::mma(af[ft], bl[at], acc[ft][at]);
-            }
-        }
-      }
-    }
+    if (wact)
+      gemm_tiles<MODE, MT, FT>(acc, wb + BL.f1, p1, D / 32, rt, sm.Gt, (long)MT * SG, lane, SG);
+    __syncthreads();   // the tile shares LDS with X, written by the epilogue
   }
   // epilogue 1 -> X, s1
   if (wact) {
@@ -309,13 +325,10 @@ mlp_mfma_kernel(MlpArgs a) {
 #pragma unroll
       for (int at = 0; at < AT; ++at) {
         int f = 64 * w + 16 * ft + fq;
-        float v[4];
+        float v[4], d[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          float z = acc[ft][at][i] + b1[f + i];
-          v[i] = softplus_a(z, a.alpha);
-          s1[ft][at][i] = sigmoid_a(z, a.alpha);
-        }
+        for (int i = 0; i < 4; ++i) act_pair(acc[ft][at][i] + b1[f + i], a.alpha, inv_alpha, v[i], d[i]);
+        s1.set4(ft, at, d);
         put4<MODE, MT>(sm.X, at * 16 + ca, f, v);
       }
   }
@@ -327,20 +340,17 @@ mlp_mfma_kernel(MlpArgs a) {
     for (int ft = 0; ft < FT; ++ft)
 #pragma unroll
       for (int at = 0; at < AT; ++at) acc[ft][at] = floatx4{0.f, 0.f, 0.f, 0.f};
-    gemm_tiles<MODE, MT, FT, SH>(acc, wb + BL.f2, phh, H / 32, rt, &sm.X[0][0][0], (long)MT * SH,
-                                 lane);
+    gemm_tiles<MODE, MT, FT>(acc, wb + BL.f2, phh, H / 32, rt, &sm.X[0][0][0], (long)MT * SH,
+                                 lane, SH);
 #pragma unroll
     for (int ft = 0; ft < FT; ++ft)
 #pragma unroll
       for (int at = 0; at < AT; ++at) {
         int f = 64 * w + 16 * ft + fq;
-        float v[4];
+        float v[4], d[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          float z = acc[ft][at][i] + b2[f + i];
-          v[i] = softplus_a(z, a.alpha);
-          s2[ft][at][i] = sigmoid_a(z, a.alpha);
-        }
+        for (int i = 0; i < 4; ++i) act_pair(acc[ft][at][i] + b2[f + i], a.alpha, inv_alpha, v[i], d[i]);
+        s2.set4(ft, at, d);
         put4<MODE, MT>(sm.Y, at * 16 + ca, f, v);
       }
   }
@@ -351,8 +361,8 @@ mlp_mfma_kernel(MlpArgs a) {
     for (int ft = 0; ft < FT; ++ft)
 #pragma unroll
       for (int at = 0; at < AT; ++at) acc[ft][at] = floatx4{0.f, 0.f, 0.f, 0.f};
-    gemm_tiles<MODE, MT, FT, SH>(acc, wb + BL.f3, phh, H / 32, rt, &sm.Y[0][0][0], (long)MT * SH,
-                                 lane);
+    gemm_tiles<MODE, MT, FT>(acc, wb + BL.f3, phh, H / 32, rt, &sm.Y[0][0][0], (long)MT * SH,
+                                 lane, SH);
     float ep[AT];
 #pragma unroll
     for (int at = 0; at < AT; ++at) ep[at] = 0.f;
@@ -364,10 +374,11 @@ mlp_mfma_kernel(MlpArgs a) {
         float v[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          float z = acc[ft][at][i] + b3[f + i];
           float wv = wo[f + i];
-          ep[at] += softplus_a(z, a.alpha) * wv;
-          v[i] = wv * sigmoid_a(z, a.alpha);   // dZ3 with upstream 1
+          float act, d;
+          act_pair(acc[ft][at][i] + b3[f + i], a.alpha, inv_alpha, act, d);
+          ep[at] += act * wv;
+          v[i] = wv * d;   // dZ3 with upstream 1
         }
         put4<MODE, MT>(sm.X, at * 16 + ca, f, v);
       }
@@ -397,8 +408,8 @@ mlp_mfma_kernel(MlpArgs a) {
     for (int ft = 0; ft < FT; ++ft)
 #pragma unroll
       for (int at = 0; at < AT; ++at) acc[ft][at] = floatx4{0.f, 0.f, 0.f, 0.f};
-    gemm_tiles<MODE, MT, FT, SH>(acc, wb + BL.b3, phh, H / 32, rt, &sm.X[0][0][0], (long)MT * SH,
-                                 lane);
+    gemm_tiles<MODE, MT, FT>(acc, wb + BL.b3, phh, H / 32, rt, &sm.X[0][0][0], (long)MT * SH,
+                                 lane, SH);
 #pragma unroll
     for (int ft = 0; ft < FT; ++ft)
 #pragma unroll
@@ -406,7 +417,7 @@ mlp_mfma_kernel(MlpArgs a) {
         int f = 64 * w + 16 * ft + fq;
         float v[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) v[i] = acc[ft][at][i] * s2[ft][at][i];
+        for (int i = 0; i < 4; ++i) v[i] = acc[ft][at][i] * s2.get(ft, at, i);
         put4<MODE, MT>(sm.Y, at * 16 + ca, f, v);
       }
   }
@@ -417,8 +428,8 @@ mlp_mfma_kernel(MlpArgs a) {
     for (int ft = 0; ft < FT; ++ft)
 #pragma unroll
       for (int at = 0; at < AT; ++at) acc[ft][at] = floatx4{0.f, 0.f, 0.f, 0.f};
-    gemm_tiles<MODE, MT, FT, SH>(acc, wb + BL.b2, phh, H / 32, rt, &sm.Y[0][0][0], (long)MT * SH,
-                                 lane);
+    gemm_tiles<MODE, MT, FT>(acc, wb + BL.b2, phh, H / 32, rt, &sm.Y[0][0][0], (long)MT * SH,
+                                 lane, SH);
 #pragma unroll
     for (int ft = 0; ft < FT; ++ft)
 #pragma unroll
@@ -426,7 +437,7 @@ mlp_mfma_kernel(MlpArgs a) {
         int f = 64 * w + 16 * ft + fq;
         float v[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) v[i] = acc[ft][at][i] * s1[ft][at][i];
+        for (int i = 0; i < 4; ++i) v[i] = acc[ft][at][i] * s1.get(ft, at, i);
         put4<MODE, MT>(sm.X, at * 16 + ca, f, v);
       }
   }
@@ -443,8 +454,8 @@ mlp_mfma_kernel(MlpArgs a) {
       for (int u = 0; u < 2; ++u)
 #pragma unroll
         for (int at = 0; at < AT; ++at) acc2[u][at] = floatx4{0.f, 0.f, 0.f, 0.f};
-      gemm_tiles<MODE, MT, 2, SH>(acc2, wb + BL.b1, pd, H / 32, rt2, &sm.X[0][0][0], (long)MT * SH,
-                                  lane);
+      gemm_tiles<MODE, MT, 2>(acc2, wb + BL.b1, pd, H / 32, rt2, &sm.X[0][0][0], (long)MT * SH,
+                                  lane, SH);
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
         if (u == 1 && rt2[1] == rt2[0]) continue;
@@ -471,7 +482,9 @@ extern "C" {
 int tm_mlp_mfma_pack_info(int D, int H, int nh, int mode, long* out) {
   TM_REQUIRE(nh == 3, "tm_mlp_mfma: exactly 3 hidden layers supported (got %d)", nh);
   TM_REQUIRE(H % 64 == 0 && H <= HMAX && H >= 64, "tm_mlp_mfma: hidden width %d not a multiple of 64 in 64..512", H);
-  TM_REQUIRE(D % KC == 0 && D > 0, "tm_mlp_mfma: descriptor length %d not a multiple of %d", D, KC);
+  TM_REQUIRE(D % KC == 0 && D > 0 && D + 8 <= 2 * (HMAX + 8),
+             "tm_mlp_mfma: descriptor length %d not a multiple of %d in %d..%d", D, KC, KC,
+             2 * HMAX + 8);
   TM_REQUIRE(mode >= 0 && mode <= 2, "tm_mlp_mfma: bad mode %d", mode);
   int planes = mode == 2 ? 2 : 1;
   BlobLayout L = blob_layout(D, H, planes);
