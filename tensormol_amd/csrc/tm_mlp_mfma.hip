@@ -107,14 +107,26 @@ template <>
 struct Ty<2> : Ty<0> {};
 
 // act(z) = log(1 + e^{a z}) / a and act'(z) = sigmoid(a z), sharing e = e^{-|a z|}:
-// act = (max(a z, 0) + log(1 + e)) / a, sigmoid = 1/(1+e) for a z >= 0, e/(1+e) below
-__device__ __forceinline__ void act_pair(float z, float alpha, float inv_alpha, float& act,
-                                         float& dact) {
-  const float x = alpha * z;
-  const float e = __expf(-fabsf(x));
+// act = (max(a z, 0) + log(1 + e)) / a, sigmoid = 1/(1+e) for a z >= 0, e/(1+e) below.
+// The hardware exp2 / log2 are used directly: the exp2 argument is <= 0 (a
+// result below the normal range is flushed to 0, harmless here) and the log2
+// argument lies in [1, 2], so the library's range fix-ups are not needed.
+struct ActConst {
+  float alpha, inv_alpha, k_exp, k_log;   // k_exp = -alpha log2(e), k_log = ln(2) / alpha
+};
+__device__ __forceinline__ ActConst act_const(float alpha) {
+  ActConst c;
+  c.alpha = alpha;
+  c.inv_alpha = 1.0f / alpha;
+  c.k_exp = -alpha * 1.4426950408889634f;
+  c.k_log = 0.6931471805599453f / alpha;
+  return c;
+}
+__device__ __forceinline__ void act_pair(float z, const ActConst& c, float& act, float& dact) {
+  const float e = __builtin_amdgcn_exp2f(fabsf(z) * c.k_exp);
   const float r = __builtin_amdgcn_rcpf(1.0f + e);
-  act = (fmaxf(x, 0.f) + __logf(1.0f + e)) * inv_alpha;
-  dact = x >= 0.f ? r : e * r;
+  act = fmaxf(z, 0.f) + __builtin_amdgcn_logf(1.0f + e) * c.k_log;
+  dact = z >= 0.f ? r : e * r;
 }
 
 // Activation derivatives of a wave's features, kept from the forward to the
@@ -184,11 +196,22 @@ __device__ __forceinline__ void put4(unsigned short (*buf)[MT][HMAX + 8], int at
 // The weight fragments of k-step ks+1 are in flight while the MFMAs of ks run:
 // two register sets alternate (the loop is unrolled by two; nks is even), and
 // scheduling barriers keep the loads ahead of the MFMAs that do not need them.
+// First weight fragments of a GEMM (k-step 0).  Weights do not depend on LDS,
+// so a wave loads them before the epilogue and barrier that precede the GEMM.
+template <int NFT>
+__device__ __forceinline__ void gemm_first(uint4 (&a)[NFT], const unsigned short* __restrict__ wf,
+                                           int nks, const int* rtile, int lane) {
+#pragma unroll
+  for (int ft = 0; ft < NFT; ++ft)
+    a[ft] = *reinterpret_cast<const uint4*>(wf + ((long)(rtile[ft] * nks) * 64 + lane) * 8);
+}
+
 template <int MODE, int MT, int NFT>
 __device__ __forceinline__ void gemm_tiles(floatx4 (&acc)[NFT][MT / 16],
                                            const unsigned short* __restrict__ wf, long plane_elems,
                                            int nks, const int* rtile, const unsigned short* bbase,
-                                           long bplane, int lane, int STRIDE) {
+                                           long bplane, int lane, int STRIDE,
+                                           const uint4 (&a_first)[NFT]) {
   constexpr int AT = MT / 16;
   const int brow = lane & 15;
   const int bk = (lane >> 4) * 8;
@@ -223,7 +246,7 @@ __device__ __forceinline__ void gemm_tiles(floatx4 (&acc)[NFT][MT / 16],
   };
   uint4 a0[NFT], a1[NFT];
 #pragma unroll
-  for (int ft = 0; ft < NFT; ++ft) a0[ft] = afrag(ft, 0, 0);
+  for (int ft = 0; ft < NFT; ++ft) a0[ft] = a_first[ft];
   for (int ks = 0; ks < nks; ks += 2) {
 #pragma unroll
     for (int ft = 0; ft < NFT; ++ft) a1[ft] = afrag(ft, ks + 1, 0);
@@ -276,8 +299,11 @@ mlp_mfma_kernel(MlpArgs a) {
   const int ca = lane & 15;         // atom column of this lane in a C tile
   // activation derivatives of layers 1 and 2 for this wave's features
   DerivRegs<MODE, FT, AT> s1, s2;
-  const float inv_alpha = 1.0f / a.alpha;
+  const ActConst ac = act_const(a.alpha);
   floatx4 acc[FT][AT];
+  const long phh = frag_elems(H / 16, H / 32);
+  uint4 pf[FT];   // first weight fragments of the next GEMM, loaded early
+  if (wact) gemm_first<FT>(pf, wb + BL.f1, D / 32, rt, lane);
 
   // ------------------------------------------------------------ layer 1
 #pragma unroll
@@ -315,7 +341,8 @@ mlp_mfma_kernel(MlpArgs a) {
     }
     __syncthreads();
     if (wact)
-      gemm_tiles<MODE, MT, FT>(acc, wb + BL.f1, p1, D / 32, rt, sm.Gt, (long)MT * SG, lane, SG);
+      gemm_tiles<MODE, MT, FT>(acc, wb + BL.f1, p1, D / 32, rt, sm.Gt, (long)MT * SG, lane, SG, pf);
+    if (wact) gemm_first<FT>(pf, wb + BL.f2, H / 32, rt, lane);
     __syncthreads();   // the tile shares LDS with X, written by the epilogue
   }
   // epilogue 1 -> X, s1
@@ -327,13 +354,12 @@ mlp_mfma_kernel(MlpArgs a) {
         int f = 64 * w + 16 * ft + fq;
         float v[4], d[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) act_pair(acc[ft][at][i] + b1[f + i], a.alpha, inv_alpha, v[i], d[i]);
+        for (int i = 0; i < 4; ++i) act_pair(acc[ft][at][i] + b1[f + i], ac, v[i], d[i]);
         s1.set4(ft, at, d);
         put4<MODE, MT>(sm.X, at * 16 + ca, f, v);
       }
   }
   __syncthreads();
-  const long phh = frag_elems(H / 16, H / 32);
   // ------------------------------------------------------------ layer 2
   if (wact) {
 #pragma unroll
@@ -341,7 +367,8 @@ mlp_mfma_kernel(MlpArgs a) {
 #pragma unroll
       for (int at = 0; at < AT; ++at) acc[ft][at] = floatx4{0.f, 0.f, 0.f, 0.f};
     gemm_tiles<MODE, MT, FT>(acc, wb + BL.f2, phh, H / 32, rt, &sm.X[0][0][0], (long)MT * SH,
-                                 lane, SH);
+                                 lane, SH, pf);
+    gemm_first<FT>(pf, wb + BL.f3, H / 32, rt, lane);
 #pragma unroll
     for (int ft = 0; ft < FT; ++ft)
 #pragma unroll
@@ -349,7 +376,7 @@ mlp_mfma_kernel(MlpArgs a) {
         int f = 64 * w + 16 * ft + fq;
         float v[4], d[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) act_pair(acc[ft][at][i] + b2[f + i], a.alpha, inv_alpha, v[i], d[i]);
+        for (int i = 0; i < 4; ++i) act_pair(acc[ft][at][i] + b2[f + i], ac, v[i], d[i]);
         s2.set4(ft, at, d);
         put4<MODE, MT>(sm.Y, at * 16 + ca, f, v);
       }
@@ -362,7 +389,8 @@ mlp_mfma_kernel(MlpArgs a) {
 #pragma unroll
       for (int at = 0; at < AT; ++at) acc[ft][at] = floatx4{0.f, 0.f, 0.f, 0.f};
     gemm_tiles<MODE, MT, FT>(acc, wb + BL.f3, phh, H / 32, rt, &sm.Y[0][0][0], (long)MT * SH,
-                                 lane, SH);
+                                 lane, SH, pf);
+    gemm_first<FT>(pf, wb + BL.b3, H / 32, rt, lane);
     float ep[AT];
 #pragma unroll
     for (int at = 0; at < AT; ++at) ep[at] = 0.f;
@@ -376,7 +404,7 @@ mlp_mfma_kernel(MlpArgs a) {
         for (int i = 0; i < 4; ++i) {
           float wv = wo[f + i];
           float act, d;
-          act_pair(acc[ft][at][i] + b3[f + i], a.alpha, inv_alpha, act, d);
+          act_pair(acc[ft][at][i] + b3[f + i], ac, act, d);
           ep[at] += act * wv;
           v[i] = wv * d;   // dZ3 with upstream 1
         }
@@ -409,7 +437,8 @@ mlp_mfma_kernel(MlpArgs a) {
 #pragma unroll
       for (int at = 0; at < AT; ++at) acc[ft][at] = floatx4{0.f, 0.f, 0.f, 0.f};
     gemm_tiles<MODE, MT, FT>(acc, wb + BL.b3, phh, H / 32, rt, &sm.X[0][0][0], (long)MT * SH,
-                                 lane, SH);
+                                 lane, SH, pf);
+    gemm_first<FT>(pf, wb + BL.b2, H / 32, rt, lane);
 #pragma unroll
     for (int ft = 0; ft < FT; ++ft)
 #pragma unroll
@@ -429,7 +458,7 @@ mlp_mfma_kernel(MlpArgs a) {
 #pragma unroll
       for (int at = 0; at < AT; ++at) acc[ft][at] = floatx4{0.f, 0.f, 0.f, 0.f};
     gemm_tiles<MODE, MT, FT>(acc, wb + BL.b2, phh, H / 32, rt, &sm.Y[0][0][0], (long)MT * SH,
-                                 lane, SH);
+                                 lane, SH, pf);
 #pragma unroll
     for (int ft = 0; ft < FT; ++ft)
 #pragma unroll
@@ -440,6 +469,11 @@ mlp_mfma_kernel(MlpArgs a) {
         for (int i = 0; i < 4; ++i) v[i] = acc[ft][at][i] * s1.get(ft, at, i);
         put4<MODE, MT>(sm.X, at * 16 + ca, f, v);
       }
+  }
+  uint4 pf2[2];   // first fragments of backward 1 (rows of dG tiles w, w + NW)
+  if (w < D / 16) {
+    int r0[2] = {w, w + NW < D / 16 ? w + NW : w};
+    gemm_first<2>(pf2, wb + BL.b1, H / 32, r0, lane);
   }
   __syncthreads();
   // ------------------------------------------------------------ backward 1: X (dZ1) -> dG
@@ -455,7 +489,12 @@ mlp_mfma_kernel(MlpArgs a) {
 #pragma unroll
         for (int at = 0; at < AT; ++at) acc2[u][at] = floatx4{0.f, 0.f, 0.f, 0.f};
       gemm_tiles<MODE, MT, 2>(acc2, wb + BL.b1, pd, H / 32, rt2, &sm.X[0][0][0], (long)MT * SH,
-                                  lane, SH);
+                                  lane, SH, pf2);
+      const int tn = t0 + 2 * NW;   // the next iteration's first fragments
+      if (tn < ntile) {
+        int rn[2] = {tn, tn + NW < ntile ? tn + NW : tn};
+        gemm_first<2>(pf2, wb + BL.b1, H / 32, rn, lane);
+      }
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
         if (u == 1 && rt2[1] == rt2[0]) continue;
@@ -506,6 +545,7 @@ int tm_mlp_mfma(int N, int D, int H, int ntiles, const int* tiles, const int* se
   int st = tm_mlp_mfma_pack_info(D, H, 3, mode, info);
   if (st) return st;
   TM_REQUIRE(ldg % 4 == 0 && ldg >= D, "tm_mlp_mfma: ldg %d", ldg);
+  TM_REQUIRE(alpha > 0.f, "tm_mlp_mfma: activation alpha %g must be positive", (double)alpha);
   TM_REQUIRE(ntiles >= 0, "tm_mlp_mfma: bad tile count");
   if (ntiles == 0) return TM_OK;
   MlpArgs a;
