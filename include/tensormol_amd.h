@@ -143,10 +143,11 @@ int tm_mlp_f32(int M, int D, int nh, const int* hid, const float* G, int ldg, co
  * L2 serves few weight blobs); seg (device int32 [nseg][4]) = (net,
  * blob index, first row in perm, rows); blobs: device array of weight-blob
  * pointers packed as tm_mlp_mfma_pack_info describes; mode 0 = bf16, 1 = fp16,
- * 2 = bf16 hi/lo split.  Writes y[net*N + atom] and dG[(net*N + atom)*ldg + f]. */
+ * 2 = bf16 hi/lo split; tile_rows = atoms per tile, 64 or 32 (32 in the split
+ * mode).  Writes y[net*N + atom] and dG[(net*N + atom)*ldg + f]. */
 int tm_mlp_mfma(int N, int D, int H, int ntiles, const int* tiles, const int* seg, int ldg,
                 const float* G, const int* perm, const void* blobs, float alpha, int mode,
-                float* y, float* dG, void* stream);
+                int tile_rows, float* y, float* dG, void* stream);
 /* out[0] blob bytes, out[1] byte offset of the float section (b1, b2, b3, wo,
  * bo), out[2..7] element offsets of the forward (layers 1-3) and backward
  * (layers 3, 2, 1) fragment arrays, out[8] planes (2 for the split mode). */

@@ -45,7 +45,7 @@ SIGNATURES = {
     "tm_nh_kick": [I, D, P, P, P, P, P, P, P, P, P],
     "tm_fire_dots": [I, P, P, P, P],
     "tm_fire_update": [I, I, P, I, P, P, P, P, P, P, P, D, I, D, D, D, D, D, P],
-    "tm_mlp_mfma": [I, I, I, I, P, P, I, P, P, P, F, I, P, P, P],
+    "tm_mlp_mfma": [I, I, I, I, P, P, I, P, P, P, F, I, I, P, P, P],
     "tm_mlp_mfma_pack_info": [I, I, I, I, P],
 }
 

@@ -263,8 +263,11 @@ __device__ __forceinline__ void gemm_tiles(floatx4 (&acc)[NFT][MT / 16],
   }
 }
 
+// 64-row tiles: one workgroup per CU (LDS); 32-row tiles in the BF16 / FP16
+// modes fit two workgroups per CU when the waves stay within 128 VGPRs.
 template <int MODE, int MT>
 __global__ void __launch_bounds__(NW * 64)
+__attribute__((amdgpu_waves_per_eu(MT == 32 && MODE != 2 ? 4 : 2)))
 mlp_mfma_kernel(MlpArgs a) {
   constexpr int AT = MT / 16;
   constexpr int PL = MODE == 2 ? 2 : 1;
@@ -540,13 +543,15 @@ int tm_mlp_mfma_pack_info(int D, int H, int nh, int mode, long* out) {
 // of per-(net, element) weight blob pointers (layout of tm_mlp_mfma_pack_info).
 int tm_mlp_mfma(int N, int D, int H, int ntiles, const int* tiles, const int* seg, int ldg,
                 const float* G, const int* perm, const void* blobs, float alpha, int mode,
-                float* y, float* dG, void* stream) {
+                int tile_rows, float* y, float* dG, void* stream) {
   long info[9];
   int st = tm_mlp_mfma_pack_info(D, H, 3, mode, info);
   if (st) return st;
   TM_REQUIRE(ldg % 4 == 0 && ldg >= D, "tm_mlp_mfma: ldg %d", ldg);
   TM_REQUIRE(alpha > 0.f, "tm_mlp_mfma: activation alpha %g must be positive", (double)alpha);
   TM_REQUIRE(ntiles >= 0, "tm_mlp_mfma: bad tile count");
+  TM_REQUIRE(tile_rows == 32 || (tile_rows == 64 && mode != 2),
+             "tm_mlp_mfma: tile_rows %d (64 or 32; 32 in the split mode)", tile_rows);
   if (ntiles == 0) return TM_OK;
   MlpArgs a;
   a.N = N; a.D = D; a.H = H; a.ldg = ldg;
@@ -561,8 +566,10 @@ int tm_mlp_mfma(int N, int D, int H, int ntiles, const int* tiles, const int* se
   a.dG = dG;
   a.flags = nullptr;
   hipStream_t s = (hipStream_t)stream;
-  if (mode == 0) mlp_mfma_kernel<0, 64><<<ntiles, NW * 64, 0, s>>>(a);
-  else if (mode == 1) mlp_mfma_kernel<1, 64><<<ntiles, NW * 64, 0, s>>>(a);
+  if (mode == 0 && tile_rows == 64) mlp_mfma_kernel<0, 64><<<ntiles, NW * 64, 0, s>>>(a);
+  else if (mode == 0) mlp_mfma_kernel<0, 32><<<ntiles, NW * 64, 0, s>>>(a);
+  else if (mode == 1 && tile_rows == 64) mlp_mfma_kernel<1, 64><<<ntiles, NW * 64, 0, s>>>(a);
+  else if (mode == 1) mlp_mfma_kernel<1, 32><<<ntiles, NW * 64, 0, s>>>(a);
   else mlp_mfma_kernel<2, 32><<<ntiles, NW * 64, 0, s>>>(a);
   TM_LAUNCH_CHECK();
   return TM_OK;

@@ -62,6 +62,9 @@ class MFMAWeights:
         self.engine = engine
         self.mode = MODES[precision]
         self.mt = TILE_ROWS[self.mode]
+        dev = engine.device
+        self.ncu = (torch.cuda.get_device_properties(dev).multi_processor_count
+                    if dev.type == "cuda" and torch.cuda.is_available() else 256)
         model = engine.model
         self.D = model.sym.D
         if len(model.hidden) != 3 or len(set(model.hidden)) != 1:
@@ -105,10 +108,14 @@ class MFMAWeights:
 
     def workspace(self, ws, upload=None):
         """Segment and tile tables for one packed batch (rows are element-sorted):
-        seg[s] = (net, net * nele + element, first row, rows), tiles[t] = (seg, first row)."""
+        seg[s] = (net, net * nele + element, first row, rows), tiles[t] = (seg, first row).
+
+        64-row tiles fill whole rounds of the device (one workgroup per CU); the
+        remaining rows go to 32-row tiles in a second launch, so the last round
+        is not half empty.  The split mode uses 32-row tiles throughout."""
         nele = self.engine.nele
         counts = np.diff(ws.eoff).astype(np.int64)
-        segs, tiles = [], []
+        segs, t64 = [], []
         for net in range(2):
             for a in range(nele):
                 M = int(counts[a])
@@ -116,20 +123,31 @@ class MFMAWeights:
                     continue
                 s = len(segs)
                 segs.append((net, net * nele + a, int(ws.eoff[a]), M))
-                r0 = np.arange(0, M, self.mt, dtype=np.int32)
-                tiles.append(np.stack([np.full_like(r0, s), r0], axis=1))
+                r0 = np.arange(0, M, 64, dtype=np.int32)
+                t64.append(np.stack([np.full_like(r0, s), r0], axis=1))
         seg = np.array(segs, dtype=np.int32).reshape(-1, 4)
-        til = np.concatenate(tiles).astype(np.int32) if tiles else np.zeros((0, 2), np.int32)
-        til = xcd_order(til)
+        t64 = np.concatenate(t64).astype(np.int32) if t64 else np.zeros((0, 2), np.int32)
+        full = 0 if self.mode == 2 else (len(t64) // self.ncu) * self.ncu
+        head, rest = t64[:full], t64[full:]
+        # each remaining 64-row tile becomes two 32-row tiles (the second only if rows remain)
+        halves = np.repeat(rest, 2, axis=0)
+        halves[1::2, 1] += 32
+        rows = seg[halves[:, 0], 3] if len(halves) else np.zeros(0, np.int32)
+        t32 = halves[halves[:, 1] < rows] if len(halves) else halves
+        t64x, t32x = xcd_order(head), xcd_order(t32)
         if upload is None:
             dev = self.engine.device
             upload = lambda a, out=None: torch.from_numpy(a).to(dev)
-        return dict(seg=upload(seg), tiles=upload(til), ntiles=len(til))
+        return dict(seg=upload(seg), tiles64=upload(t64x), n64=len(t64x),
+                    tiles32=upload(t32x), n32=len(t32x))
 
     def run(self, ws):
         eng = self.engine
         m = ws.mlp_ws
-        eng.lib.tm_mlp_mfma(ws.N, self.D, self.H, m["ntiles"], _lib.ptr(m["tiles"]),
-                            _lib.ptr(m["seg"]), self.D, _lib.ptr(ws.G), _lib.ptr(ws.perm),
-                            _lib.ptr(self.blob_ptrs), float(eng.model.alpha), self.mode,
-                            _lib.ptr(ws.Y2), _lib.ptr(ws.J2), eng._stream())
+        for rows, key, n in ((64, "tiles64", m["n64"]), (32, "tiles32", m["n32"])):
+            if n == 0:
+                continue
+            eng.lib.tm_mlp_mfma(ws.N, self.D, self.H, n, _lib.ptr(m[key]), _lib.ptr(m["seg"]),
+                                self.D, _lib.ptr(ws.G), _lib.ptr(ws.perm), _lib.ptr(self.blob_ptrs),
+                                float(eng.model.alpha), self.mode, rows, _lib.ptr(ws.Y2),
+                                _lib.ptr(ws.J2), eng._stream())

@@ -85,3 +85,27 @@ def test_energy_forces_mfma(cuda, precision, name, modelname):
     assert frms < 1e-3
     if precision == "bf16x3":
         assert np.all(dE / natom < 2e-5)
+
+
+@pytest.mark.parametrize("precision", ["bf16", "fp16"])
+def test_tile_split_bit_identical(cuda, precision):
+    """Whole rounds of 64-row tiles plus 32-row tiles for the rest give the
+    same outputs, bit for bit, as 32-row tiles only: every output row's
+    arithmetic is independent of the tile it sits in."""
+    from tensormol_amd.engine import BPEEEngine
+    Z, x, ms, boxes = H.system("waterbox10")
+    pos = torch.tensor(x, dtype=torch.float64, device=cuda)
+    outs = []
+    for ncu in (1 << 20, 40):   # all 32-row tiles / two 64-row rounds of 40 tiles + 32-row rest
+        eng = BPEEEngine(H.model_for("water"), precision=precision)
+        eng.mfma.ncu = ncu
+        ws = eng.prepare(Z, ms, boxes, pos0=x)
+        if ncu == 40:
+            assert ws.mlp_ws["n64"] > 0 and ws.mlp_ws["n32"] > 0
+        eng.build_lists(ws, pos)
+        eng.descriptors(ws, pos)
+        eng.networks(ws)
+        torch.cuda.synchronize()
+        outs.append((ws.Y2[:2 * ws.N].clone(), ws.J2[:2 * ws.N * eng.D].clone()))
+    assert torch.equal(outs[0][0], outs[1][0])
+    assert torch.equal(outs[0][1], outs[1][1])
