@@ -230,26 +230,39 @@ __global__ void cell_gather_kernel(int N, const int* __restrict__ cell_atoms,
 
 // Per-molecule column sums of an [N, C] array (float or double input) into
 // out[B, C] (double); one block per molecule, fixed reduction tree.
-template <class T>
-__global__ void __launch_bounds__(256)
+template <class T, int NT>
+__global__ void __launch_bounds__(NT)
 mol_sum_kernel(const int* __restrict__ mol_start, int C, const T* __restrict__ in, int ldin,
                double* __restrict__ out, int scale_by_count) {
-  __shared__ double part[256];
+  constexpr int CB = 4;   // columns accumulated per pass over the atoms
+  __shared__ double part[CB][NT / 64];
   const int m = blockIdx.x;
   const int lo = mol_start[m], hi = mol_start[m + 1];
-  for (int c = 0; c < C; ++c) {
-    double acc = 0.0;
-    for (int a = lo + threadIdx.x; a < hi; a += 256) acc += (double)in[(long)a * ldin + c];
-    part[threadIdx.x] = acc;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-      if (threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
-      __syncthreads();
+  for (int c0 = 0; c0 < C; c0 += CB) {
+    const int nc = min(CB, C - c0);
+    double acc[CB] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int a = lo + threadIdx.x; a < hi; a += NT) {
+      const T* row = in + (long)a * ldin + c0;
+#pragma unroll
+      for (int c = 0; c < CB; ++c)
+        if (c < nc) acc[c] += (double)row[c];
     }
-    if (threadIdx.x == 0) {
-      double v = part[0];
+    // fixed-order reduction: butterfly inside each wave, then the wave sums in order
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < CB; ++c) {
+      double v = acc[c];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      if (lane == 0) part[c][wv] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < nc) {
+      double v = 0.0;
+      for (int q = 0; q < NT / 64; ++q) v += part[threadIdx.x][q];
       if (scale_by_count) v = hi > lo ? v / (double)(hi - lo) : 0.0;
-      out[(long)m * C + c] = v;
+      out[(long)m * C + c0 + threadIdx.x] = v;
     }
     __syncthreads();
   }
@@ -290,6 +303,12 @@ __global__ void chunk_final_kernel(int nchunk, int C, int n, const double* __res
 template <class T>
 int mol_sum(int B, const int* mol_start, int C, const T* in, int ldin, double* out, int mean,
             double* scratch, int n_total, hipStream_t s) {
+  if (B == 1 && n_total > 2048 && (scratch == nullptr || n_total <= (1 << 18))) {
+    // one system of up to 256k atoms: a single 1024-thread block, one launch
+    mol_sum_kernel<T, 1024><<<1, 1024, 0, s>>>(mol_start, C, in, ldin, out, mean);
+    TM_LAUNCH_CHECK();
+    return TM_OK;
+  }
   if (B == 1 && scratch != nullptr && n_total > 2048) {
     int nchunk = (n_total + 1023) / 1024;
     chunk_sum_kernel<T><<<nchunk, 256, 0, s>>>(n_total, C, in, ldin, scratch);
@@ -298,7 +317,7 @@ int mol_sum(int B, const int* mol_start, int C, const T* in, int ldin, double* o
     TM_LAUNCH_CHECK();
     return TM_OK;
   }
-  mol_sum_kernel<T><<<B, 256, 0, s>>>(mol_start, C, in, ldin, out, mean);
+  mol_sum_kernel<T, 256><<<B, 256, 0, s>>>(mol_start, C, in, ldin, out, mean);
   TM_LAUNCH_CHECK();
   return TM_OK;
 }
@@ -330,8 +349,9 @@ __global__ void charge_upstream_kernel(int N, const int* __restrict__ mol,
 
 extern "C" {
 
-// scratch (optional, >= ceil(N/1024)*C doubles) enables the multi-block path
-// for a single large molecule; n_total = N.
+// One molecule of up to 262,144 atoms is summed by a single 1024-thread block;
+// above that, scratch (>= ceil(N/1024)*C doubles) enables the two-pass
+// multi-block path; n_total = N.
 int tm_mol_sum_f32(int B, const int* mol_start, int C, const float* in, int ldin, double* out,
                    int mean, double* scratch, int n_total, void* stream) {
   TM_REQUIRE(B >= 0 && C > 0 && C <= 64, "tm_mol_sum_f32: bad sizes");

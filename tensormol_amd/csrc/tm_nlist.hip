@@ -212,30 +212,53 @@ __global__ void zero2_kernel(long n, int* __restrict__ a, int* __restrict__ b) {
   if (t < n) { a[t] = 0; b[t] = 0; }
 }
 
-// exclusive scan of cell_count into cell_start (single block): each thread
-// sums a contiguous chunk, one block scan of the 1024 partials, then each
-// thread writes its chunk
+// Exclusive prefix sum of n ints (out[n] = total) by one 1024-thread block:
+// tiles of 4096 elements, 4 consecutive per thread (coalesced), a shuffle scan
+// inside each wave, the 16 wave totals scanned through LDS, and a running
+// carry between tiles.
 __global__ void __launch_bounds__(1024) scan_kernel(int n, const int* __restrict__ in,
                                                     int* __restrict__ out) {
-  __shared__ int part[1024];
-  const int per = (n + 1023) / 1024;
-  const int lo = threadIdx.x * per, hi = min(n, lo + per);
-  int sum = 0;
-  for (int a = lo; a < hi; ++a) sum += in[a];
-  part[threadIdx.x] = sum;
+  __shared__ int wtot[16];
+  __shared__ int carry_s;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  if (tid == 0) carry_s = 0;
   __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    int v = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
+  for (int base = 0; base < n; base += 4096) {
+    const int i0 = base + 4 * tid;
+    int v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = i0 + k < n ? in[i0 + k] : 0;
+    const int t = v[0] + v[1] + v[2] + v[3];
+    int incl = t;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      int u = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += u;
+    }
+    if (lane == 63) wtot[w] = incl;
     __syncthreads();
-    part[threadIdx.x] += v;
+    if (w == 0) {
+      int x = lane < 16 ? wtot[lane] : 0;
+#pragma unroll
+      for (int o = 1; o < 16; o <<= 1) {
+        int u = __shfl_up(x, o, 64);
+        if (lane >= o) x += u;
+      }
+      if (lane < 16) wtot[lane] = x;   // inclusive over waves
+    }
+    __syncthreads();
+    const int carry = carry_s;
+    int run = carry + (w > 0 ? wtot[w - 1] : 0) + incl - t;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (i0 + k < n) out[i0 + k] = run;
+      run += v[k];
+    }
+    __syncthreads();   // every thread has read carry_s and wtot
+    if (tid == 0) carry_s = carry + wtot[15];
     __syncthreads();
   }
-  int run = part[threadIdx.x] - sum;   // exclusive prefix of this chunk
-  for (int a = lo; a < hi; ++a) {
-    out[a] = run;
-    run += in[a];
-  }
-  if (threadIdx.x == 1023) out[n] = part[1023];
+  if (tid == 0) out[n] = carry_s;
 }
 
 __global__ void cell_fill_kernel(int N, const int* __restrict__ cell_of,
