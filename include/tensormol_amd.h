@@ -54,15 +54,18 @@ int tm_cell_build(int N, const double* pos, const int* mol, double ox, double oy
  * first doubles failing sqrt(s) + 1e-13 < rc (MolEmb.cpp:1214-1218), so the
  * sets are bit-exact; rc (>= the pair cutoff) sizes the cell stencil.  Rows
  * are sorted by (element of j, j).  Candidates come from the molecule
- * (cell_atoms == NULL) or from the grid of tm_cell_build.
+ * (cell_atoms == NULL) or from the grid of tm_cell_build (nx..nz cells of edges
+ * csx..csz from the origin ox..oz, the arguments given to tm_cell_build; each
+ * centre's stencil rows are trimmed with its own position).
  * Outputs: nbr[i*ldr+n], ncnt[i], rseg[i*(nele+1)+e], ang[i*lda+a] (position
  * in row i), acnt[i], aseg[i*(nele+1)+e], rev[i*ldr+n] (mirror slot), flags
  * (0: row overflow, 1: sub-list overflow, 3: non-finite input). */
 int tm_nl_build(int N, const double* pos, const int* elem, const int* mol, const int* mol_start,
                 const double* box, const int* cell_start, const int* cell_atoms,
                 const int* cell_of, const int* cell_pos, const double* posc, const int* molc,
-                int nx, int ny, int nz, double csx, double csy, double csz, int periodic, int nele,
-                double rc, double s_r, double s_a, int ldr, int lda, int* nbr, int* ncnt,
+                int nx, int ny, int nz, double csx, double csy, double csz, double ox, double oy,
+                double oz, int periodic, int nele, double rc, double s_r, double s_a, int ldr,
+                int lda, int* nbr, int* ncnt,
                 int* rseg, int* ang, int* acnt, int* aseg, int* rev, int* flags, void* stream);
 
 /* Materialises the sorted triples of every centre (Neighbors.py:369-420):
@@ -80,7 +83,8 @@ int tm_triples(int N, int ldr, int lda, int nele, const int* nbr, const int* ang
 int tm_pairs(int N, const double* pos, const int* mol, const int* mol_start, const int* nreal,
              const double* box, const int* cell_start, const int* cell_atoms, const int* cell_of,
              const int* cell_pos, const double* posc, const int* molc, int nx, int ny, int nz,
-             double csx, double csy, double csz, int periodic, double rc, double s_c, int half,
+             double csx, double csy, double csz, double ox, double oy, double oz, int periodic,
+             double rc, double s_c, int half,
              int* count, const long* start, int* out, int* flags, void* stream);
 
 /* ---- descriptors -------------------------------------------------------
@@ -170,7 +174,8 @@ int tm_charges(int N, const int* mol, const double* qmean, const float* qraw, co
 int tm_coulomb(int N, const float* xq, const int* elem, const int* mol, const int* mol_start,
                const double* box, const int* cell_start, const int* cell_atoms, const int* cell_of,
                const int* cell_pos, const int* molc, int nx, int ny, int nz, double csx, double csy,
-               double csz, int periodic, const double* consts, double rc_ee, double width,
+               double csz, double ox, double oy, double oz, int periodic, const double* consts,
+               double rc_ee, double width,
                int add_ecc, int has_vdw, int nele, const double* c6, const double* r0, float* xqc,
                int* elc, double* phi, double* grad, double* eatom, int* flags, void* stream);
 

@@ -22,10 +22,10 @@ SIGNATURES = {
     "tm_version": [],
     "tm_zero_i32": [I, P, P],
     "tm_cell_build": [I, P, P, D, D, D, D, D, D, I, I, I, I, P, P, P, P, P, P, P, P, P, P],
-    "tm_nl_build": [I, P, P, P, P, P, P, P, P, P, P, P, I, I, I, D, D, D, I, I, D, D, D, I, I, P, P, P,
+    "tm_nl_build": [I, P, P, P, P, P, P, P, P, P, P, P, I, I, I, D, D, D, D, D, D, I, I, D, D, D, I, I, P, P, P,
                     P, P, P, P, P, P],
     "tm_triples": [I, I, I, I, P, P, P, P, P, P, P],
-    "tm_pairs": [I, P, P, P, P, P, P, P, P, P, P, P, I, I, I, D, D, D, I, D, D, I, P, P, P, P, P],
+    "tm_pairs": [I, P, P, P, P, P, P, P, P, P, P, P, I, I, I, D, D, D, D, D, D, I, D, D, I, P, P, P, P, P],
     "tm_sym_forward": [I, P, P, P, P, P, P, P, P, P, P, I, I, I, I, F, F, F, F, P, P, P, P, I, P],
     "tm_sym_adjoint": [I, P, P, P, P, P, P, P, P, P, P, I, I, I, I, F, F, F, F, P, P, P, P, P, P,
                        I, P, P],
@@ -36,7 +36,7 @@ SIGNATURES = {
     "tm_mol_sum_f64": [I, P, I, P, I, P, I, P, I, P],
     "tm_charges": [I, P, P, P, P, P, P, P, P],
     "tm_charge_upstream": [I, P, P, P, P, P],
-    "tm_coulomb": [I, P, P, P, P, P, P, P, P, P, P, I, I, I, D, D, D, I, P, D, D, I, I, I, P, P, P, P,
+    "tm_coulomb": [I, P, P, P, P, P, P, P, P, P, P, I, I, I, D, D, D, D, D, D, I, P, D, D, I, I, I, P, P, P, P,
                    P, P, P, P, P],
     "tm_vv_drift": [I, D, P, P, P, P],
     "tm_vv_kick": [I, D, P, P, P, P, P, P],

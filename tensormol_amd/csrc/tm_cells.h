@@ -1,10 +1,10 @@
 // Candidate generation over a cell grid, shared by the neighbour-row,
 // pair-list and Coulomb sweeps.
 //
-// The stencil of a centre is the set of x-rows (dy, dz) whose minimum
-// cell-to-cell distance is below the cutoff, each trimmed in x to the cells
-// that can still hold a neighbour; with cells of about half the short cutoff
-// this is a close cover of the cutoff sphere.  Every row is one contiguous
+// The stencil of a centre is the set of x-rows (dy, dz) whose band of cells
+// comes within the cutoff of the centre, each trimmed in x to the cells that
+// can still hold a neighbour; with cells of about half the short cutoff this
+// is a close cover of the cutoff sphere.  Every row is one contiguous
 // range of the cell-ordered atom array (two when it wraps a periodic edge).
 // The ranges of one centre are collected in LDS with a prefix sum of their
 // lengths, so a wavefront sweeps the concatenated candidates 64 at a time
@@ -25,8 +25,21 @@ struct CellGrid {
   const int* molc;         // molecule of each cell-ordered atom (NULL: one system)
   int nc[3];               // cells per axis
   float cs[3];             // cell edges (A)
+  float org[3];            // grid origin (A): cell k of an axis spans [org + k cs, org + (k+1) cs)
   int periodic;            // 1: wrap, 0: clamp
 };
+
+// Margin (A) added to the cutoff when rows are trimmed with the centre's own
+// position: it covers the FP32 rounding of positions and cell bounds, so the
+// candidates stay a superset of the atoms inside the cutoff.
+constexpr float TM_TRIM_MARGIN = 2e-3f;
+
+// Distance along one axis from a centre at offset f inside its cell to the band
+// of cells d cells away (0 for its own band).
+__device__ __forceinline__ float tm_band_dist(int d, float f, float cs) {
+  const float v = d > 0 ? (float)d * cs - f : (d < 0 ? f - (float)(d + 1) * cs : 0.f);
+  return fmaxf(v, 0.f);
+}
 
 // Ranges per centre (at most two per stencil row).  A sweep of cutoff rc over
 // cells of edge cs has at most (2*ceil(rc/cs)+3)^2 rows.
@@ -40,20 +53,38 @@ struct RangeLds {
 // Builds the candidate ranges of atom i for cutoff rc into R (one wavefront,
 // all lanes participate).  Returns the number of candidates; *nrange gets the
 // number of ranges (capped at CAP, overflow flagged).
+// Rows are trimmed with the centre's position (px, py, pz): along y and z the
+// distance from the centre to a row's band of cells, along x the cells that
+// [x - s, x + s] covers with s^2 = (rc + margin)^2 - dy^2 - dz^2.  On an axis
+// whose stencil wraps the whole periodic box the cell-gap bound (|d| - 1) cs is
+// used instead (two images of a band can be equally near there).
 template <int CAP>
 __device__ inline int tm_build_ranges(int i, const CellGrid& g, float rc, RangeLds<CAP>& R, int lane,
-                                      int* flags, int* nrange) {
+                                      int* flags, int* nrange, double px, double py, double pz) {
   const int c = g.cell_of[i];
   const int cx = c % g.nc[0], cy = (c / g.nc[0]) % g.nc[1], cz = c / (g.nc[0] * g.nc[1]);
   int k[3];
   bool full[3];
-  for (int d = 0; d < 3; ++d) {
-    k[d] = (int)ceilf(rc / g.cs[d]) + 1;
-    full[d] = g.periodic && (2 * k[d] + 1 >= g.nc[d]);
+  float f[3];
+  {
+    const int ci[3] = {cx, cy, cz};
+    const double p[3] = {px, py, pz};
+    for (int d = 0; d < 3; ++d) {
+      k[d] = (int)ceilf(rc / g.cs[d]) + 1;
+      full[d] = g.periodic && (2 * k[d] + 1 >= g.nc[d]);
+      double u = p[d] - (double)g.org[d];
+      if (g.periodic) {   // wrapped in FP64, so unwrapped coordinates keep their precision
+        const double Ld = (double)g.nc[d] * (double)g.cs[d];
+        u -= Ld * floor(u / Ld);
+      }
+      f[d] = (float)(u - (double)ci[d] * (double)g.cs[d]);   // offset inside the centre's cell
+    }
   }
   const int ny = full[1] ? g.nc[1] : 2 * k[1] + 1;
   const int nz = full[2] ? g.nc[2] : 2 * k[2] + 1;
-  const float rc2 = rc * rc * 1.0001f;
+  const float rcm = rc + TM_TRIM_MARGIN;
+  const float rc2 = rcm * rcm;
+  const float inv_cs0 = 1.0f / g.cs[0];
   int nr = 0;   // wave-uniform
   for (int base = 0; base < ny * nz; base += 64) {
     int t = base + lane;
@@ -67,19 +98,23 @@ __device__ inline int tm_build_ranges(int i, const CellGrid& g, float rc, RangeL
       if (full[2]) { z = oz; dz = z - cz; if (g.periodic) { if (dz > g.nc[2] / 2) dz -= g.nc[2]; if (dz < -g.nc[2] / 2) dz += g.nc[2]; } }
       else { dz = oz - k[2]; z = cz + dz; if (g.periodic) z = ((z % g.nc[2]) + g.nc[2]) % g.nc[2]; else ok = ok && z >= 0 && z < g.nc[2]; }
       if (ok) {
-        float my = fmaxf(0.f, (float)(abs(dy) - 1)) * g.cs[1];
-        float mz = fmaxf(0.f, (float)(abs(dz) - 1)) * g.cs[2];
+        float my = full[1] ? fmaxf(0.f, (float)(abs(dy) - 1)) * g.cs[1] : tm_band_dist(dy, f[1], g.cs[1]);
+        float mz = full[2] ? fmaxf(0.f, (float)(abs(dz) - 1)) * g.cs[2] : tm_band_dist(dz, f[2], g.cs[2]);
         float rem = rc2 - my * my - mz * mz;
         if (rem > 0.f) {
-          int kx = (int)floorf(sqrtf(rem) / g.cs[0]) + 1;
+          const float sx = sqrtf(rem);
+          int xlo = cx + (int)floorf((f[0] - sx) * inv_cs0);
+          int xhi = cx + (int)floorf((f[0] + sx) * inv_cs0);
           int row = g.nc[0] * (y + g.nc[1] * z);
-          if (g.periodic && 2 * kx + 1 >= g.nc[0]) {
+          if (g.periodic && xhi - xlo + 1 >= g.nc[0]) {
             lo0 = g.cell_start[row]; hi0 = g.cell_start[row + g.nc[0]]; cnt = 1;
           } else if (!g.periodic) {
-            int xlo = max(cx - kx, 0), xhi = min(cx + kx, g.nc[0] - 1);
-            lo0 = g.cell_start[row + xlo]; hi0 = g.cell_start[row + xhi + 1]; cnt = 1;
+            xlo = max(xlo, 0);
+            xhi = min(xhi, g.nc[0] - 1);
+            if (xlo <= xhi) {
+              lo0 = g.cell_start[row + xlo]; hi0 = g.cell_start[row + xhi + 1]; cnt = 1;
+            }
           } else {
-            int xlo = cx - kx, xhi = cx + kx;
             if (xlo < 0) {
               lo0 = g.cell_start[row + xlo + g.nc[0]]; hi0 = g.cell_start[row + g.nc[0]];
               lo1 = g.cell_start[row]; hi1 = g.cell_start[row + xhi + 1]; cnt = 2;
@@ -145,7 +180,8 @@ __device__ __forceinline__ int tm_candidate(const RangeLds<CAP>& R, int nr, int 
 template <int CAP, class F>
 __device__ __forceinline__ void tm_sweep(int i, const int* __restrict__ mol,
                                          const int* __restrict__ mol_start, const CellGrid& g,
-                                         float rc, RangeLds<CAP>& R, int lane, int* flags, F&& f) {
+                                         float rc, RangeLds<CAP>& R, int lane, int* flags,
+                                         double px, double py, double pz, F&& f) {
   if (g.cell_atoms == nullptr) {
     const int m = mol[i];
     const int lo = mol_start[m], hi = mol_start[m + 1];
@@ -156,7 +192,7 @@ __device__ __forceinline__ void tm_sweep(int i, const int* __restrict__ mol,
     return;
   }
   int nr = 0;
-  const int total = tm_build_ranges(i, g, rc, R, lane, flags, &nr);
+  const int total = tm_build_ranges(i, g, rc, R, lane, flags, &nr, px, py, pz);
   // Walk the concatenated ranges 64 candidates at a time.  The ranges that a
   // chunk touches start at r0 (wave-uniform); lane k holds range r0 + k, and
   // each lane finds its own range by counting, with shuffles, the range

@@ -202,7 +202,8 @@ coulomb_kernel(int N, const float4* __restrict__ xq, const int* __restrict__ ele
       __syncwarp();
     }
   };
-  tm_sweep(i, mol, mol_start, grid, a.rc, rng[w], lane, flags, visit);
+  tm_sweep(i, mol, mol_start, grid, a.rc, rng[w], lane, flags, (double)pi.x, (double)pi.y,
+           (double)pi.z, visit);
   drain(qn);
   double sphi = tm_wave_sum_d(acc.phi);
   double gx = tm_wave_sum_d(acc.gx);
@@ -392,7 +393,8 @@ int tm_charge_upstream(int N, const int* mol, const double* phi, const double* p
 int tm_coulomb(int N, const float* xq, const int* elem, const int* mol, const int* mol_start,
                const double* box, const int* cell_start, const int* cell_atoms, const int* cell_of,
                const int* cell_pos, const int* molc, int nx, int ny, int nz, double csx, double csy,
-               double csz, int periodic, const double* consts, double rc_ee, double width,
+               double csz, double ox, double oy, double oz, int periodic, const double* consts,
+               double rc_ee, double width,
                int add_ecc, int has_vdw, int nele, const double* c6, const double* r0, float* xqc,
                int* elc, double* phi, double* grad, double* eatom, int* flags, void* stream) {
   TM_REQUIRE(nele > 0 && nele <= TM_MAX_ELE, "tm_coulomb: bad nele");
@@ -424,6 +426,7 @@ int tm_coulomb(int N, const float* xq, const int* elem, const int* mol, const in
   g.cell_pos = cell_pos; g.posc = nullptr; g.molc = molc;
   g.nc[0] = nx; g.nc[1] = ny; g.nc[2] = nz;
   g.cs[0] = (float)csx; g.cs[1] = (float)csy; g.cs[2] = (float)csz;
+  g.org[0] = (float)ox; g.org[1] = (float)oy; g.org[2] = (float)oz;
   g.periodic = periodic;
   hipStream_t s = (hipStream_t)stream;
   if (cell_atoms != nullptr) {

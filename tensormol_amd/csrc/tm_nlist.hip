@@ -87,7 +87,7 @@ nl_rows_kernel(int N, const double* __restrict__ pos, const int* __restrict__ el
       }
       count += __popcll(bal);
     };
-    tm_sweep(i, mol, mol_start, grid, rc, rng[w], lane, flags, visit);
+    tm_sweep(i, mol, mol_start, grid, rc, rng[w], lane, flags, xi, yi, zi, visit);
     if (count > NL_LDS_CAP || count > ldr) {
       if (lane == 0) atomicOr(&flags[TM_FLAG_NB_OVERFLOW], 1);
       count = min(count, min(NL_LDS_CAP, ldr));
@@ -375,7 +375,7 @@ pairs_kernel(int N, const double* __restrict__ pos, const int* __restrict__ mol,
     if (FILL && ok) out[o + cnt + __popcll(bal & ((1ull << lane) - 1ull))] = j;
     cnt += __popcll(bal);
   };
-  tm_sweep(i, mol, mol_start, grid, rc, rng[w], lane, flags, visit);
+  tm_sweep(i, mol, mol_start, grid, rc, rng[w], lane, flags, xi, yi, zi, visit);
   if (!FILL && lane == 0) count[i] = cnt;
 }
 
@@ -385,12 +385,14 @@ extern "C" {
 
 static CellGrid make_grid(const int* cell_start, const int* cell_atoms, const int* cell_of,
                           const int* cell_pos, const double* posc, const int* molc, int nx, int ny,
-                          int nz, double csx, double csy, double csz, int periodic) {
+                          int nz, double csx, double csy, double csz, double ox, double oy,
+                          double oz, int periodic) {
   CellGrid g;
   g.cell_start = cell_start; g.cell_atoms = cell_atoms; g.cell_of = cell_of;
   g.cell_pos = cell_pos; g.posc = posc; g.molc = molc;
   g.nc[0] = nx; g.nc[1] = ny; g.nc[2] = nz;
   g.cs[0] = (float)csx; g.cs[1] = (float)csy; g.cs[2] = (float)csz;
+  g.org[0] = (float)ox; g.org[1] = (float)oy; g.org[2] = (float)oz;
   g.periodic = periodic;
   return g;
 }
@@ -430,7 +432,7 @@ int tm_cell_build(int N, const double* pos, const int* mol, double ox, double oy
 int tm_nl_build(int N, const double* pos, const int* elem, const int* mol, const int* mol_start,
                 const double* box, const int* cell_start, const int* cell_atoms,
                 const int* cell_of, const int* cell_pos, const double* posc, const int* molc,
-                int nx, int ny, int nz, double csx, double csy, double csz, int periodic, int nele,
+                int nx, int ny, int nz, double csx, double csy, double csz, double ox, double oy, double oz, int periodic, int nele,
                 double rc, double s_r, double s_a, int ldr, int lda, int* nbr, int* ncnt,
                 int* rseg, int* ang, int* acnt, int* aseg, int* rev, int* flags, void* stream) {
   TM_REQUIRE(N >= 0 && nele > 0 && nele <= TM_MAX_ELE, "tm_nl_build: bad nele %d", nele);
@@ -446,7 +448,7 @@ int tm_nl_build(int N, const double* pos, const int* elem, const int* mol, const
   if (N == 0) return TM_OK;
   hipStream_t s = (hipStream_t)stream;
   CellGrid g = make_grid(cell_start, cell_atoms, cell_of, cell_pos, posc, molc, nx, ny, nz, csx, csy,
-                         csz, periodic);
+                         csz, ox, oy, oz, periodic);
   nl_rows_kernel<<<tm_div_up(N, NL_WAVES), NL_WAVES * 64, 0, s>>>(
       N, pos, elem, mol, mol_start, box, g, nele, s_r, s_a, (float)rc, ldr, lda, nbr, ncnt, rseg,
       ang, acnt, aseg, flags);
@@ -473,7 +475,7 @@ int tm_triples(int N, int ldr, int lda, int nele, const int* nbr, const int* ang
 int tm_pairs(int N, const double* pos, const int* mol, const int* mol_start, const int* nreal,
              const double* box, const int* cell_start, const int* cell_atoms, const int* cell_of,
              const int* cell_pos, const double* posc, const int* molc, int nx, int ny, int nz,
-             double csx, double csy, double csz, int periodic, double rc, double s_c, int half,
+             double csx, double csy, double csz, double ox, double oy, double oz, int periodic, double rc, double s_c, int half,
              int* count, const long* start, int* out, int* flags, void* stream) {
   TM_REQUIRE(N >= 0, "tm_pairs: bad sizes");
   TM_REQUIRE(start == nullptr || out != nullptr, "tm_pairs: fill pass needs an output array");
@@ -485,7 +487,7 @@ int tm_pairs(int N, const double* pos, const int* mol, const int* mol_start, con
   if (N == 0) return TM_OK;
   hipStream_t s = (hipStream_t)stream;
   CellGrid g = make_grid(cell_start, cell_atoms, cell_of, cell_pos, posc, molc, nx, ny, nz, csx, csy,
-                         csz, periodic);
+                         csz, ox, oy, oz, periodic);
   if (start == nullptr)
     pairs_kernel<false><<<tm_div_up(N, NL_WAVES), NL_WAVES * 64, 0, s>>>(
         N, pos, mol, mol_start, nreal, box, g, (float)rc, s_c, half, count, nullptr, nullptr, flags);

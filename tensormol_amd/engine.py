@@ -268,12 +268,13 @@ class BPEEEngine:
         ws.molc = torch.zeros(ws.N, **i32) if ws.B > 1 else None
 
     def _grid_args(self, ws):
-        """(cell_start, cell_atoms, cell_of, cell_pos, posc, molc, nx, ny, nz, cx, cy, cz)."""
+        """(cell_start, cell_atoms, cell_of, cell_pos, posc, molc, nx, ny, nz, cx, cy, cz,
+        ox, oy, oz)."""
         if not ws.use_cells:
-            return (None, None, None, None, None, None, 1, 1, 1, 1.0, 1.0, 1.0)
-        n, c = ws.grid_nc, ws.grid_cs
+            return (None, None, None, None, None, None, 1, 1, 1, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0)
+        n, c, o = ws.grid_nc, ws.grid_cs, ws.grid_origin
         return (_P(ws.cell_start), _P(ws.cell_atoms), _P(ws.cell_of), _P(ws.cell_pos), _P(ws.posc),
-                _P(ws.molc), n[0], n[1], n[2], c[0], c[1], c[2])
+                _P(ws.molc), n[0], n[1], n[2], c[0], c[1], c[2], o[0], o[1], o[2])
 
     # ---------------------------------------------------------- stages
     def _stream(self):
@@ -334,7 +335,7 @@ class BPEEEngine:
         g = self._grid_args(ws)
         L.tm_coulomb(ws.N, _P(ws.xq), _P(ws.elem), _P(ws.mol), _P(ws.mol_start), _P(ws.box),
                      g[0], g[1], g[2], g[3], g[5], g[6], g[7], g[8], g[9], g[10], g[11],
-                     ws.periodic, self.dsf.ctypes.data_as(_lib.P), self.model.ee_off, self.vdw_width,
+                     g[12], g[13], g[14], ws.periodic, self.dsf.ctypes.data_as(_lib.P), self.model.ee_off, self.vdw_width,
                      int(bool(PARAMS["AddEcc"])), int(self.model.has_vdw), self.nele,
                      self.c6.ctypes.data_as(_lib.P), self.r0.ctypes.data_as(_lib.P),
                      _P(ws.xqc), _P(ws.elc), _P(ws.phi), _P(ws.gradc), _P(ws.eatom), _P(ws.flags),

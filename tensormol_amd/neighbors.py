@@ -76,7 +76,7 @@ def _pairs(pk, rc, half):
     count = torch.zeros(max(1, pk.N), dtype=torch.int32, device=pk.dev)
     flags = torch.zeros(8, dtype=torch.int32, device=pk.dev)
     args = (pk.N, _P(pk.pos), _P(pk.mol), _P(pk.mol_start), _P(pk.nreal), None, None, None, None,
-            None, None, None, 1, 1, 1, 1.0, 1.0, 1.0, 0, float(rc), thr, int(half))
+            None, None, None, 1, 1, 1, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, 0, float(rc), thr, int(half))
     L.tm_pairs(*args, _P(count), None, None, _P(flags), st)
     cnt = count[:pk.N].cpu().numpy().astype(np.int64)
     start = np.concatenate([[0], np.cumsum(cnt)]).astype(np.int64)
@@ -232,7 +232,8 @@ class NeighborListSet:
         rev = torch.zeros(max(1, N * ldr), **i32)
         flags = torch.zeros(8, **i32)
         L.tm_nl_build(N, _P(pk.pos), _P(el), _P(pk.mol), _P(pk.mol_start), None, None, None, None,
-                      None, None, None, 1, 1, 1, 1.0, 1.0, 1.0, 0, nele, float(rr), nl_threshold(rr),
+                      None, None, None, 1, 1, 1, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0, 0, nele, float(rr),
+                      nl_threshold(rr),
                       nl_threshold(ra), ldr, lda, _P(nbr), _P(ncnt), _P(rseg), _P(ang), _P(acnt),
                       _P(aseg), _P(rev), _P(flags), st)
         f = flags.cpu().numpy()

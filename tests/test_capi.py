@@ -50,8 +50,8 @@ def test_version_and_argument_errors(lib):
     assert lib.tm_version() == 1
     with pytest.raises(RuntimeError, match="bad nele"):
         lib.tm_nl_build(10, None, None, None, None, None, None, None, None, None, None, None, 1, 1, 1,
-                        1.0, 1.0, 1.0, 0, 0, 4.6, 1.0, 1.0, 128, 48, None, None, None, None, None,
-                        None, None, None, None)
+                        1.0, 1.0, 1.0, 0.0, 0.0, 0.0, 0, 0, 4.6, 1.0, 1.0, 128, 48, None, None, None,
+                        None, None, None, None, None, None)
     with pytest.raises(RuntimeError, match="3 hidden layers"):
         out = (ctypes.c_long * 9)()
         lib.tm_mlp_mfma_pack_info(256, 512, 2, 0, out)
