@@ -566,10 +566,20 @@ int tm_mlp_mfma(int N, int D, int H, int ntiles, const int* tiles, const int* se
   a.dG = dG;
   a.flags = nullptr;
   hipStream_t s = (hipStream_t)stream;
+  // 32-row tiles fit two workgroups per CU.  With no more tiles than CUs that
+  // would let the dispatcher stack two on one CU while others idle, so the
+  // launch then asks for extra LDS: one workgroup per CU.
+  size_t pad = 0;
+  if (tile_rows == 32 && mode != 2) {
+    int dev = 0, ncu = 0;
+    TM_HIP_CHECK(hipGetDevice(&dev));
+    TM_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    if (ntiles <= ncu) pad = 16384;
+  }
   if (mode == 0 && tile_rows == 64) mlp_mfma_kernel<0, 64><<<ntiles, NW * 64, 0, s>>>(a);
-  else if (mode == 0) mlp_mfma_kernel<0, 32><<<ntiles, NW * 64, 0, s>>>(a);
+  else if (mode == 0) mlp_mfma_kernel<0, 32><<<ntiles, NW * 64, pad, s>>>(a);
   else if (mode == 1 && tile_rows == 64) mlp_mfma_kernel<1, 64><<<ntiles, NW * 64, 0, s>>>(a);
-  else if (mode == 1) mlp_mfma_kernel<1, 32><<<ntiles, NW * 64, 0, s>>>(a);
+  else if (mode == 1) mlp_mfma_kernel<1, 32><<<ntiles, NW * 64, pad, s>>>(a);
   else mlp_mfma_kernel<2, 32><<<ntiles, NW * 64, 0, s>>>(a);
   TM_LAUNCH_CHECK();
   return TM_OK;

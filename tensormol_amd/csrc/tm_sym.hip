@@ -38,6 +38,7 @@ struct SymArgs {
   float eta, zeta, rr, ra;        // AN1 parameters and cutoffs (A)
   float two_1mz;                  // 2^(1-zeta)
   int zeta_int;                   // zeta when it is a small positive integer, else 0
+  float kr, ka;                    // pi / Rc_r, pi / Rc_a (pi as the reference's literal)
   float rs_r[NR_MAX];
   float rs_a[NA];
   float cos_t[NT], sin_t[NT];
@@ -58,6 +59,25 @@ __device__ __forceinline__ float powz(float b, const SymArgs& a) {
   return b > 0.f ? __expf(a.zeta * __logf(b)) : 0.f;
 }
 
+// (1 + v)^zeta and zeta (1 + v)^(zeta - 1) for b = 1 + v >= 0, without a division
+// when zeta is a small integer: r = b^(zeta - 1) by square-and-multiply.
+__device__ __forceinline__ void powz_pair(float b, const SymArgs& a, float& Az, float& dAz) {
+  if (a.zeta_int > 0) {
+    const int e = a.zeta_int - 1;
+    float r = 1.f, x = b;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      r = ((e >> k) & 1) ? r * x : r;
+      x *= x;
+    }
+    Az = r * b;
+    dAz = a.zeta * r;
+    return;
+  }
+  Az = b > 0.f ? __expf(a.zeta * __logf(b)) : 0.f;
+  dAz = b > 0.f ? a.zeta * Az * __builtin_amdgcn_rcpf(b) : 0.f;
+}
+
 struct RowLds {
   float4 d[SYM_WAVES][MAX_ROW];      // (x_j - x_i, r)
   float fc[SYM_WAVES][MAX_ROW];      // radial cutoff function
@@ -69,21 +89,25 @@ __device__ __forceinline__ void load_row(int i, int w, int lane, const double* _
                                          const double* __restrict__ box, const int* __restrict__ mol,
                                          const int* __restrict__ elem, const int* __restrict__ nbr,
                                          int n, int ldr, const SymArgs& a, RowLds& L) {
-  double Lx = 0, Ly = 0, Lz = 0;
+  // minimum image with the reciprocal edge (iL = 0 for open boundaries): it
+  // differs from d - L rint(d / L) only at |d| = L/2, far beyond the cutoff
+  double Lx = 0, Ly = 0, Lz = 0, iLx = 0, iLy = 0, iLz = 0;
   if (box != nullptr) {
     int m = mol[i];
     Lx = box[3 * m]; Ly = box[3 * m + 1]; Lz = box[3 * m + 2];
+    iLx = 1.0 / Lx; iLy = 1.0 / Ly; iLz = 1.0 / Lz;
   }
   const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
   for (int q = lane; q < n; q += 64) {
     int j = nbr[(long)i * ldr + q];
-    float dx = (float)tm_min_image(pos[3 * j] - xi, Lx);
-    float dy = (float)tm_min_image(pos[3 * j + 1] - yi, Ly);
-    float dz = (float)tm_min_image(pos[3 * j + 2] - zi, Lz);
+    double ex = pos[3 * j] - xi, ey = pos[3 * j + 1] - yi, ez = pos[3 * j + 2] - zi;
+    float dx = (float)(ex - Lx * rint(ex * iLx));
+    float dy = (float)(ey - Ly * rint(ey * iLy));
+    float dz = (float)(ez - Lz * rint(ez * iLz));
     float r = sqrtf(dx * dx + dy * dy + dz * dz + 1e-27f);
     L.d[w][q] = make_float4(dx, dy, dz, r);
-    L.fc[w][q] = 0.5f * (__cosf(PI_LIT * r / a.rr) + 1.0f);
-    L.fa[w][q] = 0.5f * (__cosf(PI_LIT * r / a.ra) + 1.0f);
+    L.fc[w][q] = 0.5f * (__cosf(a.kr * r) + 1.0f);
+    L.fa[w][q] = 0.5f * (__cosf(a.ka * r) + 1.0f);
     L.el[w][q] = elem[j];
   }
 }
@@ -178,7 +202,8 @@ sym_fwd_kernel(int N, const double* __restrict__ pos, const double* __restrict__
       decode_triple(t, segs[w][1], a.nele, ja, kb, p);
       int qj = arow[ja], qk = arow[kb];
       float4 dj = L.d[w][qj], dk = L.d[w][qk];
-      float c = (dj.x * dk.x + dj.y * dk.y + dj.z * dk.z) / (dj.w * dk.w);
+      float c = (dj.x * dk.x + dj.y * dk.y + dj.z * dk.z) *
+                __builtin_amdgcn_rcpf(dj.w * dk.w);
       c = fminf(fmaxf(c, -1.0f), 1.0f);
       float sn = sqrtf(fmaxf(1.0f - c * c, 0.0f));
       float ravg = 0.5f * (dj.w + dk.w);
@@ -219,7 +244,7 @@ sym_fwd_kernel(int N, const double* __restrict__ pos, const double* __restrict__
 }
 
 // dE/dG (two contributions combined: gE + u_i * gQ) -> w slots
-__global__ void __launch_bounds__(SYM_WAVES * 64)
+__global__ void __launch_bounds__(SYM_WAVES * 64) __attribute__((amdgpu_waves_per_eu(4)))
 sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__ box,
                const int* __restrict__ mol, const int* __restrict__ elem,
                const int* __restrict__ nbr, const int* __restrict__ ncnt,
@@ -251,7 +276,7 @@ sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__
   load_row(i, w, lane, pos, box, mol, elem, nbr, n, ldr, a, L);
   __syncwarp();
   // ---------------- radial: lane = neighbour
-  const float kr = PI_LIT / a.rr;
+  const float kr = a.kr;
   for (int q = lane; q < n; q += 64) {
     float4 d = L.d[w][q];
     float r = d.w;
@@ -264,7 +289,7 @@ sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__
       float ex = __expf(-a.eta * x * x);
       dEdr += gw[e * a.nr + s] * ex * (dfc - 2.0f * a.eta * x * fc);
     }
-    float sc = dEdr / r;
+    float sc = dEdr * __builtin_amdgcn_rcpf(r);
     wacc[w][q] = make_float4(sc * d.x, sc * d.y, sc * d.z, 0.f);
   }
   __syncwarp();
@@ -273,7 +298,7 @@ sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__
   const int* arow = ang + (long)i * lda;
   const int ntrip = m * (m - 1) / 2;
   const int off = a.nele * a.nr;
-  const float ka = PI_LIT / a.ra;
+  const float ka = a.ka;
   float4 wang = make_float4(0.f, 0.f, 0.f, 0.f);   // lane = sub-list member
   for (int base = 0; base < ntrip; base += 64) {
     int t = base + lane;
@@ -283,12 +308,13 @@ sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__
       int qj = arow[ja], qk = arow[kb];
       float4 dj = L.d[w][qj], dk = L.d[w][qk];
       float rj = dj.w, rk = dk.w;
-      float inv = 1.0f / (rj * rk);
+      const float irj = __builtin_amdgcn_rcpf(rj), irk = __builtin_amdgcn_rcpf(rk);
+      float inv = irj * irk;
       float c = (dj.x * dk.x + dj.y * dk.y + dj.z * dk.z) * inv;
       c = fminf(fmaxf(c, -1.0f), 1.0f);
       float s2 = fmaxf(1.0f - c * c, 0.0f);
       float sn = sqrtf(s2);
-      float cos_over_sin = sn > 1e-6f ? c / sn : 0.f;
+      float cos_over_sin = sn > 1e-6f ? c * __builtin_amdgcn_rcpf(sn) : 0.f;
       float ravg = 0.5f * (rj + rk);
       float fj = L.fa[w][qj], fk = L.fa[w][qk];
       float dfj = -0.5f * ka * __sinf(ka * rj), dfk = -0.5f * ka * __sinf(ka * rk);
@@ -313,8 +339,8 @@ sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__
         }
         float v = c * a.cos_t[tt] + sn * a.sin_t[tt];
         float b = 1.0f + v;
-        float Az = powz(b, a);                     // (1+v)^zeta
-        float dAz = b > 0.f ? a.zeta * Az / b : 0.f;  // zeta (1+v)^(zeta-1)
+        float Az, dAz;                             // (1+v)^zeta, zeta (1+v)^(zeta-1)
+        powz_pair(b, a, Az, dAz);
         float dv = a.cos_t[tt] - cos_over_sin * a.sin_t[tt];
         dEdc += a.two_1mz * dAz * dv * S;
         dEdrav += a.two_1mz * Az * Sp;
@@ -323,8 +349,8 @@ sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__
       dEdc *= F;
       dEdrav *= F;
       // chain rule into d_j, d_k
-      float cj = dEdrav * 0.5f / rj + dEdF * dfj * fk / rj - dEdc * c / (rj * rj);
-      float ck = dEdrav * 0.5f / rk + dEdF * fj * dfk / rk - dEdc * c / (rk * rk);
+      float cj = (dEdrav * 0.5f + dEdF * dfj * fk - dEdc * c * irj) * irj;
+      float ck = (dEdrav * 0.5f + dEdF * fj * dfk - dEdc * c * irk) * irk;
       float x = dEdc * inv;
       ca[w][lane] = make_float4(cj * dj.x + x * dk.x, cj * dj.y + x * dk.y, cj * dj.z + x * dk.z, 0.f);
       cb[w][lane] = make_float4(ck * dk.x + x * dj.x, ck * dk.y + x * dj.y, ck * dk.z + x * dj.z, 0.f);
@@ -430,7 +456,7 @@ sym_jvp_kernel(int N, const double* __restrict__ pos, const double* __restrict__
     segs[w][1][lane] = aseg[(long)i * (a.nele + 1) + lane];
   }
   load_row(i, w, lane, pos, box, mol, elem, nbr, n, ldr, a, L);
-  const float kr = PI_LIT / a.rr, ka = PI_LIT / a.ra;
+  const float kr = a.kr, ka = a.ka;
   {
     const double uxi = du[3 * i], uyi = du[3 * i + 1], uzi = du[3 * i + 2];
     __syncwarp();
@@ -439,7 +465,7 @@ sym_jvp_kernel(int N, const double* __restrict__ pos, const double* __restrict__
       const float tx = (float)(du[3 * j] - uxi), ty = (float)(du[3 * j + 1] - uyi);
       const float tz = (float)(du[3 * j + 2] - uzi);
       const float4 d = L.d[w][q];
-      T.t[w][q] = make_float4(tx, ty, tz, (d.x * tx + d.y * ty + d.z * tz) / d.w);
+      T.t[w][q] = make_float4(tx, ty, tz, (d.x * tx + d.y * ty + d.z * tz) * __builtin_amdgcn_rcpf(d.w));
       T.dfc[w][q] = -0.5f * kr * __sinf(kr * d.w);
       T.dfa[w][q] = -0.5f * ka * __sinf(ka * d.w);
     }
@@ -484,17 +510,18 @@ sym_jvp_kernel(int N, const double* __restrict__ pos, const double* __restrict__
       float4 dj = L.d[w][qj], dk = L.d[w][qk];
       float4 uj = T.t[w][qj], uk = T.t[w][qk];
       float rj = dj.w, rk = dk.w;
-      float inv = 1.0f / (rj * rk);
+      const float irj = __builtin_amdgcn_rcpf(rj), irk = __builtin_amdgcn_rcpf(rk);
+      float inv = irj * irk;
       float c = (dj.x * dk.x + dj.y * dk.y + dj.z * dk.z) * inv;
       c = fminf(fmaxf(c, -1.0f), 1.0f);
       float sn = sqrtf(fmaxf(1.0f - c * c, 0.0f));
       float dc = (uj.x * dk.x + uj.y * dk.y + uj.z * dk.z + dj.x * uk.x + dj.y * uk.y +
-                  dj.z * uk.z) * inv - c * (uj.w / rj + uk.w / rk);
+                  dj.z * uk.z) * inv - c * (uj.w * irj + uk.w * irk);
       float fj = L.fa[w][qj], fk = L.fa[w][qk];
       float ravg = 0.5f * (rj + rk);
       tc[w][lane] = c;
       tsn[w][lane] = sn;
-      tcs[w][lane] = sn > 1e-6f ? c / sn : 0.f;
+      tcs[w][lane] = sn > 1e-6f ? c * __builtin_amdgcn_rcpf(sn) : 0.f;
       tf[w][lane] = fj * fk;
       tdf[w][lane] = T.dfa[w][qj] * uj.w * fk + fj * T.dfa[w][qk] * uk.w;
       tdc[w][lane] = dc;
@@ -523,8 +550,8 @@ sym_jvp_kernel(int N, const double* __restrict__ pos, const double* __restrict__
       }
       float v = tc[w][u] * ct + tsn[w][u] * st;
       float b = 1.0f + v;
-      float Az = powz(b, a);
-      float dAz = b > 0.f ? a.zeta * Az / b : 0.f;
+      float Az, dAz;
+      powz_pair(b, a, Az, dAz);
       float dv = ct - tcs[w][u] * st;
       float R = tR[w][os][u], F = tf[w][u];
       acc += a.two_1mz * (dAz * dv * tdc[w][u] * R * F +
@@ -550,6 +577,8 @@ int fill_args(SymArgs& a, int nele, int nr, float eta, float zeta, float rr, flo
   a.D = nele * nr + a.nelep * NA * NT;
   TM_REQUIRE(a.D <= 1024, "sym: descriptor length %d above 1024", a.D);
   a.eta = eta; a.zeta = zeta; a.rr = rr; a.ra = ra;
+  a.kr = PI_LIT / rr;
+  a.ka = PI_LIT / ra;
   a.two_1mz = powf(2.0f, 1.0f - zeta);
   a.zeta_int = (zeta == floorf(zeta) && zeta >= 1.f && zeta <= 64.f) ? (int)zeta : 0;
   for (int s = 0; s < nr; ++s) a.rs_r[s] = rs_r[s];
