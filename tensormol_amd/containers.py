@@ -31,12 +31,31 @@ class Mol:
     def AtomTypes(self):
         return np.unique(self.atoms)
 
+    @staticmethod
+    def ParseProperties(s_):
+        """Comment line 'Comment: ;;;energy E;;;...' -> {'energy': E}
+        (Mol.py:268-285: only the lowercase 'energy' property is read here;
+        the reference's binary 'Lattice' field is not portable text)."""
+        props = {}
+        if "Comment:" not in s_:
+            return props
+        for prop in s_.split("Comment:", 1)[1].split(";;;"):
+            f = prop.split()
+            if len(f) >= 2 and f[0] == "energy":
+                props["energy"] = float(f[1])
+        return props
+
+    def PropertyString(self):
+        """';;;name value' for every property (Mol.py:286-299)."""
+        return "".join(f";;;{k} {v}" for k, v in self.properties.items())
+
     def FromXYZString(self, string):
+        """XYZ block -> atoms, coords, properties (Mol.py:300-330)."""
         lines = string.split("\n")
         natoms = int(lines[0].split()[0])
         if len(lines[1].split()) > 1:
             try:
-                self.properties["energy"] = float(lines[1].split()[1])
+                self.properties = self.ParseProperties(lines[1])
             except ValueError:
                 pass
         atoms, coords = [], []
@@ -48,7 +67,7 @@ class Mol:
         self.atoms = np.array(atoms, dtype=np.uint8)
         self.coords = np.array(coords, dtype=np.float64)
 
-    def XYZString(self, comment=""):
+    def XYZString(self, comment="Comment: "):
         out = [str(self.NAtoms()), comment]
         for z, c in zip(self.atoms, self.coords):
             out.append(f"{_NAMES.get(int(z), str(int(z)))} {c[0]:.10f} {c[1]:.10f} {c[2]:.10f}")
@@ -56,7 +75,7 @@ class Mol:
 
     def WriteXYZfile(self, fpath=".", fname="mol", mode="a", wprop=False):
         os.makedirs(fpath, exist_ok=True)
-        comment = " ".join(f"{k}: {v}" for k, v in self.properties.items()) if wprop else ""
+        comment = "Comment: " + (self.PropertyString() if wprop else "")
         with open(os.path.join(fpath, fname + ".xyz"), mode) as f:
             f.write(self.XYZString(comment))
 

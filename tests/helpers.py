@@ -32,7 +32,23 @@ def system(name):
     if name == "chno_small":
         Z, x = W.chno_sphere(natom=1200, seed=11)
         return Z, x, np.array([0, len(Z)]), None
+    if name in GOLDEN:
+        mols = golden_mols(GOLDEN[name][0])[: GOLDEN[name][1]]
+        return W.pack([(m.atoms.astype(np.int32), m.coords) for m in mols])[:3] + (None,)
     raise KeyError(name)
+
+
+# geometries from the reference's datasets/ (tests/golden): file, molecules used
+GOLDEN = {"morphine": ("morphine", 1), "water10": ("water10", 1), "peptide188": ("2mzx_tm_opt", 1),
+          "johnsonmols": ("johnsonmols", 10)}
+
+
+def golden_mols(fname):
+    import os
+    from tensormol_amd.containers import MSet
+    ms = MSet(fname, path_=os.path.join(os.path.dirname(__file__), "golden"), center_=False)
+    ms.ReadXYZ()
+    return ms.mols
 
 
 @functools.lru_cache(maxsize=None)

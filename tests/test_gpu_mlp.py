@@ -66,7 +66,8 @@ def test_mfma_mlp_vs_torch_fp32(cuda, precision, name, modelname):
 
 @pytest.mark.parametrize("precision", ["bf16", "fp16", "bf16x3"])
 @pytest.mark.parametrize("name,modelname", [("water64", "water"), ("chains", "chno"),
-                                            ("waterbox10", "water"), ("chno_small", "chno")])
+                                            ("waterbox10", "water"), ("chno_small", "chno"),
+                                            ("johnsonmols", "chno"), ("peptide188", "chno")])
 def test_energy_forces_mfma(cuda, precision, name, modelname):
     from tensormol_amd.engine import BPEEEngine
     eng = BPEEEngine(H.model_for(modelname), precision=precision)

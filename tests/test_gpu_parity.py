@@ -13,7 +13,8 @@ from tests import helpers as H
 
 pytestmark = pytest.mark.gpu
 
-SYSTEMS = [("water64", "water"), ("chains", "chno"), ("waterbox10", "water"), ("chno_small", "chno")]
+SYSTEMS = [("water64", "water"), ("chains", "chno"), ("waterbox10", "water"), ("chno_small", "chno"),
+           ("morphine", "chno"), ("water10", "water"), ("peptide188", "chno"), ("johnsonmols", "chno")]
 
 
 def _engine(modelname, precision="fp32"):
