@@ -46,12 +46,16 @@ struct SymArgs {
 
 constexpr float PI_LIT = 3.14159265359f;
 
+// b^zeta: ZI > 0 is zeta known at compile time (a few multiplies); ZI = 0 reads
+// zeta from the arguments (square-and-multiply when it is a small integer).
+template <int ZI>
 __device__ __forceinline__ float powz(float b, const SymArgs& a) {
-  if (a.zeta_int > 0) {   // square-and-multiply over the 7 bits of zeta <= 64
+  const int zi = ZI > 0 ? ZI : a.zeta_int;
+  if (zi > 0) {   // square-and-multiply over the 7 bits of zeta <= 64
     float r = 1.f, x = b;
 #pragma unroll
     for (int k = 0; k < 7; ++k) {
-      r = ((a.zeta_int >> k) & 1) ? r * x : r;
+      r = ((zi >> k) & 1) ? r * x : r;
       x *= x;
     }
     return r;
@@ -61,9 +65,11 @@ __device__ __forceinline__ float powz(float b, const SymArgs& a) {
 
 // (1 + v)^zeta and zeta (1 + v)^(zeta - 1) for b = 1 + v >= 0, without a division
 // when zeta is a small integer: r = b^(zeta - 1) by square-and-multiply.
+template <int ZI>
 __device__ __forceinline__ void powz_pair(float b, const SymArgs& a, float& Az, float& dAz) {
-  if (a.zeta_int > 0) {
-    const int e = a.zeta_int - 1;
+  const int zi = ZI > 0 ? ZI : a.zeta_int;
+  if (zi > 0) {
+    const int e = zi - 1;
     float r = 1.f, x = b;
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
@@ -71,7 +77,7 @@ __device__ __forceinline__ void powz_pair(float b, const SymArgs& a, float& Az, 
       x *= x;
     }
     Az = r * b;
-    dAz = a.zeta * r;
+    dAz = (ZI > 0 ? (float)ZI : a.zeta) * r;
     return;
   }
   Az = b > 0.f ? __expf(a.zeta * __logf(b)) : 0.f;
@@ -144,6 +150,7 @@ __device__ __forceinline__ void decode_triple(int t, const int* seg, int nele, i
   p = -1;
 }
 
+template <int ZI>
 __global__ void __launch_bounds__(SYM_WAVES * 64)
 sym_fwd_kernel(int N, const double* __restrict__ pos, const double* __restrict__ box,
                const int* __restrict__ mol, const int* __restrict__ elem,
@@ -231,7 +238,7 @@ sym_fwd_kernel(int N, const double* __restrict__ pos, const double* __restrict__
         acc = 0.f;
       }
       float v = tc[w][u] * ct + ts[w][u] * st;
-      acc += a.two_1mz * powz(1.0f + v, a) * tR[w][os][u] * tf[w][u];
+      acc += a.two_1mz * powz<ZI>(1.0f + v, a) * tR[w][os][u] * tf[w][u];
     }
     __syncwarp();
   }
@@ -244,6 +251,7 @@ sym_fwd_kernel(int N, const double* __restrict__ pos, const double* __restrict__
 }
 
 // dE/dG (two contributions combined: gE + u_i * gQ) -> w slots
+template <int ZI>
 __global__ void __launch_bounds__(SYM_WAVES * 64) __attribute__((amdgpu_waves_per_eu(4)))
 sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__ box,
                const int* __restrict__ mol, const int* __restrict__ elem,
@@ -257,7 +265,6 @@ sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__
   float* gw = gdyn + (threadIdx.x >> 6) * a.D;
   __shared__ float4 wacc[SYM_WAVES][MAX_ROW];
   __shared__ float4 ca[SYM_WAVES][64], cb[SYM_WAVES][64];
-  __shared__ int tja[SYM_WAVES][64], tkb[SYM_WAVES][64];
   __shared__ int segs[SYM_WAVES][2][TM_MAX_ELE + 1];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i = blockIdx.x * SYM_WAVES + w;
@@ -300,6 +307,11 @@ sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__
   const int off = a.nele * a.nr;
   const float ka = a.ka;
   float4 wang = make_float4(0.f, 0.f, 0.f, 0.f);   // lane = sub-list member
+  const int* sg = segs[w][1];
+  int eq = -1, lq = 0;   // element and in-element index of member `lane`
+  if (lane < m)
+    for (int e = 0; e < a.nele; ++e)
+      if (lane >= sg[e] && lane < sg[e + 1]) { eq = e; lq = lane - sg[e]; }
   for (int base = 0; base < ntrip; base += 64) {
     int t = base + lane;
     if (t < ntrip) {
@@ -340,7 +352,7 @@ sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__
         float v = c * a.cos_t[tt] + sn * a.sin_t[tt];
         float b = 1.0f + v;
         float Az, dAz;                             // (1+v)^zeta, zeta (1+v)^(zeta-1)
-        powz_pair(b, a, Az, dAz);
+        powz_pair<ZI>(b, a, Az, dAz);
         float dv = a.cos_t[tt] - cos_over_sin * a.sin_t[tt];
         dEdc += a.two_1mz * dAz * dv * S;
         dEdrav += a.two_1mz * Az * Sp;
@@ -354,19 +366,46 @@ sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__
       float x = dEdc * inv;
       ca[w][lane] = make_float4(cj * dj.x + x * dk.x, cj * dj.y + x * dk.y, cj * dj.z + x * dk.z, 0.f);
       cb[w][lane] = make_float4(ck * dk.x + x * dj.x, ck * dk.y + x * dj.y, ck * dk.z + x * dj.z, 0.f);
-      tja[w][lane] = ja;
-      tkb[w][lane] = kb;
     }
     __syncwarp();
-    int cnt = min(64, ntrip - base);
-    for (int v = 0; v < cnt; ++v) {
-      if (tja[w][v] == lane) {
-        float4 c4 = ca[w][v];
-        wang.x += c4.x; wang.y += c4.y; wang.z += c4.z;
-      }
-      if (tkb[w][v] == lane) {
-        float4 c4 = cb[w][v];
-        wang.x += c4.x; wang.y += c4.y; wang.z += c4.z;
+    // Member `lane` gathers its contributions from the triples [base, base + cnt)
+    // in ascending triple order, walking only its own partners: in a same-element
+    // block it is k of (j < k) at t = k(k-1)/2 + j, then j at t = k'(k'-1)/2 + j
+    // for k' > j; in a cross block (ea < eb) t = kb_local * la + ja_local.
+    if (eq >= 0) {
+      const int cnt = min(64, ntrip - base);
+      auto add = [&](const float4& c4) { wang.x += c4.x; wang.y += c4.y; wang.z += c4.z; };
+      int bo = 0;   // first triple of the block
+      for (int ea = 0; ea < a.nele; ++ea) {
+        const int la = sg[ea + 1] - sg[ea];
+        for (int eb = ea; eb < a.nele; ++eb) {
+          const int lb = sg[eb + 1] - sg[eb];
+          const int nb = ea == eb ? la * (la - 1) / 2 : la * lb;
+          const int lo = base - bo, hi = lo + cnt;   // chunk window, block-local
+          if (hi > 0 && lo < nb) {
+            if (ea == eb) {
+              if (eq == ea) {
+                const int t0 = lq * (lq - 1) / 2;
+                for (int r = max(0, lo - t0); r < min(lq, hi - t0); ++r) add(cb[w][t0 + r - lo]);
+                for (int r = lq + 1; r < la; ++r) {
+                  const int t = r * (r - 1) / 2 + lq;
+                  if (t >= hi) break;
+                  if (t >= lo) add(ca[w][t - lo]);
+                }
+              }
+            } else if (eq == ea) {
+              for (int kl = 0; kl < lb; ++kl) {
+                const int t = kl * la + lq;
+                if (t >= hi) break;
+                if (t >= lo) add(ca[w][t - lo]);
+              }
+            } else if (eq == eb) {
+              const int t0 = lq * la;
+              for (int r = max(0, lo - t0); r < min(la, hi - t0); ++r) add(cb[w][t0 + r - lo]);
+            }
+          }
+          bo += nb;
+        }
       }
     }
     __syncwarp();
@@ -433,6 +472,7 @@ struct JvpLds {
   float dfa[SYM_WAVES][MAX_ROW];    // d fc_a / dr
 };
 
+template <int ZI>
 __global__ void __launch_bounds__(SYM_WAVES * 64)
 sym_jvp_kernel(int N, const double* __restrict__ pos, const double* __restrict__ box,
                const int* __restrict__ mol, const int* __restrict__ elem,
@@ -551,7 +591,7 @@ sym_jvp_kernel(int N, const double* __restrict__ pos, const double* __restrict__
       float v = tc[w][u] * ct + tsn[w][u] * st;
       float b = 1.0f + v;
       float Az, dAz;
-      powz_pair(b, a, Az, dAz);
+      powz_pair<ZI>(b, a, Az, dAz);
       float dv = ct - tcs[w][u] * st;
       float R = tR[w][os][u], F = tf[w][u];
       acc += a.two_1mz * (dAz * dv * tdc[w][u] * R * F +
@@ -609,8 +649,9 @@ int tm_sym_forward(int N, const double* pos, const double* box, const int* mol, 
              ldr, lda, MAX_ROW, MAX_ANG);
   if (N == 0) return TM_OK;
   hipStream_t s = (hipStream_t)stream;
-  sym_fwd_kernel<<<tm_div_up(N, SYM_WAVES), SYM_WAVES * 64, 0, s>>>(
-      N, pos, box, mol, elem, nbr, ncnt, rseg, ang, acnt, aseg, ldr, lda, a, G, ldg);
+  auto k = a.zeta_int == 8 ? sym_fwd_kernel<8> : sym_fwd_kernel<0>;
+  k<<<tm_div_up(N, SYM_WAVES), SYM_WAVES * 64, 0, s>>>(N, pos, box, mol, elem, nbr, ncnt, rseg,
+                                                       ang, acnt, aseg, ldr, lda, a, G, ldg);
   TM_LAUNCH_CHECK();
   return TM_OK;
 }
@@ -629,7 +670,8 @@ int tm_sym_adjoint(int N, const double* pos, const double* box, const int* mol, 
              lda, MAX_ROW, MAX_ANG);
   if (N == 0) return TM_OK;
   hipStream_t s = (hipStream_t)stream;
-  sym_adj_kernel<<<tm_div_up(N, SYM_WAVES), SYM_WAVES * 64, sizeof(float) * SYM_WAVES * a.D, s>>>(
+  auto k = a.zeta_int == 8 ? sym_adj_kernel<8> : sym_adj_kernel<0>;
+  k<<<tm_div_up(N, SYM_WAVES), SYM_WAVES * 64, sizeof(float) * SYM_WAVES * a.D, s>>>(
       N, pos, box, mol, elem, nbr, ncnt, rseg, ang, acnt, aseg, ldr, lda, a, gE, gQ, u, ldg,
       (float4*)wslot);
   TM_LAUNCH_CHECK();
@@ -649,8 +691,9 @@ int tm_sym_jvp(int N, const double* pos, const double* box, const int* mol, cons
              MAX_ROW, MAX_ANG);
   if (N == 0) return TM_OK;
   hipStream_t s = (hipStream_t)stream;
-  sym_jvp_kernel<<<tm_div_up(N, SYM_WAVES), SYM_WAVES * 64, 0, s>>>(
-      N, pos, box, mol, elem, nbr, ncnt, rseg, ang, acnt, aseg, ldr, lda, a, du, V, ldg);
+  auto k = a.zeta_int == 8 ? sym_jvp_kernel<8> : sym_jvp_kernel<0>;
+  k<<<tm_div_up(N, SYM_WAVES), SYM_WAVES * 64, 0, s>>>(N, pos, box, mol, elem, nbr, ncnt, rseg,
+                                                       ang, acnt, aseg, ldr, lda, a, du, V, ldg);
   TM_LAUNCH_CHECK();
   return TM_OK;
 }
