@@ -186,8 +186,10 @@ int tm_charge_upstream(int N, const int* mol, const double* phi, const double* p
 /* Per-molecule column sums (or means, mean != 0) of an [N, C] array:
  * the sparse_reduce_sum / reduce_sum per molecule of RawSymFunc.py:1354-1359
  * and TFMolInstanceDirect.py:5212, in a fixed FP64 order.  scratch (may be
- * NULL; ceil(N/1024)*C doubles) lets one large molecule (B == 1, n_total =
- * N > 2048) be summed by many blocks. */
+ * NULL; 1 + ceil(N/1024)*C doubles, zero-filled before its first use) lets one
+ * large molecule (B == 1, n_total = N > 2048) be summed by many blocks in one
+ * launch: its first word is a completion counter that each call leaves at
+ * zero, so calls sharing a scratch buffer must not overlap. */
 int tm_mol_sum_f32(int B, const int* mol_start, int C, const float* in, int ldin, double* out,
                    int mean, double* scratch, int n_total, void* stream);
 int tm_mol_sum_f64(int B, const int* mol_start, int C, const double* in, int ldin, double* out,

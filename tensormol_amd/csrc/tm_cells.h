@@ -218,3 +218,60 @@ __device__ __forceinline__ void tm_sweep(int i, const int* __restrict__ mol,
     if (r0 + 1 < nr && R.pre[r0 + 1] <= base + 64) r0 += 1;
   }
 }
+
+// tm_sweep with the candidate loads of the next chunk in flight while the
+// current chunk is processed: load(p, valid, mapped) fetches what f needs for
+// one candidate and returns it by value; f(p, valid, mapped, payload) consumes
+// it.  Same candidates in the same order as tm_sweep.
+template <int CAP, class LD, class F>
+__device__ __forceinline__ void tm_sweep_pf(int i, const int* __restrict__ mol,
+                                            const int* __restrict__ mol_start, const CellGrid& g,
+                                            float rc, RangeLds<CAP>& R, int lane, int* flags,
+                                            double px, double py, double pz, LD&& load, F&& f) {
+  if (g.cell_atoms == nullptr) {
+    const int m = mol[i];
+    const int lo = mol_start[m], hi = mol_start[m + 1];
+    auto cur = load(lo + lane, lo + lane < hi, 0);
+    for (int base = lo; base < hi; base += 64) {
+      const int pn = base + 64 + lane;
+      auto nxt = load(pn, pn < hi, 0);
+      f(base + lane, base + lane < hi, 0, cur);
+      cur = nxt;
+    }
+    return;
+  }
+  int nr = 0;
+  const int total = tm_build_ranges(i, g, rc, R, lane, flags, &nr, px, py, pz);
+  int r0 = 0;
+  // cell-order position of candidate base + lane; advances r0 to the range
+  // holding candidate base + 64 (call with base = 0, 64, 128, ... in turn)
+  auto locate = [&](int base) -> int {
+    const int t = base + lane;
+    const int rk = r0 + lane;
+    const int pre_k = rk < nr ? R.pre[rk] : 0x7fffffff;
+    const int lo_k = rk < nr ? R.lo[rk] : 0;
+    const unsigned long long inside = __ballot(rk < nr && pre_k <= base + 63);
+    const int span = __popcll(inside);
+    int idx = 0;
+    for (int k = 1; k < span; ++k) idx += (__shfl(pre_k, k, 64) <= t) ? 1 : 0;
+    const int lo_r = __shfl(lo_k, idx, 64);
+    const int pre_r = __shfl(pre_k, idx, 64);
+    r0 += span - 1;
+    if (r0 + 1 < nr && R.pre[r0 + 1] <= base + 64) r0 += 1;
+    return t < total ? lo_r + (t - pre_r) : 0;
+  };
+  if (total <= 0) return;
+  int p = locate(0);
+  auto cur = load(p, lane < total, 1);
+  for (int base = 0; base < total; base += 64) {
+    const int pc = p;
+    const bool vc = base + lane < total;
+    // issued unconditionally (one chunk past the end on the last pass: r0
+    // stays in [0, nr - 1] and every lane is invalid), so the loads in flight
+    // are the same on every path
+    p = locate(base + 64);
+    auto nxt = load(p, base + 64 + lane < total, 1);
+    f(pc, vc, 1, cur);
+    cur = nxt;
+  }
+}

@@ -125,6 +125,9 @@ constexpr int QCAP = 128;   // compacted pair queue per wave
 // evaluated 64 at a time with every lane busy.  Queue order = candidate
 // order, so the per-lane sums and the final wave reduction are formed in a
 // fixed order.
+// MOLC: the cell-ordered candidates carry molecule ids (several systems share
+// the grid); a template parameter so the candidate loads are branch-free.
+template <bool MOLC>
 __global__ void __launch_bounds__(C_WAVES * 64)
 coulomb_kernel(int N, const float4* __restrict__ xq, const int* __restrict__ elem,
                const float4* __restrict__ xqc, const int* __restrict__ elc,
@@ -168,23 +171,37 @@ coulomb_kernel(int N, const float4* __restrict__ xq, const int* __restrict__ ele
       pair_terms(a, pi.w, d.w, vci[ej], vqi[ej], d.x, d.y, d.z, acc);
     }
   };
-  auto visit = [&](int p, bool valid, int mapped) {
-    bool ok = false;
+  // candidate payload, fetched one chunk ahead of its test
+  struct Cand {
+    float4 x;
+    int e, m;
+  };
+  // Branch-free, so the same number of loads is in flight on every path and
+  // the wait for a chunk's data does not also wait for the next chunk's.
+  // Invalid lanes read entry 0 (N >= 1) and are masked by `valid` in visit.
+  auto load = [&](int p, bool valid, int mapped) {
+    const int q = valid ? p : 0;
+    Cand c;
+    c.x = mapped ? xqc[q] : xq[q];
+    c.e = mapped ? elc[q] : elem[q];
+    c.m = (MOLC && mapped) ? grid.molc[q] : mi;
+    return c;
+  };
+  auto visit = [&](int p, bool valid, int mapped, const Cand& c) {
     float4 d = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (valid && (mapped ? p != cpi : p != i) &&
-        (!mapped || grid.molc == nullptr || grid.molc[p] == mi)) {
-      const float4 pj = mapped ? xqc[p] : xq[p];
-      d.x = min_image_f(pj.x - pi.x, Lx, iLx);
-      d.y = min_image_f(pj.y - pi.y, Ly, iLy);
-      d.z = min_image_f(pj.z - pi.z, Lz, iLz);
-      d.w = pj.w;
+    bool ok = false;
+    if (valid && (mapped ? p != cpi : p != i) && c.m == mi) {
+      d.x = min_image_f(c.x.x - pi.x, Lx, iLx);
+      d.y = min_image_f(c.x.y - pi.y, Ly, iLy);
+      d.z = min_image_f(c.x.z - pi.z, Lz, iLz);
+      d.w = c.x.w;
       ok = d.x * d.x + d.y * d.y + d.z * d.z < a.rc2;
     }
     unsigned long long bal = __ballot(ok);
     int before = __popcll(bal & ((1ull << lane) - 1ull));
     if (ok) {
       qx[w][qn + before] = d;
-      qe[w][qn + before] = mapped ? elc[p] : elem[p];
+      qe[w][qn + before] = c.e;
     }
     qn += __popcll(bal);
     __syncwarp();
@@ -202,8 +219,8 @@ coulomb_kernel(int N, const float4* __restrict__ xq, const int* __restrict__ ele
       __syncwarp();
     }
   };
-  tm_sweep(i, mol, mol_start, grid, a.rc, rng[w], lane, flags, (double)pi.x, (double)pi.y,
-           (double)pi.z, visit);
+  tm_sweep_pf(i, mol, mol_start, grid, a.rc, rng[w], lane, flags, (double)pi.x, (double)pi.y,
+              (double)pi.z, load, visit);
   drain(qn);
   double sphi = tm_wave_sum_d(acc.phi);
   double gx = tm_wave_sum_d(acc.gx);
@@ -269,52 +286,95 @@ mol_sum_kernel(const int* __restrict__ mol_start, int C, const T* __restrict__ i
   }
 }
 
-// One large molecule: stage 1 sums fixed 1024-atom chunks into partial[k][c],
-// stage 2 sums the partials in chunk order.  Deterministic.
+// One large molecule in one launch: block k sums atoms [1024 k, 1024 (k + 1))
+// for every column into partial[k][c]; the last block to finish (completion
+// counter in scratch, agent-scope acquire/release) sums the partials in block
+// order and resets the counter to zero.  Deterministic whatever the block
+// that finishes last.
 template <class T>
 __global__ void __launch_bounds__(256)
-chunk_sum_kernel(int n, int C, const T* __restrict__ in, int ldin, double* __restrict__ partial) {
-  __shared__ double part[256];
+sum_last_block_kernel(int n, int C, const T* __restrict__ in, int ldin, double* scratch,
+                      double* __restrict__ out, int scale_by_count) {
+  constexpr int CB = 4;                    // columns per pass
+  __shared__ double part[CB][256 / 64];
+  __shared__ double stage[256];
+  __shared__ int last;
+  unsigned int* counter = reinterpret_cast<unsigned int*>(scratch);
+  double* partial = scratch + 1;
   const int k = blockIdx.x;
   const int lo = k * 1024, hi = min(n, lo + 1024);
-  for (int c = 0; c < C; ++c) {
-    double acc = 0.0;
-    for (int a = lo + threadIdx.x; a < hi; a += 256) acc += (double)in[(long)a * ldin + c];
-    part[threadIdx.x] = acc;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-      if (threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
-      __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int c0 = 0; c0 < C; c0 += CB) {
+    const int nc = min(CB, C - c0);
+    // all loads issued before any is used (rows past the end read row n - 1)
+    T v[4][CB];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const long a = min(lo + u * 256 + (int)threadIdx.x, n - 1);
+#pragma unroll
+      for (int c = 0; c < CB; ++c) v[u][c] = in[a * ldin + c0 + min(c, nc - 1)];
     }
-    if (threadIdx.x == 0) partial[(long)k * C + c] = part[0];
+#pragma unroll
+    for (int c = 0; c < CB; ++c) {
+      double acc = 0.0;
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (lo + u * 256 + (int)threadIdx.x < hi) acc += (double)v[u][c];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+      if (lane == 0) part[c][wv] = acc;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < nc)
+      partial[(long)k * C + c0 + threadIdx.x] =
+          ((part[threadIdx.x][0] + part[threadIdx.x][1]) + part[threadIdx.x][2]) +
+          part[threadIdx.x][3];
     __syncthreads();
   }
-}
-
-__global__ void chunk_final_kernel(int nchunk, int C, int n, const double* __restrict__ partial,
-                                   double* __restrict__ out, int scale_by_count) {
-  int c = threadIdx.x;
-  if (c >= C) return;
-  double v = 0.0;
-  for (int k = 0; k < nchunk; ++k) v += partial[(long)k * C + c];
-  if (scale_by_count) v = n > 0 ? v / (double)n : 0.0;
-  out[c] = v;
+  if (threadIdx.x == 0) {
+    const unsigned int done =
+        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    last = done == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  // the other blocks' partials are visible after the acquire: stage them in
+  // LDS 256 at a time, then sum each column in block order
+  double tot = 0.0;   // thread c < C: column c
+  const int nb = gridDim.x;
+  for (int c = 0; c < C; ++c) {
+    for (int b0 = 0; b0 < nb; b0 += 256) {
+      const int b = b0 + threadIdx.x;
+      stage[threadIdx.x] = b < nb ? partial[(long)b * C + c] : 0.0;
+      __syncthreads();
+      if ((int)threadIdx.x == c) {
+        const int m = min(256, nb - b0);
+        for (int q = 0; q < m; ++q) tot += stage[q];
+      }
+      __syncthreads();
+    }
+  }
+  if ((int)threadIdx.x < C) {
+    if (scale_by_count) tot = n > 0 ? tot / (double)n : 0.0;
+    out[threadIdx.x] = tot;
+  }
+  if (threadIdx.x == 0)
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <class T>
 int mol_sum(int B, const int* mol_start, int C, const T* in, int ldin, double* out, int mean,
             double* scratch, int n_total, hipStream_t s) {
-  if (B == 1 && n_total > 2048 && (scratch == nullptr || n_total <= (1 << 18))) {
-    // one system of up to 256k atoms: a single 1024-thread block, one launch
-    mol_sum_kernel<T, 1024><<<1, 1024, 0, s>>>(mol_start, C, in, ldin, out, mean);
+  if (B == 1 && n_total > 2048 && scratch != nullptr) {
+    sum_last_block_kernel<T><<<(n_total + 1023) / 1024, 256, 0, s>>>(n_total, C, in, ldin,
+                                                                     scratch, out, mean);
     TM_LAUNCH_CHECK();
     return TM_OK;
   }
-  if (B == 1 && scratch != nullptr && n_total > 2048) {
-    int nchunk = (n_total + 1023) / 1024;
-    chunk_sum_kernel<T><<<nchunk, 256, 0, s>>>(n_total, C, in, ldin, scratch);
-    TM_LAUNCH_CHECK();
-    chunk_final_kernel<<<1, 64, 0, s>>>(nchunk, C, n_total, scratch, out, mean);
+  if (B == 1 && n_total > 2048) {
+    // one system without scratch: a single 1024-thread block, one launch
+    mol_sum_kernel<T, 1024><<<1, 1024, 0, s>>>(mol_start, C, in, ldin, out, mean);
     TM_LAUNCH_CHECK();
     return TM_OK;
   }
@@ -350,9 +410,10 @@ __global__ void charge_upstream_kernel(int N, const int* __restrict__ mol,
 
 extern "C" {
 
-// One molecule of up to 262,144 atoms is summed by a single 1024-thread block;
-// above that, scratch (>= ceil(N/1024)*C doubles) enables the two-pass
-// multi-block path; n_total = N.
+// One large molecule (B == 1, n_total = N > 2048) is summed by ceil(N/1024)
+// blocks in one launch when scratch is given (1 + ceil(N/1024)*C doubles, its
+// first word a completion counter that must be zero before the first call and
+// is zero again after each), else by a single 1024-thread block.
 int tm_mol_sum_f32(int B, const int* mol_start, int C, const float* in, int ldin, double* out,
                    int mean, double* scratch, int n_total, void* stream) {
   TM_REQUIRE(B >= 0 && C > 0 && C <= 64, "tm_mol_sum_f32: bad sizes");
@@ -434,9 +495,11 @@ int tm_coulomb(int N, const float* xq, const int* elem, const int* mol, const in
                                                          (float4*)xqc, elc);
     TM_LAUNCH_CHECK();
   }
-  coulomb_kernel<<<tm_div_up(N, C_WAVES), C_WAVES * 64, 0, s>>>(
-      N, (const float4*)xq, elem, (const float4*)xqc, elc, mol, mol_start, box, g, a, phi, grad,
-      eatom, flags);
+  auto k = (cell_atoms != nullptr && molc != nullptr) ? coulomb_kernel<true>
+                                                      : coulomb_kernel<false>;
+  k<<<tm_div_up(N, C_WAVES), C_WAVES * 64, 0, s>>>(N, (const float4*)xq, elem,
+                                                   (const float4*)xqc, elc, mol, mol_start, box, g,
+                                                   a, phi, grad, eatom, flags);
   TM_LAUNCH_CHECK();
   return TM_OK;
 }
