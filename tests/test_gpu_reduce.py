@@ -1,0 +1,86 @@
+"""Per-molecule sums through the C ABI (tm_mol_sum_f32 / tm_mol_sum_f64).
+
+The reductions are restatements of reduce_sum over a molecule's atoms
+(RawSymFunc.py:1354-1359, TFMolInstanceDirect.py:5212).  Checked against
+exactly rounded sums (math.fsum) of the same inputs: one large system summed
+by many blocks in one launch (completion counter in the scratch buffer, so
+back-to-back calls, a smaller call after a larger one, and strided inputs are
+covered), one system without scratch (single block), and many molecules (one
+block each).  Results must also be bitwise repeatable.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from tensormol_amd import _lib
+
+pytestmark = pytest.mark.gpu
+
+
+def _ref(x, starts, mean):
+    out = []
+    for m in range(len(starts) - 1):
+        lo, hi = starts[m], starts[m + 1]
+        row = [math.fsum(x[lo:hi, c].astype(np.float64)) for c in range(x.shape[1])]
+        if mean:
+            row = [v / (hi - lo) if hi > lo else 0.0 for v in row]
+        out.append(row)
+    return np.array(out, dtype=np.float64)
+
+
+def _call(L, dtype, starts, x, C, mean, scratch, dev):
+    B = len(starts) - 1
+    st = torch.tensor(starts, dtype=torch.int32, device=dev)
+    xin = torch.tensor(x, dtype=dtype, device=dev)
+    out = torch.full((B, C), float("nan"), dtype=torch.float64, device=dev)
+    fn = L.tm_mol_sum_f32 if dtype == torch.float32 else L.tm_mol_sum_f64
+    fn(B, _lib.ptr(st), C, _lib.ptr(xin), x.shape[1], _lib.ptr(out), int(mean), _lib.ptr(scratch),
+       int(starts[-1]), None)
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_single_system_many_blocks(cuda, dtype):
+    L = _lib.lib()
+    rng = np.random.default_rng(7)
+    scratch = torch.zeros(1 + 300 * 8, dtype=torch.float64, device=cuda)
+    # N, columns used, stored columns (ldin), mean; the second call is smaller
+    cases = [(12288, 3, 3, 0), (2049, 1, 2, 1), (300000, 5, 7, 0), (12288, 2, 2, 1),
+             (4097, 4, 4, 0)]
+    for N, C, ld, mean in cases:
+        x = rng.standard_normal((N, ld)) * 10.0 + 0.5
+        if dtype == torch.float32:
+            x = x.astype(np.float32)
+        got = _call(L, dtype, [0, N], x, C, mean, scratch, cuda)
+        want = _ref(x[:, :C], [0, N], mean)
+        scale = np.abs(x[:, :C]).astype(np.float64).sum(axis=0) / (N if mean else 1)
+        assert np.all(np.abs(got - want) <= 1e-13 * scale + 1e-300), (N, C, got, want)
+        again = _call(L, dtype, [0, N], x, C, mean, scratch, cuda)
+        assert np.array_equal(got, again), "not bitwise repeatable"
+    # the completion counter is back at zero after every call
+    assert scratch[:1].view(torch.int32)[0].item() == 0
+
+
+def test_single_system_without_scratch(cuda):
+    L = _lib.lib()
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((5000, 2))
+    got = _call(L, torch.float64, [0, 5000], x, 2, 0, None, cuda)
+    want = _ref(x, [0, 5000], 0)
+    assert np.allclose(got, want, rtol=0, atol=1e-11)
+
+
+def test_many_molecules(cuda):
+    L = _lib.lib()
+    rng = np.random.default_rng(5)
+    sizes = rng.integers(0, 300, size=37)
+    starts = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    x = rng.standard_normal((int(starts[-1]), 3))
+    scratch = torch.zeros(1 + 8 * 64, dtype=torch.float64, device=cuda)
+    for mean in (0, 1):
+        got = _call(L, torch.float64, list(starts), x, 3, mean, scratch, cuda)
+        want = _ref(x, list(starts), mean)
+        assert np.allclose(got, want, rtol=0, atol=1e-11)
