@@ -159,9 +159,8 @@ sym_fwd_kernel(int N, const double* __restrict__ pos, const double* __restrict__
                const int* __restrict__ acnt, const int* __restrict__ aseg, int ldr, int lda,
                SymArgs a, float* __restrict__ G, int ldg) {
   __shared__ RowLds L;
-  __shared__ float tc[SYM_WAVES][64], ts[SYM_WAVES][64], tf[SYM_WAVES][64];
-  __shared__ float tR[SYM_WAVES][NA][64];
-  __shared__ int tp[SYM_WAVES][64];
+  __shared__ float4 tq[SYM_WAVES][64];        // per triple: (cos theta, sin theta, fc_j fc_k, -)
+  __shared__ float4 tR[SYM_WAVES][64][NA / 4];   // per triple: exp(-eta (ravg - Rs_s)^2), s = 0..7
   __shared__ int segs[SYM_WAVES][2][TM_MAX_ELE + 1];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i = blockIdx.x * SYM_WAVES + w;
@@ -193,20 +192,46 @@ sym_fwd_kernel(int N, const double* __restrict__ pos, const double* __restrict__
       }
     }
   }
-  // ---------------- angular: phase A lane = triple, phase B lane = (t, s)
+  // ---------------- angular
+  // Phase A: lane = triple, stages its geometry in LDS.  Phase B: lane =
+  // (theta point t, triple group g); a lane takes every 8th triple of an
+  // element-pair segment, forms (1 + cos(theta - theta_t))^zeta fc_j fc_k once
+  // and adds it times R_s to one accumulator per shell s.  When a segment
+  // ends, a butterfly over g sums the 8 groups and lane (t, g) writes shell
+  // s = g.  Triples are ordered by element pair (blocks ea <= eb), so each
+  // segment is a contiguous range of the enumeration.
   const int m = min(acnt[i], MAX_ANG);
   const int* arow = ang + (long)i * lda;
   const int ntrip = m * (m - 1) / 2;
   const int off = a.nele * a.nr;
-  const int ot = lane >> 3, os = lane & 7;
+  const int ot = lane >> 3, g = lane & 7;
   const float ct = a.cos_t[ot], st = a.sin_t[ot];
-  int next_p = 0, cur_p = -1;
-  float acc = 0.f;
+  const int* sg = segs[w][1];
+  float acc[NA];
+#pragma unroll
+  for (int s = 0; s < NA; ++s) acc[s] = 0.f;
+  int pdone = 0;   // segments [0, pdone) are written
+  auto flush = [&](int p) {
+    float v[NA];
+#pragma unroll
+    for (int s = 0; s < NA; ++s) {
+      float x = acc[s];
+      x += __shfl_xor(x, 1, 64);
+      x += __shfl_xor(x, 2, 64);
+      x += __shfl_xor(x, 4, 64);
+      v[s] = x;
+      acc[s] = 0.f;
+    }
+    float out = v[0];
+#pragma unroll
+    for (int s = 1; s < NA; ++s) out = g == s ? v[s] : out;
+    Gi[off + p * 64 + ot * NA + g] = a.two_1mz * out;
+  };
   for (int base = 0; base < ntrip; base += 64) {
     int t = base + lane;
     if (t < ntrip) {
       int ja, kb, p;
-      decode_triple(t, segs[w][1], a.nele, ja, kb, p);
+      decode_triple(t, sg, a.nele, ja, kb, p);
       int qj = arow[ja], qk = arow[kb];
       float4 dj = L.d[w][qj], dk = L.d[w][qk];
       float c = (dj.x * dk.x + dj.y * dk.y + dj.z * dk.z) *
@@ -214,40 +239,42 @@ sym_fwd_kernel(int N, const double* __restrict__ pos, const double* __restrict__
       c = fminf(fmaxf(c, -1.0f), 1.0f);
       float sn = sqrtf(fmaxf(1.0f - c * c, 0.0f));
       float ravg = 0.5f * (dj.w + dk.w);
-      tc[w][lane] = c;
-      ts[w][lane] = sn;
-      tf[w][lane] = L.fa[w][qj] * L.fa[w][qk];
-      tp[w][lane] = p;
+      tq[w][lane] = make_float4(c, sn, L.fa[w][qj] * L.fa[w][qk], 0.f);
+      float R[NA];
 #pragma unroll
       for (int s = 0; s < NA; ++s) {
         float x = ravg - a.rs_a[s];
-        tR[w][s][lane] = __expf(-a.eta * x * x);
+        R[s] = __expf(-a.eta * x * x);
       }
+      tR[w][lane][0] = make_float4(R[0], R[1], R[2], R[3]);
+      tR[w][lane][1] = make_float4(R[4], R[5], R[6], R[7]);
     }
     __syncwarp();
-    int cnt = min(64, ntrip - base);
-    for (int u = 0; u < cnt; ++u) {
-      int p = tp[w][u];
-      if (p != cur_p) {
-        if (cur_p >= 0) {
-          for (; next_p < cur_p; ++next_p) Gi[off + next_p * 64 + lane] = 0.f;
-          Gi[off + cur_p * 64 + lane] = acc;
-          next_p = cur_p + 1;
+    const int cnt = min(64, ntrip - base);
+    int T = 0, p = 0;   // first triple of segment p
+    for (int ea = 0; ea < a.nele; ++ea) {
+      const int la = sg[ea + 1] - sg[ea];
+      for (int eb = ea; eb < a.nele; ++eb, ++p) {
+        const int lb = sg[eb + 1] - sg[eb];
+        const int Te = T + (ea == eb ? la * (la - 1) / 2 : la * lb);
+        const int lo = max(T - base, 0), hi = min(Te - base, cnt);   // chunk-local part
+        for (int u = lo + ((g - lo) & 7); u < hi; u += 8) {
+          const float4 q = tq[w][u];
+          const float4 r0 = tR[w][u][0], r1 = tR[w][u][1];
+          const float AF = powz<ZI>(1.0f + (q.x * ct + q.y * st), a) * q.z;
+          acc[0] += AF * r0.x; acc[1] += AF * r0.y; acc[2] += AF * r0.z; acc[3] += AF * r0.w;
+          acc[4] += AF * r1.x; acc[5] += AF * r1.y; acc[6] += AF * r1.z; acc[7] += AF * r1.w;
         }
-        cur_p = p;
-        acc = 0.f;
+        if (p >= pdone && Te <= base + cnt) {   // segment complete (wave-uniform)
+          flush(p);
+          pdone = p + 1;
+        }
+        T = Te;
       }
-      float v = tc[w][u] * ct + ts[w][u] * st;
-      acc += a.two_1mz * powz<ZI>(1.0f + v, a) * tR[w][os][u] * tf[w][u];
     }
     __syncwarp();
   }
-  if (cur_p >= 0) {
-    for (; next_p < cur_p; ++next_p) Gi[off + next_p * 64 + lane] = 0.f;
-    Gi[off + cur_p * 64 + lane] = acc;
-    next_p = cur_p + 1;
-  }
-  for (; next_p < a.nelep; ++next_p) Gi[off + next_p * 64 + lane] = 0.f;
+  for (int p = pdone; p < a.nelep; ++p) Gi[off + p * 64 + lane] = 0.f;
 }
 
 // dE/dG (two contributions combined: gE + u_i * gQ) -> w slots
@@ -259,7 +286,8 @@ sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__
                const int* __restrict__ rseg, const int* __restrict__ ang,
                const int* __restrict__ acnt, const int* __restrict__ aseg, int ldr, int lda,
                SymArgs a, const float* __restrict__ gE, const float* __restrict__ gQ,
-               const float* __restrict__ u, int ldg, float4* __restrict__ wslot) {
+               const float* __restrict__ u, const double* __restrict__ phi,
+               const double* __restrict__ phimean, int ldg, float4* __restrict__ wslot) {
   __shared__ RowLds L;
   extern __shared__ float gdyn[];   // SYM_WAVES * D floats: dE/dG of each wave's centre
   float* gw = gdyn + (threadIdx.x >> 6) * a.D;
@@ -274,7 +302,9 @@ sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__
     segs[w][0][lane] = min(rseg[(long)i * (a.nele + 1) + lane], n);
     segs[w][1][lane] = aseg[(long)i * (a.nele + 1) + lane];
   }
-  const float ui = (gQ != nullptr) ? u[i] : 0.f;
+  // charge upstream u_i = phi_i - mean_m(phi) (as tm_charge_upstream) unless given
+  const float ui = gQ == nullptr ? 0.f
+                   : (u != nullptr ? u[i] : (float)(phi[i] - phimean[mol[i]]));
   for (int f = lane; f < a.D; f += 64) {
     float v = gE[(long)i * ldg + f];
     if (gQ != nullptr) v += ui * gQ[(long)i * ldg + f];
@@ -661,19 +691,22 @@ int tm_sym_adjoint(int N, const double* pos, const double* box, const int* mol, 
                    const int* acnt, const int* aseg, int ldr, int lda, int nele, int nr,
                    float eta, float zeta, float rr, float ra, const float* rs_r,
                    const float* rs_a, const float* thetas, const float* gE, const float* gQ,
-                   const float* u, int ldg, float* wslot, void* stream) {
+                   const float* u, const double* phi, const double* phimean, int ldg,
+                   float* wslot, void* stream) {
   SymArgs a;
   int st = fill_args(a, nele, nr, eta, zeta, rr, ra, rs_r, rs_a, thetas);
   if (st) return st;
   TM_REQUIRE(ldg >= a.D, "tm_sym_adjoint: ldg %d < D %d", ldg, a.D);
+  TM_REQUIRE(gQ == nullptr || u != nullptr || (phi != nullptr && phimean != nullptr),
+             "tm_sym_adjoint: gQ needs u or phi and phimean");
   TM_REQUIRE(ldr <= MAX_ROW && lda <= MAX_ANG, "tm_sym_adjoint: strides %d/%d above %d/%d", ldr,
              lda, MAX_ROW, MAX_ANG);
   if (N == 0) return TM_OK;
   hipStream_t s = (hipStream_t)stream;
   auto k = a.zeta_int == 8 ? sym_adj_kernel<8> : sym_adj_kernel<0>;
   k<<<tm_div_up(N, SYM_WAVES), SYM_WAVES * 64, sizeof(float) * SYM_WAVES * a.D, s>>>(
-      N, pos, box, mol, elem, nbr, ncnt, rseg, ang, acnt, aseg, ldr, lda, a, gE, gQ, u, ldg,
-      (float4*)wslot);
+      N, pos, box, mol, elem, nbr, ncnt, rseg, ang, acnt, aseg, ldr, lda, a, gE, gQ, u, phi,
+      phimean, ldg, (float4*)wslot);
   TM_LAUNCH_CHECK();
   return TM_OK;
 }
