@@ -37,10 +37,12 @@ int tm_zero_i32(int n, int* p, void* stream);
  * Cell grid (one large open or periodic system, or a batch): tm_cell_build
  * bins atoms into an nx*ny*nz grid of cells with origin o and edges c
  * (periodic: positions wrapped; open: clamped, flag 2 raised) and sorts each
- * cell's atoms by index.  Buffers: cell_of[N], cell_count[ncell],
- * cell_start[ncell+1], cell_fill[max(ncell, N)], cell_atoms[N] (atoms in cell
- * order), cell_pos[N] (cell-order position of each atom), posc[3N] (positions
- * in cell order), molc[N] (molecules in cell order; NULL for one system).
+ * cell's atoms by index.  Buffers: cell_of[N], cell_count[ncell] (zero-filled
+ * before the first call; every call leaves it zero, so calls sharing it must
+ * not overlap), cell_start[ncell+1], cell_fill[N] (staging), cell_atoms[N]
+ * (atoms in cell order), cell_pos[N] (cell-order position of each atom),
+ * posc[3N] (positions in cell order), molc[N] (molecules in cell order; NULL
+ * for one system).
  * Sweeps over the grid cover the cutoff sphere with pruned stencil rows; cells
  * of about half the short cutoff are the intended size.
  */
@@ -101,13 +103,17 @@ int tm_sym_forward(int N, const double* pos, const double* box, const int* mol, 
 
 /* Descriptor adjoint (replaces tf.gradients through the descriptor graph,
  * TFMolInstanceDirect.py:5761): with g_i = gE_i + u_i * gQ_i (gQ may be NULL),
- * writes wslot[(i*ldr+n)*4 + 0..2] = d(sum_f g_if G_if)/d(x_j - x_i). */
+ * writes wslot[(i*ldr+n)*4 + 0..2] = d(sum_f g_if G_if)/d(x_j - x_i).  u (float
+ * [N]) may be NULL when phi (double [N]) and phimean (double [B]) are given:
+ * u_i = phi_i - phimean[mol_i] is then formed in the kernel, as
+ * tm_charge_upstream forms it. */
 int tm_sym_adjoint(int N, const double* pos, const double* box, const int* mol, const int* elem,
                    const int* nbr, const int* ncnt, const int* rseg, const int* ang,
                    const int* acnt, const int* aseg, int ldr, int lda, int nele, int nr,
                    float eta, float zeta, float rr, float ra, const float* rs_r,
                    const float* rs_a, const float* thetas, const float* gE, const float* gQ,
-                   const float* u, int ldg, float* wslot, void* stream);
+                   const float* u, const double* phi, const double* phimean, int ldg,
+                   float* wslot, void* stream);
 
 /* grad_a = sum_n (w[rev(a,n)] - w[(a,n)]) + extra_a (Ha/A);
  * force = -scale * grad (scale = JOULEPERHARTREE gives J/mol/A as returned by
@@ -159,9 +165,13 @@ int tm_mlp_mfma_pack_info(int D, int H, int nh, int mode, long* out);
 
 /* ---- electrostatics ----------------------------------------------------
  * Neutral charges q = qraw - mean_m(qraw) and dipole terms
- * (TFMolInstanceDirect.py:5274-5279); xq = float32 (x, y, z, q). */
+ * (TFMolInstanceDirect.py:5274-5279); xq = float32 (x, y, z, q) (may be NULL
+ * when cell_pos is given).  With cell_pos (tm_cell_build's) the same values
+ * and elem are also written in cell order to xqc (float4 [N]) and elc (int
+ * [N]), the copies tm_coulomb sweeps (it then takes xq = NULL). */
 int tm_charges(int N, const int* mol, const double* qmean, const float* qraw, const double* pos,
-               float* xq, double* q, double* qx, void* stream);
+               float* xq, double* q, double* qx, const int* cell_pos, const int* elem, float* xqc,
+               int* elc, void* stream);
 
 /* DSF-elu Coulomb (RawSymFunc.py:1307-1359) and Grimme-C6 vdW
  * (RawSymFunc.py:1361-1414) over all pairs within rc_ee, evaluated from the
@@ -170,7 +180,8 @@ int tm_charges(int N, const int* mol, const double* qmean, const float* qraw, co
  * arguments are those of tm_cell_build (NULL cell_atoms: molecule sweeps).  Outputs
  * phi[i] = sum_j K(r_ij) q_j, grad (dE/dx at fixed q, Ha/A), eatom[2i] =
  * q_i phi_i / 2, eatom[2i+1] = vdW energy share.  With a cell grid xqc
- * (float4 [N]) and elc (int [N]) are scratch for the cell-ordered copies. */
+ * (float4 [N]) and elc (int [N]) hold the cell-ordered copies of xq and elem:
+ * gathered here from xq, or, when xq is NULL, already written by tm_charges. */
 int tm_coulomb(int N, const float* xq, const int* elem, const int* mol, const int* mol_start,
                const double* box, const int* cell_start, const int* cell_atoms, const int* cell_of,
                const int* cell_pos, const int* molc, int nx, int ny, int nz, double csx, double csy,
@@ -194,6 +205,17 @@ int tm_mol_sum_f32(int B, const int* mol_start, int C, const float* in, int ldin
                    int mean, double* scratch, int n_total, void* stream);
 int tm_mol_sum_f64(int B, const int* mol_start, int C, const double* in, int ldin, double* out,
                    int mean, double* scratch, int n_total, void* stream);
+
+/* Several per-molecule column sums in one launch, as tm_mol_sum_f64 per
+ * source: the closing reductions of one evaluation (per-molecule BP energy,
+ * Coulomb and vdW energies, dipole; TFMolInstanceDirect.py:5212, 5278-5279,
+ * RawSymFunc.py:1354-1359).  Host arrays of nsrc <= 4 entries: source k is an
+ * [N, ld[k]] device array (float64 when f64[k], else float32) whose first
+ * cols[k] columns are summed into out[k] (device, [B, cols[k]]); at most 16
+ * columns in all.  scratch as for tm_mol_sum_f64 with C = total columns. */
+int tm_mol_sums(int B, const int* mol_start, int n_total, int nsrc, const void* const* in,
+                const int* f64, const int* cols, const int* ld, double* const* out,
+                double* scratch, void* stream);
 
 /* ---- integrators (TensorMol/Simulations/SimpleMD.py) --------------------
  * VelocityVerletStep (SimpleMD.py:14-38) split into drift and kick;

@@ -28,13 +28,14 @@ SIGNATURES = {
     "tm_pairs": [I, P, P, P, P, P, P, P, P, P, P, P, I, I, I, D, D, D, D, D, D, I, D, D, I, P, P, P, P, P],
     "tm_sym_forward": [I, P, P, P, P, P, P, P, P, P, P, I, I, I, I, F, F, F, F, P, P, P, P, I, P],
     "tm_sym_adjoint": [I, P, P, P, P, P, P, P, P, P, P, I, I, I, I, F, F, F, F, P, P, P, P, P, P,
-                       I, P, P],
+                       P, P, I, P, P],
     "tm_sym_jvp": [I, P, P, P, P, P, P, P, P, P, P, I, I, I, I, F, F, F, F, P, P, P, P, P, I, P],
     "tm_force_gather": [I, I, P, P, P, P, D, P, P, P],
     "tm_mlp_f32": [I, I, I, P, P, I, P, P, F, P, P, P, P],
     "tm_mol_sum_f32": [I, P, I, P, I, P, I, P, I, P],
     "tm_mol_sum_f64": [I, P, I, P, I, P, I, P, I, P],
-    "tm_charges": [I, P, P, P, P, P, P, P, P],
+    "tm_mol_sums": [I, P, I, I, P, P, P, P, P, P, P],
+    "tm_charges": [I, P, P, P, P, P, P, P, P, P, P, P, P],
     "tm_charge_upstream": [I, P, P, P, P, P],
     "tm_coulomb": [I, P, P, P, P, P, P, P, P, P, P, I, I, I, D, D, D, D, D, D, I, P, D, D, I, I, I, P, P, P, P,
                    P, P, P, P, P],

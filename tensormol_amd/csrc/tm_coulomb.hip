@@ -158,8 +158,8 @@ coulomb_kernel(int N, const float4* __restrict__ xq, const int* __restrict__ ele
     Lx = (float)box[3 * mi]; Ly = (float)box[3 * mi + 1]; Lz = (float)box[3 * mi + 2];
     iLx = 1.0f / Lx; iLy = 1.0f / Ly; iLz = 1.0f / Lz;
   }
-  const float4 pi = xq[i];
-  const int ei = elem[i];
+  const float4 pi = cpi >= 0 ? xqc[cpi] : xq[i];
+  const int ei = cpi >= 0 ? elc[cpi] : elem[i];
   const float* vci = vdw_c + ei * TM_MAX_ELE;
   const float* vqi = vdw_q + ei * TM_MAX_ELE;
   PairAcc acc = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -363,6 +363,106 @@ sum_last_block_kernel(int n, int C, const T* __restrict__ in, int ldin, double* 
     __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Several per-molecule column sums in one launch.  Source k is an [N, ld[k]]
+// array (float64 when f64[k], else float32) of which the first cols[k]
+// columns are summed; columns are numbered across sources in order.
+constexpr int SUMS_MAX_SRC = 4, SUMS_MAX_COL = 16;
+struct SumSrc {
+  const void* in[SUMS_MAX_SRC];
+  double* out[SUMS_MAX_SRC];
+  int f64[SUMS_MAX_SRC], cols[SUMS_MAX_SRC], ld[SUMS_MAX_SRC];
+  int nsrc, ctot;
+};
+
+__device__ __forceinline__ double src_val(const SumSrc& S, int k, long a, int c) {
+  return S.f64[k] ? ((const double*)S.in[k])[a * S.ld[k] + c]
+                  : (double)((const float*)S.in[k])[a * S.ld[k] + c];
+}
+
+// one large system: block k sums atoms [1024 k, 1024 (k + 1)) of every column;
+// the last block to finish sums the partials in block order (as
+// sum_last_block_kernel) and resets the counter
+__global__ void __launch_bounds__(256)
+sums_last_block_kernel(int n, SumSrc S, double* scratch) {
+  __shared__ double part[SUMS_MAX_COL][4];
+  __shared__ double stage[256];
+  __shared__ int last;
+  unsigned int* counter = reinterpret_cast<unsigned int*>(scratch);
+  double* partial = scratch + 1;
+  const int k = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int lo = k * 1024, hi = min(n, lo + 1024);
+  int cg = 0;
+  for (int q = 0; q < S.nsrc; ++q) {
+    for (int c = 0; c < S.cols[q]; ++c, ++cg) {
+      double v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = src_val(S, q, min(lo + u * 256 + tid, n - 1), c);
+      double acc = 0.0;
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (lo + u * 256 + tid < hi) acc += v[u];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+      if (lane == 0) part[cg][wv] = acc;
+    }
+  }
+  __syncthreads();
+  const int C = S.ctot;
+  if (tid < C) partial[(long)k * C + tid] = ((part[tid][0] + part[tid][1]) + part[tid][2]) + part[tid][3];
+  __syncthreads();
+  if (tid == 0) {
+    const unsigned int done =
+        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    last = done == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  // partials [nb][C] staged 256 at a time; thread c < C adds column c in block order
+  const int tot_n = (int)gridDim.x * C;
+  double tot = 0.0;
+  for (int e0 = 0; e0 < tot_n; e0 += 256) {
+    stage[tid] = e0 + tid < tot_n ? partial[e0 + tid] : 0.0;
+    __syncthreads();
+    if (tid < C) {
+      const int e1 = min(e0 + 256, tot_n);
+      for (int e = e0 + ((tid - e0 % C) + C) % C; e < e1; e += C) tot += stage[e - e0];
+    }
+    __syncthreads();
+  }
+  if (tid < C) {
+    int q = 0, c = tid;
+    while (c >= S.cols[q]) { c -= S.cols[q]; ++q; }
+    S.out[q][c] = tot;
+  }
+  if (tid == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// one block per molecule, every column; fixed reduction tree
+template <int NT>
+__global__ void __launch_bounds__(NT)
+sums_per_mol_kernel(const int* __restrict__ mol_start, SumSrc S) {
+  __shared__ double part[NT / 64];
+  const int m = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int lo = mol_start[m], hi = mol_start[m + 1];
+  for (int q = 0; q < S.nsrc; ++q) {
+    for (int c = 0; c < S.cols[q]; ++c) {
+      double acc = 0.0;
+      for (int a = lo + tid; a < hi; a += NT) acc += src_val(S, q, a, c);
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+      if (lane == 0) part[wv] = acc;
+      __syncthreads();
+      if (tid == 0) {
+        double v = 0.0;
+        for (int w = 0; w < NT / 64; ++w) v += part[w];
+        S.out[q][(long)m * S.cols[q] + c] = v;
+      }
+      __syncthreads();
+    }
+  }
+}
+
 template <class T>
 int mol_sum(int B, const int* mol_start, int C, const T* in, int ldin, double* out, int mean,
             double* scratch, int n_total, hipStream_t s) {
@@ -386,13 +486,21 @@ int mol_sum(int B, const int* mol_start, int C, const T* in, int ldin, double* o
 __global__ void charges_kernel(int N, const int* __restrict__ mol, const double* __restrict__ qmean,
                                const float* __restrict__ qraw, const double* __restrict__ pos,
                                float4* __restrict__ xq, double* __restrict__ q,
-                               double* __restrict__ qx) {
+                               double* __restrict__ qx, const int* __restrict__ cell_pos,
+                               const int* __restrict__ elem, float4* __restrict__ xqc,
+                               int* __restrict__ elc) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   double qi = (double)qraw[i] - qmean[mol[i]];
   q[i] = qi;
   double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
-  xq[i] = make_float4((float)x, (float)y, (float)z, (float)qi);
+  const float4 v = make_float4((float)x, (float)y, (float)z, (float)qi);
+  if (xq != nullptr) xq[i] = v;
+  if (cell_pos != nullptr) {   // the cell-ordered copies the Coulomb sweep reads
+    const int p = cell_pos[i];
+    xqc[p] = v;
+    elc[p] = elem[i];
+  }
   qx[3 * i] = qi * x * BOHRPERA_D;
   qx[3 * i + 1] = qi * y * BOHRPERA_D;
   qx[3 * i + 2] = qi * z * BOHRPERA_D;
@@ -429,11 +537,50 @@ int tm_mol_sum_f64(int B, const int* mol_start, int C, const double* in, int ldi
                          (hipStream_t)stream);
 }
 
+int tm_mol_sums(int B, const int* mol_start, int n_total, int nsrc, const void* const* in,
+                const int* f64, const int* cols, const int* ld, double* const* out,
+                double* scratch, void* stream) {
+  TM_REQUIRE(B >= 0 && n_total >= 0 && nsrc > 0 && nsrc <= SUMS_MAX_SRC,
+             "tm_mol_sums: bad sizes");
+  SumSrc S;
+  S.nsrc = nsrc;
+  S.ctot = 0;
+  for (int k = 0; k < SUMS_MAX_SRC; ++k) {
+    const bool on = k < nsrc;
+    S.in[k] = on ? in[k] : nullptr;
+    S.out[k] = on ? out[k] : nullptr;
+    S.f64[k] = on ? f64[k] : 0;
+    S.cols[k] = on ? cols[k] : 0;
+    S.ld[k] = on ? ld[k] : 0;
+    if (on) {
+      TM_REQUIRE(cols[k] > 0 && ld[k] >= cols[k] && in[k] != nullptr && out[k] != nullptr,
+                 "tm_mol_sums: bad source %d", k);
+      S.ctot += cols[k];
+    }
+  }
+  TM_REQUIRE(S.ctot <= SUMS_MAX_COL, "tm_mol_sums: %d columns above %d", S.ctot, SUMS_MAX_COL);
+  if (B == 0) return TM_OK;
+  hipStream_t s = (hipStream_t)stream;
+  if (B == 1 && n_total > 2048 && scratch != nullptr) {
+    sums_last_block_kernel<<<(n_total + 1023) / 1024, 256, 0, s>>>(n_total, S, scratch);
+  } else if (B == 1 && n_total > 2048) {
+    sums_per_mol_kernel<1024><<<1, 1024, 0, s>>>(mol_start, S);
+  } else {
+    sums_per_mol_kernel<256><<<B, 256, 0, s>>>(mol_start, S);
+  }
+  TM_LAUNCH_CHECK();
+  return TM_OK;
+}
+
 int tm_charges(int N, const int* mol, const double* qmean, const float* qraw, const double* pos,
-               float* xq, double* q, double* qx, void* stream) {
+               float* xq, double* q, double* qx, const int* cell_pos, const int* elem, float* xqc,
+               int* elc, void* stream) {
+  TM_REQUIRE(cell_pos == nullptr || (elem != nullptr && xqc != nullptr && elc != nullptr),
+             "tm_charges: cell_pos needs elem, xqc and elc");
+  TM_REQUIRE(xq != nullptr || cell_pos != nullptr, "tm_charges: no position output");
   if (N == 0) return TM_OK;
-  charges_kernel<<<tm_div_up(N, 256), 256, 0, (hipStream_t)stream>>>(N, mol, qmean, qraw, pos,
-                                                                     (float4*)xq, q, qx);
+  charges_kernel<<<tm_div_up(N, 256), 256, 0, (hipStream_t)stream>>>(
+      N, mol, qmean, qraw, pos, (float4*)xq, q, qx, cell_pos, elem, (float4*)xqc, elc);
   TM_LAUNCH_CHECK();
   return TM_OK;
 }
@@ -459,6 +606,7 @@ int tm_coulomb(int N, const float* xq, const int* elem, const int* mol, const in
                int add_ecc, int has_vdw, int nele, const double* c6, const double* r0, float* xqc,
                int* elc, double* phi, double* grad, double* eatom, int* flags, void* stream) {
   TM_REQUIRE(nele > 0 && nele <= TM_MAX_ELE, "tm_coulomb: bad nele");
+  TM_REQUIRE(xq != nullptr || cell_atoms != nullptr, "tm_coulomb: xq is needed without a cell grid");
   TM_REQUIRE(cell_atoms == nullptr || (xqc != nullptr && elc != nullptr && cell_pos != nullptr),
              "tm_coulomb: cell sweeps need cell_pos and the xqc/elc buffers");
   if (cell_atoms != nullptr) {
@@ -490,7 +638,7 @@ int tm_coulomb(int N, const float* xq, const int* elem, const int* mol, const in
   g.org[0] = (float)ox; g.org[1] = (float)oy; g.org[2] = (float)oz;
   g.periodic = periodic;
   hipStream_t s = (hipStream_t)stream;
-  if (cell_atoms != nullptr) {
+  if (cell_atoms != nullptr && xq != nullptr) {
     cell_gather_kernel<<<tm_div_up(N, 256), 256, 0, s>>>(N, cell_atoms, (const float4*)xq, elem,
                                                          (float4*)xqc, elc);
     TM_LAUNCH_CHECK();

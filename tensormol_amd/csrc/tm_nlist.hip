@@ -202,21 +202,12 @@ __global__ void cell_assign_kernel(int N, const double* __restrict__ pos, double
   atomicAdd(&cell_count[cell], 1);
 }
 
-__global__ void copy_i32_kernel(int n, const int* __restrict__ src, int* __restrict__ dst) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) dst[t] = src[t];
-}
-
-__global__ void zero2_kernel(long n, int* __restrict__ a, int* __restrict__ b) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) { a[t] = 0; b[t] = 0; }
-}
-
 // Exclusive prefix sum of n ints (out[n] = total) by one 1024-thread block:
 // tiles of 4096 elements, 4 consecutive per thread (coalesced), a shuffle scan
 // inside each wave, the 16 wave totals scanned through LDS, and a running
-// carry between tiles.
-__global__ void __launch_bounds__(1024) scan_kernel(int n, const int* __restrict__ in,
+// carry between tiles.  The input is zeroed as it is read (the per-cell
+// counts become the fill counters).
+__global__ void __launch_bounds__(1024) scan_kernel(int n, int* __restrict__ in,
                                                     int* __restrict__ out) {
   __shared__ int wtot[16];
   __shared__ int carry_s;
@@ -228,6 +219,9 @@ __global__ void __launch_bounds__(1024) scan_kernel(int n, const int* __restrict
     int v[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) v[k] = i0 + k < n ? in[i0 + k] : 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (i0 + k < n) in[i0 + k] = 0;
     const int t = v[0] + v[1] + v[2] + v[3];
     int incl = t;
 #pragma unroll
@@ -261,27 +255,30 @@ __global__ void __launch_bounds__(1024) scan_kernel(int n, const int* __restrict
   if (tid == 0) out[n] = carry_s;
 }
 
+// unsorted cell order into the staging array; the counters start at zero
 __global__ void cell_fill_kernel(int N, const int* __restrict__ cell_of,
-                                 const int* __restrict__ cell_start, int* __restrict__ cell_fill,
-                                 int* __restrict__ cell_atoms) {
+                                 const int* __restrict__ cell_start, int* __restrict__ counter,
+                                 int* __restrict__ staging) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   int c = cell_of[i];
-  int slot = cell_start[c] + atomicAdd(&cell_fill[c], 1);
-  cell_atoms[slot] = i;
+  int slot = cell_start[c] + atomicAdd(&counter[c], 1);
+  staging[slot] = i;
 }
 
 // order atoms inside each cell by index, so every later sweep is
 // deterministic: one wavefront per cell ranks its entries (cells hold a few
-// dozen atoms; larger cells are ranked in 64-entry rounds)
+// dozen atoms; larger cells are ranked in 64-entry rounds).  Each wave also
+// clears its cell's counter, so the counters are zero for the next build.
 __global__ void cell_sort_kernel(int ncell, const int* __restrict__ cell_start,
                                  const int* __restrict__ unsorted, int* __restrict__ cell_atoms,
                                  int* __restrict__ cell_pos, const double* __restrict__ pos,
                                  double* __restrict__ posc, const int* __restrict__ mol,
-                                 int* __restrict__ molc) {
+                                 int* __restrict__ molc, int* __restrict__ counter) {
   const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (c >= ncell) return;
+  if (lane == 0) counter[c] = 0;
   const int lo = cell_start[c], hi = cell_start[c + 1];
   for (int a = lo + lane; a < hi; a += 64) {
     int v = unsorted[a];
@@ -397,10 +394,12 @@ static CellGrid make_grid(const int* cell_start, const int* cell_atoms, const in
   return g;
 }
 
-// cell_fill must hold max(ncell, N) ints: per-cell counters, then the
-// unsorted staging copy of cell_atoms.  Outputs cell_atoms (index order inside
-// each cell), cell_pos (position of every atom in cell order), posc (positions
-// in cell order) and molc (molecules in cell order; may be NULL for one system).
+// cell_count (ncell ints) must be zero on entry (zero-fill it once; every call
+// leaves it zero); it holds the per-cell counts, then the fill counters.
+// cell_fill (at least N ints) receives the unsorted staging copy of cell_atoms.
+// Outputs cell_atoms (index order inside each cell), cell_pos (position of
+// every atom in cell order), posc (positions in cell order) and molc
+// (molecules in cell order; may be NULL for one system).
 int tm_cell_build(int N, const double* pos, const int* mol, double ox, double oy, double oz,
                   double cx, double cy, double cz, int nx, int ny, int nz, int periodic,
                   int* cell_of, int* cell_count, int* cell_start, int* cell_fill, int* cell_atoms,
@@ -410,21 +409,19 @@ int tm_cell_build(int N, const double* pos, const int* mol, double ox, double oy
   hipStream_t s = (hipStream_t)stream;
   long ncell = (long)nx * ny * nz;
   if (N == 0) return TM_OK;
-  // counters are cleared by a kernel (no memset nodes when the caller captures a graph)
-  zero2_kernel<<<tm_div_up(ncell, 256), 256, 0, s>>>(ncell, cell_count, cell_fill);
-  TM_LAUNCH_CHECK();
+  // cell_count is zero here: the previous call's sort cleared it (no memset
+  // nodes when the caller captures a graph)
   cell_assign_kernel<<<tm_div_up(N, 256), 256, 0, s>>>(N, pos, ox, oy, oz, cx, cy, cz, nx, ny, nz,
                                                        periodic, cell_of, cell_count, flags);
   TM_LAUNCH_CHECK();
   scan_kernel<<<1, 1024, 0, s>>>((int)ncell, cell_count, cell_start);
   TM_LAUNCH_CHECK();
-  cell_fill_kernel<<<tm_div_up(N, 256), 256, 0, s>>>(N, cell_of, cell_start, cell_fill, cell_atoms);
-  TM_LAUNCH_CHECK();
-  // cell_fill doubles as the unsorted staging array once its counters are spent
-  copy_i32_kernel<<<tm_div_up(N, 256), 256, 0, s>>>(N, cell_atoms, cell_fill);
+  cell_fill_kernel<<<tm_div_up(N, 256), 256, 0, s>>>(N, cell_of, cell_start, cell_count,
+                                                     cell_fill);
   TM_LAUNCH_CHECK();
   cell_sort_kernel<<<tm_div_up(ncell, 4), 256, 0, s>>>((int)ncell, cell_start, cell_fill,
-                                                       cell_atoms, cell_pos, pos, posc, mol, molc);
+                                                       cell_atoms, cell_pos, pos, posc, mol, molc,
+                                                       cell_count);
   TM_LAUNCH_CHECK();
   return TM_OK;
 }

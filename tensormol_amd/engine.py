@@ -330,10 +330,14 @@ class BPEEEngine:
     def electrostatics(self, ws, pos):
         L, st = self.lib, self._stream()
         L.tm_mol_sum_f32(ws.B, _P(ws.mol_start), 1, _P(ws.qraw), 1, _P(ws.qmean), 1, _P(ws.red), ws.N, st)
-        L.tm_charges(ws.N, _P(ws.mol), _P(ws.qmean), _P(ws.qraw), _P(pos), _P(ws.xq), _P(ws.q),
-                     _P(ws.qx), st)
         g = self._grid_args(ws)
-        L.tm_coulomb(ws.N, _P(ws.xq), _P(ws.elem), _P(ws.mol), _P(ws.mol_start), _P(ws.box),
+        if ws.use_cells:   # charges written straight into cell order for the sweep
+            L.tm_charges(ws.N, _P(ws.mol), _P(ws.qmean), _P(ws.qraw), _P(pos), None, _P(ws.q),
+                         _P(ws.qx), _P(ws.cell_pos), _P(ws.elem), _P(ws.xqc), _P(ws.elc), st)
+        else:
+            L.tm_charges(ws.N, _P(ws.mol), _P(ws.qmean), _P(ws.qraw), _P(pos), _P(ws.xq), _P(ws.q),
+                         _P(ws.qx), None, None, None, None, st)
+        L.tm_coulomb(ws.N, None if ws.use_cells else _P(ws.xq), _P(ws.elem), _P(ws.mol), _P(ws.mol_start), _P(ws.box),
                      g[0], g[1], g[2], g[3], g[5], g[6], g[7], g[8], g[9], g[10], g[11],
                      g[12], g[13], g[14], ws.periodic, self.dsf.ctypes.data_as(_lib.P), self.model.ee_off, self.vdw_width,
                      int(bool(PARAMS["AddEcc"])), int(self.model.has_vdw), self.nele,
@@ -341,7 +345,6 @@ class BPEEEngine:
                      _P(ws.xqc), _P(ws.elc), _P(ws.phi), _P(ws.gradc), _P(ws.eatom), _P(ws.flags),
                      st)
         L.tm_mol_sum_f64(ws.B, _P(ws.mol_start), 1, _P(ws.phi), 1, _P(ws.phimean), 1, _P(ws.red), ws.N, st)
-        L.tm_charge_upstream(ws.N, _P(ws.mol), _P(ws.phi), _P(ws.phimean), _P(ws.u), st)
 
     def adjoint(self, ws, pos):
         L, st = self.lib, self._stream()
@@ -350,7 +353,8 @@ class BPEEEngine:
                          _P(ws.ncnt), _P(ws.rseg), _P(ws.ang), _P(ws.acnt), _P(ws.aseg), self.ldr,
                          self.lda, self.nele, sym.nr_r, sym.eta, sym.zeta, sym.rr, sym.ra,
                          _lib.fptr(self.sym_rs_r), _lib.fptr(self.sym_rs_a), _lib.fptr(self.sym_th),
-                         _P(ws.JE), _P(ws.JQ), _P(ws.u), self.D, _P(ws.wslot), st)
+                         _P(ws.JE), _P(ws.JQ), None, _P(ws.phi), _P(ws.phimean), self.D,
+                         _P(ws.wslot), st)
         L.tm_force_gather(ws.N, self.ldr, _P(ws.ncnt), _P(ws.rev), _P(ws.wslot), _P(ws.gradc),
                           JOULEPERHARTREE, _P(ws.grad), _P(ws.force), st)
 
@@ -376,15 +380,27 @@ class BPEEEngine:
                          _P(ws.ncnt), _P(ws.rseg), _P(ws.ang), _P(ws.acnt), _P(ws.aseg), self.ldr,
                          self.lda, self.nele, sym.nr_r, sym.eta, sym.zeta, sym.rr, sym.ra,
                          _lib.fptr(self.sym_rs_r), _lib.fptr(self.sym_rs_a), _lib.fptr(self.sym_th),
-                         _P(gE), None, None, self.D, _P(ws.wslot), st)
+                         _P(gE), None, None, None, None, self.D, _P(ws.wslot), st)
         L.tm_force_gather(ws.N, self.ldr, _P(ws.ncnt), _P(ws.rev), _P(ws.wslot), None, 1.0,
                           _P(grad), None, st)
 
     def reductions(self, ws):
-        L, st = self.lib, self._stream()
-        L.tm_mol_sum_f32(ws.B, _P(ws.mol_start), 1, _P(ws.eps), 1, _P(ws.Ebp), 0, _P(ws.red), ws.N, st)
-        L.tm_mol_sum_f64(ws.B, _P(ws.mol_start), 2, _P(ws.eatom), 2, _P(ws.Eext), 0, _P(ws.red), ws.N, st)
-        L.tm_mol_sum_f64(ws.B, _P(ws.mol_start), 3, _P(ws.qx), 3, _P(ws.dipole), 0, _P(ws.red), ws.N, st)
+        """Per-molecule BP energy, Coulomb/vdW energies and dipole in one launch."""
+        key = (ws.eps.data_ptr(), ws.eatom.data_ptr(), ws.qx.data_ptr(), ws.Ebp.data_ptr(),
+               ws.Eext.data_ptr(), ws.dipole.data_ptr())
+        arr = getattr(ws, "_sums_args", None)
+        if arr is None or arr[0] != key:
+            c = _lib.ctypes
+            srcs = ((ws.eps, 0, 1, 1, ws.Ebp), (ws.eatom, 1, 2, 2, ws.Eext), (ws.qx, 1, 3, 3, ws.dipole))
+            arr = (key, (c.c_void_p * 3)(*[t.data_ptr() for t, *_ in srcs]),
+                   (c.c_int * 3)(*[f for _, f, _, _, _ in srcs]),
+                   (c.c_int * 3)(*[k for _, _, k, _, _ in srcs]),
+                   (c.c_int * 3)(*[ld for _, _, _, ld, _ in srcs]),
+                   (c.c_void_p * 3)(*[o.data_ptr() for *_, o in srcs]))
+            ws._sums_args = arr
+        _, ins, f64, cols, ld, outs = arr
+        self.lib.tm_mol_sums(ws.B, _P(ws.mol_start), ws.N, 3, ins, f64, cols, ld, outs, _P(ws.red),
+                             self._stream())
 
     def run(self, ws, pos, forces=True, check=None):
         """Enqueue one full evaluation for positions pos (cuda float64 [N,3])."""

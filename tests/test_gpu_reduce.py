@@ -84,3 +84,69 @@ def test_many_molecules(cuda):
         got = _call(L, torch.float64, list(starts), x, 3, mean, scratch, cuda)
         want = _ref(x, list(starts), mean)
         assert np.allclose(got, want, rtol=0, atol=1e-11)
+
+
+@pytest.mark.parametrize("sizes", [[12288], [2500], list(range(1, 40))])
+def test_fused_sums_match_single_sums(cuda, sizes):
+    """tm_mol_sums over (float32 [N], float64 [N,2], float64 [N,3]) gives the
+    exactly rounded per-molecule sums, like three single-source calls."""
+    import ctypes
+    L = _lib.lib()
+    rng = np.random.default_rng(11)
+    starts = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    N, B = int(starts[-1]), len(sizes)
+    a = rng.standard_normal((N, 1)).astype(np.float32)
+    b = rng.standard_normal((N, 2)) * 3.0
+    c = rng.standard_normal((N, 3)) - 0.25
+    st = torch.tensor(starts, dtype=torch.int32, device=cuda)
+    ta = torch.tensor(a, device=cuda)
+    tb = torch.tensor(b, device=cuda)
+    tc = torch.tensor(c, device=cuda)
+    outs = [torch.full((B, k), float("nan"), dtype=torch.float64, device=cuda) for k in (1, 2, 3)]
+    scratch = torch.zeros(1 + 6 * (N // 1024 + 1), dtype=torch.float64, device=cuda)
+    ins = (ctypes.c_void_p * 3)(ta.data_ptr(), tb.data_ptr(), tc.data_ptr())
+    ous = (ctypes.c_void_p * 3)(*[o.data_ptr() for o in outs])
+    f64 = (ctypes.c_int * 3)(0, 1, 1)
+    cols = (ctypes.c_int * 3)(1, 2, 3)
+    ld = (ctypes.c_int * 3)(1, 2, 3)
+    for _ in range(2):
+        L.tm_mol_sums(B, _lib.ptr(st), N, 3, ins, f64, cols, ld, ous, _lib.ptr(scratch), None)
+        torch.cuda.synchronize()
+        for x, o in zip((a, b, c), outs):
+            want = _ref(x, list(starts), 0)
+            scale = np.abs(x).astype(np.float64).sum()
+            assert np.all(np.abs(o.cpu().numpy() - want) <= 1e-13 * scale)
+    assert scratch[:1].view(torch.int32)[0].item() == 0
+
+
+def test_cell_build_leaves_counts_zero(cuda):
+    """tm_cell_build needs cell_count zero on entry and leaves it zero; cells
+    list every atom once, in index order inside each cell."""
+    L = _lib.lib()
+    rng = np.random.default_rng(4)
+    N, n = 3000, 6
+    edge = 20.0
+    pos = torch.tensor(rng.uniform(0, edge, (N, 3)), dtype=torch.float64, device=cuda)
+    mol = torch.zeros(N, dtype=torch.int32, device=cuda)
+    i32 = dict(dtype=torch.int32, device=cuda)
+    ncell = n ** 3
+    cell_of, count = torch.zeros(N, **i32), torch.zeros(ncell, **i32)
+    start, fill = torch.zeros(ncell + 1, **i32), torch.zeros(N, **i32)
+    atoms, cpos = torch.zeros(N, **i32), torch.zeros(N, **i32)
+    posc = torch.zeros(N, 3, dtype=torch.float64, device=cuda)
+    flags = torch.zeros(8, **i32)
+    cs = edge / n
+    for _ in range(2):
+        L.tm_cell_build(N, _lib.ptr(pos), _lib.ptr(mol), 0.0, 0.0, 0.0, cs, cs, cs, n, n, n, 1,
+                        _lib.ptr(cell_of), _lib.ptr(count), _lib.ptr(start), _lib.ptr(fill),
+                        _lib.ptr(atoms), _lib.ptr(cpos), _lib.ptr(posc), None, _lib.ptr(flags),
+                        None)
+        torch.cuda.synchronize()
+        assert int(count.abs().sum().item()) == 0
+        a, s, co = atoms.cpu().numpy(), start.cpu().numpy(), cell_of.cpu().numpy()
+        assert sorted(a.tolist()) == list(range(N))
+        for c in range(ncell):
+            seg = a[s[c]:s[c + 1]]
+            assert np.all(np.diff(seg) > 0) and np.all(co[seg] == c)
+        assert np.array_equal(cpos.cpu().numpy()[a], np.arange(N))
+        assert torch.equal(posc, pos[atoms.long()])
