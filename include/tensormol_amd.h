@@ -60,8 +60,9 @@ int tm_cell_build(int N, const double* pos, const int* mol, double ox, double oy
  * csx..csz from the origin ox..oz, the arguments given to tm_cell_build; each
  * centre's stencil rows are trimmed with its own position).
  * Outputs: nbr[i*ldr+n], ncnt[i], rseg[i*(nele+1)+e], ang[i*lda+a] (position
- * in row i), acnt[i], aseg[i*(nele+1)+e], rev[i*ldr+n] (mirror slot), flags
- * (0: row overflow, 1: sub-list overflow, 3: non-finite input). */
+ * in row i), acnt[i], aseg[i*(nele+1)+e], rev[i*ldr+n] (mirror slot; may be
+ * NULL, and tm_sym_forward can fill it instead), flags (0: row overflow, 1:
+ * sub-list overflow, 3: non-finite input). */
 int tm_nl_build(int N, const double* pos, const int* elem, const int* mol, const int* mol_start,
                 const double* box, const int* cell_start, const int* cell_atoms,
                 const int* cell_of, const int* cell_pos, const double* posc, const int* molc,
@@ -94,12 +95,15 @@ int tm_pairs(int N, const double* pos, const int* mol, const int* mol_start, con
  * TFSymSet_Scattered_Linear_WithEle (TensorMol/TFDescriptors/RawSymFunc.py:
  * 1696-1755, 868-962, 2223-2265).  G float32 [N, ldg], column l*nr + s
  * (radial) then nele*nr + p*64 + t*8 + s (angular).  Host arrays: rs_r[nr],
- * rs_a[8], thetas[8]. */
+ * rs_a[8], thetas[8].  rev (may be NULL) receives the mirror slots of the
+ * rows, as tm_nl_build's rev output (unmatched entries raise bit 2 of flags[0]),
+ * so a caller may pass rev = NULL to tm_nl_build and have it filled here. */
 int tm_sym_forward(int N, const double* pos, const double* box, const int* mol, const int* elem,
                    const int* nbr, const int* ncnt, const int* rseg, const int* ang,
                    const int* acnt, const int* aseg, int ldr, int lda, int nele, int nr,
                    float eta, float zeta, float rr, float ra, const float* rs_r,
-                   const float* rs_a, const float* thetas, float* G, int ldg, void* stream);
+                   const float* rs_a, const float* thetas, float* G, int ldg, int* rev,
+                   int* flags, void* stream);
 
 /* Descriptor adjoint (replaces tf.gradients through the descriptor graph,
  * TFMolInstanceDirect.py:5761): with g_i = gE_i + u_i * gQ_i (gQ may be NULL),

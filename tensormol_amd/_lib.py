@@ -26,7 +26,7 @@ SIGNATURES = {
                     P, P, P, P, P, P],
     "tm_triples": [I, I, I, I, P, P, P, P, P, P, P],
     "tm_pairs": [I, P, P, P, P, P, P, P, P, P, P, P, I, I, I, D, D, D, D, D, D, I, D, D, I, P, P, P, P, P],
-    "tm_sym_forward": [I, P, P, P, P, P, P, P, P, P, P, I, I, I, I, F, F, F, F, P, P, P, P, I, P],
+    "tm_sym_forward": [I, P, P, P, P, P, P, P, P, P, P, I, I, I, I, F, F, F, F, P, P, P, P, I, P, P, P],
     "tm_sym_adjoint": [I, P, P, P, P, P, P, P, P, P, P, I, I, I, I, F, F, F, F, P, P, P, P, P, P,
                        P, P, I, P, P],
     "tm_sym_jvp": [I, P, P, P, P, P, P, P, P, P, P, I, I, I, I, F, F, F, F, P, P, P, P, P, I, P],

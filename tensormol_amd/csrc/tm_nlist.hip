@@ -450,10 +450,12 @@ int tm_nl_build(int N, const double* pos, const int* elem, const int* mol, const
       N, pos, elem, mol, mol_start, box, g, nele, s_r, s_a, (float)rc, ldr, lda, nbr, ncnt, rseg,
       ang, acnt, aseg, flags);
   TM_LAUNCH_CHECK();
-  long total = (long)N * ldr;
-  nl_rev_kernel<<<tm_div_up(total, 256), 256, 0, s>>>(N, ldr, nele, elem, nbr, ncnt, rseg, rev,
-                                                      flags);
-  TM_LAUNCH_CHECK();
+  if (rev != nullptr) {   // else tm_sym_forward fills rev
+    long total = (long)N * ldr;
+    nl_rev_kernel<<<tm_div_up(total, 256), 256, 0, s>>>(N, ldr, nele, elem, nbr, ncnt, rseg, rev,
+                                                        flags);
+    TM_LAUNCH_CHECK();
+  }
   return TM_OK;
 }
 

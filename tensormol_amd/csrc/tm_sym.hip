@@ -157,7 +157,8 @@ sym_fwd_kernel(int N, const double* __restrict__ pos, const double* __restrict__
                const int* __restrict__ nbr, const int* __restrict__ ncnt,
                const int* __restrict__ rseg, const int* __restrict__ ang,
                const int* __restrict__ acnt, const int* __restrict__ aseg, int ldr, int lda,
-               SymArgs a, float* __restrict__ G, int ldg) {
+               SymArgs a, float* __restrict__ G, int ldg, int* __restrict__ rev,
+               int* __restrict__ flags) {
   __shared__ RowLds L;
   __shared__ float4 tq[SYM_WAVES][64];        // per triple: (cos theta, sin theta, fc_j fc_k, -)
   __shared__ float4 tR[SYM_WAVES][64][NA / 4];   // per triple: exp(-eta (ravg - Rs_s)^2), s = 0..7
@@ -275,6 +276,27 @@ sym_fwd_kernel(int N, const double* __restrict__ pos, const double* __restrict__
     __syncwarp();
   }
   for (int p = pdone; p < a.nelep; ++p) Gi[off + p * 64 + lane] = 0.f;
+  if (rev == nullptr) return;
+  // Mirror slots (the rev output of the neighbour build): rev[i*ldr+q] =
+  // j*ldr + position of i in row j, found by binary search in the element
+  // segment of row j that holds i (rows are sorted by (element, index)).
+  const int ei = elem[i];
+  for (int q = lane; q < n; q += 64) {
+    const int j = nbr[(long)i * ldr + q];
+    int lo = rseg[(long)j * (a.nele + 1) + ei];
+    int hi = rseg[(long)j * (a.nele + 1) + ei + 1];
+    const int* row = nbr + (long)j * ldr;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (row[mid] < i) lo = mid + 1; else hi = mid;
+    }
+    if (lo < ncnt[j] && row[lo] == i) {
+      rev[(long)i * ldr + q] = (int)((long)j * ldr + lo);
+    } else {
+      rev[(long)i * ldr + q] = (int)((long)i * ldr + q);   // unmatched (truncated row)
+      atomicOr(&flags[TM_FLAG_NB_OVERFLOW], 2);
+    }
+  }
 }
 
 // dE/dG (two contributions combined: gE + u_i * gQ) -> w slots
@@ -292,6 +314,7 @@ sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__
   extern __shared__ float gdyn[];   // SYM_WAVES * D floats: dE/dG of each wave's centre
   float* gw = gdyn + (threadIdx.x >> 6) * a.D;
   __shared__ float4 wacc[SYM_WAVES][MAX_ROW];
+  __shared__ float dfa_s[SYM_WAVES][MAX_ROW];   // d fc_a / dr per neighbour
   __shared__ float4 ca[SYM_WAVES][64], cb[SYM_WAVES][64];
   __shared__ int segs[SYM_WAVES][2][TM_MAX_ELE + 1];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -320,6 +343,7 @@ sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__
     int e = L.el[w][q];
     float fc = L.fc[w][q];
     float dfc = -0.5f * kr * __sinf(kr * r);
+    dfa_s[w][q] = -0.5f * a.ka * __sinf(a.ka * r);
     float dEdr = 0.f;
     for (int s = 0; s < a.nr; ++s) {
       float x = r - a.rs_r[s];
@@ -335,7 +359,6 @@ sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__
   const int* arow = ang + (long)i * lda;
   const int ntrip = m * (m - 1) / 2;
   const int off = a.nele * a.nr;
-  const float ka = a.ka;
   float4 wang = make_float4(0.f, 0.f, 0.f, 0.f);   // lane = sub-list member
   const int* sg = segs[w][1];
   int eq = -1, lq = 0;   // element and in-element index of member `lane`
@@ -359,7 +382,7 @@ sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__
       float cos_over_sin = sn > 1e-6f ? c * __builtin_amdgcn_rcpf(sn) : 0.f;
       float ravg = 0.5f * (rj + rk);
       float fj = L.fa[w][qj], fk = L.fa[w][qk];
-      float dfj = -0.5f * ka * __sinf(ka * rj), dfk = -0.5f * ka * __sinf(ka * rk);
+      float dfj = dfa_s[w][qj], dfk = dfa_s[w][qk];
       float F = fj * fk;
       float R[NA], Rp[NA];
 #pragma unroll
@@ -384,12 +407,14 @@ sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__
         float Az, dAz;                             // (1+v)^zeta, zeta (1+v)^(zeta-1)
         powz_pair<ZI>(b, a, Az, dAz);
         float dv = a.cos_t[tt] - cos_over_sin * a.sin_t[tt];
-        dEdc += a.two_1mz * dAz * dv * S;
-        dEdrav += a.two_1mz * Az * Sp;
-        dEdF += a.two_1mz * Az * S;
+        dEdc += dAz * dv * S;
+        dEdrav += Az * Sp;
+        dEdF += Az * S;
       }
-      dEdc *= F;
-      dEdrav *= F;
+      // the common factor 2^(1 - zeta) of every term, applied once
+      dEdc *= a.two_1mz * F;
+      dEdrav *= a.two_1mz * F;
+      dEdF *= a.two_1mz;
       // chain rule into d_j, d_k
       float cj = (dEdrav * 0.5f + dEdF * dfj * fk - dEdc * c * irj) * irj;
       float ck = (dEdrav * 0.5f + dEdF * fj * dfk - dEdc * c * irk) * irk;
@@ -670,18 +695,21 @@ int tm_sym_forward(int N, const double* pos, const double* box, const int* mol, 
                    const int* nbr, const int* ncnt, const int* rseg, const int* ang,
                    const int* acnt, const int* aseg, int ldr, int lda, int nele, int nr,
                    float eta, float zeta, float rr, float ra, const float* rs_r,
-                   const float* rs_a, const float* thetas, float* G, int ldg, void* stream) {
+                   const float* rs_a, const float* thetas, float* G, int ldg, int* rev,
+                   int* flags, void* stream) {
   SymArgs a;
   int st = fill_args(a, nele, nr, eta, zeta, rr, ra, rs_r, rs_a, thetas);
   if (st) return st;
   TM_REQUIRE(ldg >= a.D, "tm_sym_forward: ldg %d < D %d", ldg, a.D);
+  TM_REQUIRE(rev == nullptr || flags != nullptr, "tm_sym_forward: rev needs flags");
   TM_REQUIRE(ldr <= MAX_ROW && lda <= MAX_ANG, "tm_sym_forward: strides %d/%d above %d/%d",
              ldr, lda, MAX_ROW, MAX_ANG);
   if (N == 0) return TM_OK;
   hipStream_t s = (hipStream_t)stream;
   auto k = a.zeta_int == 8 ? sym_fwd_kernel<8> : sym_fwd_kernel<0>;
   k<<<tm_div_up(N, SYM_WAVES), SYM_WAVES * 64, 0, s>>>(N, pos, box, mol, elem, nbr, ncnt, rseg,
-                                                       ang, acnt, aseg, ldr, lda, a, G, ldg);
+                                                       ang, acnt, aseg, ldr, lda, a, G, ldg, rev,
+                                                       flags);
   TM_LAUNCH_CHECK();
   return TM_OK;
 }

@@ -282,6 +282,7 @@ class BPEEEngine:
 
     def build_lists(self, ws, pos):
         L, st = self.lib, self._stream()
+        ws.rev_ok = False   # the mirror slots are filled by the next descriptor forward
         L.tm_zero_i32(8, _P(ws.flags), st)
         if ws.use_cells:
             o, c, n = ws.grid_origin, ws.grid_cs, ws.grid_nc
@@ -293,7 +294,7 @@ class BPEEEngine:
         L.tm_nl_build(ws.N, _P(pos), _P(ws.elem), _P(ws.mol), _P(ws.mol_start), _P(ws.box), *g,
                       ws.periodic, self.nele, self.model.sym.rr, self.s_r, self.s_a, self.ldr,
                       self.lda, _P(ws.nbr), _P(ws.ncnt), _P(ws.rseg), _P(ws.ang), _P(ws.acnt),
-                      _P(ws.aseg), _P(ws.rev), _P(ws.flags), st)
+                      _P(ws.aseg), None, _P(ws.flags), st)   # rev: filled by tm_sym_forward
 
     def check_flags(self, ws):
         f = ws.flags.cpu().numpy()
@@ -309,7 +310,9 @@ class BPEEEngine:
                                 _P(ws.ncnt), _P(ws.rseg), _P(ws.ang), _P(ws.acnt), _P(ws.aseg),
                                 self.ldr, self.lda, self.nele, sym.nr_r, sym.eta, sym.zeta, sym.rr,
                                 sym.ra, _lib.fptr(self.sym_rs_r), _lib.fptr(self.sym_rs_a),
-                                _lib.fptr(self.sym_th), _P(ws.G), self.D, self._stream())
+                                _lib.fptr(self.sym_th), _P(ws.G), self.D, _P(ws.rev),
+                                _P(ws.flags), self._stream())
+        ws.rev_ok = True
 
     def networks(self, ws):
         if self.precision == "fp32":
@@ -346,7 +349,14 @@ class BPEEEngine:
                      st)
         L.tm_mol_sum_f64(ws.B, _P(ws.mol_start), 1, _P(ws.phi), 1, _P(ws.phimean), 1, _P(ws.red), ws.N, st)
 
+    def _need_rev(self, ws, pos):
+        """The force gather reads the mirror slots of the current lists, which
+        the descriptor forward fills."""
+        if not getattr(ws, "rev_ok", False):
+            self.descriptors(ws, pos)
+
     def adjoint(self, ws, pos):
+        self._need_rev(ws, pos)
         L, st = self.lib, self._stream()
         sym = self.model.sym
         L.tm_sym_adjoint(ws.N, _P(pos), _P(ws.box), _P(ws.mol), _P(ws.elem), _P(ws.nbr),
@@ -373,6 +383,7 @@ class BPEEEngine:
     def descriptor_vjp(self, ws, pos, gE, grad):
         """grad[:N] = (dG/dx)^T gE: the per-atom gradient of sum_i gE_i . G_i
         (gE cuda float32 [N, D]); the BP-only adjoint (no charge branch)."""
+        self._need_rev(ws, pos)
         L, st = self.lib, self._stream()
         sym = self.model.sym
         assert gE.dtype == torch.float32 and gE.is_contiguous() and gE.shape[1] == self.D

@@ -87,6 +87,36 @@ def test_neighbour_sets_bit_exact(cuda, name, modelname):
 
 
 @pytest.mark.parametrize("name,modelname", SYSTEMS)
+def test_mirror_slots(cuda, name, modelname):
+    """rev from the descriptor forward equals the neighbour build's own pass,
+    and every slot (i -> j) points at the slot (j -> i)."""
+    from tensormol_amd._lib import ptr
+    eng = _engine(modelname)
+    ws, pos = _prepare(eng, name, cuda)
+    eng.build_lists(ws, pos)
+    eng.descriptors(ws, pos)          # fills ws.rev
+    eng.check_flags(ws)
+    torch.cuda.synchronize()
+    rev_fwd = ws.rev.clone()
+    ws.rev.fill_(-7)
+    L, st = eng.lib, eng._stream()
+    g = eng._grid_args(ws)
+    L.tm_nl_build(ws.N, ptr(pos), ptr(ws.elem), ptr(ws.mol), ptr(ws.mol_start),
+                  ptr(ws.box), *g, ws.periodic, eng.nele, eng.model.sym.rr, eng.s_r, eng.s_a,
+                  eng.ldr, eng.lda, ptr(ws.nbr), ptr(ws.ncnt), ptr(ws.rseg), ptr(ws.ang),
+                  ptr(ws.acnt), ptr(ws.aseg), ptr(ws.rev), ptr(ws.flags), st)
+    torch.cuda.synchronize()
+    ncnt = ws.ncnt.cpu().numpy()
+    valid = (np.arange(eng.ldr)[None, :] < ncnt[:, None]).reshape(-1)
+    a = rev_fwd.cpu().numpy()[valid]
+    b = ws.rev.cpu().numpy()[valid]
+    np.testing.assert_array_equal(a, b)
+    nbr = ws.nbr.cpu().numpy()
+    centre = np.repeat(np.arange(ws.N), eng.ldr)[valid]
+    np.testing.assert_array_equal(nbr[a], centre)
+
+
+@pytest.mark.parametrize("name,modelname", SYSTEMS)
 def test_descriptors(cuda, name, modelname):
     eng = _engine(modelname)
     ws, pos = _prepare(eng, name, cuda)
