@@ -174,6 +174,17 @@ __device__ __forceinline__ int tm_candidate(const RangeLds<CAP>& R, int nr, int 
   return R.lo[lo] + (t - R.pre[lo]);
 }
 
+// Range of candidate base + lane inside a chunk, as an index k into the ranges
+// the chunk touches (lane k holds range r0 + k; ranges 1 .. span-1 start at
+// chunk offsets off_k = pre_k - base in [1, 63], strictly increasing): the
+// number of those starts at or below the lane.  The offsets are read with
+// readlane (k is wave-uniform), so the loop has no LDS round trips.
+__device__ __forceinline__ int tm_chunk_range(int lane, int span, int off_k) {
+  int idx = 0;
+  for (int k = 1; k < span; ++k) idx += __builtin_amdgcn_readlane(off_k, k) <= lane ? 1 : 0;
+  return idx;
+}
+
 // Sweep the candidates of centre i: calls f(p, valid, mapped) with all 64
 // lanes in lockstep (so f may use ballots); p indexes the cell-ordered arrays
 // when mapped, the atom arrays otherwise (molecule range, no grid).
@@ -195,8 +206,7 @@ __device__ __forceinline__ void tm_sweep(int i, const int* __restrict__ mol,
   const int total = tm_build_ranges(i, g, rc, R, lane, flags, &nr, px, py, pz);
   // Walk the concatenated ranges 64 candidates at a time.  The ranges that a
   // chunk touches start at r0 (wave-uniform); lane k holds range r0 + k, and
-  // each lane finds its own range by counting, with shuffles, the range
-  // starts at or below its candidate index.
+  // each lane finds its own range with tm_chunk_range.
   int r0 = 0;
   for (int base = 0; base < total; base += 64) {
     const int t = base + lane;
@@ -206,8 +216,7 @@ __device__ __forceinline__ void tm_sweep(int i, const int* __restrict__ mol,
     // ranges starting inside this chunk: those with pre <= base + 63
     const unsigned long long inside = __ballot(rk < nr && pre_k <= base + 63);
     const int span = __popcll(inside);   // >= 1: range r0 contains base
-    int idx = 0;
-    for (int k = 1; k < span; ++k) idx += (__shfl(pre_k, k, 64) <= t) ? 1 : 0;
+    const int idx = tm_chunk_range(lane, span, pre_k - base);
     const int lo_r = __shfl(lo_k, idx, 64);
     const int pre_r = __shfl(pre_k, idx, 64);
     const bool valid = t < total;
@@ -252,8 +261,7 @@ __device__ __forceinline__ void tm_sweep_pf(int i, const int* __restrict__ mol,
     const int lo_k = rk < nr ? R.lo[rk] : 0;
     const unsigned long long inside = __ballot(rk < nr && pre_k <= base + 63);
     const int span = __popcll(inside);
-    int idx = 0;
-    for (int k = 1; k < span; ++k) idx += (__shfl(pre_k, k, 64) <= t) ? 1 : 0;
+    const int idx = tm_chunk_range(lane, span, pre_k - base);
     const int lo_r = __shfl(lo_k, idx, 64);
     const int pre_r = __shfl(pre_k, idx, 64);
     r0 += span - 1;

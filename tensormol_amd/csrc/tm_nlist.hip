@@ -51,8 +51,10 @@ nl_rows_kernel(int N, const double* __restrict__ pos, const int* __restrict__ el
                int* __restrict__ nbr, int* __restrict__ ncnt, int* __restrict__ rseg,
                int* __restrict__ ang, int* __restrict__ acnt, int* __restrict__ aseg,
                int* __restrict__ flags) {
-  __shared__ unsigned long long keys[NL_WAVES][NL_LDS_CAP];
-  __shared__ unsigned long long sorted[NL_WAVES][NL_LDS_CAP];
+  // key = element << 28 | j << 1 | (within the angular cutoff): the order of
+  // keys is the order of (element, j) for distinct atoms (j < 2^27)
+  __shared__ unsigned keys[NL_WAVES][NL_LDS_CAP];
+  __shared__ unsigned sorted[NL_WAVES][NL_LDS_CAP];
   __shared__ RangeLds<NL_RANGES> rng[NL_WAVES];
   const int w = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
@@ -82,8 +84,7 @@ nl_rows_kernel(int N, const double* __restrict__ pos, const int* __restrict__ el
         int slot = count + before;
         int j = mapped ? grid.cell_atoms[p] : p;
         if (slot < NL_LDS_CAP)
-          keys[w][slot] = ((unsigned long long)(unsigned)elem[j] << 40) |
-                          ((unsigned long long)(in_a ? 1u : 0u) << 32) | (unsigned)j;
+          keys[w][slot] = ((unsigned)elem[j] << 28) | ((unsigned)j << 1) | (in_a ? 1u : 0u);
       }
       count += __popcll(bal);
     };
@@ -94,17 +95,11 @@ nl_rows_kernel(int N, const double* __restrict__ pos, const int* __restrict__ el
     }
   }
   __syncthreads();
-  // rank sort by (element, atom index); the in_a bit sits below the element
-  // and above the index and never decides the order of distinct atoms
+  // rank sort by (element, atom index)
   for (int a = lane; a < count; a += 64) {
-    unsigned long long ka = keys[w][a];
-    unsigned long long ord_a = (ka >> 40 << 32) | (ka & 0xffffffffull);
+    const unsigned ka = keys[w][a];
     int rank = 0;
-    for (int b = 0; b < count; ++b) {
-      unsigned long long kb = keys[w][b];
-      unsigned long long ord_b = (kb >> 40 << 32) | (kb & 0xffffffffull);
-      rank += ord_b < ord_a;
-    }
+    for (int b = 0; b < count; ++b) rank += keys[w][b] < ka;
     sorted[w][rank] = ka;
   }
   __syncthreads();
@@ -112,12 +107,13 @@ nl_rows_kernel(int N, const double* __restrict__ pos, const int* __restrict__ el
   long rowr = (long)i * ldr;
   long rowa = (long)i * lda;
   int acount = 0;
+  int cr = 0, ca = 0;   // lane e <= nele: entries (all / angular) with element < e
   for (int base = 0; base < count; base += 64) {
     int a = base + lane;
     bool have = a < count;
-    unsigned long long k = have ? sorted[w][a] : 0ull;
-    if (have) nbr[rowr + a] = (int)(k & 0xffffffffull);
-    bool isang = have && ((k >> 32) & 1ull);
+    unsigned k = have ? sorted[w][a] : 0u;
+    if (have) nbr[rowr + a] = (int)((k >> 1) & ((1u << 27) - 1u));
+    bool isang = have && (k & 1u);
     unsigned long long bal = __ballot(isang);
     int before = __popcll(bal & ((1ull << lane) - 1ull));
     if (isang) {
@@ -125,24 +121,22 @@ nl_rows_kernel(int N, const double* __restrict__ pos, const int* __restrict__ el
       if (slot < lda) ang[rowa + slot] = a;
     }
     acount += __popcll(bal);
+    const int el = (int)(k >> 28);
+    for (int e = 1; e <= nele; ++e) {   // element segment starts of this chunk
+      const unsigned long long br = __ballot(have && el < e);
+      const unsigned long long ba = __ballot(isang && el < e);
+      if (lane == e) { cr += __popcll(br); ca += __popcll(ba); }
+    }
   }
   if (acount > lda) {
     if (lane == 0) atomicOr(&flags[TM_FLAG_ANG_OVERFLOW], 1);
     acount = lda;
   }
   if (lane == 0) { ncnt[i] = count; acnt[i] = acount; }
-  // element segment starts: number of entries with element < e
   if (lane <= nele) {
-    int e = lane;
-    int cr = 0, ca = 0;
-    for (int a = 0; a < count; ++a) {
-      unsigned long long k = sorted[w][a];
-      int el = (int)(k >> 40);
-      if (el < e) { cr++; if ((k >> 32) & 1ull) ca++; }
-    }
     if (ca > acount) ca = acount;
-    rseg[(long)i * (nele + 1) + e] = cr;
-    aseg[(long)i * (nele + 1) + e] = ca;
+    rseg[(long)i * (nele + 1) + lane] = cr;
+    aseg[(long)i * (nele + 1) + lane] = ca;
   }
 }
 
@@ -436,6 +430,7 @@ int tm_nl_build(int N, const double* pos, const int* elem, const int* mol, const
   TM_REQUIRE(ldr > 0 && ldr <= 256 && lda > 0 && lda <= ldr, "tm_nl_build: bad strides %d %d", ldr,
              lda);
   TM_REQUIRE((long)N * ldr < (1l << 31), "tm_nl_build: N*ldr overflows int32 slots");
+  TM_REQUIRE(N <= (1 << 27), "tm_nl_build: %d atoms above 2^27", N);
   if (cell_atoms != nullptr) {
     TM_REQUIRE(cell_start && cell_of && cell_pos && posc, "tm_nl_build: incomplete cell grid");
     int k = (int)ceil(rc / fmin(csx, fmin(csy, csz))) + 1;
