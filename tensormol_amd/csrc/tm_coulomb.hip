@@ -75,17 +75,18 @@ __device__ __forceinline__ void pair_terms(const CoulArgs& a, float qi, float qj
   float dEdr = 0.f;               // dE_pair/dr_A
   if (a.add_ecc) {
     const float r = rA * B1, inv_r = inv_rA * iB1;
-    float K = 0.f, dK = 0.f;
-    if (r > a.r_sr) {
-      if (r <= a.r_lr) {
-        const float z = a.alpha * r;
-        const float e2 = __expf(-z * z);
-        const float ec = erfc_cheb(z, e2);
-        K = ec * inv_r - a.zz + (r - a.r_lr) * a.lin;
-        dK = -ec * inv_r * inv_r - 1.1283791671f * a.alpha * e2 * inv_r + a.lin;
-      }
+    // one exponential serves both regions: exp(-z^2) beyond R_sr, exp(r - R_sr)
+    // inside it (a wave usually holds pairs of both, so no branch)
+    const bool lr = r > a.r_sr;
+    const float z = a.alpha * r;
+    const float ex = __expf(lr ? -z * z : r - a.r_sr);
+    float K, dK;
+    if (lr) {
+      const float ec = erfc_cheb(z, ex);
+      K = ec * inv_r - a.zz + (r - a.r_lr) * a.lin;
+      dK = -ec * inv_r * inv_r - 1.1283791671f * a.alpha * ex * inv_r + a.lin;
+      if (r > a.r_lr) { K = 0.f; dK = 0.f; }
     } else {
-      const float ex = __expf(r - a.r_sr);
       K = a.ea * (ex - 1.0f) + a.shift;
       dK = a.ea * ex;
     }
