@@ -196,57 +196,129 @@ __global__ void cell_assign_kernel(int N, const double* __restrict__ pos, double
   atomicAdd(&cell_count[cell], 1);
 }
 
-// Exclusive prefix sum of n ints (out[n] = total) by one 1024-thread block:
-// tiles of 4096 elements, 4 consecutive per thread (coalesced), a shuffle scan
-// inside each wave, the 16 wave totals scanned through LDS, and a running
-// carry between tiles.  The input is zeroed as it is read (the per-cell
-// counts become the fill counters).
+// Exclusive prefix sum of n ints (out[n] = total), zeroing the input as it is
+// read (the per-cell counts become the fill counters).  A tile of 16384
+// entries is scanned by a 1024-thread block: 16 consecutive entries per thread
+// (four 16-byte loads; in and out are 16-byte aligned), a shuffle scan of the
+// thread totals inside each wave, the 16 wave totals scanned through LDS.
+// One block walks the tiles with a running carry when there are few; for
+// more, a first launch writes the tile totals and a second scans every tile
+// from the sum of the totals before it.
+constexpr int SCAN_PT = 16;
+constexpr int SCAN_TILE = 1024 * SCAN_PT;
+
+__device__ __forceinline__ int scan_tile(int n, int* __restrict__ in, int* __restrict__ out,
+                                         int base, int carry, int* wtot) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int i0 = base + SCAN_PT * tid;
+  int v[SCAN_PT];
+  if (i0 + SCAN_PT <= n) {
+#pragma unroll
+    for (int q = 0; q < SCAN_PT / 4; ++q) {
+      const int4 x = reinterpret_cast<const int4*>(in + i0)[q];
+      v[4 * q] = x.x; v[4 * q + 1] = x.y; v[4 * q + 2] = x.z; v[4 * q + 3] = x.w;
+    }
+#pragma unroll
+    for (int q = 0; q < SCAN_PT / 4; ++q) reinterpret_cast<int4*>(in + i0)[q] = make_int4(0, 0, 0, 0);
+  } else {
+#pragma unroll
+    for (int k = 0; k < SCAN_PT; ++k) v[k] = i0 + k < n ? in[i0 + k] : 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_PT; ++k)
+      if (i0 + k < n) in[i0 + k] = 0;
+  }
+  int t = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_PT; ++k) t += v[k];
+  int incl = t;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int u = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += u;
+  }
+  if (lane == 63) wtot[w] = incl;
+  __syncthreads();
+  if (w == 0) {
+    int x = lane < 16 ? wtot[lane] : 0;
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+      int u = __shfl_up(x, o, 64);
+      if (lane >= o) x += u;
+    }
+    if (lane < 16) wtot[lane] = x;   // inclusive over waves
+  }
+  __syncthreads();
+  int run = carry + (w > 0 ? wtot[w - 1] : 0) + incl - t;
+  int e[SCAN_PT];
+#pragma unroll
+  for (int k = 0; k < SCAN_PT; ++k) { e[k] = run; run += v[k]; }
+  if (i0 + SCAN_PT <= n) {
+#pragma unroll
+    for (int q = 0; q < SCAN_PT / 4; ++q)
+      reinterpret_cast<int4*>(out + i0)[q] = make_int4(e[4 * q], e[4 * q + 1], e[4 * q + 2], e[4 * q + 3]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < SCAN_PT; ++k)
+      if (i0 + k < n) out[i0 + k] = e[k];
+  }
+  const int total = wtot[15];
+  __syncthreads();   // wtot is reused by the next tile
+  return total;
+}
+
 __global__ void __launch_bounds__(1024) scan_kernel(int n, int* __restrict__ in,
                                                     int* __restrict__ out) {
   __shared__ int wtot[16];
-  __shared__ int carry_s;
+  int carry = 0;
+  for (int base = 0; base < n; base += SCAN_TILE) carry += scan_tile(n, in, out, base, carry, wtot);
+  if (threadIdx.x == 0) out[n] = carry;
+}
+
+// totals of the 16384-entry tiles (fixed-order block reduction)
+__global__ void __launch_bounds__(1024) scan_sums_kernel(int n, const int* __restrict__ in,
+                                                         int* __restrict__ sums) {
+  __shared__ int wsum[16];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  if (tid == 0) carry_s = 0;
-  __syncthreads();
-  for (int base = 0; base < n; base += 4096) {
-    const int i0 = base + 4 * tid;
-    int v[4];
+  const int i0 = blockIdx.x * SCAN_TILE + SCAN_PT * tid;
+  int t = 0;
+  if (i0 + SCAN_PT <= n) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) v[k] = i0 + k < n ? in[i0 + k] : 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-      if (i0 + k < n) in[i0 + k] = 0;
-    const int t = v[0] + v[1] + v[2] + v[3];
-    int incl = t;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      int u = __shfl_up(incl, o, 64);
-      if (lane >= o) incl += u;
+    for (int q = 0; q < SCAN_PT / 4; ++q) {
+      const int4 x = reinterpret_cast<const int4*>(in + i0)[q];
+      t += (x.x + x.y) + (x.z + x.w);
     }
-    if (lane == 63) wtot[w] = incl;
-    __syncthreads();
-    if (w == 0) {
-      int x = lane < 16 ? wtot[lane] : 0;
-#pragma unroll
-      for (int o = 1; o < 16; o <<= 1) {
-        int u = __shfl_up(x, o, 64);
-        if (lane >= o) x += u;
-      }
-      if (lane < 16) wtot[lane] = x;   // inclusive over waves
-    }
-    __syncthreads();
-    const int carry = carry_s;
-    int run = carry + (w > 0 ? wtot[w - 1] : 0) + incl - t;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (i0 + k < n) out[i0 + k] = run;
-      run += v[k];
-    }
-    __syncthreads();   // every thread has read carry_s and wtot
-    if (tid == 0) carry_s = carry + wtot[15];
-    __syncthreads();
+  } else {
+    for (int k = 0; k < SCAN_PT; ++k) t += i0 + k < n ? in[i0 + k] : 0;
   }
-  if (tid == 0) out[n] = carry_s;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+  if (lane == 0) wsum[w] = t;
+  __syncthreads();
+  if (tid == 0) {
+    int x = 0;
+    for (int q = 0; q < 16; ++q) x += wsum[q];
+    sums[blockIdx.x] = x;
+  }
+}
+
+// each block scans its tile from the sum of the totals of the tiles before it
+__global__ void __launch_bounds__(1024) scan_apply_kernel(int n, int* __restrict__ in,
+                                                          int* __restrict__ out,
+                                                          const int* __restrict__ sums) {
+  __shared__ int wtot[16];
+  __shared__ int carry_s;
+  const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
+  if (tid < 64) {
+    int x = 0;
+    for (int q = lane; q < b; q += 64) x += sums[q];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    if (tid == 0) carry_s = x;
+  }
+  __syncthreads();
+  const int carry = carry_s;
+  const int total = scan_tile(n, in, out, b * SCAN_TILE, carry, wtot);
+  if (b == (int)gridDim.x - 1 && tid == 0) out[n] = carry + total;
 }
 
 // unsorted cell order into the staging array; the counters start at zero
@@ -408,8 +480,16 @@ int tm_cell_build(int N, const double* pos, const int* mol, double ox, double oy
   cell_assign_kernel<<<tm_div_up(N, 256), 256, 0, s>>>(N, pos, ox, oy, oz, cx, cy, cz, nx, ny, nz,
                                                        periodic, cell_of, cell_count, flags);
   TM_LAUNCH_CHECK();
-  scan_kernel<<<1, 1024, 0, s>>>((int)ncell, cell_count, cell_start);
-  TM_LAUNCH_CHECK();
+  const int ntile = (int)((ncell + SCAN_TILE - 1) / SCAN_TILE);
+  if (ntile <= 2 || ntile > N) {
+    scan_kernel<<<1, 1024, 0, s>>>((int)ncell, cell_count, cell_start);
+    TM_LAUNCH_CHECK();
+  } else {   // tile totals in cell_fill (not yet in use: the fill comes next)
+    scan_sums_kernel<<<ntile, 1024, 0, s>>>((int)ncell, cell_count, cell_fill);
+    TM_LAUNCH_CHECK();
+    scan_apply_kernel<<<ntile, 1024, 0, s>>>((int)ncell, cell_count, cell_start, cell_fill);
+    TM_LAUNCH_CHECK();
+  }
   cell_fill_kernel<<<tm_div_up(N, 256), 256, 0, s>>>(N, cell_of, cell_start, cell_count,
                                                      cell_fill);
   TM_LAUNCH_CHECK();

@@ -149,3 +149,43 @@ def test_energy_forces_fp32(cuda, name, modelname):
     assert frms < 1e-4
     assert qerr < 1e-4
     np.testing.assert_allclose(r["dipole"], ref["dipole"], atol=5e-3 * np.sqrt(natom.max()))
+
+
+@pytest.mark.parametrize("name,modelname", [("waterbox10", "water")])
+def test_fused_abi_paths_bit_identical(cuda, name, modelname):
+    """The engine's fused paths equal the separate ones bit for bit: charges
+    written in cell order by tm_charges (tm_coulomb with xq = NULL) against the
+    gather inside tm_coulomb, and the charge upstream formed inside
+    tm_sym_adjoint (u = NULL, phi and phimean given) against tm_charge_upstream."""
+    from tensormol_amd._lib import P, ptr
+    from tensormol_amd.params import PARAMS
+    eng = _engine(modelname, "fp32")
+    ws, pos = _prepare(eng, name, cuda)
+    assert ws.use_cells
+    eng.run(ws, pos)
+    torch.cuda.synchronize()
+    fused = {k: getattr(ws, k).clone() for k in ("phi", "gradc", "eatom", "wslot", "q")}
+    L, st = eng.lib, eng._stream()
+    g = eng._grid_args(ws)
+    L.tm_charges(ws.N, ptr(ws.mol), ptr(ws.qmean), ptr(ws.qraw), ptr(pos), ptr(ws.xq), ptr(ws.q),
+                 ptr(ws.qx), None, None, None, None, st)
+    ws.xqc.zero_()
+    ws.elc.zero_()
+    L.tm_coulomb(ws.N, ptr(ws.xq), ptr(ws.elem), ptr(ws.mol), ptr(ws.mol_start), ptr(ws.box),
+                 g[0], g[1], g[2], g[3], g[5], g[6], g[7], g[8], g[9], g[10], g[11], g[12], g[13],
+                 g[14], ws.periodic, eng.dsf.ctypes.data_as(P),
+                 eng.model.ee_off, eng.vdw_width, int(bool(PARAMS["AddEcc"])),
+                 int(eng.model.has_vdw), eng.nele, eng.c6.ctypes.data_as(P),
+                 eng.r0.ctypes.data_as(P), ptr(ws.xqc), ptr(ws.elc), ptr(ws.phi),
+                 ptr(ws.gradc), ptr(ws.eatom), ptr(ws.flags), st)
+    L.tm_charge_upstream(ws.N, ptr(ws.mol), ptr(ws.phi), ptr(ws.phimean), ptr(ws.u), st)
+    sym = eng.model.sym
+    from tensormol_amd._lib import fptr
+    L.tm_sym_adjoint(ws.N, ptr(pos), ptr(ws.box), ptr(ws.mol), ptr(ws.elem), ptr(ws.nbr),
+                     ptr(ws.ncnt), ptr(ws.rseg), ptr(ws.ang), ptr(ws.acnt), ptr(ws.aseg), eng.ldr,
+                     eng.lda, eng.nele, sym.nr_r, sym.eta, sym.zeta, sym.rr, sym.ra,
+                     fptr(eng.sym_rs_r), fptr(eng.sym_rs_a), fptr(eng.sym_th), ptr(ws.JE),
+                     ptr(ws.JQ), ptr(ws.u), None, None, eng.D, ptr(ws.wslot), st)
+    torch.cuda.synchronize()
+    for k, v in fused.items():
+        assert torch.equal(getattr(ws, k), v), k

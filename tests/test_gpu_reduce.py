@@ -119,12 +119,13 @@ def test_fused_sums_match_single_sums(cuda, sizes):
     assert scratch[:1].view(torch.int32)[0].item() == 0
 
 
-def test_cell_build_leaves_counts_zero(cuda):
+@pytest.mark.parametrize("N,n", [(3000, 6), (20000, 40)])
+def test_cell_build_leaves_counts_zero(cuda, N, n):
     """tm_cell_build needs cell_count zero on entry and leaves it zero; cells
-    list every atom once, in index order inside each cell."""
+    list every atom once, in index order inside each cell.  64,000 cells take
+    the two-launch scan (four 16384-entry tiles), 216 the single block."""
     L = _lib.lib()
     rng = np.random.default_rng(4)
-    N, n = 3000, 6
     edge = 20.0
     pos = torch.tensor(rng.uniform(0, edge, (N, 3)), dtype=torch.float64, device=cuda)
     mol = torch.zeros(N, dtype=torch.int32, device=cuda)
@@ -145,7 +146,9 @@ def test_cell_build_leaves_counts_zero(cuda):
         assert int(count.abs().sum().item()) == 0
         a, s, co = atoms.cpu().numpy(), start.cpu().numpy(), cell_of.cpu().numpy()
         assert sorted(a.tolist()) == list(range(N))
-        for c in range(ncell):
+        assert s[0] == 0 and s[-1] == N and np.all(np.diff(s) >= 0)
+        np.testing.assert_array_equal(np.diff(s), np.bincount(co, minlength=ncell))
+        for c in np.flatnonzero(np.diff(s)):
             seg = a[s[c]:s[c + 1]]
             assert np.all(np.diff(seg) > 0) and np.all(co[seg] == c)
         assert np.array_equal(cpos.cpu().numpy()[a], np.arange(N))
