@@ -139,10 +139,10 @@ class BPEEEngine:
         i32 = dict(dtype=torch.int32, device=dev)
         f32 = dict(dtype=torch.float32, device=dev)
         f64 = dict(dtype=torch.float64, device=dev)
-        ws.elem = torch.zeros(N, **i32)
-        ws.mol = torch.zeros(N, **i32)
-        ws.mol_start = torch.zeros(B + 1, **i32)
-        ws.perm = torch.zeros(N, **i32)
+        ws.elem = ws.buf_elem = torch.zeros(N, **i32)
+        ws.mol = ws.buf_mol = torch.zeros(N, **i32)
+        ws.mol_start = ws.buf_mol_start = torch.zeros(B + 1, **i32)
+        ws.perm = ws.buf_perm = torch.zeros(N, **i32)
         ws.nbr = torch.zeros(N * self.ldr, **i32)
         ws.ncnt = torch.zeros(N, **i32)
         ws.rseg = torch.zeros(N * (self.nele + 1), **i32)
@@ -180,7 +180,9 @@ class BPEEEngine:
         """Point a workspace at a new packed batch (N <= ncap, B <= bcap).
         Only small host-built index arrays are copied; no buffer is reallocated
         unless a cell grid is (re)built for a single large system.  With an
-        open PinnedStager the copies are queued without blocking the host."""
+        open PinnedStager the arrays join its batch: the workspace then points
+        at their device copies, which are valid for work queued after the
+        stager's flush()."""
         up = (lambda a, out=None: stager.put(a, out)) if stager is not None else _plain_upload(self.device)
         dev = self.device
         Z = np.asarray(Z)
@@ -197,11 +199,22 @@ class BPEEEngine:
         perm = np.argsort(elem, kind="stable").astype(np.int32)
         counts = np.bincount(elem, minlength=self.nele)
         ws.eoff = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
-        if N:
-            up(elem, ws.elem[:N])
-            up(ws.mol_np, ws.mol[:N])
-            up(perm, ws.perm[:N])
-        up(ms.astype(np.int32), ws.mol_start[:B + 1])
+        if stager is not None:
+            # per-batch index arrays are views of the stager's device arena,
+            # valid for the work queued after its flush
+            if N:
+                ws.elem = up(elem.astype(np.int32))
+                ws.mol = up(ws.mol_np)
+                ws.perm = up(perm)
+            ws.mol_start = up(ms.astype(np.int32))
+        else:
+            ws.elem, ws.mol, ws.perm = ws.buf_elem, ws.buf_mol, ws.buf_perm
+            ws.mol_start = ws.buf_mol_start
+            if N:
+                up(elem, ws.elem[:N])
+                up(ws.mol_np, ws.mol[:N])
+                up(perm, ws.perm[:N])
+            up(ms.astype(np.int32), ws.mol_start[:B + 1])
         ws.box = None
         ws.periodic = 0
         ws.boxes_np = None

@@ -136,13 +136,14 @@ class Server:
         return np.repeat(starts, lens) + rel
 
     def _repack(self, keep, new):
-        """Order continuing + new requests as [nvt | opt | sp], move device state
-        (every copy queued behind the running step)."""
+        """Order continuing + new requests as [nvt | opt | sp] and move device
+        state.  Every host array of the batch is staged first and lands with one
+        host->device copy; the device gathers and scatters are queued after it
+        (all behind the running step)."""
         reqs = sorted(keep + new, key=lambda r: (KIND_ORDER[r.kind], r.rid))
         up = self.stager.put
         self.stager.begin()
         B = len(reqs)
-        dev = self.eng.device
         sizes = np.fromiter((len(r.Z) for r in reqs), dtype=np.int64, count=B)
         ms = np.zeros(B + 1, dtype=np.int64)
         np.cumsum(sizes, out=ms[1:])
@@ -151,17 +152,13 @@ class Server:
         old_off = np.fromiter((r._off for r in reqs), dtype=np.int64, count=B)
         old_slot = np.fromiter((r._mol for r in reqs), dtype=np.int64, count=B)
         cont = old_off >= 0
-        eta_c = fs_c = None
-        if cont.any():
-            # continuing requests: atom state and per-molecule state gathered on the device
+        # ---- stage (host only)
+        has_cont = bool(cont.any())
+        if has_cont:   # continuing requests: their atom and molecule state is gathered on the device
             s_idx = up(self._ranges(old_off[cont], sizes[cont]))
             d_idx = up(self._ranges(ms[:-1][cont], sizes[cont]))
-            for t in (self.pos, self.v, self.a):
-                t.index_copy_(0, d_idx, t.index_select(0, s_idx))
             so = up(old_slot[cont])
             sn = up(np.nonzero(cont)[0])
-            eta_c = self.eta.index_select(0, so)
-            fs_c = self.fstate.index_select(0, so)
         newk = np.nonzero(~cont)[0]
         if len(newk):
             xs = np.concatenate([reqs[k].xyz for k in newk]).astype(np.float64)
@@ -172,13 +169,9 @@ class Server:
                 if r.kind == "nvt":
                     vs[o:o + len(r.Z)] = maxwell_boltzmann(r.Z, self.T, r.seed, atomic_masses(r.Z))
                 o += len(r.Z)
-            d_idx = up(self._ranges(ms[:-1][newk], sizes[newk]))
-            self.pos.index_copy_(0, d_idx, up(xs))
-            self.v.index_copy_(0, d_idx, up(vs))
-            self.a.index_fill_(0, d_idx, 0.0)
+            d_new = up(self._ranges(ms[:-1][newk], sizes[newk]))
+            xs_d, vs_d = up(xs), up(vs)
         Zall = np.concatenate([r.Z for r in reqs]) if reqs else np.zeros(0, np.int32)
-        if N:
-            up(atomic_masses(Zall), self.m[:N])
         # per-molecule state: thermostat constants from kind and size (SimpleMD.py:109-125),
         # fresh FIRE state for new optimisations, carried state for continuing requests
         kT = np.where(kinds == 0, IDEALGASR * 1e-10 * self.T, 0.0)
@@ -186,12 +179,29 @@ class Server:
         Q = np.where(kinds == 0, kT * tau * tau, 1.0)
         fs = np.zeros((B, 4))
         fs[kinds == 1] = (self.fire[0], 0.1, 0.0, 1.0)
+        if N:
+            self.m = up(atomic_masses(Zall))
+        if B:
+            self.kT = up(kT)
+            self.Q = up(Q)
+            fs_d = up(fs)
+        if reqs:
+            self.eng.set_topology(self.ws, Zall, ms, stager=self.stager)
+        self.stager.flush()
+        # ---- device work, queued after the upload
+        if has_cont:
+            for t in (self.pos, self.v, self.a):
+                t.index_copy_(0, d_idx, t.index_select(0, s_idx))
+            eta_c = self.eta.index_select(0, so)
+            fs_c = self.fstate.index_select(0, so)
+        if len(newk):
+            self.pos.index_copy_(0, d_new, xs_d)
+            self.v.index_copy_(0, d_new, vs_d)
+            self.a.index_fill_(0, d_new, 0.0)
         if B:
             self.eta[:B].zero_()
-            up(kT, self.kT[:B])
-            up(Q, self.Q[:B])
-            up(fs, self.fstate[:B])
-            if eta_c is not None:
+            self.fstate[:B].copy_(fs_d)
+            if has_cont:
                 self.eta.index_copy_(0, sn, eta_c)
                 self.fstate.index_copy_(0, sn, fs_c)
         for k, r in enumerate(reqs):
@@ -201,8 +211,6 @@ class Server:
         self.ms = ms
         self.natoms = N
         self.groups = [int(c) for c in np.bincount(kinds, minlength=3)]
-        if reqs:
-            self.eng.set_topology(self.ws, Zall, ms, stager=self.stager)
         self.stager.end()
 
     # ----------------------------------------------------------------- step

@@ -9,21 +9,31 @@ import torch
 
 
 class PinnedStager:
-    """Host->device uploads through a ring of pinned slots.
+    """Host->device uploads of one batch through a ring of pinned slots and a
+    single device arena, in one copy.
 
-    begin() opens the next slot (waiting, if needed, until the copies queued
-    from it last time have completed), put() stages an array and queues its
-    copy on the current stream, end() records the slot's completion event.
+    begin() opens the next slot (waiting, if needed, until the copy queued from
+    it last time has completed).  put() stages an array in the slot and returns
+    the device view where it will land (or, with `out`, a copy into `out` is
+    queued after the upload).  flush() queues the one host->device copy of the
+    slot, then those copies; device work that reads a put() result must be
+    queued after flush().  end() flushes if needed and records the slot's
+    completion event.  Everything runs on the current stream, so the arena is
+    rewritten only after the kernels queued before the next flush() have read it.
     """
 
     def __init__(self, device, slot_bytes=16 << 20, slots=2):
         self.device = device
         self.size = int(slot_bytes)
         self.bufs = [torch.empty(self.size, dtype=torch.uint8, pin_memory=True) for _ in range(slots)]
+        self.np_bufs = [b.numpy() for b in self.bufs]
+        self.arena = torch.empty(self.size, dtype=torch.uint8, device=device)
         self.events = [None] * slots
         self.cur = 0
         self.off = 0
         self.open = False
+        self.flushed = True
+        self.pending = []
 
     def begin(self):
         self.cur = (self.cur + 1) % len(self.bufs)
@@ -32,30 +42,49 @@ class PinnedStager:
             ev.synchronize()
         self.off = 0
         self.open = True
+        self.flushed = False
+        self.pending = []
+
+    def flush(self):
+        if self.flushed:
+            return
+        if self.off:
+            self.arena[:self.off].copy_(self.bufs[self.cur][:self.off], non_blocking=True)
+        for dst, src in self.pending:
+            dst.copy_(src)
+        self.pending = []
+        self.flushed = True
 
     def end(self):
+        self.flush()
         ev = torch.cuda.Event()
         ev.record(torch.cuda.current_stream(self.device))
         self.events[self.cur] = ev
         self.open = False
 
     def put(self, arr, out=None):
-        """Queue a copy of `arr` into `out` (or a new device tensor); returns it."""
+        """Stage `arr`; returns its device copy (valid for work queued after
+        flush()), or `out`, into which it is copied at flush()."""
         arr = np.ascontiguousarray(arr)
-        src = torch.from_numpy(arr)
-        dst = out if out is not None else torch.empty(arr.shape, dtype=src.dtype, device=self.device)
         n = arr.nbytes
-        if n == 0:
-            return dst
         off = (self.off + 255) & ~255
-        if not self.open or off + n > self.size:
-            dst.copy_(src)   # outside a slot, or too large: a plain copy
+        if not self.open or self.flushed or off + n > self.size:
+            # outside an open batch, or too large: a plain (blocking) copy
+            src = torch.from_numpy(arr)
+            dst = out if out is not None else torch.empty(arr.shape, dtype=src.dtype, device=self.device)
+            if n:
+                dst.copy_(src)
             return dst
-        host = self.bufs[self.cur][off:off + n].view(src.dtype).view(arr.shape)
-        host.copy_(src)
-        dst.copy_(host, non_blocking=True)
+        if n >= (64 << 10):   # large: torch's multi-threaded host copy
+            self.bufs[self.cur][off:off + n].copy_(torch.from_numpy(arr.reshape(-1).view(np.uint8)))
+        else:                 # small: numpy's lower per-call cost
+            self.np_bufs[self.cur][off:off + n] = arr.reshape(-1).view(np.uint8)
         self.off = off + n
-        return dst
+        dev = self.arena[off:off + n].view(torch.from_numpy(arr[:0].reshape(-1)).dtype).view(arr.shape)
+        if out is None:
+            return dev
+        self.pending.append((out, dev))
+        return out
 
 
 class PinnedOutputs:
