@@ -16,6 +16,7 @@ elu_Normalize_Dropout.evaluate (TensorMol/TFNetworks/TFMolInstanceDirect.py:
 Memory layout in HBM: positions float64 [N,3]; descriptors float32 [N,D];
 neighbour rows int32 ELL [N, ldr]; per-atom scalars float32/float64 [N].
 """
+import ctypes as _ctypes
 import math
 import struct
 
@@ -86,6 +87,7 @@ class BPEEEngine:
                  check_overflow=True):
         self.model = model
         self.device = torch.device(device)
+        self._dev_index = self.device.index   # None: the current device at each call
         self.precision = precision or PARAMS["MLPPrecision"]
         self.ldr, self.lda = int(ldr), int(lda)
         self.check_overflow = check_overflow
@@ -196,7 +198,7 @@ class BPEEEngine:
         ws.elem_np = elem
         ws.mol_np = np.repeat(np.arange(B, dtype=np.int32), np.diff(ms))
         ws.ms_np = ms
-        perm = np.argsort(elem, kind="stable").astype(np.int32)
+        perm = np.argsort(elem.astype(np.uint8), kind="stable").astype(np.int32)   # radix sort
         counts = np.bincount(elem, minlength=self.nele)
         ws.eoff = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
         if stager is not None:
@@ -291,7 +293,10 @@ class BPEEEngine:
 
     # ---------------------------------------------------------- stages
     def _stream(self):
-        return ctypes_stream(torch.cuda.current_stream(self.device))
+        """The current stream of the engine's device as a ctypes pointer (read
+        without resolving a device object: it is asked for at every launch)."""
+        idx = self._dev_index if self._dev_index is not None else torch.cuda.current_device()
+        return _ctypes.c_void_p(torch._C._cuda_getCurrentRawStream(idx))
 
     def build_lists(self, ws, pos):
         L, st = self.lib, self._stream()
