@@ -81,8 +81,14 @@ class Server:
     """Continuous batching over one engine workspace of fixed capacity."""
 
     def __init__(self, engine, max_atoms=65536, max_mols=8192, hbm_budget_bytes=None, T=300.0,
-                 dt=0.5, fire_dt0=1.0, fire_fmax=5e-3, fire_maxstep=0.1):
+                 dt=0.5, fire_dt0=1.0, fire_fmax=5e-3, fire_maxstep=0.1, repack_every=1):
+        """repack_every: with K > 1, a batch whose membership changes only by
+        departures and arrivals is repacked at most every K iterations; in
+        between, the previous layout is relaunched with departed requests as
+        discarded ghosts and arrivals wait (at most K - 1 iterations).  A repack
+        is forced when ghosts exceed a quarter of the batch's atoms."""
         self.eng = engine
+        self.repack_every = max(1, int(repack_every))
         per_atom = self.bytes_per_atom()
         if hbm_budget_bytes is not None:
             max_atoms = min(max_atoms, int(hbm_budget_bytes // per_atom))
@@ -343,6 +349,7 @@ class Server:
         self.natoms = 0
         self.atom_steps = 0
         inflight = None
+        since = 0   # iterations since the last repack
         while it < max_iters:
             # continuing requests: those of the launched step that outlive it
             if inflight is not None:
@@ -352,8 +359,13 @@ class Server:
             natoms = sum(len(r.Z) for r in keep)
             nmol = len(keep)
             now = time.perf_counter() - t0
+            # relaunch the previous layout (departed requests as ghosts, arrivals
+            # waiting) unless a repack is due
+            relaunch = False
+            if self.repack_every > 1 and inflight is not None and keep and since + 1 < self.repack_every:
+                relaunch = 4 * natoms >= 3 * inflight["N"]
             new = []
-            while pi < len(pending) and (not realtime or pending[pi].t_arrival <= now):
+            while not relaunch and pi < len(pending) and (not realtime or pending[pi].t_arrival <= now):
                 r = pending[pi]
                 if natoms + len(r.Z) > self.max_atoms or nmol + 1 > self.max_mols:
                     break   # admission by capacity (HBM budget); retried next iteration
@@ -366,8 +378,13 @@ class Server:
                 pi += 1
             nxt = None
             if keep or new:
-                if new or inflight is None or len(keep) != len(inflight["reqs"]):
+                if relaunch:
+                    # requests whose last step is the one on the GPU are retired by
+                    # its _finish, before this launch's results are read
+                    since += 1
+                elif new or inflight is None or len(keep) != len(inflight["reqs"]):
                     self._repack(keep, new)
+                    since = 0
                 self.outs.reverse()
                 nxt = self._launch(self.outs[0])
                 it += 1

@@ -181,3 +181,24 @@ def test_failure_stays_with_its_molecule(served):
             np.testing.assert_array_equal(o.forces, d.forces)
         else:
             np.testing.assert_array_equal(o.positions, d.positions)
+
+
+def test_lazy_repack_same_results(served):
+    """With repack_every = 4 (departed requests ride along as discarded ghosts,
+    arrivals wait for the next repack) every request ends with the same steps,
+    energy and coordinates/forces, bit for bit."""
+    from tensormol_amd.serving import Server
+    model, eng, srv, reqs, done = served
+    lazy = Server(eng, max_atoms=4096, max_mols=256, repack_every=4)
+    got = {r.rid: r for r in lazy.run(_copy(reqs), realtime=False)}
+    torch.cuda.synchronize()
+    assert sorted(got) == sorted(done)
+    assert lazy.iters > 0
+    for rid, d in done.items():
+        g = got[rid]
+        assert g.error == d.error and g.steps_done == d.steps_done, rid
+        assert g.energy == d.energy, rid
+        if d.kind == "sp":
+            assert np.array_equal(g.forces, d.forces), rid
+        else:
+            assert np.array_equal(g.positions, d.positions), rid
