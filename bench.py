@@ -52,11 +52,14 @@ def build_system(args):
     return Z, x, box, model
 
 
-def run_md(eng, ws, Z, x, box, steps, warmup, dt, graph=True, mlp_samples=20):
+def run_md(eng, ws, Z, x, box, steps, warmup, dt, graph=True, mlp_samples=20, steps_per_graph=10):
     """NVE Velocity Verlet on the device.  Returns wall time of the timed steps,
-    per-step times (HIP events between steps), live MLP-kernel launch times
-    (HIP events around the launch, on the stream it runs on) and the total
-    energy before and after the timed steps."""
+    per-step times (HIP events between graph replays, divided by the steps a
+    replay holds), live MLP-kernel launch times (HIP events around the launch,
+    on the stream it runs on) and the total energy before and after the timed
+    steps.  With graph, `steps_per_graph` consecutive steps (the largest
+    divisor of `steps` not above it) are captured into one HIP graph, so the
+    replay's launch cost is paid once per that many steps."""
     import torch
     from tensormol_amd import _lib, params
     from tensormol_amd import workloads as W
@@ -105,9 +108,14 @@ def run_md(eng, ws, Z, x, box, steps, warmup, dt, graph=True, mlp_samples=20):
         if graph:
             g = torch.cuda.CUDAGraph()
             side.synchronize()
+    spg = 1
     if graph:
+        spg = max(1, min(int(steps_per_graph), steps))
+        while steps % spg:
+            spg -= 1
         with torch.cuda.graph(g, stream=side):
-            step()
+            for _ in range(spg):
+                step()
         replay = g.replay
     else:
         replay = step
@@ -119,7 +127,7 @@ def run_md(eng, ws, Z, x, box, steps, warmup, dt, graph=True, mlp_samples=20):
         side.synchronize()
         t0 = time.perf_counter()
         first.record(side)
-        for _ in range(steps):
+        for _ in range(steps // spg):
             replay()
             ev = torch.cuda.Event(enable_timing=True)
             ev.record(side)
@@ -131,19 +139,20 @@ def run_md(eng, ws, Z, x, box, steps, warmup, dt, graph=True, mlp_samples=20):
     per_step = []
     prev = first
     for ev in step_ev:
-        per_step.append(prev.elapsed_time(ev))
+        per_step.append(prev.elapsed_time(ev) / spg)
         prev = ev
     mlp = np.array([a_.elapsed_time(b_) for a_, b_ in mlp_ms])
     return dict(wall=t1 - t0, per_step_ms=np.array(per_step), mlp_ms=mlp, e_start=e_start,
-                e_end=e_end, graph=graph)
+                e_end=e_end, graph=graph, steps_per_graph=spg)
 
 
 STAGES = ("build_lists", "descriptors", "networks", "electrostatics", "adjoint", "reductions")
 
 
 def stage_times(eng, ws, pos, reps, stream):
-    """Mean GPU time (ms) of each engine stage, from HIP events recorded on
-    the stream the stages run on, over `reps` eager evaluations."""
+    """Median GPU time (ms) of each engine stage, from HIP events recorded on
+    the stream the stages run on, over `reps` eager evaluations (the median
+    keeps one stalled launch from the host out of the figure)."""
     import torch
     recs = []
     with torch.cuda.stream(stream):
@@ -158,7 +167,7 @@ def stage_times(eng, ws, pos, reps, stream):
     stream.synchronize()
     out = {}
     for k, name in enumerate(STAGES):
-        out[name] = float(np.mean([e[k].elapsed_time(e[k + 1]) for e in recs]))
+        out[name] = float(np.median([e[k].elapsed_time(e[k + 1]) for e in recs]))
     return out
 
 
@@ -458,6 +467,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--steps-per-graph", type=int, default=10,
+                    help="MD steps captured into one HIP graph (a divisor of --steps)")
     ap.add_argument("--serve-requests", type=int, default=4096, help="0 disables the serving leg")
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-atoms", type=int, default=65536)
@@ -519,7 +530,8 @@ def main():
     ws = eng.prepare(Z, np.array([0, N]), box[None])
     print(f"[bench] {N} atoms, precision {args.precision}, warmup {args.warmup}, steps {args.steps}",
           file=sys.stderr, flush=True)
-    res = run_md(eng, ws, Z, x, box, args.steps, args.warmup, args.dt, graph=not args.no_graph)
+    res = run_md(eng, ws, Z, x, box, args.steps, args.warmup, args.dt, graph=not args.no_graph,
+                 steps_per_graph=args.steps_per_graph)
     value = N * args.steps / res["wall"]
     ms_per_step = 1e3 * res["wall"] / args.steps
     print(f"[bench] {value:.4g} atom-steps/s, {ms_per_step:.3f} ms/step", file=sys.stderr, flush=True)
@@ -556,7 +568,7 @@ def main():
         # same warmup, step count and HIP-graph replay as the timed run: the
         # random-weight system heats, so a shorter or earlier segment is cheaper
         r32 = run_md(eng32, ws32, Z, x, box, args.steps, args.warmup, args.dt,
-                     graph=not args.no_graph, mlp_samples=0)
+                     graph=not args.no_graph, mlp_samples=0, steps_per_graph=args.steps_per_graph)
         fp32_value = N * args.steps / r32["wall"]
         print(f"[bench] plain fp32 path {fp32_value:.4g} atom-steps/s", file=sys.stderr, flush=True)
 
@@ -611,6 +623,7 @@ def main():
         "config": {"workload": f"periodic {N // 3}-water box NVE MD ({N} atoms, {args.edge} A cube), "
                                "BP+EE 256-512-512-512-1 x2 nets, DSF 15 A",
                    "atoms": N, "dt_fs": args.dt, "parallelism": "single GPU",
+                   "steps_per_graph": res["steps_per_graph"],
                    "hip_graph": not args.no_graph},
         "nve_energy_drift_Ha": round(float(drift), 6),
         "force_rms_err_vs_fp64": force_rms_err,
