@@ -304,7 +304,8 @@ def run_train(args):
     dev = torch.device("cuda")
     pos = torch.tensor(x, dtype=torch.float64, device=dev)
     for dtype in ("bf16", "fp32"):
-        tr = ForceMatchingTrainer(student, mlp_dtype=dtype, lr=1e-4)
+        # eager steps, then the same step captured into a HIP graph and replayed
+        tr = ForceMatchingTrainer(student, mlp_dtype=dtype, lr=1e-4, capturable=True)
         tb = tr.batch(Z, ms, E_ref, F_ref)
         for _ in range(args.warmup):
             tr.step(tb, pos)
@@ -314,11 +315,27 @@ def run_train(args):
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         losses = [float(v.detach()) for v in losses]
-        out[dtype] = dict(ms_per_step=1e3 * wall / args.train_steps,
-                          atoms_per_s=N * args.train_steps / wall, loss_first=losses[0],
-                          loss_last=losses[-1])
-        log(f"{dtype}: {out[dtype]['ms_per_step']:.2f} ms/step, {out[dtype]['atoms_per_s']:.4g} "
-            f"atoms/s, loss {losses[0]:.4e} -> {losses[-1]:.4e}")
+        rec = dict(eager_ms_per_step=1e3 * wall / args.train_steps,
+                   eager_atoms_per_s=N * args.train_steps / wall)
+        if not args.train_no_graph:
+            g, Lg = tr.capture(tb, pos, warmup=3)
+            for _ in range(2):
+                g.replay()
+            torch.cuda.synchronize()
+            gl = []
+            t0 = time.perf_counter()
+            for _ in range(args.train_steps):
+                g.replay()
+                gl.append(Lg.clone())
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+            losses += [float(v) for v in gl]
+        rec.update(ms_per_step=1e3 * wall / args.train_steps, atoms_per_s=N * args.train_steps / wall,
+                   hip_graph=not args.train_no_graph, loss_first=losses[0], loss_last=losses[-1])
+        out[dtype] = rec
+        log(f"{dtype}: {rec['ms_per_step']:.2f} ms/step ({'graph replay' if rec['hip_graph'] else 'eager'}), "
+            f"eager {rec['eager_ms_per_step']:.2f} ms/step, {rec['atoms_per_s']:.4g} atoms/s, "
+            f"loss {losses[0]:.4e} -> {losses[-1]:.4e}")
     # gradient check on the first 16 molecules against the FP64 oracle's double backward
     k = 16
     Zs, xs, mss = Z[:ms[k]], x[:ms[k]], ms[:k + 1]
@@ -345,7 +362,9 @@ def run_train(args):
         "dtype": "bf16 GEMM operands (rounded), fp32 products and accumulation", "data": "synthetic",
         "config": {"workload": f"{B} C/H/N/O chain molecules of 20-60 atoms ({N} atoms), "
                                "BP nets 768-512-512-512-1, loss MSE(E) + 0.5 MSE(F), Adam"},
-        "fp32_mlp": {k_: round(v, 4) for k_, v in out["fp32"].items()},
+        "eager_ms_per_step": round(out["bf16"]["eager_ms_per_step"], 3),
+        "hip_graph": out["bf16"]["hip_graph"],
+        "fp32_mlp": {k_: (round(v, 4) if isinstance(v, float) else v) for k_, v in out["fp32"].items()},
         "loss_first_last_bf16": [out["bf16"]["loss_first"], out["bf16"]["loss_last"]],
         "weight_grad_rel_err_vs_fp64": errs,
     }
@@ -485,6 +504,8 @@ def main():
                     help="force-matching training steps on 256 chain molecules instead (own JSON line)")
     ap.add_argument("--train-mols", type=int, default=256)
     ap.add_argument("--train-steps", type=int, default=20)
+    ap.add_argument("--train-no-graph", action="store_true",
+                    help="time eager training steps only (no HIP-graph capture)")
     ap.add_argument("--sphere", action="store_true",
                     help="single-point E+F of the 24,000-atom C/H/N/O sphere instead (own JSON line)")
     ap.add_argument("--sphere-atoms", type=int, default=24000)

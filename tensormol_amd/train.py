@@ -71,7 +71,10 @@ class ForceMatchingTrainer:
     stack torch's BF16 GEMMs stall the host for milliseconds per call, and the
     step is bound by launch latency, not by GEMM throughput."""
 
-    def __init__(self, model, device="cuda", mlp_dtype="fp32", lr=1e-4, ldr=128, lda=48):
+    def __init__(self, model, device="cuda", mlp_dtype="fp32", lr=1e-4, ldr=128, lda=48,
+                 capturable=False):
+        """capturable: keep the optimizer's state on the device so that a whole
+        step can be captured into a HIP graph (capture())."""
         if mlp_dtype not in ("fp32", "bf16"):
             raise ValueError(f"mlp_dtype {mlp_dtype!r}: use 'fp32' or 'bf16'")
         self.model = model
@@ -83,7 +86,7 @@ class ForceMatchingTrainer:
         self.params = {e: [(torch.tensor(np.asarray(W), **f32).requires_grad_(),
                             torch.tensor(np.asarray(b), **f32).requires_grad_())
                            for (W, b) in model.energy_nets[e]] for e in model.eles}
-        self.opt = torch.optim.Adam(self.parameters(), lr=lr)
+        self.opt = torch.optim.Adam(self.parameters(), lr=lr, capturable=bool(capturable))
 
     def parameters(self):
         """Flat list: elements in model order, layers in order, (W, b) each."""
@@ -126,12 +129,14 @@ class ForceMatchingTrainer:
                 h = z
         return h[:, 0]
 
-    def loss(self, tb, pos):
+    def loss(self, tb, pos, check=True):
         """(L, E, F) at positions pos (cuda float64 [N, 3]); differentiable in
-        the network weights."""
+        the network weights.  check reads the neighbour-list flags on the host
+        (skipped inside a graph capture)."""
         eng, ws = self.eng, tb.ws
         eng.build_lists(ws, pos)
-        eng.check_flags(ws)
+        if check:
+            eng.check_flags(ws)
         eng.descriptors(ws, pos)
         G = ws.G[:tb.N].detach().clone().requires_grad_(True)
         eps = torch.zeros(tb.N, dtype=torch.float32, device=self.device)
@@ -148,10 +153,34 @@ class ForceMatchingTrainer:
         L, E, F = self.loss(tb, pos)
         return L, torch.autograd.grad(L, self.parameters())
 
-    def step(self, tb, pos):
+    def step(self, tb, pos, check=True):
         """One Adam step; returns the loss before the update."""
         self.opt.zero_grad(set_to_none=True)
-        L, _, _ = self.loss(tb, pos)
+        L, _, _ = self.loss(tb, pos, check=check)
         L.backward()
         self.opt.step()
         return L
+
+    def capture(self, tb, pos, warmup=3):
+        """Capture one whole step (lists, descriptors, MLP, double backward,
+        Adam update) for a fixed batch and positions into a HIP graph; returns
+        (graph, loss) where each graph.replay() performs one more step and
+        leaves its loss (before the update) in `loss`.  Needs capturable=True.
+        The `warmup` eager steps before the capture are training steps too."""
+        if not self.opt.param_groups[0].get("capturable", False):
+            raise RuntimeError("capture() needs ForceMatchingTrainer(..., capturable=True)")
+        cur = torch.cuda.current_stream(self.device)
+        side = torch.cuda.Stream(self.device)
+        side.wait_stream(cur)
+        with torch.cuda.stream(side):
+            for _ in range(warmup):
+                self.step(tb, pos)
+        cur.wait_stream(side)
+        torch.cuda.synchronize(self.device)
+        g = torch.cuda.CUDAGraph()
+        self.opt.zero_grad(set_to_none=True)
+        with torch.cuda.graph(g):
+            L, _, _ = self.loss(tb, pos, check=False)
+            L.backward()
+            self.opt.step()
+        return g, L
