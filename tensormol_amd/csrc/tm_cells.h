@@ -41,6 +41,12 @@ __device__ __forceinline__ float tm_band_dist(int d, float f, float cs) {
   return fmaxf(v, 0.f);
 }
 
+// Lanes below this one whose bit is set in a wave-wide ballot.
+__device__ __forceinline__ int tm_lanes_below(unsigned long long bal) {
+  return __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
+}
+
 // Ranges per centre (at most two per stencil row).  A sweep of cutoff rc over
 // cells of edge cs has at most (2*ceil(rc/cs)+3)^2 rows.
 template <int CAP>
@@ -48,11 +54,13 @@ struct RangeLds {
   static constexpr int cap = CAP;
   int lo[CAP];
   int pre[CAP + 1];
+  int mark[64];   // range starts inside the current chunk, tagged with the chunk's base
 };
 
 // Builds the candidate ranges of atom i for cutoff rc into R (one wavefront,
 // all lanes participate).  Returns the number of candidates; *nrange gets the
-// number of ranges (capped at CAP, overflow flagged).
+// number of ranges (capped at CAP, overflow flagged).  Every range is
+// non-empty, so the range starts pre[] increase strictly.
 // Rows are trimmed with the centre's position (px, py, pz): along y and z the
 // distance from the centre to a row's band of cells, along x the cells that
 // [x - s, x + s] covers with s^2 = (rc + margin)^2 - dy^2 - dz^2.  On an axis
@@ -80,6 +88,7 @@ __device__ inline int tm_build_ranges(int i, const CellGrid& g, float rc, RangeL
       f[d] = (float)(u - (double)ci[d] * (double)g.cs[d]);   // offset inside the centre's cell
     }
   }
+  R.mark[lane] = -1;   // no chunk base is negative; ordered by the wave barriers below
   const int ny = full[1] ? g.nc[1] : 2 * k[1] + 1;
   const int nz = full[2] ? g.nc[2] : 2 * k[2] + 1;
   const float rcm = rc + TM_TRIM_MARGIN;
@@ -125,15 +134,16 @@ __device__ inline int tm_build_ranges(int i, const CellGrid& g, float rc, RangeL
               lo0 = g.cell_start[row + xlo]; hi0 = g.cell_start[row + xhi + 1]; cnt = 1;
             }
           }
-          if (cnt >= 1 && hi0 <= lo0) { lo0 = lo1; hi0 = hi1; cnt -= 1; }
+          // keep only non-empty ranges (both halves of a wrapped row can be empty)
           if (cnt == 2 && hi1 <= lo1) cnt = 1;
+          if (cnt >= 1 && hi0 <= lo0) { lo0 = lo1; hi0 = hi1; cnt -= 1; }
         }
       }
     }
     // compact the non-empty ranges of this round into R (lane order = row order)
     unsigned long long b1 = __ballot(cnt >= 1);
     unsigned long long b2 = __ballot(cnt >= 2);
-    int before = __popcll(b1 & ((1ull << lane) - 1ull)) + __popcll(b2 & ((1ull << lane) - 1ull));
+    int before = tm_lanes_below(b1) + tm_lanes_below(b2);
     int slot = nr + before;
     if (cnt >= 1 && slot < CAP) { R.lo[slot] = lo0; R.pre[slot + 1] = hi0 - lo0; }
     if (cnt >= 2 && slot + 1 < CAP) { R.lo[slot + 1] = lo1; R.pre[slot + 2] = hi1 - lo1; }
@@ -160,7 +170,7 @@ __device__ inline int tm_build_ranges(int i, const CellGrid& g, float rc, RangeL
   }
   __syncwarp();
   *nrange = nr;
-  return carry;
+  return __builtin_amdgcn_readfirstlane(carry);   // the same in every lane
 }
 
 // Position (in cell order) of the t-th candidate: binary search in pre.
@@ -174,15 +184,38 @@ __device__ __forceinline__ int tm_candidate(const RangeLds<CAP>& R, int nr, int 
   return R.lo[lo] + (t - R.pre[lo]);
 }
 
-// Range of candidate base + lane inside a chunk, as an index k into the ranges
-// the chunk touches (lane k holds range r0 + k; ranges 1 .. span-1 start at
-// chunk offsets off_k = pre_k - base in [1, 63], strictly increasing): the
-// number of those starts at or below the lane.  The offsets are read with
-// readlane (k is wave-uniform), so the loop has no LDS round trips.
-__device__ __forceinline__ int tm_chunk_range(int lane, int span, int off_k) {
-  int idx = 0;
-  for (int k = 1; k < span; ++k) idx += __builtin_amdgcn_readlane(off_k, k) <= lane ? 1 : 0;
-  return idx;
+// Cell-order position of candidate base + lane of a sweep over the ranges in R
+// (0 for base + lane >= total), and r0 advanced from the range holding
+// candidate base to the range holding base + 64.  Call with base = 0, 64,
+// 128, ... in turn, r0 = 0 first (wave-uniform); needs nr >= 1.
+//
+// Lane k looks at range r0 + k.  Ranges r0 + 1 .. r0 + span - 1 start inside
+// the chunk, at offsets pre - base in [1, 63], distinct because every range
+// is non-empty.  Each such lane tags slot (pre - base) of the wave's mark array
+// with the chunk's base; every lane reads its own slot, and a ballot gives the
+// start positions, so lane L's range is r0 + (number of starts at or below L).
+// mark[0] is never written and older tags are other chunks' bases, so only
+// this chunk's starts match.
+template <int CAP>
+__device__ __forceinline__ int tm_locate(RangeLds<CAP>& R, int nr, int total, int lane, int base,
+                                         int& r0) {
+  const int rk = r0 + lane;
+  const int pv = R.pre[min(rk, nr)];
+  const int lo_k = R.lo[min(rk, nr - 1)];
+  const int pre_k = rk < nr ? pv : 0x7fffffff;   // past the last range: never inside
+  const int span = __popcll(__ballot(pre_k <= base + 63));   // >= 1: range r0 holds base
+  if (lane >= 1 && lane < span) R.mark[pre_k - base] = base;
+  __syncwarp();
+  const unsigned long long starts = __ballot(R.mark[lane] == base);
+  const int idx = __popcll(starts & ((2ull << lane) - 1ull));
+  const int lo_r = __shfl(lo_k, idx, 64);
+  const int pre_r = __shfl(pre_k, idx, 64);
+  // start of range r0 + span, the first one past the chunk (lane span holds it)
+  const int nxt = span < 64 ? __builtin_amdgcn_readlane(pre_k, span)
+                            : (r0 + 64 < nr ? R.pre[r0 + 64] : 0x7fffffff);
+  r0 += span - 1 + (nxt <= base + 64 ? 1 : 0);
+  const int t = base + lane;
+  return t < total ? lo_r + (t - pre_r) : 0;
 }
 
 // Sweep the candidates of centre i: calls f(p, valid, mapped) with all 64
@@ -204,27 +237,11 @@ __device__ __forceinline__ void tm_sweep(int i, const int* __restrict__ mol,
   }
   int nr = 0;
   const int total = tm_build_ranges(i, g, rc, R, lane, flags, &nr, px, py, pz);
-  // Walk the concatenated ranges 64 candidates at a time.  The ranges that a
-  // chunk touches start at r0 (wave-uniform); lane k holds range r0 + k, and
-  // each lane finds its own range with tm_chunk_range.
+  // Walk the concatenated ranges 64 candidates at a time.
   int r0 = 0;
   for (int base = 0; base < total; base += 64) {
-    const int t = base + lane;
-    const int rk = r0 + lane;
-    const int pre_k = rk < nr ? R.pre[rk] : 0x7fffffff;
-    const int lo_k = rk < nr ? R.lo[rk] : 0;
-    // ranges starting inside this chunk: those with pre <= base + 63
-    const unsigned long long inside = __ballot(rk < nr && pre_k <= base + 63);
-    const int span = __popcll(inside);   // >= 1: range r0 contains base
-    const int idx = tm_chunk_range(lane, span, pre_k - base);
-    const int lo_r = __shfl(lo_k, idx, 64);
-    const int pre_r = __shfl(pre_k, idx, 64);
-    const bool valid = t < total;
-    f(valid ? lo_r + (t - pre_r) : 0, valid, 1);
-    // range holding candidate base + 64: ranges r0 .. r0+span-1 start at or
-    // below base + 63; range r0+span (if any) starts after base + 63
-    r0 += span - 1;
-    if (r0 + 1 < nr && R.pre[r0 + 1] <= base + 64) r0 += 1;
+    const int p = tm_locate(R, nr, total, lane, base, r0);
+    f(p, base + lane < total, 1);
   }
 }
 
@@ -252,22 +269,7 @@ __device__ __forceinline__ void tm_sweep_pf(int i, const int* __restrict__ mol,
   int nr = 0;
   const int total = tm_build_ranges(i, g, rc, R, lane, flags, &nr, px, py, pz);
   int r0 = 0;
-  // cell-order position of candidate base + lane; advances r0 to the range
-  // holding candidate base + 64 (call with base = 0, 64, 128, ... in turn)
-  auto locate = [&](int base) -> int {
-    const int t = base + lane;
-    const int rk = r0 + lane;
-    const int pre_k = rk < nr ? R.pre[rk] : 0x7fffffff;
-    const int lo_k = rk < nr ? R.lo[rk] : 0;
-    const unsigned long long inside = __ballot(rk < nr && pre_k <= base + 63);
-    const int span = __popcll(inside);
-    const int idx = tm_chunk_range(lane, span, pre_k - base);
-    const int lo_r = __shfl(lo_k, idx, 64);
-    const int pre_r = __shfl(pre_k, idx, 64);
-    r0 += span - 1;
-    if (r0 + 1 < nr && R.pre[r0 + 1] <= base + 64) r0 += 1;
-    return t < total ? lo_r + (t - pre_r) : 0;
-  };
+  auto locate = [&](int base) { return tm_locate(R, nr, total, lane, base, r0); };
   if (total <= 0) return;
   int p = locate(0);
   auto cur = load(p, lane < total, 1);

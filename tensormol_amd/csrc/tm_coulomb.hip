@@ -24,7 +24,7 @@ namespace {
 
 constexpr int C_WAVES = 4;
 constexpr double BOHRPERA_D = 1.889725989;
-constexpr int C_RANGES = 512;    // 15 A stencil over ~2.3 A cells: ~230 rows (+ wraps)
+constexpr int C_RANGES = 480;    // 15 A stencil over ~2.3 A cells: <= ~230 rows (+ wraps)
 
 struct CoulArgs {
   float alpha, r_sr, r_lr, zz, yy, shift, ea;     // bohr units
@@ -199,7 +199,7 @@ coulomb_kernel(int N, const float4* __restrict__ xq, const int* __restrict__ ele
       ok = d.x * d.x + d.y * d.y + d.z * d.z < a.rc2;
     }
     unsigned long long bal = __ballot(ok);
-    int before = __popcll(bal & ((1ull << lane) - 1ull));
+    int before = tm_lanes_below(bal);
     if (ok) {
       qx[w][qn + before] = d;
       qe[w][qn + before] = c.e;

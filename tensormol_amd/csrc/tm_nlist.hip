@@ -79,7 +79,7 @@ nl_rows_kernel(int N, const double* __restrict__ pos, const int* __restrict__ el
         in_a = s < s_a;
       }
       unsigned long long bal = __ballot(ok);
-      int before = __popcll(bal & ((1ull << lane) - 1ull));
+      int before = tm_lanes_below(bal);
       if (ok) {
         int slot = count + before;
         int j = mapped ? grid.cell_atoms[p] : p;
@@ -115,7 +115,7 @@ nl_rows_kernel(int N, const double* __restrict__ pos, const int* __restrict__ el
     if (have) nbr[rowr + a] = (int)((k >> 1) & ((1u << 27) - 1u));
     bool isang = have && (k & 1u);
     unsigned long long bal = __ballot(isang);
-    int before = __popcll(bal & ((1ull << lane) - 1ull));
+    int before = tm_lanes_below(bal);
     if (isang) {
       int slot = acount + before;
       if (slot < lda) ang[rowa + slot] = a;
@@ -435,7 +435,7 @@ pairs_kernel(int N, const double* __restrict__ pos, const int* __restrict__ mol,
       ok = (!half || j > i) && sq_dist_nocontract(xi, yi, zi, q[0], q[1], q[2], Lx, Ly, Lz) < s_c;
     }
     unsigned long long bal = __ballot(ok);
-    if (FILL && ok) out[o + cnt + __popcll(bal & ((1ull << lane) - 1ull))] = j;
+    if (FILL && ok) out[o + cnt + tm_lanes_below(bal)] = j;
     cnt += __popcll(bal);
   };
   tm_sweep(i, mol, mol_start, grid, rc, rng[w], lane, flags, xi, yi, zi, visit);
