@@ -95,11 +95,11 @@ __device__ __forceinline__ void pair_terms(const CoulArgs& a, float qi, float qj
   }
   if (a.has_vdw) {
     const float r = rA * B2, inv_r = inv_rA * iB2;
-    const float t = r * a.inv_width;
-    float S, dS;
-    if (t > 1.f) { S = 1.f; dS = 0.f; }
-    else if (t > 0.f) { S = -t * t * (2.f * t - 3.f); dS = 6.f * t * (1.f - t) * a.inv_width; }
-    else { S = 0.f; dS = 0.f; }
+    // t clamped to [0, 1] gives S = 1, dS = 0 beyond the width exactly, and
+    // S = dS = 0 at t = 0, without branches
+    const float t = fminf(fmaxf(r * a.inv_width, 0.f), 1.f);
+    const float S = -t * t * (2.f * t - 3.f);
+    const float dS = 6.f * t * (1.f - t) * a.inv_width;
     // C r^-6 / (1 + 6 (r/R0)^-12) written as C r^6 / (r^12 + 6 R0^12): the same
     // function, finite for every r >= 0 in FP32 (r <= 15 A BOHRPERA^2 here)
     const float r2 = r * r, r6 = r2 * r2 * r2, r12 = r6 * r6;
@@ -189,15 +189,15 @@ coulomb_kernel(int N, const float4* __restrict__ xq, const int* __restrict__ ele
     return c;
   };
   auto visit = [&](int p, bool valid, int mapped, const Cand& c) {
-    float4 d = make_float4(0.f, 0.f, 0.f, 0.f);
-    bool ok = false;
-    if (valid && (mapped ? p != cpi : p != i) && c.m == mi) {
-      d.x = min_image_f(c.x.x - pi.x, Lx, iLx);
-      d.y = min_image_f(c.x.y - pi.y, Ly, iLy);
-      d.z = min_image_f(c.x.z - pi.z, Lz, iLz);
-      d.w = c.x.w;
-      ok = d.x * d.x + d.y * d.y + d.z * d.z < a.rc2;
-    }
+    // every lane holds a loaded entry (invalid lanes entry 0), so the
+    // separation is formed branch-free and validity enters the test
+    float4 d;
+    d.x = min_image_f(c.x.x - pi.x, Lx, iLx);
+    d.y = min_image_f(c.x.y - pi.y, Ly, iLy);
+    d.z = min_image_f(c.x.z - pi.z, Lz, iLz);
+    d.w = c.x.w;
+    const bool ok = valid && (mapped ? p != cpi : p != i) && c.m == mi &&
+                    d.x * d.x + d.y * d.y + d.z * d.z < a.rc2;
     unsigned long long bal = __ballot(ok);
     int before = tm_lanes_below(bal);
     if (ok) {

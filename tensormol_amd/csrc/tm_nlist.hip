@@ -70,14 +70,14 @@ nl_rows_kernel(int N, const double* __restrict__ pos, const int* __restrict__ el
     const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
     const int cpi = grid.cell_pos != nullptr ? grid.cell_pos[i] : -1;
     auto visit = [&](int p, bool valid, int mapped) {
-      bool ok = false, in_a = false;
-      if (valid && (mapped ? p != cpi : p != i) &&
-          (!mapped || grid.molc == nullptr || grid.molc[p] == mi)) {
-        const double* q = mapped ? grid.posc + 3 * (long)p : pos + 3 * (long)p;
-        double s = sq_dist_nocontract(xi, yi, zi, q[0], q[1], q[2], Lx, Ly, Lz);
-        ok = s < s_r;
-        in_a = s < s_a;
-      }
+      // invalid lanes read entry 0, so the loads and the distance are branch-free
+      const int pq = valid ? p : 0;
+      const double* q = mapped ? grid.posc + 3 * (long)pq : pos + 3 * (long)pq;
+      const double s = sq_dist_nocontract(xi, yi, zi, q[0], q[1], q[2], Lx, Ly, Lz);
+      const bool same_mol = !mapped || grid.molc == nullptr || grid.molc[pq] == mi;
+      const bool take = valid && (mapped ? p != cpi : p != i) && same_mol;
+      const bool ok = take && s < s_r;
+      const bool in_a = take && s < s_a;
       unsigned long long bal = __ballot(ok);
       int before = tm_lanes_below(bal);
       if (ok) {
