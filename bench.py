@@ -81,9 +81,9 @@ def run_md(eng, ws, Z, x, box, steps, warmup, dt, graph=True, mlp_samples=20, st
         eng.run(ws, pos)
         L.tm_vv_kick(ws.N, 0.0, P(ws.force), P(m), P(v), P(a), P(ke), st)
 
-        def step(record=False):
+        def step(record=False, clear_flags=True):
             L.tm_vv_drift(ws.N, dt, P(v), P(a), P(pos), st)
-            eng.build_lists(ws, pos)
+            eng.build_lists(ws, pos, clear_flags=clear_flags)
             eng.descriptors(ws, pos)
             if record:
                 e0 = torch.cuda.Event(enable_timing=True)
@@ -113,9 +113,11 @@ def run_md(eng, ws, Z, x, box, steps, warmup, dt, graph=True, mlp_samples=20, st
         spg = max(1, min(int(steps_per_graph), steps))
         while steps % spg:
             spg -= 1
+        # the status flags are cleared by the first step of a replay and
+        # accumulate over its steps, so the check after the replays covers all
         with torch.cuda.graph(g, stream=side):
-            for _ in range(spg):
-                step()
+            for k in range(spg):
+                step(clear_flags=k == 0)
         replay = g.replay
     else:
         replay = step

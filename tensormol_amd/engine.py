@@ -298,10 +298,15 @@ class BPEEEngine:
         idx = self._dev_index if self._dev_index is not None else torch.cuda.current_device()
         return _ctypes.c_void_p(torch._C._cuda_getCurrentRawStream(idx))
 
-    def build_lists(self, ws, pos):
+    def build_lists(self, ws, pos, clear_flags=True):
+        """Cell grid and neighbour rows for positions pos.  The status flags
+        (row overflow, non-finite coordinates) are cleared first unless
+        clear_flags is False: steps captured together into one graph then
+        accumulate them, and a check after the replay sees every step's."""
         L, st = self.lib, self._stream()
         ws.rev_ok = False   # the mirror slots are filled by the next descriptor forward
-        L.tm_zero_i32(8, _P(ws.flags), st)
+        if clear_flags:
+            L.tm_zero_i32(8, _P(ws.flags), st)
         if ws.use_cells:
             o, c, n = ws.grid_origin, ws.grid_cs, ws.grid_nc
             L.tm_cell_build(ws.N, _P(pos), _P(ws.mol), o[0], o[1], o[2], c[0], c[1], c[2], n[0], n[1],

@@ -60,6 +60,27 @@ def gpu_pairs_triples(eng, ws):
     return trp, trt, mil
 
 
+def test_flags_kept_without_clear(cuda):
+    """build_lists(clear_flags=False) keeps the status flags of earlier builds
+    (steps captured into one graph accumulate them; the check after a replay
+    sees every step's), and the default clears them.  The overflow comes from
+    the same periodic box with its coordinates halved: eight times the density,
+    rows longer than their 128 slots (a flagged, clipped case)."""
+    eng = _engine("water")
+    ws, pos = _prepare(eng, "waterbox10", cuda)
+    Z, x, ms, boxes = H.system("waterbox10")
+    lo = x.min(axis=0)
+    dense = torch.tensor(lo + 0.5 * (x - lo), dtype=torch.float64, device=cuda)
+    eng.build_lists(ws, dense)
+    f1 = ws.flags.cpu().numpy().copy()
+    assert f1[0] != 0
+    eng.build_lists(ws, pos, clear_flags=False)
+    np.testing.assert_array_equal(ws.flags.cpu().numpy(), f1)
+    eng.build_lists(ws, pos)
+    assert not ws.flags.cpu().numpy().any()
+    eng.check_flags(ws)
+
+
 @pytest.mark.parametrize("name,modelname", SYSTEMS)
 def test_neighbour_sets_bit_exact(cuda, name, modelname):
     eng = _engine(modelname)

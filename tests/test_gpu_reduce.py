@@ -119,11 +119,12 @@ def test_fused_sums_match_single_sums(cuda, sizes):
     assert scratch[:1].view(torch.int32)[0].item() == 0
 
 
-@pytest.mark.parametrize("N,n", [(3000, 6), (20000, 40)])
+@pytest.mark.parametrize("N,n", [(3000, 6), (12288, 21), (20000, 40)])
 def test_cell_build_leaves_counts_zero(cuda, N, n):
     """tm_cell_build needs cell_count zero on entry and leaves it zero; cells
-    list every atom once, in index order inside each cell.  64,000 cells take
-    the two-launch scan (four 16384-entry tiles), 216 the single block."""
+    list every atom once, in index order inside each cell, and cell_of is
+    floor((x - origin) / edge) wrapped into the grid.  64,000 cells take the
+    two-launch scan (four 16384-entry tiles), 216 and 9,261 the single block."""
     L = _lib.lib()
     rng = np.random.default_rng(4)
     edge = 20.0
@@ -145,6 +146,8 @@ def test_cell_build_leaves_counts_zero(cuda, N, n):
         torch.cuda.synchronize()
         assert int(count.abs().sum().item()) == 0
         a, s, co = atoms.cpu().numpy(), start.cpu().numpy(), cell_of.cpu().numpy()
+        q = np.mod(np.floor((pos.cpu().numpy() - 0.0) / cs).astype(np.int64), n)
+        np.testing.assert_array_equal(co, q[:, 0] + n * (q[:, 1] + n * q[:, 2]))
         assert sorted(a.tolist()) == list(range(N))
         assert s[0] == 0 and s[-1] == N and np.all(np.diff(s) >= 0)
         np.testing.assert_array_equal(np.diff(s), np.bincount(co, minlength=ncell))
