@@ -77,12 +77,14 @@ def run_md(eng, ws, Z, x, box, steps, warmup, dt, graph=True, mlp_samples=20, st
     mlp_ms = []
     with torch.cuda.stream(side):
         st = eng._stream()
-        # initial forces -> accelerations (a = F/m before the first drift)
+        # initial forces -> accelerations (a = F/m before the first drift), and
+        # the first drift: every step is then evaluate + kick + the next drift
+        # (one launch), the same operations as drift + evaluate + kick
         eng.run(ws, pos)
         L.tm_vv_kick(ws.N, 0.0, P(ws.force), P(m), P(v), P(a), P(ke), st)
+        L.tm_vv_drift(ws.N, dt, P(v), P(a), P(pos), st)
 
         def step(record=False, clear_flags=True):
-            L.tm_vv_drift(ws.N, dt, P(v), P(a), P(pos), st)
             eng.build_lists(ws, pos, clear_flags=clear_flags)
             eng.descriptors(ws, pos)
             if record:
@@ -96,7 +98,7 @@ def run_md(eng, ws, Z, x, box, steps, warmup, dt, graph=True, mlp_samples=20, st
             eng.electrostatics(ws, pos)
             eng.adjoint(ws, pos)
             eng.reductions(ws)
-            L.tm_vv_kick(ws.N, dt, P(ws.force), P(m), P(v), P(a), P(ke), st)
+            L.tm_vv_kick_drift(ws.N, dt, P(ws.force), P(m), P(v), P(a), P(ke), P(pos), st)
 
         for _ in range(warmup):
             step()

@@ -231,6 +231,11 @@ int tm_mol_sums(int B, const int* mol_start, int n_total, int nsrc, const void* 
 int tm_vv_drift(int N, double dt, const double* v, const double* a, double* x, void* stream);
 int tm_vv_kick(int N, double dt, const double* f, const double* m, double* v, double* a,
                double* ke, void* stream);
+/* tm_vv_kick with the forces of step k, then tm_vv_drift of step k + 1, in one
+ * launch (bit-identical to the two calls in turn): a loop of steps that starts
+ * with one drift runs each step as evaluate + kick-drift. */
+int tm_vv_kick_drift(int N, double dt, const double* f, const double* m, double* v, double* a,
+                     double* ke, double* x, void* stream);
 int tm_nh_drift(int N, double dt, const int* mol, const double* eta, const double* m, double* x,
                 const double* v, const double* a, double* vh, double* ke2, void* stream);
 int tm_nh_eta(int B, const int* mol_start, double dt, const double* kT, const double* Q,

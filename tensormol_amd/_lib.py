@@ -41,6 +41,7 @@ SIGNATURES = {
                    P, P, P, P, P],
     "tm_vv_drift": [I, D, P, P, P, P],
     "tm_vv_kick": [I, D, P, P, P, P, P, P],
+    "tm_vv_kick_drift": [I, D, P, P, P, P, P, P, P],
     "tm_nh_drift": [I, D, P, P, P, P, P, P, P, P, P],
     "tm_nh_eta": [I, P, D, P, P, P, P, P],
     "tm_nh_kick": [I, D, P, P, P, P, P, P, P, P, P],

@@ -36,6 +36,26 @@ __global__ void vv_kick_kernel(int N, double dt, const double* __restrict__ f,
   if (ke) ke[i] = 0.5 * mi * s;
 }
 
+// vv_kick of one step and vv_drift of the next in one pass over the atoms:
+// the same operations on the same values as the two kernels in turn
+__global__ void vv_kick_drift_kernel(int N, double dt, const double* __restrict__ f,
+                                     const double* __restrict__ m, double* __restrict__ v,
+                                     double* __restrict__ a, double* __restrict__ ke,
+                                     double* __restrict__ x) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double mi = m[i], s = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    double an = 1e-10 * f[3 * i + d] / mi;
+    double vn = v[3 * i + d] + 0.5 * (a[3 * i + d] + an) * dt;
+    v[3 * i + d] = vn;
+    a[3 * i + d] = an;
+    s += vn * vn;
+    x[3 * i + d] = x[3 * i + d] + vn * dt + 0.5 * an * dt * dt;
+  }
+  if (ke) ke[i] = 0.5 * mi * s;
+}
+
 // Nose-Hoover first half: x, vh from (x, v, a, eta[mol]); ke2[i] = (m|v|^2/2, m|vh|^2/2)
 __global__ void nh_drift_kernel(int N, double dt, const int* __restrict__ mol,
                                 const double* __restrict__ eta, const double* __restrict__ m,
@@ -176,6 +196,15 @@ int tm_vv_kick(int N, double dt, const double* f, const double* m, double* v, do
                double* ke, void* stream) {
   if (N == 0) return TM_OK;
   vv_kick_kernel<<<tm_div_up(N, 256), 256, 0, (hipStream_t)stream>>>(N, dt, f, m, v, a, ke);
+  TM_LAUNCH_CHECK();
+  return TM_OK;
+}
+
+int tm_vv_kick_drift(int N, double dt, const double* f, const double* m, double* v, double* a,
+                     double* ke, double* x, void* stream) {
+  if (N == 0) return TM_OK;
+  vv_kick_drift_kernel<<<tm_div_up(N, 256), 256, 0, (hipStream_t)stream>>>(N, dt, f, m, v, a, ke,
+                                                                           x);
   TM_LAUNCH_CHECK();
   return TM_OK;
 }

@@ -1,0 +1,37 @@
+"""Velocity Verlet entry points through the C ABI (VelocityVerletStep,
+TensorMol/Simulations/SimpleMD.py:14-38, split into drift and kick): the fused
+kick-drift of one launch equals tm_vv_kick followed by tm_vv_drift, bit for
+bit, on every output."""
+import numpy as np
+import pytest
+import torch
+
+from tensormol_amd import _lib
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("N", [1, 1000, 12288])
+def test_kick_drift_equals_kick_then_drift(cuda, N):
+    L, P = _lib.lib(), _lib.ptr
+    rng = np.random.default_rng(N)
+    f = torch.tensor(rng.standard_normal((N, 3)) * 50.0, device=cuda)
+    m = torch.tensor(rng.uniform(1.0, 16.0, N), device=cuda)
+    v0 = torch.tensor(rng.standard_normal((N, 3)) * 1e-3, device=cuda)
+    a0 = torch.tensor(rng.standard_normal((N, 3)) * 1e-9, device=cuda)
+    x0 = torch.tensor(rng.uniform(0.0, 30.0, (N, 3)), device=cuda)
+    dt = 0.5
+    outs = []
+    for fused in (False, True):
+        v, a, x = v0.clone(), a0.clone(), x0.clone()
+        ke = torch.zeros(N, dtype=torch.float64, device=cuda)
+        if fused:
+            assert L.tm_vv_kick_drift(N, dt, P(f), P(m), P(v), P(a), P(ke), P(x), None) == 0
+        else:
+            assert L.tm_vv_kick(N, dt, P(f), P(m), P(v), P(a), P(ke), None) == 0
+            assert L.tm_vv_drift(N, dt, P(v), P(a), P(x), None) == 0
+        torch.cuda.synchronize()
+        outs.append((v, a, x, ke))
+    for name, p, q in zip(("v", "a", "x", "ke"), *outs):
+        assert torch.equal(p, q), name
+    assert not torch.equal(outs[1][2], x0)   # the drift moved the atoms
