@@ -79,7 +79,8 @@ def run_md(eng, ws, Z, x, box, steps, warmup, dt, graph=True, mlp_samples=20, st
         st = eng._stream()
         # initial forces -> accelerations (a = F/m before the first drift), and
         # the first drift: every step is then evaluate + kick + the next drift
-        # (one launch), the same operations as drift + evaluate + kick
+        # (the kick and drift run inside the force gather's launch), the same
+        # operations as drift + evaluate + kick
         eng.run(ws, pos)
         L.tm_vv_kick(ws.N, 0.0, P(ws.force), P(m), P(v), P(a), P(ke), st)
         L.tm_vv_drift(ws.N, dt, P(v), P(a), P(pos), st)
@@ -96,9 +97,8 @@ def run_md(eng, ws, Z, x, box, steps, warmup, dt, graph=True, mlp_samples=20, st
                 e1.record(side)
                 mlp_ms.append((e0, e1))
             eng.electrostatics(ws, pos)
-            eng.adjoint(ws, pos)
+            eng.adjoint(ws, pos, vv=(dt, m, v, a, ke, pos))
             eng.reductions(ws)
-            L.tm_vv_kick_drift(ws.N, dt, P(ws.force), P(m), P(v), P(a), P(ke), P(pos), st)
 
         for _ in range(warmup):
             step()

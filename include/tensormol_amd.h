@@ -138,6 +138,14 @@ int tm_sym_jvp(int N, const double* pos, const double* box, const int* mol, cons
 int tm_force_gather(int N, int ldr, const int* ncnt, const int* rev, const float* wslot,
                     const double* extra, double scale, double* grad, double* force,
                     void* stream);
+/* tm_force_gather followed by tm_vv_kick_drift with the gathered force (scale
+ * = JOULEPERHARTREE gives the force tm_vv_kick takes), in one launch and
+ * bit-identical to the two calls: one MD step of VelocityVerletStep
+ * (SimpleMD.py:14-38) ends with the force evaluation.  ke may be NULL. */
+int tm_force_gather_vv(int N, int ldr, const int* ncnt, const int* rev, const float* wslot,
+                       const double* extra, double scale, double* grad, double* force, double dt,
+                       const double* m, double* v, double* a, double* ke, double* x,
+                       void* stream);
 
 /* ---- atomic networks ---------------------------------------------------
  * Energy / charge branches of energy_inference and dipole_inference

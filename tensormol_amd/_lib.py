@@ -31,6 +31,7 @@ SIGNATURES = {
                        P, P, I, P, P],
     "tm_sym_jvp": [I, P, P, P, P, P, P, P, P, P, P, I, I, I, I, F, F, F, F, P, P, P, P, P, I, P],
     "tm_force_gather": [I, I, P, P, P, P, D, P, P, P],
+    "tm_force_gather_vv": [I, I, P, P, P, P, D, P, P, D, P, P, P, P, P, P],
     "tm_mlp_f32": [I, I, I, P, P, I, P, P, F, P, P, P, P],
     "tm_mol_sum_f32": [I, P, I, P, I, P, I, P, I, P],
     "tm_mol_sum_f64": [I, P, I, P, I, P, I, P, I, P],

@@ -474,13 +474,24 @@ sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__
   for (int q = lane; q < n; q += 64) wslot[(long)i * ldr + q] = wacc[w][q];
 }
 
+// Velocity Verlet state advanced by the force gather (VV = true): the kick of
+// the step with the gathered force, then the drift of the next step, with the
+// operations of vv_kick_kernel and vv_drift_kernel.
+struct VVState {
+  double dt;
+  const double* m;
+  double *v, *a, *ke, *x;
+};
+
 // grad_a = sum_n (w[rev(a,n)] - w[(a,n)]) + extra_a ; optional force output
 // force = -scale * grad (J/mol/A when scale = JOULEPERHARTREE).  16 lanes per
 // atom, each summing every 16th slot, then a fixed butterfly over the 16.
+template <bool VV>
 __global__ void force_gather_kernel(int N, int ldr, const int* __restrict__ ncnt,
                                     const int* __restrict__ rev, const float4* __restrict__ wslot,
                                     const double* __restrict__ extra, double scale,
-                                    double* __restrict__ grad, double* __restrict__ force) {
+                                    double* __restrict__ grad, double* __restrict__ force,
+                                    VVState vv) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int a = (int)(t >> 4);
   const int sub = threadIdx.x & 15;
@@ -509,8 +520,22 @@ __global__ void force_gather_kernel(int N, int ldr, const int* __restrict__ ncnt
   if (grad != nullptr) {
     grad[3 * a] = gx; grad[3 * a + 1] = gy; grad[3 * a + 2] = gz;
   }
+  const double f[3] = {-scale * gx, -scale * gy, -scale * gz};
   if (force != nullptr) {
-    force[3 * a] = -scale * gx; force[3 * a + 1] = -scale * gy; force[3 * a + 2] = -scale * gz;
+    force[3 * a] = f[0]; force[3 * a + 1] = f[1]; force[3 * a + 2] = f[2];
+  }
+  if (VV) {
+    const double mi = vv.m[a], dt = vv.dt;
+    double s = 0.0;
+    for (int d = 0; d < 3; ++d) {
+      double an = 1e-10 * f[d] / mi;
+      double vn = vv.v[3 * a + d] + 0.5 * (vv.a[3 * a + d] + an) * dt;
+      vv.v[3 * a + d] = vn;
+      vv.a[3 * a + d] = an;
+      s += vn * vn;
+      vv.x[3 * a + d] = vv.x[3 * a + d] + vn * dt + 0.5 * an * dt * dt;
+    }
+    if (vv.ke) vv.ke[a] = 0.5 * mi * s;
   }
 }
 
@@ -764,9 +789,23 @@ int tm_force_gather(int N, int ldr, const int* ncnt, const int* rev, const float
                     void* stream) {
   if (N == 0) return TM_OK;
   hipStream_t s = (hipStream_t)stream;
-  force_gather_kernel<<<tm_div_up(16l * N, 256), 256, 0, s>>>(N, ldr, ncnt, rev,
-                                                              (const float4*)wslot, extra, scale,
-                                                              grad, force);
+  force_gather_kernel<false><<<tm_div_up(16l * N, 256), 256, 0, s>>>(
+      N, ldr, ncnt, rev, (const float4*)wslot, extra, scale, grad, force, VVState{});
+  TM_LAUNCH_CHECK();
+  return TM_OK;
+}
+
+int tm_force_gather_vv(int N, int ldr, const int* ncnt, const int* rev, const float* wslot,
+                       const double* extra, double scale, double* grad, double* force, double dt,
+                       const double* m, double* v, double* a, double* ke, double* x,
+                       void* stream) {
+  TM_REQUIRE(m != nullptr && v != nullptr && a != nullptr && x != nullptr,
+             "tm_force_gather_vv: m, v, a and x are required");
+  if (N == 0) return TM_OK;
+  hipStream_t s = (hipStream_t)stream;
+  force_gather_kernel<true><<<tm_div_up(16l * N, 256), 256, 0, s>>>(
+      N, ldr, ncnt, rev, (const float4*)wslot, extra, scale, grad, force,
+      VVState{dt, m, v, a, ke, x});
   TM_LAUNCH_CHECK();
   return TM_OK;
 }

@@ -378,7 +378,11 @@ class BPEEEngine:
         if not getattr(ws, "rev_ok", False):
             self.descriptors(ws, pos)
 
-    def adjoint(self, ws, pos):
+    def adjoint(self, ws, pos, vv=None):
+        """Descriptor adjoint and force gather.  vv = (dt, m, v, a, ke, x)
+        (device float64 arrays, ke may be None) also advances a Velocity Verlet
+        state in the same launch as the gather: the kick of this step with the
+        new forces, then the drift of the next (x is updated in place)."""
         self._need_rev(ws, pos)
         L, st = self.lib, self._stream()
         sym = self.model.sym
@@ -388,8 +392,14 @@ class BPEEEngine:
                          _lib.fptr(self.sym_rs_r), _lib.fptr(self.sym_rs_a), _lib.fptr(self.sym_th),
                          _P(ws.JE), _P(ws.JQ), None, _P(ws.phi), _P(ws.phimean), self.D,
                          _P(ws.wslot), st)
-        L.tm_force_gather(ws.N, self.ldr, _P(ws.ncnt), _P(ws.rev), _P(ws.wslot), _P(ws.gradc),
-                          JOULEPERHARTREE, _P(ws.grad), _P(ws.force), st)
+        if vv is None:
+            L.tm_force_gather(ws.N, self.ldr, _P(ws.ncnt), _P(ws.rev), _P(ws.wslot), _P(ws.gradc),
+                              JOULEPERHARTREE, _P(ws.grad), _P(ws.force), st)
+        else:
+            dt, m, v, a, ke, x = vv
+            L.tm_force_gather_vv(ws.N, self.ldr, _P(ws.ncnt), _P(ws.rev), _P(ws.wslot),
+                                 _P(ws.gradc), JOULEPERHARTREE, _P(ws.grad), _P(ws.force),
+                                 float(dt), _P(m), _P(v), _P(a), _P(ke), _P(x), st)
 
     def descriptor_jvp(self, ws, pos, du, out):
         """out[:N, :D] = dG/dx . du (float32) for per-atom displacements du

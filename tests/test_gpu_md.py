@@ -35,3 +35,41 @@ def test_kick_drift_equals_kick_then_drift(cuda, N):
     for name, p, q in zip(("v", "a", "x", "ke"), *outs):
         assert torch.equal(p, q), name
     assert not torch.equal(outs[1][2], x0)   # the drift moved the atoms
+
+
+def test_force_gather_vv_equals_gather_then_kick_drift(cuda):
+    """BPEEEngine.adjoint(..., vv=) advances the Verlet state inside the force
+    gather's launch (tm_force_gather_vv): forces, gradient, v, a, ke and the
+    drifted positions equal tm_force_gather followed by tm_vv_kick_drift."""
+    from tensormol_amd import params
+    from tensormol_amd.engine import BPEEEngine
+    from tests import helpers as H
+    L, P = _lib.lib(), _lib.ptr
+    Z, x, ms, boxes = H.system("waterbox10")
+    eng = BPEEEngine(H.model_for("water"), precision="bf16")
+    ws = eng.prepare(Z, ms, boxes, pos0=x)
+    m = torch.tensor(params.atomic_masses(Z), dtype=torch.float64, device=cuda)
+    rng = np.random.default_rng(0)
+    v0 = torch.tensor(rng.standard_normal((ws.N, 3)) * 1e-3, device=cuda)
+    a0 = torch.tensor(rng.standard_normal((ws.N, 3)) * 1e-9, device=cuda)
+    dt = 0.5
+    outs = []
+    for fused in (False, True):
+        pos = torch.tensor(x, dtype=torch.float64, device=cuda)
+        v, a = v0.clone(), a0.clone()
+        ke = torch.zeros(ws.N, dtype=torch.float64, device=cuda)
+        eng.build_lists(ws, pos)
+        eng.descriptors(ws, pos)
+        eng.networks(ws)
+        eng.electrostatics(ws, pos)
+        if fused:
+            eng.adjoint(ws, pos, vv=(dt, m, v, a, ke, pos))
+        else:
+            eng.adjoint(ws, pos)
+            assert L.tm_vv_kick_drift(ws.N, dt, P(ws.force), P(m), P(v), P(a), P(ke), P(pos),
+                                      eng._stream()) == 0
+        torch.cuda.synchronize()
+        outs.append([t.clone() for t in (ws.force, ws.grad, v, a, ke, pos)])
+    for name, p, q in zip(("force", "grad", "v", "a", "ke", "x"), *outs):
+        assert torch.equal(p, q), name
+    assert not torch.equal(outs[1][5], torch.tensor(x, dtype=torch.float64, device=cuda))
