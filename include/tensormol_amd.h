@@ -178,12 +178,16 @@ int tm_mlp_mfma_pack_info(int D, int H, int nh, int mode, long* out);
 /* ---- electrostatics ----------------------------------------------------
  * Neutral charges q = qraw - mean_m(qraw) and dipole terms
  * (TFMolInstanceDirect.py:5274-5279); xq = float32 (x, y, z, q) (may be NULL
- * when cell_pos is given).  With cell_pos (tm_cell_build's) the same values
- * and elem are also written in cell order to xqc (float4 [N]) and elc (int
- * [N]), the copies tm_coulomb sweeps (it then takes xq = NULL). */
-int tm_charges(int N, const int* mol, const double* qmean, const float* qraw, const double* pos,
-               float* xq, double* q, double* qx, const int* cell_pos, const int* elem, float* xqc,
-               int* elc, void* stream);
+ * when cell_pos is given).  With mol_start the molecule means are formed in
+ * the launch (every block sums its molecules in one fixed order, FP64) and
+ * written to qmean when it is not NULL (molecules with atoms); with
+ * mol_start = NULL qmean is the input (tm_mol_sum_f32 with mean = 1).  With
+ * cell_pos (tm_cell_build's) the same values and elem are also written in
+ * cell order to xqc (float4 [N]) and elc (int [N]), the copies tm_coulomb
+ * sweeps (it then takes xq = NULL). */
+int tm_charges(int N, const int* mol, const int* mol_start, double* qmean, const float* qraw,
+               const double* pos, float* xq, double* q, double* qx, const int* cell_pos,
+               const int* elem, float* xqc, int* elc, void* stream);
 
 /* DSF-elu Coulomb (RawSymFunc.py:1307-1359) and Grimme-C6 vdW
  * (RawSymFunc.py:1361-1414) over all pairs within rc_ee, evaluated from the

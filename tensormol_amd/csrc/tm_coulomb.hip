@@ -484,15 +484,60 @@ int mol_sum(int B, const int* mol_start, int C, const T* in, int ldin, double* o
   return TM_OK;
 }
 
-__global__ void charges_kernel(int N, const int* __restrict__ mol, const double* __restrict__ qmean,
-                               const float* __restrict__ qraw, const double* __restrict__ pos,
-                               float4* __restrict__ xq, double* __restrict__ q,
-                               double* __restrict__ qx, const int* __restrict__ cell_pos,
-                               const int* __restrict__ elem, float4* __restrict__ xqc,
-                               int* __restrict__ elc) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+// Charges and their cell-ordered copies.  MEANS: the molecule means of qraw
+// are formed in the block itself: every block sums each molecule its atoms
+// belong to over all of that molecule's atoms, in one fixed order (thread t
+// takes atoms lo + t + 256 j into partial sum j mod 8, the eight partials in a
+// fixed tree, a butterfly per wave, then the four wave totals in order), so
+// all blocks agree bit for bit and no launch or cross-block wait precedes the
+// charges.  The block that holds a molecule's
+// first atom writes its mean to qmean (when given).  Otherwise qmean is input.
+// The wave totals alternate between two LDS slots, so one barrier per
+// molecule suffices: a slot is rewritten two molecules later, after the
+// barrier that every reader of its previous contents has passed.
+template <bool MEANS>
+__global__ void __launch_bounds__(256)
+charges_kernel(int N, const int* __restrict__ mol, const int* __restrict__ mol_start,
+               double* __restrict__ qmean, const float* __restrict__ qraw,
+               const double* __restrict__ pos, float4* __restrict__ xq, double* __restrict__ q,
+               double* __restrict__ qx, const int* __restrict__ cell_pos,
+               const int* __restrict__ elem, float4* __restrict__ xqc, int* __restrict__ elc) {
+  __shared__ double wpart[2][4];   // wave totals, alternating between molecules
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double mine = 0.0;   // the mean of this thread's molecule
+  if (MEANS) {
+    const int i0 = blockIdx.x * blockDim.x;
+    const int m0 = mol[i0];
+    const int m1 = mol[min(i0 + (int)blockDim.x, N) - 1];
+    const int my_m = i < N ? mol[i] : -1;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int mm = m0; mm <= m1; ++mm) {   // molecules without atoms cost one pass
+      const int lo = mol_start[mm], hi = mol_start[mm + 1];
+      // eight independent partial sums (loads in flight), combined in a fixed tree
+      double p8[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      int a = lo + (int)threadIdx.x;
+      for (; a + 7 * 256 < hi; a += 8 * 256) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) p8[k] += (double)qraw[a + k * 256];
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (a + k * 256 < hi) p8[k] += (double)qraw[a + k * 256];
+      double acc = ((p8[0] + p8[1]) + (p8[2] + p8[3])) + ((p8[4] + p8[5]) + (p8[6] + p8[7]));
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+      double* part = wpart[(mm - m0) & 1];
+      if (lane == 0) part[w] = acc;
+      __syncthreads();   // part of the molecule before this one is no longer read
+      const double tot = ((part[0] + part[1]) + part[2]) + part[3];
+      const double mean = hi > lo ? tot / (double)(hi - lo) : 0.0;
+      if (my_m == mm) mine = mean;
+      if (threadIdx.x == 0 && qmean != nullptr && lo >= i0) qmean[mm] = mean;
+    }
+  }
   if (i >= N) return;
-  double qi = (double)qraw[i] - qmean[mol[i]];
+  const double mi = MEANS ? mine : qmean[mol[i]];
+  double qi = (double)qraw[i] - mi;
   q[i] = qi;
   double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
   const float4 v = make_float4((float)x, (float)y, (float)z, (float)qi);
@@ -573,15 +618,19 @@ int tm_mol_sums(int B, const int* mol_start, int n_total, int nsrc, const void* 
   return TM_OK;
 }
 
-int tm_charges(int N, const int* mol, const double* qmean, const float* qraw, const double* pos,
-               float* xq, double* q, double* qx, const int* cell_pos, const int* elem, float* xqc,
-               int* elc, void* stream) {
+int tm_charges(int N, const int* mol, const int* mol_start, double* qmean, const float* qraw,
+               const double* pos, float* xq, double* q, double* qx, const int* cell_pos,
+               const int* elem, float* xqc, int* elc, void* stream) {
   TM_REQUIRE(cell_pos == nullptr || (elem != nullptr && xqc != nullptr && elc != nullptr),
              "tm_charges: cell_pos needs elem, xqc and elc");
   TM_REQUIRE(xq != nullptr || cell_pos != nullptr, "tm_charges: no position output");
+  TM_REQUIRE(mol_start != nullptr || qmean != nullptr,
+             "tm_charges: needs qmean, or mol_start to form the means");
   if (N == 0) return TM_OK;
-  charges_kernel<<<tm_div_up(N, 256), 256, 0, (hipStream_t)stream>>>(
-      N, mol, qmean, qraw, pos, (float4*)xq, q, qx, cell_pos, elem, (float4*)xqc, elc);
+  hipStream_t s = (hipStream_t)stream;
+  auto k = mol_start != nullptr ? charges_kernel<true> : charges_kernel<false>;
+  k<<<tm_div_up(N, 256), 256, 0, s>>>(N, mol, mol_start, qmean, qraw, pos, (float4*)xq, q, qx,
+                                      cell_pos, elem, (float4*)xqc, elc);
   TM_LAUNCH_CHECK();
   return TM_OK;
 }

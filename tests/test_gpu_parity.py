@@ -176,8 +176,11 @@ def test_energy_forces_fp32(cuda, name, modelname):
 def test_fused_abi_paths_bit_identical(cuda, name, modelname):
     """The engine's fused paths equal the separate ones bit for bit: charges
     written in cell order by tm_charges (tm_coulomb with xq = NULL) against the
-    gather inside tm_coulomb, and the charge upstream formed inside
-    tm_sym_adjoint (u = NULL, phi and phimean given) against tm_charge_upstream."""
+    gather inside tm_coulomb, the molecule means formed inside tm_charges
+    (mol_start given; they also land in qmean) against qmean as input, and the
+    charge upstream formed inside tm_sym_adjoint (u = NULL, phi and phimean
+    given) against tm_charge_upstream.  The in-launch means agree with
+    tm_mol_sum_f32 to rounding."""
     from tensormol_amd._lib import P, ptr
     from tensormol_amd.params import PARAMS
     eng = _engine(modelname, "fp32")
@@ -188,8 +191,14 @@ def test_fused_abi_paths_bit_identical(cuda, name, modelname):
     fused = {k: getattr(ws, k).clone() for k in ("phi", "gradc", "eatom", "wslot", "q")}
     L, st = eng.lib, eng._stream()
     g = eng._grid_args(ws)
-    L.tm_charges(ws.N, ptr(ws.mol), ptr(ws.qmean), ptr(ws.qraw), ptr(pos), ptr(ws.xq), ptr(ws.q),
-                 ptr(ws.qx), None, None, None, None, st)
+    mean_ref = torch.full_like(ws.qmean, float("nan"))
+    L.tm_mol_sum_f32(ws.B, ptr(ws.mol_start), 1, ptr(ws.qraw), 1, ptr(mean_ref), 1, ptr(ws.red),
+                     ws.N, st)
+    torch.cuda.synchronize()
+    scale = ws.qraw[:ws.N].abs().double().mean().item()
+    assert (ws.qmean[:ws.B] - mean_ref[:ws.B]).abs().max().item() <= 1e-13 * scale
+    L.tm_charges(ws.N, ptr(ws.mol), None, ptr(ws.qmean), ptr(ws.qraw), ptr(pos), ptr(ws.xq),
+                 ptr(ws.q), ptr(ws.qx), None, None, None, None, st)
     ws.xqc.zero_()
     ws.elc.zero_()
     L.tm_coulomb(ws.N, ptr(ws.xq), ptr(ws.elem), ptr(ws.mol), ptr(ws.mol_start), ptr(ws.box),
