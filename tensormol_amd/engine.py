@@ -356,13 +356,22 @@ class BPEEEngine:
     def electrostatics(self, ws, pos):
         L, st = self.lib, self._stream()
         g = self._grid_args(ws)
-        # molecule means of the raw charges formed inside the charges launch
+        # One system: the mean of the raw charges is formed inside the charges
+        # launch (each block sums it).  A batch of many small molecules takes
+        # one block per molecule in a reduction launch first instead, since a
+        # block would otherwise step through every molecule it touches.
+        ms = None
+        if ws.B == 1:
+            ms = ws.mol_start
+        else:
+            L.tm_mol_sum_f32(ws.B, _P(ws.mol_start), 1, _P(ws.qraw), 1, _P(ws.qmean), 1,
+                             _P(ws.red), ws.N, st)
         if ws.use_cells:   # charges written straight into cell order for the sweep
-            L.tm_charges(ws.N, _P(ws.mol), _P(ws.mol_start), _P(ws.qmean), _P(ws.qraw), _P(pos),
+            L.tm_charges(ws.N, _P(ws.mol), _P(ms), _P(ws.qmean), _P(ws.qraw), _P(pos),
                          None, _P(ws.q), _P(ws.qx), _P(ws.cell_pos), _P(ws.elem), _P(ws.xqc),
                          _P(ws.elc), st)
         else:
-            L.tm_charges(ws.N, _P(ws.mol), _P(ws.mol_start), _P(ws.qmean), _P(ws.qraw), _P(pos),
+            L.tm_charges(ws.N, _P(ws.mol), _P(ms), _P(ws.qmean), _P(ws.qraw), _P(pos),
                          _P(ws.xq), _P(ws.q), _P(ws.qx), None, None, None, None, st)
         L.tm_coulomb(ws.N, None if ws.use_cells else _P(ws.xq), _P(ws.elem), _P(ws.mol), _P(ws.mol_start), _P(ws.box),
                      g[0], g[1], g[2], g[3], g[5], g[6], g[7], g[8], g[9], g[10], g[11],
