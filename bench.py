@@ -517,8 +517,8 @@ def main():
     ap.add_argument("--sphere-evals", type=int, default=50)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus}); "
-              "the single-GPU value is the figure of merit", file=sys.stderr)
+        print(f"bench.py measures one GPU: the library has no multi-GPU path (asked for "
+              f"{args.gpus}); the single-GPU value is the figure of merit", file=sys.stderr)
         sys.exit(2)
     if args.sphere:
         print(json.dumps(run_sphere(args)), flush=True)
