@@ -224,6 +224,33 @@ class DeviceMD:
             L.tm_nh_kick(ws.N, self.dt, _P(ws.mol), _P(self.eta), _P(ws.force), _P(self.m),
                          _P(self.vh), _P(self.v), _P(self.a), _P(self.ke), st)
 
+    def run_steps(self, n):
+        """n steps, leaving the state of n calls to step() bit for bit.  For
+        NVE the kick of every step but the last and the drift of the step after
+        it run inside the force gather's launch (tm_force_gather_vv), so a step
+        costs two launches fewer; the status flags are cleared once and
+        accumulate over the n steps.  Capturable into one HIP graph."""
+        if self.kind != "nve" or n <= 0:
+            for _ in range(n):
+                self.step()
+            return
+        e, ws, L = self.eng, self.ws, self.eng.lib
+        st = e._stream()
+        L.tm_vv_drift(ws.N, self.dt, _P(self.v), _P(self.a), _P(self.pos), st)
+        for k in range(n):
+            e.build_lists(ws, self.pos, clear_flags=k == 0)
+            e.descriptors(ws, self.pos)
+            e.networks(ws)
+            e.electrostatics(ws, self.pos)
+            if k + 1 < n:
+                e.adjoint(ws, self.pos, vv=(self.dt, self.m, self.v, self.a, self.ke, self.pos))
+                e.reductions(ws)
+            else:
+                e.adjoint(ws, self.pos)
+                e.reductions(ws)
+                L.tm_vv_kick(ws.N, self.dt, _P(ws.force), _P(self.m), _P(self.v), _P(self.a),
+                             _P(self.ke), st)
+
     def capture(self, stream):
         """Capture one step into a HIP graph on `stream` (replay with .replay())."""
         self.graph = torch.cuda.CUDAGraph()

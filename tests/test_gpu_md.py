@@ -73,3 +73,28 @@ def test_force_gather_vv_equals_gather_then_kick_drift(cuda):
     for name, p, q in zip(("force", "grad", "v", "a", "ke", "x"), *outs):
         assert torch.equal(p, q), name
     assert not torch.equal(outs[1][5], torch.tensor(x, dtype=torch.float64, device=cuda))
+
+
+def test_device_md_run_steps_equals_steps(cuda):
+    """DeviceMD.run_steps(n) (kick and next drift inside the force gather) leaves
+    positions, velocities, accelerations, kinetic energies and forces equal, bit
+    for bit, to n calls of step() (drift, evaluation, kick)."""
+    from tensormol_amd.engine import BPEEEngine
+    from tensormol_amd.md import DeviceMD
+    from tests import helpers as H
+    Z, x, ms, boxes = H.system("waterbox10")
+    eng = BPEEEngine(H.model_for("water"), precision="bf16")
+    outs = []
+    for fused in (False, True):
+        ws = eng.prepare(Z, ms, boxes, pos0=x)
+        md = DeviceMD(eng, ws, Z, x, dt=0.5, kind="nve", seed=3)
+        md.start()
+        if fused:
+            md.run_steps(4)
+        else:
+            for _ in range(4):
+                md.step()
+        torch.cuda.synchronize()
+        outs.append([t.clone() for t in (md.pos, md.v, md.a, md.ke, ws.force)])
+    for name, p, q in zip(("x", "v", "a", "ke", "force"), *outs):
+        assert torch.equal(p, q), name
