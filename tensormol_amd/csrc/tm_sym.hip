@@ -85,16 +85,28 @@ __device__ __forceinline__ void powz_pair(float b, const SymArgs& a, float& Az, 
 }
 
 struct RowLds {
+  static constexpr bool full = true;
   float4 d[SYM_WAVES][MAX_ROW];      // (x_j - x_i, r)
   float fc[SYM_WAVES][MAX_ROW];      // radial cutoff function
   float fa[SYM_WAVES][MAX_ROW];      // angular cutoff function
   int el[SYM_WAVES][MAX_ROW];
 };
+// The adjoint's row: its radial pass forms fc and reads the element itself
+// (lane = neighbour), which keeps its LDS small enough for 6 waves per SIMD.
+struct RowLdsAdj {
+  static constexpr bool full = false;
+  float4 d[SYM_WAVES][MAX_ROW];
+  float fa[SYM_WAVES][MAX_ROW];
+};
 
+__device__ __forceinline__ float cutoff_fc(float k, float r) { return 0.5f * (__cosf(k * r) + 1.0f); }
+__device__ __forceinline__ float cutoff_dfc(float k, float r) { return -0.5f * k * __sinf(k * r); }
+
+template <class LDS>
 __device__ __forceinline__ void load_row(int i, int w, int lane, const double* __restrict__ pos,
                                          const double* __restrict__ box, const int* __restrict__ mol,
                                          const int* __restrict__ elem, const int* __restrict__ nbr,
-                                         int n, int ldr, const SymArgs& a, RowLds& L) {
+                                         int n, int ldr, const SymArgs& a, LDS& L) {
   // minimum image with the reciprocal edge (iL = 0 for open boundaries): it
   // differs from d - L rint(d / L) only at |d| = L/2, far beyond the cutoff
   double Lx = 0, Ly = 0, Lz = 0, iLx = 0, iLy = 0, iLz = 0;
@@ -112,9 +124,11 @@ __device__ __forceinline__ void load_row(int i, int w, int lane, const double* _
     float dz = (float)(ez - Lz * rint(ez * iLz));
     float r = sqrtf(dx * dx + dy * dy + dz * dz + 1e-27f);
     L.d[w][q] = make_float4(dx, dy, dz, r);
-    L.fc[w][q] = 0.5f * (__cosf(a.kr * r) + 1.0f);
-    L.fa[w][q] = 0.5f * (__cosf(a.ka * r) + 1.0f);
-    L.el[w][q] = elem[j];
+    L.fa[w][q] = cutoff_fc(a.ka, r);
+    if constexpr (LDS::full) {
+      L.fc[w][q] = cutoff_fc(a.kr, r);
+      L.el[w][q] = elem[j];
+    }
   }
 }
 
@@ -310,12 +324,11 @@ sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__
                SymArgs a, const float* __restrict__ gE, const float* __restrict__ gQ,
                const float* __restrict__ u, const double* __restrict__ phi,
                const double* __restrict__ phimean, int ldg, float4* __restrict__ wslot) {
-  __shared__ RowLds L;
+  __shared__ RowLdsAdj L;
   extern __shared__ float gdyn[];   // SYM_WAVES * D floats: dE/dG of each wave's centre
   float* gw = gdyn + (threadIdx.x >> 6) * a.D;
-  __shared__ float4 wacc[SYM_WAVES][MAX_ROW];
-  __shared__ float dfa_s[SYM_WAVES][MAX_ROW];   // d fc_a / dr per neighbour
-  __shared__ float4 ca[SYM_WAVES][64], cb[SYM_WAVES][64];
+  __shared__ float wacc[3][SYM_WAVES][MAX_ROW];    // w per neighbour slot (x, y, z planes)
+  __shared__ float cab[2][3][SYM_WAVES][64];       // per triple: contributions to j (0) and k (1)
   __shared__ int segs[SYM_WAVES][2][TM_MAX_ELE + 1];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i = blockIdx.x * SYM_WAVES + w;
@@ -340,10 +353,9 @@ sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__
   for (int q = lane; q < n; q += 64) {
     float4 d = L.d[w][q];
     float r = d.w;
-    int e = L.el[w][q];
-    float fc = L.fc[w][q];
-    float dfc = -0.5f * kr * __sinf(kr * r);
-    dfa_s[w][q] = -0.5f * a.ka * __sinf(a.ka * r);
+    int e = elem[nbr[(long)i * ldr + q]];
+    float fc = cutoff_fc(kr, r);
+    float dfc = cutoff_dfc(kr, r);
     float dEdr = 0.f;
     for (int s = 0; s < a.nr; ++s) {
       float x = r - a.rs_r[s];
@@ -351,7 +363,9 @@ sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__
       dEdr += gw[e * a.nr + s] * ex * (dfc - 2.0f * a.eta * x * fc);
     }
     float sc = dEdr * __builtin_amdgcn_rcpf(r);
-    wacc[w][q] = make_float4(sc * d.x, sc * d.y, sc * d.z, 0.f);
+    wacc[0][w][q] = sc * d.x;
+    wacc[1][w][q] = sc * d.y;
+    wacc[2][w][q] = sc * d.z;
   }
   __syncwarp();
   // ---------------- angular: lane = triple, contributions staged per chunk
@@ -382,7 +396,7 @@ sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__
       float cos_over_sin = sn > 1e-6f ? c * __builtin_amdgcn_rcpf(sn) : 0.f;
       float ravg = 0.5f * (rj + rk);
       float fj = L.fa[w][qj], fk = L.fa[w][qk];
-      float dfj = dfa_s[w][qj], dfk = dfa_s[w][qk];
+      float dfj = cutoff_dfc(a.ka, rj), dfk = cutoff_dfc(a.ka, rk);
       float F = fj * fk;
       float R[NA], Rp[NA];
 #pragma unroll
@@ -419,8 +433,12 @@ sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__
       float cj = (dEdrav * 0.5f + dEdF * dfj * fk - dEdc * c * irj) * irj;
       float ck = (dEdrav * 0.5f + dEdF * fj * dfk - dEdc * c * irk) * irk;
       float x = dEdc * inv;
-      ca[w][lane] = make_float4(cj * dj.x + x * dk.x, cj * dj.y + x * dk.y, cj * dj.z + x * dk.z, 0.f);
-      cb[w][lane] = make_float4(ck * dk.x + x * dj.x, ck * dk.y + x * dj.y, ck * dk.z + x * dj.z, 0.f);
+      cab[0][0][w][lane] = cj * dj.x + x * dk.x;
+      cab[0][1][w][lane] = cj * dj.y + x * dk.y;
+      cab[0][2][w][lane] = cj * dj.z + x * dk.z;
+      cab[1][0][w][lane] = ck * dk.x + x * dj.x;
+      cab[1][1][w][lane] = ck * dk.y + x * dj.y;
+      cab[1][2][w][lane] = ck * dk.z + x * dj.z;
     }
     __syncwarp();
     // Member `lane` gathers its contributions from the triples [base, base + cnt)
@@ -429,7 +447,10 @@ sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__
     // for k' > j; in a cross block (ea < eb) t = kb_local * la + ja_local.
     if (eq >= 0) {
       const int cnt = min(64, ntrip - base);
-      auto add = [&](const float4& c4) { wang.x += c4.x; wang.y += c4.y; wang.z += c4.z; };
+      // contribution of triple slot u to its j (h = 0) or k (h = 1) member
+      auto add = [&](int h, int u) {
+        wang.x += cab[h][0][w][u]; wang.y += cab[h][1][w][u]; wang.z += cab[h][2][w][u];
+      };
       int bo = 0;   // first triple of the block
       for (int ea = 0; ea < a.nele; ++ea) {
         const int la = sg[ea + 1] - sg[ea];
@@ -441,22 +462,22 @@ sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__
             if (ea == eb) {
               if (eq == ea) {
                 const int t0 = lq * (lq - 1) / 2;
-                for (int r = max(0, lo - t0); r < min(lq, hi - t0); ++r) add(cb[w][t0 + r - lo]);
+                for (int r = max(0, lo - t0); r < min(lq, hi - t0); ++r) add(1, t0 + r - lo);
                 for (int r = lq + 1; r < la; ++r) {
                   const int t = r * (r - 1) / 2 + lq;
                   if (t >= hi) break;
-                  if (t >= lo) add(ca[w][t - lo]);
+                  if (t >= lo) add(0, t - lo);
                 }
               }
             } else if (eq == ea) {
               for (int kl = 0; kl < lb; ++kl) {
                 const int t = kl * la + lq;
                 if (t >= hi) break;
-                if (t >= lo) add(ca[w][t - lo]);
+                if (t >= lo) add(0, t - lo);
               }
             } else if (eq == eb) {
               const int t0 = lq * la;
-              for (int r = max(0, lo - t0); r < min(la, hi - t0); ++r) add(cb[w][t0 + r - lo]);
+              for (int r = max(0, lo - t0); r < min(la, hi - t0); ++r) add(1, t0 + r - lo);
             }
           }
           bo += nb;
@@ -467,11 +488,13 @@ sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__
   }
   if (lane < m) {
     int q = arow[lane];
-    float4 t = wacc[w][q];
-    wacc[w][q] = make_float4(t.x + wang.x, t.y + wang.y, t.z + wang.z, 0.f);
+    wacc[0][w][q] += wang.x;
+    wacc[1][w][q] += wang.y;
+    wacc[2][w][q] += wang.z;
   }
   __syncwarp();
-  for (int q = lane; q < n; q += 64) wslot[(long)i * ldr + q] = wacc[w][q];
+  for (int q = lane; q < n; q += 64)
+    wslot[(long)i * ldr + q] = make_float4(wacc[0][w][q], wacc[1][w][q], wacc[2][w][q], 0.f);
 }
 
 // Velocity Verlet state advanced by the force gather (VV = true): the kick of
