@@ -85,16 +85,23 @@ __device__ __forceinline__ void powz_pair(float b, const SymArgs& a, float& Az, 
 }
 
 struct RowLds {
-  static constexpr bool full = true;
+  static constexpr bool has_fc = true, has_fa = true, has_el = true;
   float4 d[SYM_WAVES][MAX_ROW];      // (x_j - x_i, r)
   float fc[SYM_WAVES][MAX_ROW];      // radial cutoff function
   float fa[SYM_WAVES][MAX_ROW];      // angular cutoff function
   int el[SYM_WAVES][MAX_ROW];
 };
+// The forward's row: the angular cutoff is formed once per triple instead,
+// which keeps its LDS small enough for 7 waves per SIMD.
+struct RowLdsFwd {
+  static constexpr bool has_fc = true, has_fa = false, has_el = false;
+  float4 d[SYM_WAVES][MAX_ROW];
+  float fc[SYM_WAVES][MAX_ROW];
+};
 // The adjoint's row: its radial pass forms fc and reads the element itself
 // (lane = neighbour), which keeps its LDS small enough for 6 waves per SIMD.
 struct RowLdsAdj {
-  static constexpr bool full = false;
+  static constexpr bool has_fc = false, has_fa = true, has_el = false;
   float4 d[SYM_WAVES][MAX_ROW];
   float fa[SYM_WAVES][MAX_ROW];
 };
@@ -124,11 +131,9 @@ __device__ __forceinline__ void load_row(int i, int w, int lane, const double* _
     float dz = (float)(ez - Lz * rint(ez * iLz));
     float r = sqrtf(dx * dx + dy * dy + dz * dz + 1e-27f);
     L.d[w][q] = make_float4(dx, dy, dz, r);
-    L.fa[w][q] = cutoff_fc(a.ka, r);
-    if constexpr (LDS::full) {
-      L.fc[w][q] = cutoff_fc(a.kr, r);
-      L.el[w][q] = elem[j];
-    }
+    if constexpr (LDS::has_fa) L.fa[w][q] = cutoff_fc(a.ka, r);
+    if constexpr (LDS::has_fc) L.fc[w][q] = cutoff_fc(a.kr, r);
+    if constexpr (LDS::has_el) L.el[w][q] = elem[j];
   }
 }
 
@@ -173,7 +178,7 @@ sym_fwd_kernel(int N, const double* __restrict__ pos, const double* __restrict__
                const int* __restrict__ acnt, const int* __restrict__ aseg, int ldr, int lda,
                SymArgs a, float* __restrict__ G, int ldg, int* __restrict__ rev,
                int* __restrict__ flags) {
-  __shared__ RowLds L;
+  __shared__ RowLdsFwd L;
   __shared__ float4 tq[SYM_WAVES][64];        // per triple: (cos theta, sin theta, fc_j fc_k, -)
   __shared__ float4 tR[SYM_WAVES][64][NA / 4];   // per triple: exp(-eta (ravg - Rs_s)^2), s = 0..7
   __shared__ int segs[SYM_WAVES][2][TM_MAX_ELE + 1];
@@ -254,7 +259,7 @@ sym_fwd_kernel(int N, const double* __restrict__ pos, const double* __restrict__
       c = fminf(fmaxf(c, -1.0f), 1.0f);
       float sn = sqrtf(fmaxf(1.0f - c * c, 0.0f));
       float ravg = 0.5f * (dj.w + dk.w);
-      tq[w][lane] = make_float4(c, sn, L.fa[w][qj] * L.fa[w][qk], 0.f);
+      tq[w][lane] = make_float4(c, sn, cutoff_fc(a.ka, dj.w) * cutoff_fc(a.ka, dk.w), 0.f);
       float R[NA];
 #pragma unroll
       for (int s = 0; s < NA; ++s) {
