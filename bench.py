@@ -36,7 +36,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-MLP_PEAK_TFLOPS = {"bf16": 2500.0, "fp16": 2500.0, "bf16x3": 2500.0, "fp32": 157.3}
+MLP_PEAK_TFLOPS = {"bf16": 2500.0, "fp16": 2500.0, "bf16x3": 2500.0, "bf16x2": 2500.0, "fp16x2": 2500.0,
+                   "bf16x3f": 2500.0, "fp32": 157.3}
 
 
 def mlp_flops_per_atom(D, H):

@@ -165,14 +165,17 @@ int tm_mlp_f32(int M, int D, int nh, const int* hid, const float* G, int ldg, co
  * L2 serves few weight blobs); seg (device int32 [nseg][4]) = (net,
  * blob index, first row in perm, rows); blobs: device array of weight-blob
  * pointers packed as tm_mlp_mfma_pack_info describes; mode 0 = bf16, 1 = fp16,
- * 2 = bf16 hi/lo split; tile_rows = atoms per tile, 64 or 32 (32 in the split
- * mode).  Writes y[net*N + atom] and dG[(net*N + atom)*ldg + f]. */
+ * 2 = bf16 hi/lo split of weights and activations, 3 = bf16 with hi/lo split
+ * weights in the three forward GEMMs only (activations and the input-gradient
+ * GEMMs plain bf16); tile_rows = atoms per tile, 64 or 32 (32 in mode 2).
+ * Writes y[net*N + atom] and dG[(net*N + atom)*ldg + f]. */
 int tm_mlp_mfma(int N, int D, int H, int ntiles, const int* tiles, const int* seg, int ldg,
                 const float* G, const int* perm, const void* blobs, float alpha, int mode,
                 int tile_rows, float* y, float* dG, void* stream);
-/* out[0] blob bytes, out[1] byte offset of the float section (b1, b2, b3, wo,
- * bo), out[2..7] element offsets of the forward (layers 1-3) and backward
- * (layers 3, 2, 1) fragment arrays, out[8] planes (2 for the split mode). */
+/* out (10 entries): out[0] blob bytes, out[1] byte offset of the float section
+ * (b1, b2, b3, wo, bo), out[2..7] element offsets of the forward (layers 1-3)
+ * and backward (layers 3, 2, 1) fragment arrays, out[8] planes of the forward
+ * arrays, out[9] planes of the backward arrays (2 = hi plane then lo plane). */
 int tm_mlp_mfma_pack_info(int D, int H, int nh, int mode, long* out);
 
 /* ---- electrostatics ----------------------------------------------------

@@ -23,7 +23,17 @@
 //
 // Modes: 0 bf16, 1 fp16 (operands rounded once, FP32 accumulation),
 // 2 bf16 hi/lo split: x = hi + lo for weights and activations and
-// acc += Whi*xhi + Wlo*xhi + Whi*xlo (near-FP32 products).
+// acc += Whi*xhi + Wlo*xhi + Whi*xlo (near-FP32 products),
+// 3 bf16 with split forward weights: the three forward GEMMs take
+// acc += Whi*x + Wlo*x (weights carried to 16 significant bits, activations
+// and descriptors rounded once to BF16), the input-gradient GEMMs plain BF16;
+// 4 the same with FP16 operands (weights to 22 bits, activations to 11);
+// 5 bf16 with the forward GEMMs split in weights and activations (three
+// MFMAs, as mode 2) and plain BF16 input-gradient GEMMs.
+// Rounding the weights shifts every atom's energy the same way, so that error
+// adds up over a system; rounding activations gives an unbiased error that is
+// decorrelated across atoms, so it averages out over large systems but not
+// over a small molecule: modes 3 and 4 remove the first, mode 5 both.
 #include "tm_common.h"
 
 namespace {
@@ -59,15 +69,22 @@ struct BlobLayout {
   long f1, f2, f3, b3, b2, b1, fend;   // fragment planes (offsets in 2-byte elements)
   long bias_bytes;                     // byte offset of the float section
 };
-__host__ __device__ inline BlobLayout blob_layout(int D, int H, int planes) {
+// planes of the forward (f*) and backward (b*) fragment arrays: 1 (one rounding)
+// or 2 (hi plane, then lo plane)
+__host__ __device__ constexpr int fwd_planes(int mode) { return mode >= 2 ? 2 : 1; }
+__host__ __device__ constexpr int bwd_planes(int mode) { return mode == 2 ? 2 : 1; }
+// activation (B operand) lo planes in the forward / backward GEMMs
+__host__ __device__ constexpr bool act_split_fwd(int mode) { return mode == 2 || mode == 5; }
+__host__ __device__ constexpr bool act_split_bwd(int mode) { return mode == 2; }
+__host__ __device__ inline BlobLayout blob_layout(int D, int H, int fp, int bp) {
   BlobLayout L;
   long o = 0;
-  L.f1 = o; o += planes * frag_elems(H / 16, D / 32);
-  L.f2 = o; o += planes * frag_elems(H / 16, H / 32);
-  L.f3 = o; o += planes * frag_elems(H / 16, H / 32);
-  L.b3 = o; o += planes * frag_elems(H / 16, H / 32);
-  L.b2 = o; o += planes * frag_elems(H / 16, H / 32);
-  L.b1 = o; o += planes * frag_elems(D / 16, H / 32);
+  L.f1 = o; o += fp * frag_elems(H / 16, D / 32);
+  L.f2 = o; o += fp * frag_elems(H / 16, H / 32);
+  L.f3 = o; o += fp * frag_elems(H / 16, H / 32);
+  L.b3 = o; o += bp * frag_elems(H / 16, H / 32);
+  L.b2 = o; o += bp * frag_elems(H / 16, H / 32);
+  L.b1 = o; o += bp * frag_elems(D / 16, H / 32);
   L.fend = o;
   L.bias_bytes = ((o * 2 + 255) / 256) * 256;
   return L;
@@ -105,6 +122,12 @@ struct Ty<1> {
 };
 template <>
 struct Ty<2> : Ty<0> {};
+template <>
+struct Ty<3> : Ty<0> {};
+template <>
+struct Ty<4> : Ty<1> {};
+template <>
+struct Ty<5> : Ty<0> {};
 
 // act(z) = log(1 + e^{a z}) / a and act'(z) = sigmoid(a z), sharing e = e^{-|a z|}:
 // act = (max(a z, 0) + log(1 + e)) / a, sigmoid = 1/(1+e) for a z >= 0, e/(1+e) below.
@@ -156,7 +179,7 @@ struct DerivRegs<2, NFT, AT> {
 
 template <int MODE, int MT>
 struct Smem {
-  static constexpr int PL = MODE == 2 ? 2 : 1;
+  static constexpr int PL = act_split_fwd(MODE) ? 2 : 1;
   static constexpr int SH = HMAX + 8;                // activation row stride (elements)
   union {
     struct {
@@ -170,8 +193,9 @@ struct Smem {
   int atoms[MT];
 };
 
-// write 4 consecutive features (values v) of one atom into an activation buffer
-template <int MODE, int MT>
+// write 4 consecutive features (values v) of one atom into an activation
+// buffer; LO: also the lo plane (the residual v - hi, rounded)
+template <int MODE, int MT, bool LO>
 __device__ __forceinline__ void put4(unsigned short (*buf)[MT][HMAX + 8], int atom, int f,
                                      const float v[4]) {
   unsigned short h[4];
@@ -181,7 +205,7 @@ __device__ __forceinline__ void put4(unsigned short (*buf)[MT][HMAX + 8], int at
   w.x = (unsigned)h[0] | ((unsigned)h[1] << 16);
   w.y = (unsigned)h[2] | ((unsigned)h[3] << 16);
   *reinterpret_cast<uint2*>(&buf[0][atom][f]) = w;
-  if (MODE == 2) {
+  if constexpr (LO) {
     unsigned short l[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) l[i] = Ty<MODE>::cvt(v[i] - Ty<MODE>::back(h[i]));
@@ -206,7 +230,7 @@ __device__ __forceinline__ void gemm_first(uint4 (&a)[NFT], const unsigned short
     a[ft] = *reinterpret_cast<const uint4*>(wf + ((long)(rtile[ft] * nks) * 64 + lane) * 8);
 }
 
-template <int MODE, int MT, int NFT>
+template <int MODE, int MT, int NFT, bool SPLIT>
 __device__ __forceinline__ void gemm_tiles(floatx4 (&acc)[NFT][MT / 16],
                                            const unsigned short* __restrict__ wf, long plane_elems,
                                            int nks, const int* rtile, const unsigned short* bbase,
@@ -227,7 +251,7 @@ __device__ __forceinline__ void gemm_tiles(floatx4 (&acc)[NFT][MT / 16],
     for (int ft = 0; ft < NFT; ++ft)
 #pragma unroll
       for (int at = 0; at < AT; ++at) acc[ft][at] = Ty<MODE>::mma(a[ft], b[at], acc[ft][at]);
-    if (MODE == 2) {
+    if constexpr (SPLIT) {
       uint4 al[NFT], bl[AT];
 #pragma unroll
       for (int ft = 0; ft < NFT; ++ft) al[ft] = afrag(ft, ks, plane_elems);
@@ -263,14 +287,76 @@ __device__ __forceinline__ void gemm_tiles(floatx4 (&acc)[NFT][MT / 16],
   }
 }
 
+// Split-weight GEMM (mode 3 forward): acc += (Whi + Wlo) x B with one B plane.
+// Each k-step is two MFMA passes over the same B fragments, the hi fragments
+// (a0) then the lo fragments (a1); the lo fragments load while the hi MFMAs
+// run and the next hi fragments while the lo MFMAs run, so the register
+// footprint and the load-ahead distance are those of gemm_tiles.
+template <int MODE, int MT, int NFT>
+__device__ __forceinline__ void gemm_tiles_ws(floatx4 (&acc)[NFT][MT / 16],
+                                              const unsigned short* __restrict__ wf,
+                                              long plane_elems, int nks, const int* rtile,
+                                              const unsigned short* bbase, int lane, int STRIDE,
+                                              const uint4 (&a_first)[NFT]) {
+  constexpr int AT = MT / 16;
+  const int brow = lane & 15;
+  const int bk = (lane >> 4) * 8;
+  auto afrag = [&](int ft, int ks, long plane) {
+    return *reinterpret_cast<const uint4*>(wf + plane + ((long)(rtile[ft] * nks + ks) * 64 + lane) * 8);
+  };
+  uint4 a0[NFT], a1[NFT];
+#pragma unroll
+  for (int ft = 0; ft < NFT; ++ft) a0[ft] = a_first[ft];
+  for (int ks = 0; ks < nks; ++ks) {
+#pragma unroll
+    for (int ft = 0; ft < NFT; ++ft) a1[ft] = afrag(ft, ks, plane_elems);
+    __builtin_amdgcn_sched_barrier(0);
+    uint4 b[AT];
+#pragma unroll
+    for (int at = 0; at < AT; ++at)
+      b[at] = *reinterpret_cast<const uint4*>(bbase + (long)(at * 16 + brow) * STRIDE + ks * 32 + bk);
+#pragma unroll
+    for (int ft = 0; ft < NFT; ++ft)
+#pragma unroll
+      for (int at = 0; at < AT; ++at) acc[ft][at] = Ty<MODE>::mma(a0[ft], b[at], acc[ft][at]);
+    __builtin_amdgcn_sched_barrier(0);
+    if (ks + 1 < nks) {
+#pragma unroll
+      for (int ft = 0; ft < NFT; ++ft) a0[ft] = afrag(ft, ks + 1, 0);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int ft = 0; ft < NFT; ++ft)
+#pragma unroll
+      for (int at = 0; at < AT; ++at) acc[ft][at] = Ty<MODE>::mma(a1[ft], b[at], acc[ft][at]);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// Forward GEMM of a layer: split weights only (modes 3, 4), split weights and
+// activations (modes 2, 5), or one rounding of each (modes 0, 1).
+template <int MODE, int MT, int NFT>
+__device__ __forceinline__ void gemm_fwd(floatx4 (&acc)[NFT][MT / 16],
+                                         const unsigned short* __restrict__ wf, long plane_elems,
+                                         int nks, const int* rtile, const unsigned short* bbase,
+                                         long bplane, int lane, int STRIDE,
+                                         const uint4 (&a_first)[NFT]) {
+  if constexpr (MODE == 3 || MODE == 4)
+    gemm_tiles_ws<MODE, MT, NFT>(acc, wf, plane_elems, nks, rtile, bbase, lane, STRIDE, a_first);
+  else
+    gemm_tiles<MODE, MT, NFT, act_split_fwd(MODE)>(acc, wf, plane_elems, nks, rtile, bbase,
+                                                   bplane, lane, STRIDE, a_first);
+}
+
 // 64-row tiles: one workgroup per CU (LDS); 32-row tiles in the BF16 / FP16
 // modes fit two workgroups per CU when the waves stay within 128 VGPRs.
 template <int MODE, int MT>
 __global__ void __launch_bounds__(NW * 64)
-__attribute__((amdgpu_waves_per_eu(MT == 32 && MODE != 2 ? 4 : 2)))
+__attribute__((amdgpu_waves_per_eu(MT == 32 && !act_split_fwd(MODE) ? 4 : 2)))
 mlp_mfma_kernel(MlpArgs a) {
   constexpr int AT = MT / 16;
-  constexpr int PL = MODE == 2 ? 2 : 1;
+  constexpr bool LOF = act_split_fwd(MODE);   // forward operands carry lo planes
+  constexpr bool LOB = act_split_bwd(MODE);   // backward operands carry lo planes
   constexpr int SH = HMAX + 8;
   typedef Smem<MODE, MT> S;
   __shared__ S sm;
@@ -282,7 +368,7 @@ mlp_mfma_kernel(MlpArgs a) {
   const int net = a.seg[4 * s], blob = a.seg[4 * s + 1], first = a.seg[4 * s + 2];
   const int rows = a.seg[4 * s + 3];
   const int D = a.D, H = a.H;
-  const BlobLayout BL = blob_layout(D, H, PL);
+  const BlobLayout BL = blob_layout(D, H, fwd_planes(MODE), bwd_planes(MODE));
   const unsigned short* wb = a.blobs[blob];
   const float* fsec = reinterpret_cast<const float*>(reinterpret_cast<const char*>(wb) + BL.bias_bytes);
   const float* b1 = fsec;
@@ -332,7 +418,7 @@ mlp_mfma_kernel(MlpArgs a) {
       ww.x = (unsigned)h[0] | ((unsigned)h[1] << 16);
       ww.y = (unsigned)h[2] | ((unsigned)h[3] << 16);
       *reinterpret_cast<uint2*>(dst) = ww;
-      if (MODE == 2) {
+      if constexpr (LOF) {
         unsigned short l[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) l[i] = Ty<MODE>::cvt(v[i] - Ty<MODE>::back(h[i]));
@@ -344,7 +430,7 @@ mlp_mfma_kernel(MlpArgs a) {
     }
     __syncthreads();
     if (wact)
-      gemm_tiles<MODE, MT, FT>(acc, wb + BL.f1, p1, D / 32, rt, sm.Gt, (long)MT * SG, lane, SG, pf);
+      gemm_fwd<MODE, MT, FT>(acc, wb + BL.f1, p1, D / 32, rt, sm.Gt, (long)MT * SG, lane, SG, pf);
     if (wact) gemm_first<FT>(pf, wb + BL.f2, H / 32, rt, lane);
     __syncthreads();   // the tile shares LDS with X, written by the epilogue
   }
@@ -359,7 +445,7 @@ mlp_mfma_kernel(MlpArgs a) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) act_pair(acc[ft][at][i] + b1[f + i], ac, v[i], d[i]);
         s1.set4(ft, at, d);
-        put4<MODE, MT>(sm.X, at * 16 + ca, f, v);
+        put4<MODE, MT, LOF>(sm.X, at * 16 + ca, f, v);
       }
   }
   __syncthreads();
@@ -369,8 +455,8 @@ mlp_mfma_kernel(MlpArgs a) {
     for (int ft = 0; ft < FT; ++ft)
 #pragma unroll
       for (int at = 0; at < AT; ++at) acc[ft][at] = floatx4{0.f, 0.f, 0.f, 0.f};
-    gemm_tiles<MODE, MT, FT>(acc, wb + BL.f2, phh, H / 32, rt, &sm.X[0][0][0], (long)MT * SH,
-                                 lane, SH, pf);
+    gemm_fwd<MODE, MT, FT>(acc, wb + BL.f2, phh, H / 32, rt, &sm.X[0][0][0], (long)MT * SH,
+                               lane, SH, pf);
     gemm_first<FT>(pf, wb + BL.f3, H / 32, rt, lane);
 #pragma unroll
     for (int ft = 0; ft < FT; ++ft)
@@ -381,7 +467,7 @@ mlp_mfma_kernel(MlpArgs a) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) act_pair(acc[ft][at][i] + b2[f + i], ac, v[i], d[i]);
         s2.set4(ft, at, d);
-        put4<MODE, MT>(sm.Y, at * 16 + ca, f, v);
+        put4<MODE, MT, LOF>(sm.Y, at * 16 + ca, f, v);
       }
   }
   __syncthreads();
@@ -391,8 +477,8 @@ mlp_mfma_kernel(MlpArgs a) {
     for (int ft = 0; ft < FT; ++ft)
 #pragma unroll
       for (int at = 0; at < AT; ++at) acc[ft][at] = floatx4{0.f, 0.f, 0.f, 0.f};
-    gemm_tiles<MODE, MT, FT>(acc, wb + BL.f3, phh, H / 32, rt, &sm.Y[0][0][0], (long)MT * SH,
-                                 lane, SH, pf);
+    gemm_fwd<MODE, MT, FT>(acc, wb + BL.f3, phh, H / 32, rt, &sm.Y[0][0][0], (long)MT * SH,
+                               lane, SH, pf);
     gemm_first<FT>(pf, wb + BL.b3, H / 32, rt, lane);
     float ep[AT];
 #pragma unroll
@@ -411,7 +497,7 @@ mlp_mfma_kernel(MlpArgs a) {
           ep[at] += act * wv;
           v[i] = wv * d;   // dZ3 with upstream 1
         }
-        put4<MODE, MT>(sm.X, at * 16 + ca, f, v);
+        put4<MODE, MT, LOB>(sm.X, at * 16 + ca, f, v);
       }
 #pragma unroll
     for (int at = 0; at < AT; ++at) {
@@ -439,7 +525,7 @@ mlp_mfma_kernel(MlpArgs a) {
     for (int ft = 0; ft < FT; ++ft)
 #pragma unroll
       for (int at = 0; at < AT; ++at) acc[ft][at] = floatx4{0.f, 0.f, 0.f, 0.f};
-    gemm_tiles<MODE, MT, FT>(acc, wb + BL.b3, phh, H / 32, rt, &sm.X[0][0][0], (long)MT * SH,
+    gemm_tiles<MODE, MT, FT, LOB>(acc, wb + BL.b3, phh, H / 32, rt, &sm.X[0][0][0], (long)MT * SH,
                                  lane, SH, pf);
     gemm_first<FT>(pf, wb + BL.b2, H / 32, rt, lane);
 #pragma unroll
@@ -450,7 +536,7 @@ mlp_mfma_kernel(MlpArgs a) {
         float v[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) v[i] = acc[ft][at][i] * s2.get(ft, at, i);
-        put4<MODE, MT>(sm.Y, at * 16 + ca, f, v);
+        put4<MODE, MT, LOB>(sm.Y, at * 16 + ca, f, v);
       }
   }
   __syncthreads();
@@ -460,7 +546,7 @@ mlp_mfma_kernel(MlpArgs a) {
     for (int ft = 0; ft < FT; ++ft)
 #pragma unroll
       for (int at = 0; at < AT; ++at) acc[ft][at] = floatx4{0.f, 0.f, 0.f, 0.f};
-    gemm_tiles<MODE, MT, FT>(acc, wb + BL.b2, phh, H / 32, rt, &sm.Y[0][0][0], (long)MT * SH,
+    gemm_tiles<MODE, MT, FT, LOB>(acc, wb + BL.b2, phh, H / 32, rt, &sm.Y[0][0][0], (long)MT * SH,
                                  lane, SH, pf);
 #pragma unroll
     for (int ft = 0; ft < FT; ++ft)
@@ -470,7 +556,7 @@ mlp_mfma_kernel(MlpArgs a) {
         float v[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) v[i] = acc[ft][at][i] * s1.get(ft, at, i);
-        put4<MODE, MT>(sm.X, at * 16 + ca, f, v);
+        put4<MODE, MT, LOB>(sm.X, at * 16 + ca, f, v);
       }
   }
   uint4 pf2[2];   // first fragments of backward 1 (rows of dG tiles w, w + NW)
@@ -491,7 +577,7 @@ mlp_mfma_kernel(MlpArgs a) {
       for (int u = 0; u < 2; ++u)
 #pragma unroll
         for (int at = 0; at < AT; ++at) acc2[u][at] = floatx4{0.f, 0.f, 0.f, 0.f};
-      gemm_tiles<MODE, MT, 2>(acc2, wb + BL.b1, pd, H / 32, rt2, &sm.X[0][0][0], (long)MT * SH,
+      gemm_tiles<MODE, MT, 2, LOB>(acc2, wb + BL.b1, pd, H / 32, rt2, &sm.X[0][0][0], (long)MT * SH,
                                   lane, SH, pf2);
       const int tn = t0 + 2 * NW;   // the next iteration's first fragments
       if (tn < ntile) {
@@ -520,20 +606,20 @@ extern "C" {
 
 // out[0] = blob bytes, out[1] = byte offset of the float section,
 // out[2..7] = element offsets of f1, f2, f3, b3, b2, b1 fragment arrays,
-// out[8] = planes
+// out[8] = planes of the forward arrays, out[9] = planes of the backward arrays
 int tm_mlp_mfma_pack_info(int D, int H, int nh, int mode, long* out) {
   TM_REQUIRE(nh == 3, "tm_mlp_mfma: exactly 3 hidden layers supported (got %d)", nh);
   TM_REQUIRE(H % 64 == 0 && H <= HMAX && H >= 64, "tm_mlp_mfma: hidden width %d not a multiple of 64 in 64..512", H);
   TM_REQUIRE(D % KC == 0 && D > 0 && D + 8 <= 2 * (HMAX + 8),
              "tm_mlp_mfma: descriptor length %d not a multiple of %d in %d..%d", D, KC, KC,
              2 * HMAX + 8);
-  TM_REQUIRE(mode >= 0 && mode <= 2, "tm_mlp_mfma: bad mode %d", mode);
-  int planes = mode == 2 ? 2 : 1;
-  BlobLayout L = blob_layout(D, H, planes);
+  TM_REQUIRE(mode >= 0 && mode <= 5, "tm_mlp_mfma: bad mode %d", mode);
+  BlobLayout L = blob_layout(D, H, fwd_planes(mode), bwd_planes(mode));
   out[0] = L.bias_bytes + (4l * H + 4) * 4;
   out[1] = L.bias_bytes;
   out[2] = L.f1; out[3] = L.f2; out[4] = L.f3; out[5] = L.b3; out[6] = L.b2; out[7] = L.b1;
-  out[8] = planes;
+  out[8] = fwd_planes(mode);
+  out[9] = bwd_planes(mode);
   return TM_OK;
 }
 
@@ -544,14 +630,15 @@ int tm_mlp_mfma_pack_info(int D, int H, int nh, int mode, long* out) {
 int tm_mlp_mfma(int N, int D, int H, int ntiles, const int* tiles, const int* seg, int ldg,
                 const float* G, const int* perm, const void* blobs, float alpha, int mode,
                 int tile_rows, float* y, float* dG, void* stream) {
-  long info[9];
+  long info[10];
   int st = tm_mlp_mfma_pack_info(D, H, 3, mode, info);
   if (st) return st;
   TM_REQUIRE(ldg % 4 == 0 && ldg >= D, "tm_mlp_mfma: ldg %d", ldg);
   TM_REQUIRE(alpha > 0.f, "tm_mlp_mfma: activation alpha %g must be positive", (double)alpha);
   TM_REQUIRE(ntiles >= 0, "tm_mlp_mfma: bad tile count");
-  TM_REQUIRE(tile_rows == 32 || (tile_rows == 64 && mode != 2),
-             "tm_mlp_mfma: tile_rows %d (64 or 32; 32 in the split mode)", tile_rows);
+  const bool lof = mode == 2 || mode == 5;   // forward activations carry lo planes
+  TM_REQUIRE(tile_rows == 32 || (tile_rows == 64 && !lof),
+             "tm_mlp_mfma: tile_rows %d (64 or 32; 32 in modes 2 and 5)", tile_rows);
   if (ntiles == 0) return TM_OK;
   MlpArgs a;
   a.N = N; a.D = D; a.H = H; a.ldg = ldg;
@@ -570,7 +657,7 @@ int tm_mlp_mfma(int N, int D, int H, int ntiles, const int* tiles, const int* se
   // would let the dispatcher stack two on one CU while others idle, so the
   // launch then asks for extra LDS: one workgroup per CU.
   size_t pad = 0;
-  if (tile_rows == 32 && mode != 2) {
+  if (tile_rows == 32 && !lof) {
     int dev = 0, ncu = 0;
     TM_HIP_CHECK(hipGetDevice(&dev));
     TM_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -580,6 +667,11 @@ int tm_mlp_mfma(int N, int D, int H, int ntiles, const int* tiles, const int* se
   else if (mode == 0) mlp_mfma_kernel<0, 32><<<ntiles, NW * 64, pad, s>>>(a);
   else if (mode == 1 && tile_rows == 64) mlp_mfma_kernel<1, 64><<<ntiles, NW * 64, 0, s>>>(a);
   else if (mode == 1) mlp_mfma_kernel<1, 32><<<ntiles, NW * 64, pad, s>>>(a);
+  else if (mode == 3 && tile_rows == 64) mlp_mfma_kernel<3, 64><<<ntiles, NW * 64, 0, s>>>(a);
+  else if (mode == 3) mlp_mfma_kernel<3, 32><<<ntiles, NW * 64, pad, s>>>(a);
+  else if (mode == 4 && tile_rows == 64) mlp_mfma_kernel<4, 64><<<ntiles, NW * 64, 0, s>>>(a);
+  else if (mode == 4) mlp_mfma_kernel<4, 32><<<ntiles, NW * 64, pad, s>>>(a);
+  else if (mode == 5) mlp_mfma_kernel<5, 32><<<ntiles, NW * 64, 0, s>>>(a);
   else mlp_mfma_kernel<2, 32><<<ntiles, NW * 64, 0, s>>>(a);
   TM_LAUNCH_CHECK();
   return TM_OK;

@@ -1,165 +1,237 @@
 """Molecular dynamics drivers.
 
-Drop-in host-side drivers with the reference's signatures and units (A, fs,
-kg/mol, forces J/mol/A), restating TensorMol/Simulations/SimpleMD.py:
-VelocityVerletStep (:14-38), KineticEnergy (:40-51), Thermostat (:53-88),
-NoseThermostat (:90-129), VelocityVerlet (:322-425, including its a = 0 first
-step).  They call a user EnAndForce closure, as in the reference.
+Host drivers with the call shapes of TensorMol/Simulations/SimpleMD.py
+(VelocityVerletStep :14-38, KineticEnergy :40-51, Thermostat :53-88,
+NoseThermostat :90-129, VelocityVerlet :322-425).  They call the user's
+EnAndForce closure exactly as the reference does, and are written from the
+following specification of its behaviour.
+
+Units: positions A, time fs, masses kg/mol (PhysicalData ATOMICMASSES),
+forces J/mol/A.  Acceleration a = 1e-10 F / m in A/fs^2 (1 m/s^2 = 1e-10
+A/fs^2); kinetic energies in kg/mol A^2/fs^2 = 1e10 J/mol.
+
+- Velocity Verlet step: drift x' = x + v dt + a dt^2 / 2, force at x',
+  a' = 1e-10 F'/m, v' = v + (a + a') dt / 2; returns (x', v', a', E(x')), E = 0
+  when only a force closure is given.
+- KineticEnergy: the mean over atoms of m |v|^2 / 2, in J/mol.
+- Rescale (the velocity initialiser): every atom's velocity is scaled on its
+  own so that its temperature m |v|^2 1e10 / (3 R) is exactly T; atoms at rest
+  are left alone.
+- Rescaling thermostat step: a Velocity Verlet step, then every velocity is
+  scaled by sqrt(T / T_eff), T_eff = (2/3) KineticEnergy / R.
+- Nose-Hoover step (one thermostat variable eta): kT = R 1e-10 T,
+  tau = 20 dt_MD N, Q = kT tau^2, target g kT with g = (3N + 1) / 2,
+  K(u) = sum m |u|^2 / 2:
+      x'  = x + v dt + (a - eta v) dt^2 / 2
+      vh  = v + (a - eta v) dt / 2
+      eta <- eta + dt/(2Q) (K(v) - g kT) + dt/(2Q) (K(vh) - g kT)
+      v'  = (vh + a' dt / 2) / (1 + eta dt / 2)
+- VelocityVerlet(f, g0, name, EandF, cellsize).Prop(): the initial potential
+  EPot0 comes from EandF(g0.coords) when given.  The state starts from
+  a = 0, so the first step ignores the initial forces.  Velocities start at 0,
+  or standard-normal then Rescale'd for PARAMS["MDV0"] == "Random".  The
+  thermostat comes from PARAMS["MDThermostat"] (None, "Rescaling", "Nose").
+  Each of PARAMS["MDMaxStep"] steps wraps positions into [0, cellsize) when
+  a cell size is given.  It appends the row [t, 0, 0, 0, KE, EPot,
+  KE + (EPot - EPot0) JOULEPERHARTREE] to md_log, where KE is the per-atom
+  kinetic energy of the previous step (0 on the first), then updates KE and
+  the running mean / variance of EPot.  It writes the trajectory every third
+  step when PARAMS["MDLogTrajectory"] is set and saves md_log every 500 steps.
+  (The reference's Rescaling thermostat returns four values where Prop unpacks
+  five; here it returns five so that the thermostat is usable.)
 
 DeviceMD is the resident path: positions, velocities, accelerations and the
 thermostat state of every molecule of a packed batch live in HBM, each step is
 a fixed sequence of HIP launches (FP64 integrator kernels around one engine
 evaluation) and the whole step can be captured into a HIP graph.
 """
+import os
 import time
 
 import numpy as np
 import torch
 
 from . import _lib
+from .containers import Mol
 from .params import PARAMS, IDEALGASR, JOULEPERHARTREE, KJPERHARTREE, atomic_masses
+from .statistics import OnlineEstimator
 
 _P = _lib.ptr
 
 
 # ------------------------------------------------------------ host drivers
+def _energy_force(x, force_fn, energy_force_fn):
+    """(E, F) at x from whichever closure the caller supplied."""
+    if energy_force_fn is None:
+        return 0.0, force_fn(x)
+    return energy_force_fn(x)
+
+
+def _accel(force, masses):
+    """J/mol/A over kg/mol -> A/fs^2."""
+    return 1e-10 * np.asarray(force) / masses[:, None]
+
+
+def _mv2(v, masses):
+    """sum_i m_i |v_i|^2 in kg/mol A^2/fs^2 (twice the kinetic energy)."""
+    return float(masses @ np.square(v).sum(axis=1))
+
+
 def VelocityVerletStep(f_, a_, x_, v_, m_, dt_, fande_=None):
-    x = x_ + v_ * dt_ + 0.5 * a_ * dt_ * dt_
-    e, f_x_ = 0.0, None
-    if fande_ is None:
-        f_x_ = f_(x)
-    else:
-        e, f_x_ = fande_(x)
-    a = 1e-10 * np.einsum("ax,a->ax", f_x_, 1.0 / m_)
-    v = v_ + 0.5 * (a_ + a) * dt_
-    return x, v, a, e
+    """One Velocity Verlet step; returns (x, v, a, E) at the new positions."""
+    x = x_ + v_ * dt_ + 0.5 * a_ * dt_ ** 2
+    energy, force = _energy_force(x, f_, fande_)
+    a = _accel(force, m_)
+    return x, v_ + 0.5 * dt_ * (a_ + a), a, energy
 
 
 def KineticEnergy(v_, m_):
-    """Kinetic energy per atom in J/mol."""
-    return 0.5 * np.dot(np.einsum("ia,ia->i", v_, v_) * 1e10, m_) / len(m_)
+    """Mean kinetic energy per atom, J/mol."""
+    return 0.5e10 * _mv2(v_, m_) / len(m_)
+
+
+def _atom_temperatures(v, masses):
+    """Per-atom temperature m |v|^2 1e10 / (3 R), in K."""
+    return masses * np.square(v).sum(axis=1) * 1e10 / (3.0 * IDEALGASR)
 
 
 class Thermostat:
-    """Velocity rescaling to PARAMS["MDTemp"] (SimpleMD.py:53-88)."""
+    """Velocity-rescaling thermostat at PARAMS["MDTemp"]; constructing it
+    rescales v_ in place atom by atom."""
+
+    name = "Rescaling"
 
     def __init__(self, m_, v_):
-        self.N = len(m_)
-        self.m = m_.copy()
+        self.m = np.array(m_, dtype=np.float64)
+        self.N = len(self.m)
         self.T = PARAMS["MDTemp"]
-        self.Teff = 0.001
         self.kT = IDEALGASR * 1e-10 * self.T
         self.tau = 30 * PARAMS["MDdt"]
-        self.name = "Rescaling"
+        self.Teff = 0.001
         self.Rescale(v_)
 
-    def step(self, f_, a_, x_, v_, m_, dt_, fande_=None):
-        x = x_ + v_ * dt_ + 0.5 * a_ * dt_ * dt_
-        e, f_x_ = 0.0, None
-        if fande_ is None:
-            f_x_ = f_(x)
-        else:
-            e, f_x_ = fande_(x)
-        a = 1e-10 * np.einsum("ax,a->ax", f_x_, 1.0 / m_)
-        v = v_ + 0.5 * (a_ + a) * dt_
-        self.Teff = (2.0 / 3.0) * KineticEnergy(v, self.m) / IDEALGASR
-        v *= np.sqrt(self.T / self.Teff)
-        return x, v, a, e, f_x_
-
     def Rescale(self, v_):
-        """Each atom rescaled to exactly T (SimpleMD.py:82-88)."""
-        for i in range(self.N):
-            Teff = (2.0 / (3.0 * IDEALGASR)) * 1e10 * 0.5 * self.m[i] * np.dot(v_[i], v_[i])
-            if Teff != 0.0:
-                v_[i] *= np.sqrt(self.T / Teff)
+        """Scale each atom's velocity (in place) to temperature T."""
+        Ti = _atom_temperatures(v_, self.m)
+        moving = Ti != 0.0
+        v_[moving] *= np.sqrt(self.T / Ti[moving])[:, None]
+
+    def step(self, f_, a_, x_, v_, m_, dt_, fande_=None):
+        x, v, a, energy = VelocityVerletStep(f_, a_, x_, v_, m_, dt_, fande_)
+        self.Teff = (2.0 / 3.0) * KineticEnergy(v, self.m) / IDEALGASR
+        v = v * np.sqrt(self.T / self.Teff)
+        return x, v, a, energy, None
 
 
 class NoseThermostat(Thermostat):
-    """Single-variable Nose-Hoover (SimpleMD.py:90-129)."""
+    """Nose-Hoover thermostat with one variable eta (tau = 20 dt N)."""
+
+    name = "Nose"
 
     def __init__(self, m_, v_):
-        self.m = m_.copy()
-        self.N = len(m_)
+        self.m = np.array(m_, dtype=np.float64)
+        self.N = len(self.m)
         self.T = PARAMS["MDTemp"]
         self.eta = 0.0
-        self.name = "Nose"
         self.Rescale(v_)
 
-    def step(self, f_, a_, x_, v_, m_, dt_, fande_=None, frc_=True):
+    def _couple(self):
+        """kT, tau and Q from the current T, N and PARAMS["MDdt"]."""
         self.kT = IDEALGASR * 1e-10 * self.T
         self.tau = 20.0 * PARAMS["MDdt"] * self.N
-        self.Q = self.kT * self.tau * self.tau
-        x = x_ + v_ * dt_ + 0.5 * (a_ - self.eta * v_) * dt_ * dt_
-        vdto2 = v_ + 0.5 * (a_ - self.eta * v_) * dt_
-        e, f_x_ = 0.0, None
-        if fande_ is None:
-            f_x_ = f_(x)
-        else:
-            e, f_x_ = fande_(x)
-        a = 1e-10 * np.einsum("ax,a->ax", f_x_, 1.0 / m_)
-        ke = 0.5 * np.dot(np.einsum("ia,ia->i", v_, v_), m_)
-        etadto2 = self.eta + (dt_ / (2.0 * self.Q)) * (ke - ((3.0 * self.N + 1) / 2.0) * self.kT)
-        kedto2 = 0.5 * np.dot(np.einsum("ia,ia->i", vdto2, vdto2), m_)
-        self.eta = etadto2 + (dt_ / (2.0 * self.Q)) * (kedto2 - ((3.0 * self.N + 1) / 2.0) * self.kT)
-        v = (vdto2 + (dt_ / 2.0) * a) / (1 + (dt_ / 2.0) * self.eta)
-        if frc_:
-            return x, v, a, e, f_x_
-        return x, v, a, e
+        self.Q = self.kT * self.tau ** 2
+
+    def step(self, f_, a_, x_, v_, m_, dt_, fande_=None, frc_=True):
+        self._couple()
+        g_kT = 0.5 * (3.0 * self.N + 1.0) * self.kT
+        rate = dt_ / (2.0 * self.Q)
+        a_damped = a_ - self.eta * v_
+        x = x_ + v_ * dt_ + 0.5 * a_damped * dt_ ** 2
+        v_half = v_ + 0.5 * a_damped * dt_
+        energy, force = _energy_force(x, f_, fande_)
+        a = _accel(force, m_)
+        eta_half = self.eta + rate * (0.5 * _mv2(v_, m_) - g_kT)
+        self.eta = eta_half + rate * (0.5 * _mv2(v_half, m_) - g_kT)
+        v = (v_half + 0.5 * dt_ * a) / (1.0 + 0.5 * dt_ * self.eta)
+        return (x, v, a, energy, force) if frc_ else (x, v, a, energy)
+
+
+_THERMOSTATS = {"Rescaling": Thermostat, "Nose": NoseThermostat}
 
 
 class VelocityVerlet:
-    """VelocityVerlet(f_, g0_, name_, EandF_, cellsize_).Prop() (SimpleMD.py:322-425)."""
+    """VelocityVerlet(f_, g0_, name_, EandF_, cellsize_): .Prop() runs
+    PARAMS["MDMaxStep"] steps from the molecule g0_ and fills md_log."""
 
     def __init__(self, f_, g0_, name_="", EandF_=None, cellsize_=None):
         self.name = name_
         self.cellsize = cellsize_
-        self.maxstep = PARAMS["MDMaxStep"]
+        self.maxstep = int(PARAMS["MDMaxStep"])
         self.T = PARAMS["MDTemp"]
         self.dt = PARAMS["MDdt"]
         self.ForceFunction = f_
         self.EnergyAndForce = EandF_
+        self.atoms = g0_.atoms.copy()
+        self.natoms = len(self.atoms)
+        self.m = atomic_masses(self.atoms)
+        self.x = np.array(g0_.coords, dtype=np.float64)
         self.EPot0 = 0.0
         if EandF_ is not None:
-            self.EPot0, self.f0 = self.EnergyAndForce(g0_.coords)
+            self.EPot0, self.f0 = EandF_(self.x)
         self.EPot = self.EPot0
+        self.EnergyStat = OnlineEstimator(self.EPot0)
         self.t = 0.0
         self.KE = 0.0
-        self.atoms = g0_.atoms.copy()
-        self.m = atomic_masses(self.atoms)
-        self.natoms = len(self.atoms)
-        self.x = g0_.coords.copy()
-        self.v = np.zeros(self.x.shape)
-        self.a = np.zeros(self.x.shape)   # the reference starts from a = 0 (SimpleMD.py:359)
-        self.md_log = None
+        self.a = np.zeros_like(self.x)
+        self.v = np.zeros_like(self.x)
         if PARAMS["MDV0"] == "Random":
-            self.v = np.random.randn(*self.x.shape)
+            self.v = np.random.standard_normal(self.x.shape)
             Thermostat(self.m, self.v)
-        self.Tstat = None
-        if PARAMS["MDThermostat"] == "Rescaling":
-            self.Tstat = Thermostat(self.m, self.v)
-        elif PARAMS["MDThermostat"] == "Nose":
-            self.Tstat = NoseThermostat(self.m, self.v)
+        kind = PARAMS["MDThermostat"]
+        if kind is not None and kind not in _THERMOSTATS:
+            raise ValueError(f"MDThermostat {kind!r}: use None, 'Rescaling' or 'Nose'")
+        self.Tstat = None if kind is None else _THERMOSTATS[kind](self.m, self.v)
+        self.md_log = None
+
+    def _advance(self):
+        """One integrator step of the configured kind; updates x, v, a, EPot."""
+        if self.Tstat is None:
+            self.x, self.v, self.a, self.EPot = VelocityVerletStep(
+                self.ForceFunction, self.a, self.x, self.v, self.m, self.dt, self.EnergyAndForce)
+        else:
+            self.x, self.v, self.a, self.EPot, self.force = self.Tstat.step(
+                self.ForceFunction, self.a, self.x, self.v, self.m, self.dt, self.EnergyAndForce)
+        if self.cellsize is not None:
+            self.x = np.mod(self.x, self.cellsize)
+
+    def WriteTrajectory(self):
+        """Append the current frame to results/MDTrajectory<name>.xyz."""
+        frame = Mol(self.atoms, self.x)
+        frame.properties.update(Time=self.t, KineticEnergy=self.KE, PotEnergy=self.EPot)
+        frame.WriteXYZfile(PARAMS["results_dir"], "MDTrajectory" + self.name)
+
+    def _save_log(self):
+        os.makedirs(PARAMS["results_dir"], exist_ok=True)
+        np.savetxt(os.path.join(PARAMS["results_dir"], "MDLog" + self.name + ".txt"), self.md_log)
 
     def Prop(self):
-        step = 0
         self.md_log = np.zeros((self.maxstep, 7))
-        while step < self.maxstep:
+        for step in range(self.maxstep):
             self.t = step * self.dt
-            if PARAMS["MDThermostat"] is None:
-                self.x, self.v, self.a, self.EPot = VelocityVerletStep(
-                    self.ForceFunction, self.a, self.x, self.v, self.m, self.dt, self.EnergyAndForce)
-            else:
-                self.x, self.v, self.a, self.EPot, self.force = self.Tstat.step(
-                    self.ForceFunction, self.a, self.x, self.v, self.m, self.dt, self.EnergyAndForce)
-            if self.cellsize is not None:
-                self.x = np.mod(self.x, self.cellsize)
-            self.md_log[step, 0] = self.t
-            self.md_log[step, 4] = self.KE
-            self.md_log[step, 5] = self.EPot
-            self.md_log[step, 6] = self.KE + (self.EPot - self.EPot0) * JOULEPERHARTREE
+            self._advance()
+            row = self.md_log[step]
+            row[0] = self.t
+            row[4] = self.KE
+            row[5] = self.EPot
+            row[6] = self.KE + (self.EPot - self.EPot0) * JOULEPERHARTREE
+            self.EnergyStat(self.EPot)
             self.KE = KineticEnergy(self.v, self.m)
-            step += 1
-        return
+            if PARAMS["MDLogTrajectory"] and step % 3 == 0:
+                self.WriteTrajectory()
+            if step % 500 == 0:
+                self._save_log()
+        if self.maxstep:
+            self._save_log()
 
 
 # ------------------------------------------------------------ device path

@@ -4,7 +4,19 @@ Weights of every (net, element) pair are packed once into a blob whose layout
 tm_mlp_mfma_pack_info reports: six fragment arrays (forward W1^T, W2^T, W3^T;
 backward W3, W2, W1) in the operand order of v_mfma_f32_16x16x32 (A operand:
 lane l holds row 16*t + l%16, columns 32*s + 8*(l//16) .. +7), followed by the
-float32 biases and output layer.  The split mode stores a hi and a lo plane.
+float32 biases and output layer.  The split modes store a hi and a lo plane:
+bf16x3 for every array, bf16x2 / fp16x2 / bf16x3f for the three forward arrays
+only.
+
+Modes (operands of v_mfma_f32_16x16x32, FP32 accumulation throughout):
+  bf16     weights and activations rounded once to BF16
+  fp16     the same in FP16
+  bf16x3   hi/lo BF16 split of weights and activations in every GEMM (3 MFMAs)
+  bf16x2   forward GEMMs with hi/lo split weights (2 MFMAs), BF16 activations;
+           input-gradient GEMMs as bf16
+  fp16x2   the same with FP16 operands
+  bf16x3f  forward GEMMs split in weights and activations (3 MFMAs),
+           input-gradient GEMMs as bf16
 """
 import ctypes
 
@@ -13,8 +25,9 @@ import torch
 
 from . import _lib
 
-MODES = {"bf16": 0, "fp16": 1, "bf16x3": 2}
-TILE_ROWS = {0: 64, 1: 64, 2: 32}
+MODES = {"bf16": 0, "fp16": 1, "bf16x3": 2, "bf16x2": 3, "fp16x2": 4, "bf16x3f": 5}
+TILE_ROWS = {0: 64, 1: 64, 2: 32, 3: 64, 4: 64, 5: 32}
+HALF = {0: "bf16", 1: "fp16", 2: "bf16", 3: "bf16", 4: "fp16", 5: "bf16"}   # operand type
 
 
 def frag_pack(M):
@@ -25,7 +38,7 @@ def frag_pack(M):
 
 def _to16(x, mode):
     t = torch.tensor(np.ascontiguousarray(x), dtype=torch.float32)
-    if mode == 1:
+    if HALF[mode] == "fp16":
         h = t.to(torch.float16)
         return h.view(torch.int16).numpy(), h.to(torch.float32).numpy()
     h = t.to(torch.bfloat16)
@@ -58,7 +71,8 @@ def xcd_order(til):
 class MFMAWeights:
     def __init__(self, engine, precision):
         if precision not in MODES:
-            raise ValueError(f"unknown MLP precision {precision!r}; use fp32, bf16, fp16 or bf16x3")
+            raise ValueError(f"unknown MLP precision {precision!r}; use fp32 or one of "
+                             f"{sorted(MODES)}")
         self.engine = engine
         self.mode = MODES[precision]
         self.mt = TILE_ROWS[self.mode]
@@ -70,7 +84,7 @@ class MFMAWeights:
         if len(model.hidden) != 3 or len(set(model.hidden)) != 1:
             raise ValueError("the MFMA path needs three equal hidden layers")
         self.H = int(model.hidden[0])
-        info = (ctypes.c_long * 9)()
+        info = (ctypes.c_long * 10)()
         engine.lib.tm_mlp_mfma_pack_info(self.D, self.H, 3, self.mode, info)
         self.info = list(info)
         self.blobs = []
@@ -86,10 +100,11 @@ class MFMAWeights:
         (W1, b1), (W2, b2), (W3, b3), (Wo, bo) = layers
         nbytes, fsec = self.info[0], self.info[1]
         offs = self.info[2:8]
-        planes = self.info[8]
+        fwd_planes, bwd_planes = self.info[8], self.info[9]
         blob = np.zeros(nbytes, dtype=np.uint8)
         el = blob[:fsec].view(np.int16)
-        for off, M in zip(offs, (W1.T, W2.T, W3.T, W3, W2, W1)):
+        for k, (off, M) in enumerate(zip(offs, (W1.T, W2.T, W3.T, W3, W2, W1))):
+            planes = fwd_planes if k < 3 else bwd_planes
             hi_bits, hi_val = _to16(M, self.mode)
             packed = frag_pack(hi_bits.reshape(M.shape))
             n = packed.size
@@ -112,7 +127,7 @@ class MFMAWeights:
 
         64-row tiles fill whole rounds of the device (one workgroup per CU); the
         remaining rows go to 32-row tiles in a second launch, so the last round
-        is not half empty.  The split mode uses 32-row tiles throughout."""
+        is not half empty.  bf16x3 and bf16x3f use 32-row tiles throughout."""
         nele = self.engine.nele
         counts = np.diff(ws.eoff).astype(np.int64)
         segs, t64 = [], []
@@ -127,7 +142,7 @@ class MFMAWeights:
                 t64.append(np.stack([np.full_like(r0, s), r0], axis=1))
         seg = np.array(segs, dtype=np.int32).reshape(-1, 4)
         t64 = np.concatenate(t64).astype(np.int32) if t64 else np.zeros((0, 2), np.int32)
-        full = 0 if self.mode == 2 else (len(t64) // self.ncu) * self.ncu
+        full = 0 if self.mt == 32 else (len(t64) // self.ncu) * self.ncu
         head, rest = t64[:full], t64[full:]
         # each remaining 64-row tile becomes two 32-row tiles (the second only if rows remain)
         halves = np.repeat(rest, 2, axis=0)

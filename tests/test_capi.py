@@ -53,11 +53,17 @@ def test_version_and_argument_errors(lib):
                         1.0, 1.0, 1.0, 0.0, 0.0, 0.0, 0, 0, 4.6, 1.0, 1.0, 128, 48, None, None, None,
                         None, None, None, None, None, None)
     with pytest.raises(RuntimeError, match="3 hidden layers"):
-        out = (ctypes.c_long * 9)()
+        out = (ctypes.c_long * 10)()
         lib.tm_mlp_mfma_pack_info(256, 512, 2, 0, out)
-    out = (ctypes.c_long * 9)()
+    out = (ctypes.c_long * 10)()
     lib.tm_mlp_mfma_pack_info(256, 512, 3, 0, out)
-    assert out[8] == 1 and out[0] > 0
+    assert out[8] == 1 and out[9] == 1 and out[0] > 0
+    # split forward weights (mode 3): two forward planes, one backward plane
+    out3 = (ctypes.c_long * 10)()
+    lib.tm_mlp_mfma_pack_info(256, 512, 3, 3, out3)
+    assert out3[8] == 2 and out3[9] == 1
+    assert out3[3] - out3[2] == 2 * (out[3] - out[2])   # f1 holds two planes
+    assert out3[6] - out3[5] == out[6] - out[5]         # b3 holds one
 
 
 def test_binding_argument_counts_match_header():
