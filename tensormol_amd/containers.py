@@ -31,6 +31,16 @@ class Mol:
     def AtomTypes(self):
         return np.unique(self.atoms)
 
+    def Center(self, CenterOf="Atom", MomentOrder=1.0):
+        """Mean of coords**MomentOrder over atoms, or its mass-weighted mean
+        for CenterOf="Mass" (Mol.py:394-407)."""
+        c = np.power(self.coords, MomentOrder)
+        if CenterOf == "Mass":
+            from .params import atomic_masses
+            w = atomic_masses(self.atoms)
+            return (w[:, None] * c).sum(0) / w.sum()
+        return c.mean(0)
+
     @staticmethod
     def ParseProperties(s_):
         """Comment line 'Comment: ;;;energy E;;;...' -> {'energy': E}

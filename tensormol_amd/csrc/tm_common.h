@@ -58,9 +58,12 @@ static inline unsigned tm_div_up(long a, long b) { return (unsigned)((a + b - 1)
 #define TM_MAX_ELE 8
 
 // Minimum-image displacement along one axis for an orthorhombic box edge L
-// (L <= 0 means open boundary).  Evaluated in float64.
+// (L <= 0 means open boundary).  Evaluated in float64 with the product and
+// the difference rounded separately (no fused multiply-add), as the host
+// restatement of the neighbour predicate does.
 __device__ __forceinline__ double tm_min_image(double d, double L) {
-  if (L > 0.0) d -= L * rint(d / L);
+#pragma clang fp contract(off)
+  if (L > 0.0) d = d - L * rint(d / L);
   return d;
 }
 

@@ -320,11 +320,16 @@ class BPEEEngine:
                       _P(ws.aseg), None, _P(ws.flags), st)   # rev: filled by tm_sym_forward
 
     def check_flags(self, ws):
+        """Read the status flags (a synchronisation) and raise on any overflow
+        or non-finite coordinate; returns them otherwise."""
         f = ws.flags.cpu().numpy()
-        if f[0] or f[1]:
-            raise RuntimeError(f"neighbour row overflow (flags {f[:4]}); raise ldr/lda")
         if f[3]:
             raise FloatingPointError("non-finite coordinates")
+        if f[0] & 4:
+            raise RuntimeError(f"Coulomb sweep needed more candidate ranges than it holds "
+                               f"(flags {f[:4]}); use larger cells")
+        if f[0] or f[1]:
+            raise NeighbourOverflow(f"neighbour row overflow (flags {f[:4]}); raise ldr/lda")
         return f
 
     def descriptors(self, ws, pos):
@@ -457,17 +462,21 @@ class BPEEEngine:
                              self._stream())
 
     def run(self, ws, pos, forces=True, check=None):
-        """Enqueue one full evaluation for positions pos (cuda float64 [N,3])."""
+        """Enqueue one full evaluation for positions pos (cuda float64 [N,3]).
+        With check (default: the engine's check_overflow) the status flags are
+        read once the evaluation is queued (a synchronisation), so an overflow
+        raised by any stage (rows, mirror slots, the Coulomb sweep's ranges)
+        or non-finite coordinates raise here."""
         assert pos.dtype == torch.float64 and pos.is_contiguous() and pos.shape[0] >= ws.N
         self.build_lists(ws, pos)
-        if (self.check_overflow if check is None else check):
-            self.check_flags(ws)
         self.descriptors(ws, pos)
         self.networks(ws)
         self.electrostatics(ws, pos)
         if forces:
             self.adjoint(ws, pos)
         self.reductions(ws)
+        if self.check_overflow if check is None else check:
+            self.check_flags(ws)
 
     def results(self, ws):
         """Host copies of the outputs of the last run (float64 numpy)."""
@@ -482,6 +491,11 @@ class BPEEEngine:
     def energy_tensor(self, ws):
         """Per-molecule total energy as a device tensor (no host copy)."""
         return ws.Ebp[:ws.B] + ws.Eext[:ws.B, 0] + ws.Eext[:ws.B, 1]
+
+
+class NeighbourOverflow(RuntimeError):
+    """A neighbour row (4.6 A list or 3.1 A sub-list) exceeded its stride, so
+    the rows were truncated and the evaluation is not valid."""
 
 
 def _plain_upload(device):

@@ -185,6 +185,15 @@ class TFMolManage:
         return xyzs, Zs, natom
 
     def _eval(self, mols, HasVdw):
+        """The tuple follows HasVdw as in the reference: eight values
+        (Etotal, Ebp, Ebp_atom, Ecc, Evdw, dipole, charges, force) with it, six
+        (Etotal, Ebp, Ecc, dipole, charges, force) without.  The reference's
+        vdW instance always returns eight, so HasVdw=False fails there when
+        unpacking; here it raises ValueError for a model with the dispersion
+        term and returns the six-tuple for a model without it."""
+        if not HasVdw and self.model.has_vdw:
+            raise ValueError("this BP-EE network includes the vdW term and returns eight values; "
+                             "call with HasVdw=True (the reference fails unpacking six)")
         xyzs, Zs, natom = self._pad(mols)
         n = len(mols)
         batch = [xyzs, Zs, np.zeros(n), np.zeros((n, 3)), np.zeros_like(xyzs), None, None, None, None,
@@ -216,11 +225,24 @@ class TFMolManage:
 
     def EvalBPDirectEEUpdateSinglePeriodic(self, mol, Rr_cut, Ra_cut, Ree_cut, nreal, HasVdw=True,
                                            DoForce=True, DoCharge=False, lattice=None):
-        """TFMolManage.py:1323-1358 on the first nreal atoms of mol with
-        minimum-image lists over `lattice` (or mol.properties["lattice"]).
-        Returns Etotal, force (1, nreal, 3) J/mol/A [, charges (1, nreal)]."""
+        """TFMolManage.py:1323-1358: energy of the cell and forces on the first
+        nreal atoms of mol.  As in the reference, mol is the tessellated
+        molecule (real atoms first, then the images of Lattice.TessLattice);
+        the lattice is read back from the images (or taken from `lattice` /
+        mol.properties["Lattice"]) and the real atoms are evaluated with
+        minimum-image lists.  Returns Etotal, force (1, nreal, 3) J/mol/A
+        [, charges (1, nreal)], or Etotal alone without DoForce."""
+        from .periodic import lattice_from_images, orthorhombic_edges
         self._check_cuts(Rr_cut, Ra_cut, Ree_cut)
-        lat = lattice if lattice is not None else mol.properties.get("lattice")
+        lat = lattice
+        if lat is None:
+            lat = mol.properties.get("Lattice", mol.properties.get("lattice"))
+        if lat is None and mol.NAtoms() > nreal:
+            lat = lattice_from_images(mol.coords, nreal)
+        if lat is None:
+            raise ValueError("no lattice: pass the tessellated molecule (real atoms, then the "
+                             "images of Lattice.TessLattice), a Lattice property or lattice=")
+        lat = orthorhombic_edges(lat)
         xyzs, Zs, natom = self._pad([mol])
         batch = [xyzs, Zs, np.zeros(1), np.zeros((1, 3)), np.zeros_like(xyzs), None, None, None, None,
                  np.array([1.0 / nreal])]

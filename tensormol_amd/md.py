@@ -276,7 +276,11 @@ class DeviceMD:
         e.lib.tm_vv_kick(ws.N, 0.0, _P(ws.force), _P(self.m), _P(self.v), _P(self.a), _P(self.ke),
                          e._stream())
 
-    def step(self):
+    def step(self, check=True):
+        """One step.  With check the status flags are read afterwards (a
+        synchronisation) and an overflowed neighbour row, Coulomb range buffer
+        or a non-finite coordinate raises, as BPEEEngine.check_flags does;
+        check=False leaves the step asynchronous (and capturable)."""
         e, ws, L = self.eng, self.ws, self.eng.lib
         st = e._stream()
         if self.kind == "nve":
@@ -295,16 +299,20 @@ class DeviceMD:
                         _P(self.eta), st)
             L.tm_nh_kick(ws.N, self.dt, _P(ws.mol), _P(self.eta), _P(ws.force), _P(self.m),
                          _P(self.vh), _P(self.v), _P(self.a), _P(self.ke), st)
+        if check:
+            e.check_flags(ws)
 
-    def run_steps(self, n):
+    def run_steps(self, n, check=True):
         """n steps, leaving the state of n calls to step() bit for bit.  For
         NVE the kick of every step but the last and the drift of the step after
         it run inside the force gather's launch (tm_force_gather_vv), so a step
         costs two launches fewer; the status flags are cleared once and
-        accumulate over the n steps.  Capturable into one HIP graph."""
+        accumulate over the n steps, and with check they are read once at the
+        end (raising as step() does).  With check=False it is capturable into
+        one HIP graph; call check_flags() after the replays."""
         if self.kind != "nve" or n <= 0:
-            for _ in range(n):
-                self.step()
+            for k in range(n):
+                self.step(check=check and k == n - 1)
             return
         e, ws, L = self.eng, self.ws, self.eng.lib
         st = e._stream()
@@ -322,12 +330,20 @@ class DeviceMD:
                 e.reductions(ws)
                 L.tm_vv_kick(ws.N, self.dt, _P(ws.force), _P(self.m), _P(self.v), _P(self.a),
                              _P(self.ke), st)
+        if check:
+            e.check_flags(ws)
+
+    def check_flags(self):
+        """Raise if any step since the flags were last cleared overflowed or
+        saw a non-finite coordinate (synchronises)."""
+        return self.eng.check_flags(self.ws)
 
     def capture(self, stream):
-        """Capture one step into a HIP graph on `stream` (replay with .replay())."""
+        """Capture one step into a HIP graph on `stream` (replay with .replay();
+        call check_flags() after replays)."""
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph, stream=stream):
-            self.step()
+            self.step(check=False)
         return self.graph
 
     def energies(self):

@@ -17,6 +17,7 @@ JOULEPERHARTREE = KJPERHARTREE * 1000.0
 KCALPERHARTREE = 627.509474
 IDEALGASR = 8.3144621
 FSPERAU = 0.0241888
+AVOCONST = 6.02214086e23
 PI_LITERAL = 3.14159265359
 TWO_OVER_SQRTPI = 1.1283791671
 
@@ -74,6 +75,7 @@ class _Params(dict):
         self["OptMomentum"] = 0.0
         self["OptMomentumDecay"] = 0.8
         self["GSSearchAlpha"] = 0.05
+        self["OptLatticeStep"] = 0.050
         self["RemoveInvariant"] = True
         self["ConjGrad_Restart"] = 10
         self["FIREMaxStep"] = 0.1

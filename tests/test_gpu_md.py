@@ -98,3 +98,90 @@ def test_device_md_run_steps_equals_steps(cuda):
         outs.append([t.clone() for t in (md.pos, md.v, md.a, md.ke, ws.force)])
     for name, p, q in zip(("x", "v", "a", "ke", "force"), *outs):
         assert torch.equal(p, q), name
+
+
+# ------------------------------------------------------------------ oracle parity
+def _gentle():
+    """8 waters, 32-wide nets: a smooth potential over a short trajectory."""
+    from oracle import oracle as O
+    from tensormol_amd import workloads as W
+    from tensormol_amd.model import BPEEModel
+    Z, x = W.water_cluster(8, box=6.0, seed=1)
+    model = BPEEModel([1, 8], dsf_alpha=0.2, seed=3, hidden=[32, 32, 32])
+    sp = O.SymParams(model.eles)
+    ms = np.array([0, len(Z)])
+
+    def eandf(xx):
+        r = O.evaluate(xx, Z, ms, sp, model.energy_nets, model.charge_nets, alpha=model.alpha,
+                       dsf_alpha=model.dsf_alpha)
+        return r["Etotal"][0], -O.JOULEPERHARTREE * r["gradient"]
+    return Z, x, ms, model, eandf
+
+
+@pytest.mark.parametrize("kind", ["nve", "nvt"])
+def test_device_md_follows_oracle(cuda, kind):
+    """DeviceMD on the FP32 path against the FP64 oracle's Velocity Verlet
+    (SimpleMD.py:14-38) / Nose-Hoover step (SimpleMD.py:104-129), 24 steps of
+    0.5 fs from the same state: positions within 2e-6 A, velocities within
+    1e-4 relative, eta within 1e-3 relative, and (NVE) the total energy
+    conserved as well as the oracle conserves it."""
+    from oracle import oracle as O
+    from tensormol_amd import params
+    from tensormol_amd import workloads as W
+    from tensormol_amd.engine import BPEEEngine
+    from tensormol_amd.md import DeviceMD
+    Z, x, ms, model, eandf = _gentle()
+    m = params.atomic_masses(Z)
+    v0 = W.maxwell_boltzmann(Z, 600.0, 5, m)
+    eng = BPEEEngine(model, precision="fp32")
+    ws = eng.prepare(Z, ms, None, pos0=x)
+    md = DeviceMD(eng, ws, Z, x, v0=v0, dt=0.5, kind=kind, T=300.0)
+    md.start()
+    e0, f0 = eandf(x)
+    xo, vo, ao, eta = x.copy(), v0.copy(), 1e-10 * f0 / m[:, None], 0.0
+    et_gpu, et_orc = [], []
+    nsteps = 24
+    for k in range(nsteps):
+        md.step()
+        if kind == "nve":
+            xo, vo, ao, eo = O.vv_step(xo, vo, ao, m, 0.5, eandf)
+        else:
+            xo, vo, ao, eo, eta = O.nose_step(xo, vo, ao, m, 0.5, eandf, eta, 300.0, 0.5)
+        pe, ke = md.energies()
+        et_gpu.append(pe[0] + ke[0])
+        et_orc.append(eo + O.kinetic_energy(vo, m) * len(m) / O.JOULEPERHARTREE)
+    torch.cuda.synchronize()
+    xg, vg = md.pos.cpu().numpy(), md.v.cpu().numpy()
+    dx = np.abs(xg - xo).max()
+    dv = np.abs(vg - vo).max() / np.abs(vo).max()
+    print(f"{kind}: |dx| {dx:.2e} A, |dv|/|v| {dv:.2e}")
+    assert dx < 2e-6 and dv < 1e-4
+    if kind == "nvt":
+        eg = float(md.eta.cpu()[0])
+        print(f"eta gpu {eg:.6e} oracle {eta:.6e}")
+        assert eta != 0.0 and abs(eg - eta) < 1e-3 * abs(eta)
+    else:
+        et_gpu, et_orc = np.array(et_gpu), np.array(et_orc)
+        print(f"Etot spread gpu {np.ptp(et_gpu):.2e} oracle {np.ptp(et_orc):.2e} Ha")
+        assert np.abs(et_gpu - et_orc).max() < 1e-4
+        assert np.ptp(et_gpu) < 1.5 * np.ptp(et_orc) + 1e-5
+
+
+def test_device_md_raises_on_row_overflow(cuda):
+    """A neighbour stride far too small for the system: the rows are clamped
+    by the kernels and flagged; DeviceMD.step() raises instead of carrying on
+    with truncated rows."""
+    from tensormol_amd.engine import BPEEEngine, NeighbourOverflow
+    from tensormol_amd.md import DeviceMD
+    Z, x, ms, model, _ = _gentle()
+    eng = BPEEEngine(model, precision="fp32", ldr=4, lda=4, check_overflow=False)
+    ws = eng.prepare(Z, ms, None, pos0=x)
+    md = DeviceMD(eng, ws, Z, x, dt=0.5, kind="nve")
+    md.start()
+    with pytest.raises(NeighbourOverflow):
+        md.step()
+    md2 = DeviceMD(eng, ws, Z, x, dt=0.5, kind="nve")
+    md2.start()
+    md2.run_steps(2, check=False)   # no check asked: no raise
+    with pytest.raises(NeighbourOverflow):
+        md2.check_flags()
