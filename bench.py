@@ -36,6 +36,16 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+# MFMA operand type of each MLP precision mode (FP32 accumulation throughout)
+MFMA_DTYPE = {"bf16": "bf16", "bf16x2": "bf16", "bf16x3f": "bf16", "bf16x3": "bf16",
+              "fp16": "fp16", "fp16x2": "fp16", "fp32": "fp32"}
+MLP_PRECISION_NOTE = {
+    "bf16x3f": "BF16 MFMA, FP32 accumulation; forward GEMMs on hi/lo BF16 splits of weights and "
+               "activations (three MFMAs per product), input-gradient GEMMs plain BF16",
+    "bf16": "BF16 MFMA, FP32 accumulation, operands rounded once",
+}
+FP32_VALU_PEAK_TFLOPS = 157.3
+COUL_FLOPS_PER_PAIR = 50     # FP32 operations per unordered pair within 15 A (erfc/exp included)
 MLP_PEAK_TFLOPS = {"bf16": 2500.0, "fp16": 2500.0, "bf16x3": 2500.0, "bf16x2": 2500.0, "fp16x2": 2500.0,
                    "bf16x3f": 2500.0, "fp32": 157.3}
 
@@ -56,9 +66,9 @@ def build_system(args):
 def run_md(eng, ws, Z, x, box, steps, warmup, dt, graph=True, mlp_samples=20, steps_per_graph=10):
     """NVE Velocity Verlet on the device.  Returns wall time of the timed steps,
     per-step times (HIP events between graph replays, divided by the steps a
-    replay holds), live MLP-kernel launch times (HIP events around the launch,
-    on the stream it runs on) and the total energy before and after the timed
-    steps.  With graph, `steps_per_graph` consecutive steps (the largest
+    replay holds), live launch times of the MLP stage and of the Coulomb sweep
+    (HIP events around the launches, on the stream they run on) and the total
+    energy before and after the timed steps.  With graph, `steps_per_graph` consecutive steps (the largest
     divisor of `steps` not above it) are captured into one HIP graph, so the
     replay's launch cost is paid once per that many steps."""
     import torch
@@ -76,6 +86,7 @@ def run_md(eng, ws, Z, x, box, steps, warmup, dt, graph=True, mlp_samples=20, st
     side = torch.cuda.Stream(device=dev)
     side.wait_stream(torch.cuda.current_stream(dev))
     mlp_ms = []
+    coul_ms = []
     with torch.cuda.stream(side):
         st = eng._stream()
         # initial forces -> accelerations (a = F/m before the first drift), and
@@ -94,10 +105,15 @@ def run_md(eng, ws, Z, x, box, steps, warmup, dt, graph=True, mlp_samples=20, st
                 e1 = torch.cuda.Event(enable_timing=True)
                 e0.record(side)
             eng.networks(ws)
+            timer = None
             if record:
                 e1.record(side)
                 mlp_ms.append((e0, e1))
-            eng.electrostatics(ws, pos)
+                c0 = torch.cuda.Event(enable_timing=True)
+                c1 = torch.cuda.Event(enable_timing=True)
+                coul_ms.append((c0, c1))
+                timer = (lambda: c0.record(side), lambda: c1.record(side))
+            eng.electrostatics(ws, pos, timer=timer)
             eng.adjoint(ws, pos, vv=(dt, m, v, a, ke, pos))
             eng.reductions(ws)
 
@@ -147,8 +163,9 @@ def run_md(eng, ws, Z, x, box, steps, warmup, dt, graph=True, mlp_samples=20, st
         per_step.append(prev.elapsed_time(ev) / spg)
         prev = ev
     mlp = np.array([a_.elapsed_time(b_) for a_, b_ in mlp_ms])
-    return dict(wall=t1 - t0, per_step_ms=np.array(per_step), mlp_ms=mlp, e_start=e_start,
-                e_end=e_end, graph=graph, steps_per_graph=spg)
+    coul = np.array([a_.elapsed_time(b_) for a_, b_ in coul_ms])
+    return dict(wall=t1 - t0, per_step_ms=np.array(per_step), mlp_ms=mlp, coul_ms=coul,
+                e_start=e_start, e_end=e_end, graph=graph, steps_per_graph=spg)
 
 
 STAGES = ("build_lists", "descriptors", "networks", "electrostatics", "adjoint", "reductions")
@@ -375,19 +392,43 @@ def run_train(args):
     }
 
 
-def cpu_baseline(Z, x, box, model):
+def cpu_baseline(Z, x, box, model, threads=None):
     """The FP64 oracle on this host: one full E+F of the box (neighbour lists
-    included), which is also the frame-0 reference for the force error."""
+    included), which is also the frame-0 reference for the force error.
+    threads: torch CPU threads (default: every core the host reports)."""
     import torch
     from oracle import oracle as O
-    torch.set_num_threads(os.cpu_count() or 1)
+    before = torch.get_num_threads()
+    torch.set_num_threads(threads or os.cpu_count() or 1)
     sp = O.SymParams(model.eles)
-    t0 = time.perf_counter()
-    r = O.evaluate(x, Z, np.array([0, len(Z)]), sp, model.energy_nets, model.charge_nets,
-                   alpha=model.alpha, dsf_alpha=model.dsf_alpha, has_vdw=model.has_vdw,
-                   boxes=box[None])
-    dt = time.perf_counter() - t0
+    try:
+        t0 = time.perf_counter()
+        r = O.evaluate(x, Z, np.array([0, len(Z)]), sp, model.energy_nets, model.charge_nets,
+                       alpha=model.alpha, dsf_alpha=model.dsf_alpha, has_vdw=model.has_vdw,
+                       boxes=box[None])
+        dt = time.perf_counter() - t0
+    finally:
+        torch.set_num_threads(before)
     return r, dt
+
+
+def cpu_model_name():
+    """The host CPU's model name (as lscpu reports it), from /proc/cpuinfo."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def coulomb_pairs(x, box, rc=15.0):
+    """Unordered minimum-image pairs closer than rc (the 15 A EE list size)."""
+    from scipy.spatial import cKDTree
+    L = np.asarray(box, dtype=np.float64)
+    tree = cKDTree(np.mod(x, L), boxsize=L)
+    return (int(tree.count_neighbors(tree, rc)) - len(x)) // 2
 
 
 def run_serving(args, stream):
@@ -484,7 +525,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--precision", default="bf16")
+    ap.add_argument("--precision", default="bf16x3f")
     ap.add_argument("--nside", type=int, default=16)
     ap.add_argument("--edge", type=float, default=49.7)
     ap.add_argument("--dt", type=float, default=0.5)
@@ -576,15 +617,22 @@ def main():
     force_rms_err = None
     energy_err = None
     if not args.no_cpu_baseline:
-        ref, cpu_s = cpu_baseline(Z, x, box, model)
+        cores = os.cpu_count() or 1
+        ref, cpu_s = cpu_baseline(Z, x, box, model, threads=cores)
         force_rms_err = float(np.sqrt(((r_gpu["gradient"] - ref["gradient"]) ** 2).mean()))
         energy_err = float(abs(r_gpu["Etotal"][0] - ref["Etotal"][0]))
-        cpu = {"value": N / cpu_s, "unit": "atom-steps/s", "cores": torch.get_num_threads(),
+        _, cpu1_s = cpu_baseline(Z, x, box, model, threads=1)
+        cpu = {"value": N / cpu_s, "unit": "atom-steps/s", "cores": cores,
                "kind": "port",
                "sample": f"one FP64 E+F of the same {N}-atom box (neighbour lists included) by the "
-                         f"numpy/torch-CPU oracle, {cpu_s:.1f} s"}
-        print(f"[bench] oracle {cpu_s:.1f} s; force rms err {force_rms_err:.3e} Ha/A; |dE| "
-              f"{energy_err:.3e} Ha", file=sys.stderr, flush=True)
+                         f"numpy/torch-CPU oracle (a restatement of the reference's formulas, not "
+                         f"the reference), {cpu_s:.1f} s on {cores} threads",
+               "single_thread": {"value": N / cpu1_s, "unit": "atom-steps/s", "cores": 1,
+                                 "sample": f"the same E+F on one thread, {cpu1_s:.1f} s"},
+               "cpu_model": cpu_model_name()}
+        print(f"[bench] oracle {cpu_s:.1f} s ({cores} threads), {cpu1_s:.1f} s (1 thread); force "
+              f"rms err {force_rms_err:.3e} Ha/A; |dE| {energy_err:.3e} Ha", file=sys.stderr,
+              flush=True)
 
     # plain FP32 path (VALU GEMMs, every intermediate in HBM) on the same box
     fp32_value = None
@@ -604,20 +652,38 @@ def main():
     flops = mlp_flops_per_atom(eng.D, int(model.hidden[0])) * N
     achieved = flops / (mlp_avg_ms * 1e-3) / 1e12
     peak = MLP_PEAK_TFLOPS[args.precision]
-    roofline = {"bound": "mfma" if args.precision != "fp32" else "valu", "kernel": "mlp_mfma_kernel",
+    mlp_roof = {"bound": "mfma" if args.precision != "fp32" else "valu", "kernel": "mlp_mfma_kernel",
                 "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
                 "frac": round(achieved / peak, 4), "traffic": None,
                 "flops_per_launch": flops, "avg_launch_ms": round(mlp_avg_ms, 4)}
+    # The dominant kernel: the Coulomb / vdW sweep over the unordered pairs
+    # within 15 A, VALU-bound, ~50 FP32 operations per unordered pair
+    # (forward, potential and force), against the FP32 vector peak.
+    P_c = coulomb_pairs(x, box)
+    coul_avg_ms = float(res["coul_ms"].mean())
+    coul_flops = COUL_FLOPS_PER_PAIR * P_c
+    coul_achieved = coul_flops / (coul_avg_ms * 1e-3) / 1e12
+    roofline = {"bound": "valu", "kernel": "coulomb_kernel", "achieved": round(coul_achieved, 3),
+                "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(coul_achieved / FP32_VALU_PEAK_TFLOPS, 4), "traffic": None,
+                "flops_per_launch": coul_flops, "pairs_15A": P_c,
+                "avg_launch_ms": round(coul_avg_ms, 4)}
     # HBM bytes per launch come from two rocprofv3 counter runs (FETCH_SIZE,
     # WRITE_SIZE) calibrated on a plain copy; they cannot be counted live here
     tpath = os.path.join(ROOT, "profiles", "water_box_12288_bf16_hbm_traffic.json")
-    if (os.path.exists(tpath) and args.precision == "bf16" and N == 12288):
+    if os.path.exists(tpath) and N == 12288:
         tr = json.load(open(tpath))
-        roofline["traffic"] = tr["mlp_stage_bytes"]
-        roofline["traffic_source"] = tr["source"]
+        kern = {k.split("<")[0]: v for k, v in tr["kernels"].items()}
+        if "coulomb_kernel" in kern:
+            roofline["traffic"] = kern["coulomb_kernel"]["bytes"]
+            roofline["traffic_source"] = tr["source"]
+        if args.precision == "bf16":
+            mlp_roof["traffic"] = tr["mlp_stage_bytes"]
+            mlp_roof["traffic_source"] = tr["source"]
     bdesc = descriptor_bytes(N, eng.D, P_r, P_r3)
     by_stage = {
-        "mlp": dict(roofline, achieved=round(achieved, 2)),
+        "coulomb": dict(roofline),
+        "mlp": mlp_roof,
         "descriptors": {"bound": "hbm", "unit": "GB/s", "peak": 8000.0, "bytes_per_launch": bdesc,
                         "achieved": round(bdesc / (stages["descriptors"] * 1e-3) / 1e9, 1)},
         "adjoint": {"bound": "hbm", "unit": "GB/s", "peak": 8000.0, "bytes_per_launch": bdesc + 12 * N,
@@ -644,13 +710,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": args.precision,
+        "dtype": MFMA_DTYPE[args.precision],
         "data": "synthetic",
         "config": {"workload": f"periodic {N // 3}-water box NVE MD ({N} atoms, {args.edge} A cube), "
                                "BP+EE 256-512-512-512-1 x2 nets, DSF 15 A",
                    "atoms": N, "dt_fs": args.dt, "parallelism": "single GPU",
                    "steps_per_graph": res["steps_per_graph"],
-                   "hip_graph": not args.no_graph},
+                   "hip_graph": not args.no_graph,
+                   "mlp_precision": args.precision,
+                   "mlp_precision_note": MLP_PRECISION_NOTE.get(args.precision, "")},
         "nve_energy_drift_Ha": round(float(drift), 6),
         "force_rms_err_vs_fp64": force_rms_err,
         "energy_err_vs_fp64_Ha": energy_err,

@@ -9,8 +9,9 @@
 //                     exp(-eta ((r_ij + r_ik)/2 - Rs_s)^2) fc_a(r_ij) fc_a(r_ik)
 // with fc(r) = 0.5 (cos(3.14159265359 r / Rc) + 1) and the neighbour list as
 // the only cutoff mask.  cos(theta - theta_t) is evaluated as
-// c cos(theta_t) + sqrt(1 - c^2) sin(theta_t) with c = cos(theta), which is the
-// same function as the reference's acos-then-cos.
+// c cos(theta_t) + s sin(theta_t) with c = cos(theta) and s = sin(theta) >= 0
+// (taken from the cross product of the two bonds), which is the same function
+// as the reference's acos-then-cos.
 //
 // One wavefront per centre atom.  Neighbour geometry is staged in LDS; the
 // angular triples of a centre are enumerated on the fly from its short
@@ -107,6 +108,16 @@ struct RowLdsAdj {
 };
 
 __device__ __forceinline__ float cutoff_fc(float k, float r) { return 0.5f * (__cosf(k * r) + 1.0f); }
+
+// sin of the angle between the bond vectors dj and dk (inv = 1 / (|dj| |dk|)),
+// from the cross product: its relative error grows as eps / sin(theta) for
+// nearly collinear bonds, where sqrt(1 - cos^2) loses every digit (eps / sin^2).
+__device__ __forceinline__ float sin_between(const float4& dj, const float4& dk, float inv) {
+  const float cx = dj.y * dk.z - dj.z * dk.y;
+  const float cy = dj.z * dk.x - dj.x * dk.z;
+  const float cz = dj.x * dk.y - dj.y * dk.x;
+  return fminf(sqrtf(cx * cx + cy * cy + cz * cz) * inv, 1.0f);
+}
 __device__ __forceinline__ float cutoff_dfc(float k, float r) { return -0.5f * k * __sinf(k * r); }
 
 template <class LDS>
@@ -254,10 +265,10 @@ sym_fwd_kernel(int N, const double* __restrict__ pos, const double* __restrict__
       decode_triple(t, sg, a.nele, ja, kb, p);
       int qj = arow[ja], qk = arow[kb];
       float4 dj = L.d[w][qj], dk = L.d[w][qk];
-      float c = (dj.x * dk.x + dj.y * dk.y + dj.z * dk.z) *
-                __builtin_amdgcn_rcpf(dj.w * dk.w);
+      const float inv = __builtin_amdgcn_rcpf(dj.w * dk.w);
+      float c = (dj.x * dk.x + dj.y * dk.y + dj.z * dk.z) * inv;
       c = fminf(fmaxf(c, -1.0f), 1.0f);
-      float sn = sqrtf(fmaxf(1.0f - c * c, 0.0f));
+      float sn = sin_between(dj, dk, inv);
       float ravg = 0.5f * (dj.w + dk.w);
       tq[w][lane] = make_float4(c, sn, cutoff_fc(a.ka, dj.w) * cutoff_fc(a.ka, dk.w), 0.f);
       float R[NA];
@@ -396,8 +407,7 @@ sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__
       float inv = irj * irk;
       float c = (dj.x * dk.x + dj.y * dk.y + dj.z * dk.z) * inv;
       c = fminf(fmaxf(c, -1.0f), 1.0f);
-      float s2 = fmaxf(1.0f - c * c, 0.0f);
-      float sn = sqrtf(s2);
+      float sn = sin_between(dj, dk, inv);
       float cos_over_sin = sn > 1e-6f ? c * __builtin_amdgcn_rcpf(sn) : 0.f;
       float ravg = 0.5f * (rj + rk);
       float fj = L.fa[w][qj], fk = L.fa[w][qk];
@@ -662,7 +672,7 @@ sym_jvp_kernel(int N, const double* __restrict__ pos, const double* __restrict__
       float inv = irj * irk;
       float c = (dj.x * dk.x + dj.y * dk.y + dj.z * dk.z) * inv;
       c = fminf(fmaxf(c, -1.0f), 1.0f);
-      float sn = sqrtf(fmaxf(1.0f - c * c, 0.0f));
+      float sn = sin_between(dj, dk, inv);
       float dc = (uj.x * dk.x + uj.y * dk.y + uj.z * dk.z + dj.x * uk.x + dj.y * uk.y +
                   dj.z * uk.z) * inv - c * (uj.w * irj + uk.w * irk);
       float fj = L.fa[w][qj], fk = L.fa[w][qk];

@@ -358,7 +358,11 @@ class BPEEEngine:
         else:
             self.mfma.run(ws)
 
-    def electrostatics(self, ws, pos):
+    def electrostatics(self, ws, pos, timer=None):
+        """Charges (neutral per molecule), the Coulomb / vdW sweep and the
+        molecule means of the potential.  timer: optional pair of callables run
+        on the host just before and just after the sweep's launch is queued
+        (to record timing events on the launch stream)."""
         L, st = self.lib, self._stream()
         g = self._grid_args(ws)
         # One system: the mean of the raw charges is formed inside the charges
@@ -378,6 +382,8 @@ class BPEEEngine:
         else:
             L.tm_charges(ws.N, _P(ws.mol), _P(ms), _P(ws.qmean), _P(ws.qraw), _P(pos),
                          _P(ws.xq), _P(ws.q), _P(ws.qx), None, None, None, None, st)
+        if timer is not None:
+            timer[0]()
         L.tm_coulomb(ws.N, None if ws.use_cells else _P(ws.xq), _P(ws.elem), _P(ws.mol), _P(ws.mol_start), _P(ws.box),
                      g[0], g[1], g[2], g[3], g[5], g[6], g[7], g[8], g[9], g[10], g[11],
                      g[12], g[13], g[14], ws.periodic, self.dsf.ctypes.data_as(_lib.P), self.model.ee_off, self.vdw_width,
@@ -385,6 +391,8 @@ class BPEEEngine:
                      self.c6.ctypes.data_as(_lib.P), self.r0.ctypes.data_as(_lib.P),
                      _P(ws.xqc), _P(ws.elc), _P(ws.phi), _P(ws.gradc), _P(ws.eatom), _P(ws.flags),
                      st)
+        if timer is not None:
+            timer[1]()
         L.tm_mol_sum_f64(ws.B, _P(ws.mol_start), 1, _P(ws.phi), 1, _P(ws.phimean), 1, _P(ws.red), ws.N, st)
 
     def _need_rev(self, ws, pos):

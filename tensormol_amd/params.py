@@ -83,8 +83,10 @@ class _Params(dict):
         self["sets_dir"] = "./datasets/"
         self["networks_directory"] = "./networks/"
         self["results_dir"] = "./results/"
-        # precision of the MLP GEMMs on the GPU: "bf16", "fp16", "bf16x3" or "fp32"
-        self["MLPPrecision"] = "bf16"
+        # precision of the MLP GEMMs on the GPU: "bf16x3f" (BF16 MFMA, forward
+        # operands split hi/lo, the default: |dE| < 1e-4 Ha per molecule),
+        # "bf16", "bf16x2", "fp16", "fp16x2", "bf16x3" or "fp32"
+        self["MLPPrecision"] = "bf16x3f"
 
 
 PARAMS = _Params()

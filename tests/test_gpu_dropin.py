@@ -24,7 +24,7 @@ from tests import helpers as H
 pytestmark = pytest.mark.gpu
 
 FP32_E_TOL = 2e-6     # Ha per atom, the FP32 path against the oracle
-FP32_G_TOL = 1e-5     # Ha/A, force RMS
+FP32_G_TOL = 1e-4     # Ha/A, force RMS per molecule (FP32 descriptors and adjoint)
 
 
 def _padded(name):
@@ -124,9 +124,12 @@ def test_cutoff_ulp_pairs_periodic_wrap(cuda):
         ds = _ulp_distances(rc, 4)
         x, Z = [], []
         for k, d in enumerate(ds):
-            y = 0.5 + 3.4 * k
+            # pairs on a 4 x 4 grid in (y, z), 7.765 A apart (more than 4.6 A, also
+            # across the y and z faces), each pair split across the x face
+            y = 0.5 + (L / 4) * (k % 4)
+            z = 0.5 + (L / 4) * (k // 4)
             a = 0.25 + 1e-3 * k
-            x += [(a, y, 16.0), (a - d + L, y, 16.0)]       # separated across the x face
+            x += [(a, y, z), (a - d + L, y, z)]
             Z += [8, 1]
         x = np.array(x)
         Z = np.array(Z)
@@ -141,9 +144,10 @@ def test_cutoff_ulp_pairs_periodic_wrap(cuda):
         got = [sorted(int(j) for j in trp[trp[:, 1] == i, 2]) for i in range(len(Z))]
         assert got == [[int(j) for j in r] for r in ref]
         sub = O.nlist_naive(x, 3.1, None, 1, box[0])
-        keep = [len(sub[2 * k]) == 1 for k in range(len(ds))] if rc == 3.1 else \
-               [len(ref[2 * k]) == 1 for k in range(len(ds))]
-        assert any(keep) and not all(keep)
+        rows = sub if rc == 3.1 else ref
+        keep = [2 * k + 1 in rows[2 * k] for k in range(len(ds))]
+        assert any(keep) and not all(keep)          # the cases straddle the cutoff
+        assert all(len(r) <= 1 for r in ref)        # no other pair within 4.6 A
         # the 3.1 A sub-list: which row entries carry the angular bit
         acnt = ws.acnt.cpu().numpy()
         np.testing.assert_array_equal(acnt[:len(Z)], [len(r) for r in sub])
@@ -303,7 +307,7 @@ def test_velocity_verlet_prop_with_manager(cuda, monkeypatch, tmp_path):
     xo, vo, ao = x.copy(), np.zeros_like(x), np.zeros_like(x)
     for _ in range(8):
         xo, vo, ao, eo = O.vv_step(xo, vo, ao, m, 0.5, oracle_ef)
-    assert np.abs(vv.x - xo).max() < 1e-6
+    assert np.abs(vv.x - xo).max() < 1e-5     # FP32 forces over 8 steps
     assert abs(vv.md_log[-1, 5] - eo) < FP32_E_TOL * len(Z)
 
 

@@ -2,10 +2,18 @@
 end-to-end reduced-precision E+F vs the FP64 oracle.
 
 Tolerances (relative to the RMS of the reference quantity):
-  bf16 (8-bit mantissa operands): y 2e-2, dy/dG 3e-2
-  fp16 (11-bit):                  y 3e-3, dy/dG 5e-3
-  bf16x3 (hi/lo split):           y 2e-4, dy/dG 5e-4
-End-to-end gate of the benchmark: force RMS error < 1e-3 Ha/A vs the FP64 oracle.
+  bf16 (8-bit mantissa operands):                 y 2e-2, dy/dG 3e-2
+  fp16 (11-bit):                                  y 3e-3, dy/dG 5e-3
+  bf16x2 (forward weights split, 16 bits):        y 1e-2, dy/dG 3e-2
+  fp16x2 (forward weights split, 22 bits):        y 2e-3, dy/dG 5e-3
+  bf16x3f (forward weights and activations split): y 2e-4, dy/dG 3e-2
+  bf16x3 (hi/lo split everywhere):                y 2e-4, dy/dG 5e-4
+End-to-end gates: force RMS error < 1e-3 Ha/A and |dE| < 1e-4 Ha per molecule
+(per water for the water systems, which are clusters / boxes of waters) vs
+the FP64 oracle.  The default precision bf16x3f meets both on the water systems, the
+chain batch and the peptide (see PER_ATOM_ONLY for the rest); plain bf16 / fp16 round the weights once, which shifts every atom's
+energy the same way, so their energy error grows with the system and is
+reported, not gated.
 """
 import numpy as np
 import pytest
@@ -15,7 +23,15 @@ from tests import helpers as H
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"bf16": (2e-2, 3e-2), "fp16": (3e-3, 5e-3), "bf16x3": (2e-4, 5e-4)}
+TOL = {"bf16": (2e-2, 3e-2), "fp16": (3e-3, 5e-3), "bf16x2": (1e-2, 3e-2), "fp16x2": (2e-3, 5e-3),
+       "bf16x3f": (2e-4, 3e-2), "bf16x3": (2e-4, 5e-4)}
+ENERGY_GRADE = ("bf16x3f", "bf16x3")     # |dE| < 1e-4 Ha per molecule (see below)
+# Gated per atom (< 5e-6 Ha) instead: chno_small is one random 1,200-atom
+# cluster, not molecules; johnsonmols holds organic molecules of up to 70
+# atoms, where the two-term BF16 weights (2^-18 relative) leave ~3e-6 Ha per
+# atom, up to 2e-4 Ha on the largest molecule (a three-term weight split would
+# bring it to 9e-5).
+PER_ATOM_ONLY = ("chno_small", "johnsonmols")
 
 
 def torch_mlp_ref(G, layers, alpha):
@@ -32,7 +48,7 @@ def torch_mlp_ref(G, layers, alpha):
     return y.detach(), J
 
 
-@pytest.mark.parametrize("precision", ["bf16", "fp16", "bf16x3"])
+@pytest.mark.parametrize("precision", ["bf16", "fp16", "bf16x2", "fp16x2", "bf16x3f", "bf16x3"])
 @pytest.mark.parametrize("name,modelname", [("water64", "water"), ("chains", "chno")])
 def test_mfma_mlp_vs_torch_fp32(cuda, precision, name, modelname):
     from tensormol_amd.engine import BPEEEngine
@@ -64,10 +80,11 @@ def test_mfma_mlp_vs_torch_fp32(cuda, precision, name, modelname):
         assert ey < ty and ej < tj
 
 
-@pytest.mark.parametrize("precision", ["bf16", "fp16", "bf16x3"])
-@pytest.mark.parametrize("name,modelname", [("water64", "water"), ("chains", "chno"),
-                                            ("waterbox10", "water"), ("chno_small", "chno"),
-                                            ("johnsonmols", "chno"), ("peptide188", "chno")])
+@pytest.mark.parametrize("precision", ["bf16", "fp16", "bf16x2", "fp16x2", "bf16x3f", "bf16x3"])
+@pytest.mark.parametrize("name,modelname", [("water64", "water"), ("water10", "water"),
+                                            ("waterbox10", "water"), ("chains", "chno"),
+                                            ("chno_small", "chno"), ("johnsonmols", "chno"),
+                                            ("peptide188", "chno")])
 def test_energy_forces_mfma(cuda, precision, name, modelname):
     from tensormol_amd.engine import BPEEEngine
     eng = BPEEEngine(H.model_for(modelname), precision=precision)
@@ -80,15 +97,23 @@ def test_energy_forces_mfma(cuda, precision, name, modelname):
     ref = H.oracle_eval(name, modelname)
     natom = np.diff(ms)
     dE = np.abs(r["Etotal"] - ref["Etotal"])
+    # molecules: waters for the water systems, else the batch entries
+    nmol = natom / 3 if modelname == "water" else np.ones_like(natom)
+    per_mol = dE / nmol
     frms = np.sqrt(((r["gradient"] - ref["gradient"]) ** 2).mean())
-    print(f"{precision} {name}: |dE|/atom max {np.max(dE / natom):.2e} Ha, |dE| max {dE.max():.2e} Ha, "
-          f"force rms err {frms:.2e} Ha/A")
+    print(f"{precision} {name}: |dE| per molecule max {per_mol.max():.2e} Ha, per atom max "
+          f"{np.max(dE / natom):.2e} Ha, force rms err {frms:.2e} Ha/A")
     assert frms < 1e-3
-    if precision == "bf16x3":
-        assert np.all(dE / natom < 2e-5)
+    if precision in ENERGY_GRADE:
+        if name in PER_ATOM_ONLY:
+            assert np.all(dE / natom < 5e-6), np.max(dE / natom)
+        else:
+            assert np.all(per_mol < 1e-4), per_mol.max()
+    if precision == "fp16x2" and modelname == "water":
+        assert np.all(per_mol < 1e-4), per_mol.max()
 
 
-@pytest.mark.parametrize("precision", ["bf16", "fp16"])
+@pytest.mark.parametrize("precision", ["bf16", "fp16", "bf16x2", "fp16x2"])
 def test_tile_split_bit_identical(cuda, precision):
     """Whole rounds of 64-row tiles plus 32-row tiles for the rest give the
     same outputs, bit for bit, as 32-row tiles only: every output row's
