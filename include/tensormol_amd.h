@@ -236,6 +236,19 @@ int tm_mol_sums(int B, const int* mol_start, int n_total, int nsrc, const void* 
                 const int* f64, const int* cols, const int* ld, double* const* out,
                 double* scratch, void* stream);
 
+/* tm_mol_sums with no scratch argument and no allocation: the many-block
+ * path's partials and counter live in a library-owned static device array
+ * (512 KiB per GPU, held until the process ends), set up on the first call on
+ * each device.  No counterpart in the reference: it is for callers that
+ * cannot allocate (HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  n_total above 4,194,304 atoms returns TM_ERR_CAPACITY.  Every
+ * call on a GPU shares that GPU's array, so calls that overlap on one GPU
+ * (other streams, other threads) race; results are bit-identical to
+ * tm_mol_sums with a workspace. */
+int tm_mol_sums_static(int B, const int* mol_start, int n_total, int nsrc, const void* const* in,
+                       const int* f64, const int* cols, const int* ld, double* const* out,
+                       void* stream);
+
 /* ---- integrators (TensorMol/Simulations/SimpleMD.py) --------------------
  * VelocityVerletStep (SimpleMD.py:14-38) split into drift and kick;
  * NoseThermostat.step (SimpleMD.py:104-129) with one eta per molecule;

@@ -36,6 +36,7 @@ SIGNATURES = {
     "tm_mol_sum_f32": [I, P, I, P, I, P, I, P, I, P],
     "tm_mol_sum_f64": [I, P, I, P, I, P, I, P, I, P],
     "tm_mol_sums": [I, P, I, I, P, P, P, P, P, P, P],
+    "tm_mol_sums_static": [I, P, I, I, P, P, P, P, P, P],
     "tm_charges": [I, P, P, P, P, P, P, P, P, P, P, P, P, P],
     "tm_charge_upstream": [I, P, P, P, P, P],
     "tm_coulomb": [I, P, P, P, P, P, P, P, P, P, P, I, I, I, D, D, D, D, D, D, I, P, D, D, I, I, I, P, P, P, P,

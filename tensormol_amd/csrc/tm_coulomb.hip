@@ -18,6 +18,8 @@
 // Neighbors.py:262-282), evaluated on the fly from the cell grid; no pair
 // list is stored.  One wavefront per centre gathers its own terms, so every
 // per-atom sum is formed in a fixed order (no atomics) and accumulated in FP64.
+#include <mutex>
+
 #include "tm_cells.h"
 
 namespace {
@@ -289,19 +291,19 @@ mol_sum_kernel(const int* __restrict__ mol_start, int C, const T* __restrict__ i
 
 // One large molecule in one launch: block k sums atoms [1024 k, 1024 (k + 1))
 // for every column into partial[k][c]; the last block to finish (completion
-// counter in scratch, agent-scope acquire/release) sums the partials in block
+// counter in work, agent-scope acquire/release) sums the partials in block
 // order and resets the counter to zero.  Deterministic whatever the block
 // that finishes last.
 template <class T>
 __global__ void __launch_bounds__(256)
-sum_last_block_kernel(int n, int C, const T* __restrict__ in, int ldin, double* scratch,
+sum_last_block_kernel(int n, int C, const T* __restrict__ in, int ldin, double* work,
                       double* __restrict__ out, int scale_by_count) {
   constexpr int CB = 4;                    // columns per pass
   __shared__ double part[CB][256 / 64];
   __shared__ double stage[256];
   __shared__ int last;
-  unsigned int* counter = reinterpret_cast<unsigned int*>(scratch);
-  double* partial = scratch + 1;
+  unsigned int* counter = reinterpret_cast<unsigned int*>(work);
+  double* partial = work + 1;
   const int k = blockIdx.x;
   const int lo = k * 1024, hi = min(n, lo + 1024);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -384,12 +386,12 @@ __device__ __forceinline__ double src_val(const SumSrc& S, int k, long a, int c)
 // the last block to finish sums the partials in block order (as
 // sum_last_block_kernel) and resets the counter
 __global__ void __launch_bounds__(256)
-sums_last_block_kernel(int n, SumSrc S, double* scratch) {
+sums_last_block_kernel(int n, SumSrc S, double* work) {
   __shared__ double part[SUMS_MAX_COL][4];
   __shared__ double stage[256];
   __shared__ int last;
-  unsigned int* counter = reinterpret_cast<unsigned int*>(scratch);
-  double* partial = scratch + 1;
+  unsigned int* counter = reinterpret_cast<unsigned int*>(work);
+  double* partial = work + 1;
   const int k = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int lo = k * 1024, hi = min(n, lo + 1024);
   int cg = 0;
@@ -466,15 +468,15 @@ sums_per_mol_kernel(const int* __restrict__ mol_start, SumSrc S) {
 
 template <class T>
 int mol_sum(int B, const int* mol_start, int C, const T* in, int ldin, double* out, int mean,
-            double* scratch, int n_total, hipStream_t s) {
-  if (B == 1 && n_total > 2048 && scratch != nullptr) {
+            double* work, int n_total, hipStream_t s) {
+  if (B == 1 && n_total > 2048 && work != nullptr) {
     sum_last_block_kernel<T><<<(n_total + 1023) / 1024, 256, 0, s>>>(n_total, C, in, ldin,
-                                                                     scratch, out, mean);
+                                                                     work, out, mean);
     TM_LAUNCH_CHECK();
     return TM_OK;
   }
   if (B == 1 && n_total > 2048) {
-    // one system without scratch: a single 1024-thread block, one launch
+    // one system without work: a single 1024-thread block, one launch
     mol_sum_kernel<T, 1024><<<1, 1024, 0, s>>>(mol_start, C, in, ldin, out, mean);
     TM_LAUNCH_CHECK();
     return TM_OK;
@@ -591,29 +593,34 @@ static int coulomb_range_bound(double rc, const double cs[3], const int nc[3], i
 extern "C" {
 
 // One large molecule (B == 1, n_total = N > 2048) is summed by ceil(N/1024)
-// blocks in one launch when scratch is given (1 + ceil(N/1024)*C doubles, its
+// blocks in one launch when work is given (1 + ceil(N/1024)*C doubles, its
 // first word a completion counter that must be zero before the first call and
 // is zero again after each), else by a single 1024-thread block.
 int tm_mol_sum_f32(int B, const int* mol_start, int C, const float* in, int ldin, double* out,
-                   int mean, double* scratch, int n_total, void* stream) {
+                   int mean, double* work, int n_total, void* stream) {
   TM_REQUIRE(B >= 0 && C > 0 && C <= 64, "tm_mol_sum_f32: bad sizes");
   if (B == 0) return TM_OK;
-  return mol_sum<float>(B, mol_start, C, in, ldin, out, mean, scratch, n_total, (hipStream_t)stream);
+  return mol_sum<float>(B, mol_start, C, in, ldin, out, mean, work, n_total, (hipStream_t)stream);
 }
 
 int tm_mol_sum_f64(int B, const int* mol_start, int C, const double* in, int ldin, double* out,
-                   int mean, double* scratch, int n_total, void* stream) {
+                   int mean, double* work, int n_total, void* stream) {
   TM_REQUIRE(B >= 0 && C > 0 && C <= 64, "tm_mol_sum_f64: bad sizes");
   if (B == 0) return TM_OK;
-  return mol_sum<double>(B, mol_start, C, in, ldin, out, mean, scratch, n_total,
+  return mol_sum<double>(B, mol_start, C, in, ldin, out, mean, work, n_total,
                          (hipStream_t)stream);
 }
 
-int tm_mol_sums(int B, const int* mol_start, int n_total, int nsrc, const void* const* in,
-                const int* f64, const int* cols, const int* ld, double* const* out,
-                double* scratch, void* stream) {
-  TM_REQUIRE(B >= 0 && n_total >= 0 && nsrc > 0 && nsrc <= SUMS_MAX_SRC,
-             "tm_mol_sums: bad sizes");
+}  // extern "C"
+
+namespace {
+
+// The per-molecule sums of tm_mol_sums; `work` (may be null) is the block
+// partials and completion counter of the many-block path for one system.
+int mol_sums_impl(const char* name, int B, const int* mol_start, int n_total, int nsrc,
+                  const void* const* in, const int* f64, const int* cols, const int* ld,
+                  double* const* out, double* work, void* stream) {
+  TM_REQUIRE(B >= 0 && n_total >= 0 && nsrc > 0 && nsrc <= SUMS_MAX_SRC, "%s: bad sizes", name);
   SumSrc S;
   S.nsrc = nsrc;
   S.ctot = 0;
@@ -626,15 +633,15 @@ int tm_mol_sums(int B, const int* mol_start, int n_total, int nsrc, const void* 
     S.ld[k] = on ? ld[k] : 0;
     if (on) {
       TM_REQUIRE(cols[k] > 0 && ld[k] >= cols[k] && in[k] != nullptr && out[k] != nullptr,
-                 "tm_mol_sums: bad source %d", k);
+                 "%s: bad source %d", name, k);
       S.ctot += cols[k];
     }
   }
-  TM_REQUIRE(S.ctot <= SUMS_MAX_COL, "tm_mol_sums: %d columns above %d", S.ctot, SUMS_MAX_COL);
+  TM_REQUIRE(S.ctot <= SUMS_MAX_COL, "%s: %d columns above %d", name, S.ctot, SUMS_MAX_COL);
   if (B == 0) return TM_OK;
   hipStream_t s = (hipStream_t)stream;
-  if (B == 1 && n_total > 2048 && scratch != nullptr) {
-    sums_last_block_kernel<<<(n_total + 1023) / 1024, 256, 0, s>>>(n_total, S, scratch);
+  if (B == 1 && n_total > 2048 && work != nullptr) {
+    sums_last_block_kernel<<<(n_total + 1023) / 1024, 256, 0, s>>>(n_total, S, work);
   } else if (B == 1 && n_total > 2048) {
     sums_per_mol_kernel<1024><<<1, 1024, 0, s>>>(mol_start, S);
   } else {
@@ -642,6 +649,68 @@ int tm_mol_sums(int B, const int* mol_start, int n_total, int nsrc, const void* 
   }
   TM_LAUNCH_CHECK();
   return TM_OK;
+}
+
+// ---- the entry point that allocates nothing --------------------------------
+// The many-block path's partials and counter live in this file-scope array
+// instead of a caller's workspace: room for one system of up to
+// STATIC_SUM_MAX_ATOMS atoms and SUMS_MAX_COL columns (512 KiB per GPU, held
+// until the process ends).  Word 0 is the completion counter, cleared once per
+// device at set-up and left at zero by every call (the last block resets it).
+constexpr int STATIC_SUM_MAX_ATOMS = 1 << 22;
+constexpr int STATIC_SUM_WORDS = 1 + (STATIC_SUM_MAX_ATOMS / 1024) * SUMS_MAX_COL;
+static __device__ __align__(256) double scratch[STATIC_SUM_WORDS];
+
+constexpr int STATIC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[STATIC_MAX_DEVICES] = {};   // device address of `scratch`, per device
+
+__global__ void clear_counter_kernel(double* p) { p[0] = 0.0; }
+
+// Device address of this GPU's `scratch` (set up on the first call on a device:
+// hipGetSymbolAddress with that device current, then the counter cleared by a
+// kernel on the caller's stream).  The array's name is a host-side stand-in and
+// is never passed to a kernel.
+int static_scratch(hipStream_t s, double** out) {
+  int dev = 0;
+  TM_HIP_CHECK(hipGetDevice(&dev));
+  TM_REQUIRE(dev >= 0 && dev < STATIC_MAX_DEVICES, "tm_mol_sums_static: device %d", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    TM_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    clear_counter_kernel<<<1, 1, 0, s>>>(static_cast<double*>(p));
+    TM_LAUNCH_CHECK();
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return TM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tm_mol_sums(int B, const int* mol_start, int n_total, int nsrc, const void* const* in,
+                const int* f64, const int* cols, const int* ld, double* const* out,
+                double* work, void* stream) {
+  return mol_sums_impl("tm_mol_sums", B, mol_start, n_total, nsrc, in, f64, cols, ld, out, work,
+                       stream);
+}
+
+int tm_mol_sums_static(int B, const int* mol_start, int n_total, int nsrc, const void* const* in,
+                       const int* f64, const int* cols, const int* ld, double* const* out,
+                       void* stream) {
+  if (n_total > STATIC_SUM_MAX_ATOMS) {
+    tm_set_error("tm_mol_sums_static: %d atoms above the %d its static scratch holds", n_total,
+                 STATIC_SUM_MAX_ATOMS);
+    return TM_ERR_CAPACITY;
+  }
+  double* work = nullptr;
+  int st = static_scratch((hipStream_t)stream, &work);
+  if (st) return st;
+  return mol_sums_impl("tm_mol_sums_static", B, mol_start, n_total, nsrc, in, f64, cols, ld, out,
+                       work, stream);
 }
 
 int tm_charges(int N, const int* mol, const int* mol_start, double* qmean, const float* qraw,
