@@ -19,9 +19,16 @@ swapped index fails):
   bp_energy (MLP)             torch.nn.functional.linear + softplus
   dsf kernel                  C1 continuity at R_sr, zero value and slope at
                               R_lr, scipy erfc closed form
-  vdw kernel                  closed form on one pair
   forces                      central finite differences; sum F = 0; torque 0
-  md steps                    energy conservation, time reversibility
+  vdw kernel                  large-r limit -sqrt(C6i C6j)/r^6, zero at r -> 0,
+                              C1 switch ending at r_A = Poly_Width/BOHRPERA
+                              (the double A->bohr scaling), i/j symmetry
+  md steps (vv_step)          energy conservation, time reversibility
+  nose_step                   Q -> infinity reduces to vv_step; the extended
+                              energy's drift vanishes with dt (tests/test_drivers.py)
+  fire_step                   convergence on a harmonic bowl, the P <= 0 reset,
+                              the dt growth after nmin downhill steps, the
+                              per-atom step cap
   descriptor_jvp              central finite differences of the descriptors
   force_matching_loss         central finite differences in the weights;
                               zero at the teacher's own weights
@@ -531,3 +538,37 @@ def nose_step(x, v, a, m, dt, eandf, eta, T, mddt):
     eta = etadto2 + (dt / (2.0 * Q)) * (kedto2 - ((3.0 * N + 1) / 2.0) * kT)
     v = (vdto2 + (dt / 2.0) * a_new) / (1 + (dt / 2.0) * eta)
     return x, v, a_new, e, eta
+
+
+# ----------------------------------------------------------------- FIRE
+def fire_step(x, v, f, m, state, dtmax, nmin=5, finc=1.1, fdec=0.5, alpha0=0.1, falpha=0.99,
+              maxstep=0.1):
+    """One FIRE iteration for one molecule (Bitzek, Koskinen, Gahler, Moseler,
+    Gumbsch, PRL 97, 170201 (2006)); the reference has no FIRE.  Units as the
+    MD drivers: x A, v A/fs, f J/mol/A, m kg/mol.  state = [dt, alpha, n_pos].
+
+    With the power P = F.v: if P > 0 the velocity is mixed towards the force,
+    v <- (1 - alpha) v + alpha |v| F/|F| (norms over the whole molecule), and
+    after more than nmin such steps in a row dt grows by finc (up to dtmax) and
+    alpha shrinks by falpha; otherwise dt shrinks by fdec, alpha resets and v
+    is zeroed.  Then v <- v + dt a (a = 1e-10 F/m) and x <- x + dt v with each
+    atom's displacement capped at maxstep A.  Returns (x, v, state)."""
+    dt, alpha, npos = state
+    P = float(np.sum(f * v))
+    if P > 0.0:
+        fn, vn = float(np.linalg.norm(f)), float(np.linalg.norm(v))
+        v = (1.0 - alpha) * v + (alpha * vn / fn if fn > 0.0 else 0.0) * f
+        npos += 1
+        if npos > nmin:
+            dt = min(dt * finc, dtmax)
+            alpha *= falpha
+    else:
+        dt *= fdec
+        alpha = alpha0
+        npos = 0
+        v = np.zeros_like(v)
+    v = v + dt * 1e-10 * f / m[:, None]
+    dx = dt * v
+    n = np.linalg.norm(dx, axis=1)
+    scale = np.where(n > maxstep, maxstep / np.where(n > 0, n, 1.0), 1.0)
+    return x + dx * scale[:, None], v, [dt, alpha, npos]

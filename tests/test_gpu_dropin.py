@@ -332,3 +332,119 @@ def test_geom_optimizer_with_manager(cuda, monkeypatch, tmp_path):
     es = [float(l.split(";;;Energy ")[1].split(";;;")[0]) for l in log.splitlines() if ";;;Energy " in l]
     assert len(es) == out.properties["Step"] + 1
     assert all(b <= a + 1e-9 for a, b in zip(es, es[1:]))
+
+
+# ------------------------------------------------------------ periodic drivers
+def _periodic_setup():
+    from tensormol_amd.containers import Mol
+    from tensormol_amd.periodic import PeriodicForce
+    Z, x, ms, boxes = H.system("waterbox10")
+    L = boxes[0]
+    mgr = _manager("water")
+
+    def EnAndForce(z_, x_, nreal_, DoForce=True):
+        """The reference sample's local force (samples/test_h2o.py:1852-1862)."""
+        mtmp = Mol(z_, x_)
+        if DoForce:
+            en, f = mgr.EvalBPDirectEEUpdateSinglePeriodic(mtmp, 4.6, 3.1, 15.0, nreal_, True, DoForce)
+            return en[0], f[0]
+        en = mgr.EvalBPDirectEEUpdateSinglePeriodic(mtmp, 4.6, 3.1, 15.0, nreal_, True, DoForce)
+        return en[0]
+    PF = PeriodicForce(Mol(Z, x), np.diag(L))
+    PF.BindForce(EnAndForce, 15.0)
+    model = H.model_for("water")
+    sp = O.SymParams(model.eles)
+
+    def oracle_ef(xx):
+        r = O.evaluate(xx, Z, np.array([0, len(Z)]), sp, model.energy_nets, model.charge_nets,
+                       alpha=model.alpha, dsf_alpha=model.dsf_alpha, boxes=boxes)
+        return r["Etotal"][0], -O.JOULEPERHARTREE * r["gradient"]
+    return Z, L, PF, oracle_ef
+
+
+def test_periodic_force_vs_oracle(cuda):
+    from tensormol_amd.params import JOULEPERHARTREE
+    Z, L, PF, oracle_ef = _periodic_setup()
+    x0 = PF.mol0.coords
+    assert np.all((x0 >= 0) & (x0 < L))                 # centred and wrapped into the cell
+    E, F = PF(x0)
+    e_ref, f_ref = oracle_ef(x0)
+    assert abs(E - e_ref) < FP32_E_TOL * len(Z)
+    assert np.sqrt((((F - f_ref) / JOULEPERHARTREE) ** 2).mean()) < FP32_G_TOL
+    E2, _ = PF(x0 + L * np.array([1.0, -2.0, 0.0]))      # whole-cell shifts change nothing
+    assert abs(E2 - E) < 1e-9 * abs(E)
+    e_only, _ = PF(x0, DoForce=False)
+    assert abs(e_only - E) < 1e-9 * abs(E)
+    assert abs(PF.Density() - 0.99) < 0.05              # 1000 waters in a 31.06 A cube
+
+
+@pytest.mark.parametrize("thermostat", [None, "Nose"])
+def test_periodic_velocity_verlet_vs_oracle(cuda, monkeypatch, tmp_path, thermostat):
+    """PeriodicVelocityVerlet (PeriodicMD.py:77-146) over the periodic manager
+    call, against the oracle's minimum-image Velocity Verlet / Nose-Hoover steps
+    from the same start (a = 0 and v = 0 on the first step, as the reference)."""
+    from tensormol_amd.params import PARAMS, atomic_masses
+    from tensormol_amd.periodic import PeriodicVelocityVerlet
+    monkeypatch.setitem(PARAMS, "MDMaxStep", 4)
+    monkeypatch.setitem(PARAMS, "MDdt", 0.5)
+    monkeypatch.setitem(PARAMS, "MDV0", None)
+    monkeypatch.setitem(PARAMS, "MDThermostat", thermostat)
+    monkeypatch.setitem(PARAMS, "results_dir", str(tmp_path) + "/")
+    Z, L, PF, oracle_ef = _periodic_setup()
+    md = PeriodicVelocityVerlet(PF, "pw")
+    x0 = md.x.copy()
+    md.Prop()
+    m = atomic_masses(Z)
+    x, v, a, eta = x0.copy(), np.zeros_like(x0), np.zeros_like(x0), 0.0
+    for _ in range(4):
+        if thermostat is None:
+            x, v, a, e = O.vv_step(x, v, a, m, 0.5, oracle_ef)
+        else:
+            x, v, a, e, eta = O.nose_step(x, v, a, m, 0.5, oracle_ef, eta, PARAMS["MDTemp"], 0.5)
+    d = md.x - x
+    d -= L * np.round(d / L)
+    assert np.abs(d).max() < 1e-6
+    assert np.all((md.x >= 0) & (md.x < L))
+    assert abs(md.md_log[-1, 5] - e) < FP32_E_TOL * len(Z)
+    if thermostat == "Nose":
+        assert abs(md.Tstat.eta - eta) < 1e-3 * abs(eta) and eta != 0.0
+
+
+def test_device_fire_vs_oracle(cuda):
+    """DeviceFIRE (the serving optimiser) on the FP32 path against the oracle's
+    FIRE step (Bitzek et al. 2006) with the same parameters, 20 iterations on
+    the gentle water model, and one batch of two molecules (each with its own
+    time step and mixing state)."""
+    from tensormol_amd.engine import BPEEEngine
+    from tensormol_amd.opt import DeviceFIRE
+    from tensormol_amd.params import atomic_masses
+    from tests.test_gpu_md import _gentle
+    Z1, x1, _, model, ef1 = _gentle()
+    Z2, x2 = Z1[:12], x1[:12] + np.array([40.0, 0.0, 0.0])
+    sp = O.SymParams(model.eles)
+
+    def ef(Z, xx):
+        r = O.evaluate(xx, Z, np.array([0, len(Z)]), sp, model.energy_nets, model.charge_nets,
+                       alpha=model.alpha, dsf_alpha=model.dsf_alpha)
+        return -O.JOULEPERHARTREE * r["gradient"]
+    Z = np.concatenate([Z1, Z2])
+    ms = np.array([0, len(Z1), len(Z)])
+    eng = BPEEEngine(model, precision="fp32")
+    ws = eng.prepare(Z, ms, None, pos0=np.concatenate([x1, x2]))
+    fire = DeviceFIRE(eng, ws, Z, np.concatenate([x1, x2]), dt0=1.0, fmax=1e-9, maxstep=0.1)
+    states = [[1.0, 0.1, 0], [1.0, 0.1, 0]]
+    xs = [x1.copy(), x2.copy()]
+    vs = [np.zeros_like(x1), np.zeros_like(x2)]
+    for _ in range(20):
+        fire.step()
+        for k, Zk in enumerate((Z1, Z2)):
+            xs[k], vs[k], states[k] = O.fire_step(xs[k], vs[k], ef(Zk, xs[k]), atomic_masses(Zk),
+                                                  states[k], dtmax=10.0, maxstep=0.1)
+    torch.cuda.synchronize()
+    xg = fire.pos.cpu().numpy()
+    st = fire.state.cpu().numpy()
+    for k in range(2):
+        dx = np.abs(xg[ms[k]:ms[k + 1]] - xs[k]).max()
+        print(f"FIRE molecule {k}: |dx| {dx:.2e} A, dt gpu {st[k, 0]:.4f} oracle {states[k][0]:.4f}")
+        assert dx < 1e-5
+        assert abs(st[k, 0] - states[k][0]) < 1e-12 and st[k, 2] == states[k][2]

@@ -287,3 +287,95 @@ def test_nve_energy_conservation(small_water):
         etot.append(e + ke)
     etot = np.array(etot)
     assert np.ptp(etot) < 2e-4 * len(Z)
+
+
+# ------------------------------------------------------------ vdW: properties the mathematics fixes
+def _vdw(rA, zi=8, zj=1):
+    c6, r0 = O.vdw_params([zi, zj])
+    t64 = lambda v: torch.tensor(np.atleast_1d(v), dtype=torch.float64)
+    return O.vdw_kernel(t64(np.asarray(rA) * O.BOHRPERA), t64(c6[0]), t64(c6[1]), t64(r0[0]),
+                        t64(r0[1])).numpy(), c6, r0
+
+
+def test_vdw_large_r_limit():
+    """Far beyond the switch and the damping radius the pair term is the bare
+    dispersion -sqrt(C6_i C6_j) / r^6 (r in the kernel's own distance unit)."""
+    for rA in (20.0, 40.0):
+        V, c6, r0 = _vdw(rA)
+        r = rA * O.BOHRPERA ** 2
+        assert abs(V[0] * r ** 6 / -math.sqrt(c6[0] * c6[1]) - 1.0) < 1e-12
+
+
+def test_vdw_vanishes_at_contact_and_is_symmetric():
+    # as r -> 0 the switch starts as 3 t^2 and the damping turns r^-6 into
+    # r^6 / (6 R0^12): V ~ r^8, so doubling a small r multiplies V by 2^8
+    V, _, _ = _vdw([1e-5, 2e-5])
+    assert abs(V[1] / V[0] / 256.0 - 1.0) < 1e-4 and V[0] < 0.0
+    rs = np.linspace(0.5, 12.0, 40)
+    np.testing.assert_array_equal(_vdw(rs, 8, 1)[0], _vdw(rs, 1, 8)[0])
+
+
+def test_vdw_switch_ends_at_width_over_bohrpera():
+    """The switch S(t) = -t^2 (2t - 3) ends (S = 1, S' = 0) at t = 1.  The
+    reference scales the bohr distance by BOHRPERA once more inside the term
+    (RawSymFunc.py:1377), so in the physical distance the switch ends at
+    Poly_Width / BOHRPERA = 2.434 A, not at 4.6 A; beyond it the term equals
+    the unswitched one, before it its magnitude is smaller, and V and dV/dr
+    are continuous there."""
+    a = 4.6 / O.BOHRPERA
+    for rA in (a * 1.0001, 3.0, 4.0, 4.6):
+        V, c6, r0 = _vdw(rA)
+        r = rA * O.BOHRPERA ** 2
+        bare = -math.sqrt(c6[0] * c6[1]) / r ** 6 / (1.0 + 6.0 * (r / (r0[0] + r0[1])) ** -12.0)
+        assert abs(V[0] / bare - 1.0) < 1e-13, rA
+    for rA in (0.5 * a, 0.9 * a, 0.999 * a):
+        V, c6, r0 = _vdw(rA)
+        r = rA * O.BOHRPERA ** 2
+        bare = -math.sqrt(c6[0] * c6[1]) / r ** 6 / (1.0 + 6.0 * (r / (r0[0] + r0[1])) ** -12.0)
+        assert abs(V[0]) < abs(bare) * (1 - 1e-7), rA
+    h = 1e-6
+    (vm, v0, vp), _, _ = _vdw([a - h, a, a + h])
+    assert abs(vp - vm) < 20 * h * abs(v0)                        # continuous (|dV/dr| ~ 6|V|/r)
+    dl = (v0 - vm) / h
+    dr = (vp - v0) / h
+    assert abs(dl - dr) < 1e-3 * abs(dr)                          # C1: S'(1) = 0
+
+
+# ------------------------------------------------------------ FIRE (Bitzek et al. 2006)
+def _bowl(x0, k=2e5):
+    """E = k/2 |x - x0|^2 in J/mol; force J/mol/A."""
+    return lambda x: -k * (x - x0)
+
+
+def test_fire_converges_on_a_bowl():
+    rng = np.random.default_rng(0)
+    x0 = rng.normal(size=(5, 3))
+    m = np.full(5, 0.012)
+    force = _bowl(x0)
+    x = x0 + rng.normal(0, 0.3, size=x0.shape)
+    v = np.zeros_like(x)
+    st = [1.0, 0.1, 0]
+    for _ in range(300):
+        x, v, st = O.fire_step(x, v, force(x), m, st, dtmax=10.0)
+    assert np.abs(x - x0).max() < 1e-6
+
+
+def test_fire_reset_growth_and_cap():
+    m = np.full(2, 0.012)
+    f = np.array([[1e4, 0.0, 0.0], [0.0, -1e4, 0.0]])
+    # moving against the force (P < 0): v reset, dt * fdec, alpha reset
+    x1, v1, st = O.fire_step(np.zeros((2, 3)), -f * 1e-9, f, m, [1.0, 0.05, 3], dtmax=10.0)
+    assert st == [0.5, 0.1, 0]
+    np.testing.assert_allclose(v1, 0.5 * 1e-10 * f / m[:, None], rtol=1e-15)
+    # moving with the force: alpha and dt untouched for nmin steps, then dt * finc, alpha * falpha
+    v = f * 1e-9
+    st = [1.0, 0.1, 0]
+    dts = []
+    for _ in range(7):
+        _, v, st = O.fire_step(np.zeros((2, 3)), v, f, m, st, dtmax=1.15)
+        dts.append(st[0])
+    np.testing.assert_allclose(dts, [1.0] * 5 + [1.1, 1.15])
+    # a huge force: each atom moves exactly maxstep
+    x2, _, _ = O.fire_step(np.zeros((2, 3)), np.zeros((2, 3)), f * 1e12, m, [1.0, 0.1, 0],
+                           dtmax=10.0, maxstep=0.1)
+    np.testing.assert_allclose(np.linalg.norm(x2, axis=1), 0.1, rtol=1e-14)
