@@ -177,7 +177,15 @@ struct DerivRegs<2, NFT, AT> {
   __device__ __forceinline__ float get(int ft, int at, int i) const { return h[ft][at][i]; }
 };
 
+// Activation buffers.  Two ping-pong buffers X, Y (the descriptor tile staged
+// across both), or, where two would not fit (split activations in 64-row
+// tiles), one buffer used in place by every layer: the descriptor tile
+// streams through its first columns, and each layer's GEMM reads the whole
+// buffer before (a barrier) its epilogue overwrites it with the layer's output.
 template <int MODE, int MT>
+constexpr bool in_place() { return act_split_fwd(MODE) && MT > 32; }
+
+template <int MODE, int MT, bool INPLACE = in_place<MODE, MT>()>
 struct Smem {
   static constexpr int PL = act_split_fwd(MODE) ? 2 : 1;
   static constexpr int SH = HMAX + 8;                // activation row stride (elements)
@@ -189,6 +197,14 @@ struct Smem {
     // layer 1: the whole descriptor tile, row stride D + 8 <= 2 SH
     unsigned short Gt[2 * PL * MT * SH];
   };
+  float epart[NW][MT];
+  int atoms[MT];
+};
+template <int MODE, int MT>
+struct Smem<MODE, MT, true> {
+  static constexpr int PL = act_split_fwd(MODE) ? 2 : 1;
+  static constexpr int SH = HMAX + 8;
+  unsigned short X[PL][MT][SH];
   float epart[NW][MT];
   int atoms[MT];
 };
@@ -233,14 +249,14 @@ __device__ __forceinline__ void gemm_first(uint4 (&a)[NFT], const unsigned short
 template <int MODE, int MT, int NFT, bool SPLIT>
 __device__ __forceinline__ void gemm_tiles(floatx4 (&acc)[NFT][MT / 16],
                                            const unsigned short* __restrict__ wf, long plane_elems,
-                                           int nks, const int* rtile, const unsigned short* bbase,
-                                           long bplane, int lane, int STRIDE,
-                                           const uint4 (&a_first)[NFT]) {
+                                           int nks, int kstride, const int* rtile,
+                                           const unsigned short* bbase, long bplane, int lane,
+                                           int STRIDE, const uint4 (&a_first)[NFT]) {
   constexpr int AT = MT / 16;
   const int brow = lane & 15;
   const int bk = (lane >> 4) * 8;
   auto afrag = [&](int ft, int ks, long plane) {
-    return *reinterpret_cast<const uint4*>(wf + plane + ((long)(rtile[ft] * nks + ks) * 64 + lane) * 8);
+    return *reinterpret_cast<const uint4*>(wf + plane + ((long)(rtile[ft] * kstride + ks) * 64 + lane) * 8);
   };
   auto step = [&](const uint4 (&a)[NFT], int ks) {
     uint4 b[AT];
@@ -295,14 +311,14 @@ __device__ __forceinline__ void gemm_tiles(floatx4 (&acc)[NFT][MT / 16],
 template <int MODE, int MT, int NFT>
 __device__ __forceinline__ void gemm_tiles_ws(floatx4 (&acc)[NFT][MT / 16],
                                               const unsigned short* __restrict__ wf,
-                                              long plane_elems, int nks, const int* rtile,
-                                              const unsigned short* bbase, int lane, int STRIDE,
-                                              const uint4 (&a_first)[NFT]) {
+                                              long plane_elems, int nks, int kstride,
+                                              const int* rtile, const unsigned short* bbase,
+                                              int lane, int STRIDE, const uint4 (&a_first)[NFT]) {
   constexpr int AT = MT / 16;
   const int brow = lane & 15;
   const int bk = (lane >> 4) * 8;
   auto afrag = [&](int ft, int ks, long plane) {
-    return *reinterpret_cast<const uint4*>(wf + plane + ((long)(rtile[ft] * nks + ks) * 64 + lane) * 8);
+    return *reinterpret_cast<const uint4*>(wf + plane + ((long)(rtile[ft] * kstride + ks) * 64 + lane) * 8);
   };
   uint4 a0[NFT], a1[NFT];
 #pragma unroll
@@ -338,14 +354,15 @@ __device__ __forceinline__ void gemm_tiles_ws(floatx4 (&acc)[NFT][MT / 16],
 template <int MODE, int MT, int NFT>
 __device__ __forceinline__ void gemm_fwd(floatx4 (&acc)[NFT][MT / 16],
                                          const unsigned short* __restrict__ wf, long plane_elems,
-                                         int nks, const int* rtile, const unsigned short* bbase,
-                                         long bplane, int lane, int STRIDE,
-                                         const uint4 (&a_first)[NFT]) {
+                                         int nks, int kstride, const int* rtile,
+                                         const unsigned short* bbase, long bplane, int lane,
+                                         int STRIDE, const uint4 (&a_first)[NFT]) {
   if constexpr (MODE == 3 || MODE == 4)
-    gemm_tiles_ws<MODE, MT, NFT>(acc, wf, plane_elems, nks, rtile, bbase, lane, STRIDE, a_first);
+    gemm_tiles_ws<MODE, MT, NFT>(acc, wf, plane_elems, nks, kstride, rtile, bbase, lane, STRIDE,
+                                 a_first);
   else
-    gemm_tiles<MODE, MT, NFT, act_split_fwd(MODE)>(acc, wf, plane_elems, nks, rtile, bbase,
-                                                   bplane, lane, STRIDE, a_first);
+    gemm_tiles<MODE, MT, NFT, act_split_fwd(MODE)>(acc, wf, plane_elems, nks, kstride, rtile,
+                                                   bbase, bplane, lane, STRIDE, a_first);
 }
 
 // 64-row tiles: one workgroup per CU (LDS); 32-row tiles in the BF16 / FP16
@@ -357,9 +374,14 @@ mlp_mfma_kernel(MlpArgs a) {
   constexpr int AT = MT / 16;
   constexpr bool LOF = act_split_fwd(MODE);   // forward operands carry lo planes
   constexpr bool LOB = act_split_bwd(MODE);   // backward operands carry lo planes
+  constexpr bool INPLACE = in_place<MODE, MT>();
   constexpr int SH = HMAX + 8;
   typedef Smem<MODE, MT> S;
   __shared__ S sm;
+  // output buffer of layers 2, bwd 3 (read by layer 3, bwd 2): Y, or X in place
+  unsigned short (*const Yb)[MT][SH] = [&]() {
+    if constexpr (INPLACE) return sm.X; else return sm.Y;
+  }();
   const int tid = threadIdx.x;
   const int w = tid >> 6, lane = tid & 63;
   const int tile = blockIdx.x;
@@ -400,8 +422,31 @@ mlp_mfma_kernel(MlpArgs a) {
 #pragma unroll
     for (int at = 0; at < AT; ++at) acc[ft][at] = floatx4{0.f, 0.f, 0.f, 0.f};
   const long p1 = frag_elems(H / 16, D / 32);
-  {
-    // stage the descriptor tile G[atom][0 .. D) as bf16 (hi/lo) with row stride SG
+  const long bpl = (long)MT * SH;   // element offset of the lo plane
+  if constexpr (INPLACE) {
+    // the descriptor tile G[atom][0 .. D) streams through the buffer in
+    // column blocks of up to HMAX features (one block when D <= HMAX)
+    for (int c0 = 0; c0 < D; c0 += HMAX) {
+      const int cw = min(HMAX, D - c0);
+      __syncthreads();   // atoms known; every wave done with the previous block
+      for (int e = tid; e < MT * cw / 4; e += NW * 64) {
+        int r = e / (cw / 4), c4 = (e % (cw / 4)) * 4;
+        int atom = sm.atoms[r];
+        float4 g = atom >= 0 ? *reinterpret_cast<const float4*>(a.G + (long)atom * a.ldg + c0 + c4)
+                             : make_float4(0.f, 0.f, 0.f, 0.f);
+        float v[4] = {g.x, g.y, g.z, g.w};
+        put4<MODE, MT, LOF>(sm.X, r, c4, v);
+      }
+      __syncthreads();
+      if (wact) {
+        if (c0 > 0) gemm_first<FT>(pf, wb + BL.f1 + (long)(c0 / 32) * 512, D / 32, rt, lane);
+        gemm_fwd<MODE, MT, FT>(acc, wb + BL.f1 + (long)(c0 / 32) * 512, p1, cw / 32, D / 32, rt,
+                               &sm.X[0][0][0], bpl, lane, SH, pf);
+      }
+    }
+  } else {
+    // the whole descriptor tile G[atom][0 .. D) (hi / lo), row stride D + 8,
+    // across both buffers
     const int SG = D + 8;
     __syncthreads();
     for (int e = tid; e < MT * D / 4; e += NW * 64) {
@@ -430,10 +475,11 @@ mlp_mfma_kernel(MlpArgs a) {
     }
     __syncthreads();
     if (wact)
-      gemm_fwd<MODE, MT, FT>(acc, wb + BL.f1, p1, D / 32, rt, sm.Gt, (long)MT * SG, lane, SG, pf);
-    if (wact) gemm_first<FT>(pf, wb + BL.f2, H / 32, rt, lane);
-    __syncthreads();   // the tile shares LDS with X, written by the epilogue
+      gemm_fwd<MODE, MT, FT>(acc, wb + BL.f1, p1, D / 32, D / 32, rt, sm.Gt, (long)MT * SG, lane,
+                             SG, pf);
   }
+  if (wact) gemm_first<FT>(pf, wb + BL.f2, H / 32, rt, lane);
+  __syncthreads();   // every wave has read the tile before it is overwritten
   // epilogue 1 -> X, s1
   if (wact) {
 #pragma unroll
@@ -455,9 +501,11 @@ mlp_mfma_kernel(MlpArgs a) {
     for (int ft = 0; ft < FT; ++ft)
 #pragma unroll
       for (int at = 0; at < AT; ++at) acc[ft][at] = floatx4{0.f, 0.f, 0.f, 0.f};
-    gemm_fwd<MODE, MT, FT>(acc, wb + BL.f2, phh, H / 32, rt, &sm.X[0][0][0], (long)MT * SH,
-                               lane, SH, pf);
+    gemm_fwd<MODE, MT, FT>(acc, wb + BL.f2, phh, H / 32, H / 32, rt, &sm.X[0][0][0], bpl, lane, SH, pf);
     gemm_first<FT>(pf, wb + BL.f3, H / 32, rt, lane);
+  }
+  if constexpr (INPLACE) __syncthreads();
+  if (wact) {
 #pragma unroll
     for (int ft = 0; ft < FT; ++ft)
 #pragma unroll
@@ -467,7 +515,7 @@ mlp_mfma_kernel(MlpArgs a) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) act_pair(acc[ft][at][i] + b2[f + i], ac, v[i], d[i]);
         s2.set4(ft, at, d);
-        put4<MODE, MT, LOF>(sm.Y, at * 16 + ca, f, v);
+        put4<MODE, MT, LOF>(Yb, at * 16 + ca, f, v);
       }
   }
   __syncthreads();
@@ -477,9 +525,11 @@ mlp_mfma_kernel(MlpArgs a) {
     for (int ft = 0; ft < FT; ++ft)
 #pragma unroll
       for (int at = 0; at < AT; ++at) acc[ft][at] = floatx4{0.f, 0.f, 0.f, 0.f};
-    gemm_fwd<MODE, MT, FT>(acc, wb + BL.f3, phh, H / 32, rt, &sm.Y[0][0][0], (long)MT * SH,
-                               lane, SH, pf);
+    gemm_fwd<MODE, MT, FT>(acc, wb + BL.f3, phh, H / 32, H / 32, rt, &Yb[0][0][0], bpl, lane, SH, pf);
     gemm_first<FT>(pf, wb + BL.b3, H / 32, rt, lane);
+  }
+  if constexpr (INPLACE) __syncthreads();
+  if (wact) {
     float ep[AT];
 #pragma unroll
     for (int at = 0; at < AT; ++at) ep[at] = 0.f;
@@ -519,15 +569,18 @@ mlp_mfma_kernel(MlpArgs a) {
       a.y[(long)net * a.N + atom] = yv;
     }
   }
-  // ------------------------------------------------------------ backward 3: X (dZ3) -> Y (dZ2)
+  // ------------------------------------------------------------ backward 3: dZ3 -> dZ2
   if (wact) {
 #pragma unroll
     for (int ft = 0; ft < FT; ++ft)
 #pragma unroll
       for (int at = 0; at < AT; ++at) acc[ft][at] = floatx4{0.f, 0.f, 0.f, 0.f};
-    gemm_tiles<MODE, MT, FT, LOB>(acc, wb + BL.b3, phh, H / 32, rt, &sm.X[0][0][0], (long)MT * SH,
-                                 lane, SH, pf);
+    gemm_tiles<MODE, MT, FT, LOB>(acc, wb + BL.b3, phh, H / 32, H / 32, rt, &sm.X[0][0][0], bpl, lane,
+                                  SH, pf);
     gemm_first<FT>(pf, wb + BL.b2, H / 32, rt, lane);
+  }
+  if constexpr (INPLACE) __syncthreads();
+  if (wact) {
 #pragma unroll
     for (int ft = 0; ft < FT; ++ft)
 #pragma unroll
@@ -536,18 +589,21 @@ mlp_mfma_kernel(MlpArgs a) {
         float v[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) v[i] = acc[ft][at][i] * s2.get(ft, at, i);
-        put4<MODE, MT, LOB>(sm.Y, at * 16 + ca, f, v);
+        put4<MODE, MT, LOB>(Yb, at * 16 + ca, f, v);
       }
   }
   __syncthreads();
-  // ------------------------------------------------------------ backward 2: Y (dZ2) -> X (dZ1)
+  // ------------------------------------------------------------ backward 2: dZ2 -> dZ1
   if (wact) {
 #pragma unroll
     for (int ft = 0; ft < FT; ++ft)
 #pragma unroll
       for (int at = 0; at < AT; ++at) acc[ft][at] = floatx4{0.f, 0.f, 0.f, 0.f};
-    gemm_tiles<MODE, MT, FT, LOB>(acc, wb + BL.b2, phh, H / 32, rt, &sm.Y[0][0][0], (long)MT * SH,
-                                 lane, SH, pf);
+    gemm_tiles<MODE, MT, FT, LOB>(acc, wb + BL.b2, phh, H / 32, H / 32, rt, &Yb[0][0][0], bpl, lane,
+                                  SH, pf);
+  }
+  if constexpr (INPLACE) __syncthreads();
+  if (wact) {
 #pragma unroll
     for (int ft = 0; ft < FT; ++ft)
 #pragma unroll
@@ -565,7 +621,7 @@ mlp_mfma_kernel(MlpArgs a) {
     gemm_first<2>(pf2, wb + BL.b1, H / 32, r0, lane);
   }
   __syncthreads();
-  // ------------------------------------------------------------ backward 1: X (dZ1) -> dG
+  // ------------------------------------------------------------ backward 1: dZ1 -> dG
   {
     const long pd = frag_elems(D / 16, H / 32);
     float* dGn = a.dG + (long)net * a.N * a.ldg;
@@ -577,8 +633,8 @@ mlp_mfma_kernel(MlpArgs a) {
       for (int u = 0; u < 2; ++u)
 #pragma unroll
         for (int at = 0; at < AT; ++at) acc2[u][at] = floatx4{0.f, 0.f, 0.f, 0.f};
-      gemm_tiles<MODE, MT, 2, LOB>(acc2, wb + BL.b1, pd, H / 32, rt2, &sm.X[0][0][0], (long)MT * SH,
-                                  lane, SH, pf2);
+      gemm_tiles<MODE, MT, 2, LOB>(acc2, wb + BL.b1, pd, H / 32, H / 32, rt2, &sm.X[0][0][0], bpl, lane,
+                                   SH, pf2);
       const int tn = t0 + 2 * NW;   // the next iteration's first fragments
       if (tn < ntile) {
         int rn[2] = {tn, tn + NW < ntile ? tn + NW : tn};
@@ -636,9 +692,8 @@ int tm_mlp_mfma(int N, int D, int H, int ntiles, const int* tiles, const int* se
   TM_REQUIRE(ldg % 4 == 0 && ldg >= D, "tm_mlp_mfma: ldg %d", ldg);
   TM_REQUIRE(alpha > 0.f, "tm_mlp_mfma: activation alpha %g must be positive", (double)alpha);
   TM_REQUIRE(ntiles >= 0, "tm_mlp_mfma: bad tile count");
-  const bool lof = mode == 2 || mode == 5;   // forward activations carry lo planes
-  TM_REQUIRE(tile_rows == 32 || (tile_rows == 64 && !lof),
-             "tm_mlp_mfma: tile_rows %d (64 or 32; 32 in modes 2 and 5)", tile_rows);
+  TM_REQUIRE(tile_rows == 32 || (tile_rows == 64 && mode != 2),
+             "tm_mlp_mfma: tile_rows %d (64 or 32; 32 in mode 2)", tile_rows);
   if (ntiles == 0) return TM_OK;
   MlpArgs a;
   a.N = N; a.D = D; a.H = H; a.ldg = ldg;
@@ -653,15 +708,17 @@ int tm_mlp_mfma(int N, int D, int H, int ntiles, const int* tiles, const int* se
   a.dG = dG;
   a.flags = nullptr;
   hipStream_t s = (hipStream_t)stream;
-  // 32-row tiles fit two workgroups per CU.  With no more tiles than CUs that
-  // would let the dispatcher stack two on one CU while others idle, so the
-  // launch then asks for extra LDS: one workgroup per CU.
+  // 32-row tiles fit several workgroups per CU.  With no more tiles than CUs
+  // that would let the dispatcher stack two on one CU while others idle, so
+  // the launch then asks for extra LDS: one workgroup per CU (over half of
+  // the CU's 160 KiB).
   size_t pad = 0;
-  if (tile_rows == 32 && !lof) {
+  if (tile_rows == 32 && mode != 2) {
     int dev = 0, ncu = 0;
     TM_HIP_CHECK(hipGetDevice(&dev));
     TM_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    if (ntiles <= ncu) pad = 16384;
+    const size_t lds = mode == 5 ? sizeof(Smem<5, 32>) : sizeof(Smem<0, 32>);
+    if (ntiles <= ncu && lds < 81920) pad = 81920 - lds + 256;
   }
   if (mode == 0 && tile_rows == 64) mlp_mfma_kernel<0, 64><<<ntiles, NW * 64, 0, s>>>(a);
   else if (mode == 0) mlp_mfma_kernel<0, 32><<<ntiles, NW * 64, pad, s>>>(a);
@@ -671,7 +728,8 @@ int tm_mlp_mfma(int N, int D, int H, int ntiles, const int* tiles, const int* se
   else if (mode == 3) mlp_mfma_kernel<3, 32><<<ntiles, NW * 64, pad, s>>>(a);
   else if (mode == 4 && tile_rows == 64) mlp_mfma_kernel<4, 64><<<ntiles, NW * 64, 0, s>>>(a);
   else if (mode == 4) mlp_mfma_kernel<4, 32><<<ntiles, NW * 64, pad, s>>>(a);
-  else if (mode == 5) mlp_mfma_kernel<5, 32><<<ntiles, NW * 64, 0, s>>>(a);
+  else if (mode == 5 && tile_rows == 64) mlp_mfma_kernel<5, 64><<<ntiles, NW * 64, 0, s>>>(a);
+  else if (mode == 5) mlp_mfma_kernel<5, 32><<<ntiles, NW * 64, pad, s>>>(a);
   else mlp_mfma_kernel<2, 32><<<ntiles, NW * 64, 0, s>>>(a);
   TM_LAUNCH_CHECK();
   return TM_OK;

@@ -26,7 +26,7 @@ import torch
 from . import _lib
 
 MODES = {"bf16": 0, "fp16": 1, "bf16x3": 2, "bf16x2": 3, "fp16x2": 4, "bf16x3f": 5}
-TILE_ROWS = {0: 64, 1: 64, 2: 32, 3: 64, 4: 64, 5: 32}
+TILE_ROWS = {0: 64, 1: 64, 2: 32, 3: 64, 4: 64, 5: 64}
 HALF = {0: "bf16", 1: "fp16", 2: "bf16", 3: "bf16", 4: "fp16", 5: "bf16"}   # operand type
 
 
@@ -127,7 +127,7 @@ class MFMAWeights:
 
         64-row tiles fill whole rounds of the device (one workgroup per CU); the
         remaining rows go to 32-row tiles in a second launch, so the last round
-        is not half empty.  bf16x3 and bf16x3f use 32-row tiles throughout."""
+        is not half empty.  bf16x3 uses 32-row tiles throughout."""
         nele = self.engine.nele
         counts = np.diff(ws.eoff).astype(np.int64)
         segs, t64 = [], []

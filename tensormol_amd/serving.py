@@ -81,14 +81,25 @@ class Server:
     """Continuous batching over one engine workspace of fixed capacity."""
 
     def __init__(self, engine, max_atoms=65536, max_mols=8192, hbm_budget_bytes=None, T=300.0,
-                 dt=0.5, fire_dt0=1.0, fire_fmax=5e-3, fire_maxstep=0.1, repack_every=1):
+                 dt=0.5, fire_dt0=1.0, fire_fmax=5e-3, fire_maxstep=0.1, repack_every=1,
+                 graphs=False):
         """repack_every: with K > 1, a batch whose membership changes only by
         departures and arrivals is repacked at most every K iterations; in
         between, the previous layout is relaunched with departed requests as
         discarded ghosts and arrivals wait (at most K - 1 iterations).  A repack
-        is forced when ghosts exceed a quarter of the batch's atoms."""
+        is forced when ghosts exceed a quarter of the batch's atoms.
+        graphs: a relaunched layout's step (every kernel of the evaluation and
+        the integrators) is captured into a HIP graph on its first relaunch
+        and replayed for the following ones, so an unchanged batch costs one
+        graph launch instead of one launch per kernel."""
         self.eng = engine
         self.repack_every = max(1, int(repack_every))
+        self.use_graphs = bool(graphs)
+        self._layout = 0           # incremented by every repack
+        self._graph = None
+        self._graph_layout = -1
+        self.graph_captures = 0
+        self.graph_replays = 0
         per_atom = self.bytes_per_atom()
         if hbm_budget_bytes is not None:
             max_atoms = min(max_atoms, int(hbm_budget_bytes // per_atom))
@@ -217,6 +228,7 @@ class Server:
         self.ms = ms
         self.natoms = N
         self.groups = [int(c) for c in np.bincount(kinds, minlength=3)]
+        self._layout += 1
         self.stager.end()
 
     # ----------------------------------------------------------------- step
@@ -254,13 +266,40 @@ class Server:
                              off3(self.pos), 10.0 * fdt0, 5, 1.1, 0.5, 0.1, 0.99, fmaxstep, st)
 
     # ----------------------------------------------------------------- loop
-    def _launch(self, outs):
+    def _capture(self):
+        """Capture one step of the current layout into a HIP graph (the
+        kernels only: the result copies stay outside, since they alternate
+        between two pinned buffers)."""
+        cur = torch.cuda.current_stream(self.eng.device)
+        side = getattr(self, "_cap_stream", None)
+        if side is None:
+            side = self._cap_stream = torch.cuda.Stream(device=self.eng.device)
+        side.wait_stream(cur)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.stream(side):
+            g.capture_begin()
+            try:
+                self._step()
+            finally:
+                g.capture_end()
+        cur.wait_stream(side)
+        self._graph, self._graph_layout = g, self._layout
+        self.graph_captures += 1
+
+    def _launch(self, outs, relaunch=False):
         """Queue one step of the current batch and the copies of its status
         (flags, per-molecule energies, FIRE |F|^2), the single points' forces,
-        the nvt/opt positions and the row counts into pinned memory."""
+        the nvt/opt positions and the row counts into pinned memory.  A
+        relaunched layout runs as a HIP graph when graphs are on."""
         ws = self.ws
         bn, bo, _ = self.groups
-        self._step()
+        if self.use_graphs and relaunch:
+            if self._graph_layout != self._layout:
+                self._capture()
+            self._graph.replay()
+            self.graph_replays += 1
+        else:
+            self._step()
         parts = [ws.flags[:4].double(), self.eng.energy_tensor(ws)]
         if bo:
             parts.append(self.dsum[bn:bn + bo, 1])
@@ -386,7 +425,7 @@ class Server:
                     self._repack(keep, new)
                     since = 0
                 self.outs.reverse()
-                nxt = self._launch(self.outs[0])
+                nxt = self._launch(self.outs[0], relaunch=relaunch)
                 it += 1
                 self.atom_steps += nxt["N"]
             if inflight is not None:
@@ -409,10 +448,12 @@ class Server:
 
 
 def latency_stats(done):
-    """Arrival-to-result latency percentiles of the completed requests."""
-    lat = np.array([r.t_done - r.t_arrival for r in done if r.error is None]) * 1e3
+    """Arrival-to-answer latency percentiles over every finished request, the
+    failed ones included (a failure is an answer too, at the time it is
+    reported), and the failure count."""
+    lat = np.array([r.t_done - r.t_arrival for r in done]) * 1e3
     return {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-            "mean_ms": float(lat.mean())}
+            "mean_ms": float(lat.mean()), "failed": int(sum(r.error is not None for r in done))}
 
 
 def atom_steps(done):

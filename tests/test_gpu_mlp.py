@@ -113,7 +113,7 @@ def test_energy_forces_mfma(cuda, precision, name, modelname):
         assert np.all(per_mol < 1e-4), per_mol.max()
 
 
-@pytest.mark.parametrize("precision", ["bf16", "fp16", "bf16x2", "fp16x2"])
+@pytest.mark.parametrize("precision", ["bf16", "fp16", "bf16x2", "fp16x2", "bf16x3f"])
 def test_tile_split_bit_identical(cuda, precision):
     """Whole rounds of 64-row tiles plus 32-row tiles for the rest give the
     same outputs, bit for bit, as 32-row tiles only: every output row's

@@ -203,3 +203,28 @@ def test_lazy_repack_same_results(served):
             assert np.array_equal(g.forces, d.forces), rid
         else:
             assert np.array_equal(g.positions, d.positions), rid
+
+
+def test_graph_replayed_steps_same_results(served):
+    """Relaunched layouts replayed as HIP graphs (captured on a layout's first
+    relaunch) give every request the same steps, energy and coordinates /
+    forces, bit for bit, as eager launches; the eager program work between
+    replays (repacks, result copies) runs as usual."""
+    from tensormol_amd.serving import Server
+    model, eng, srv, reqs, done = served
+    eager = Server(eng, max_atoms=4096, max_mols=256, repack_every=4)
+    ref = {r.rid: r for r in eager.run(_copy(reqs), realtime=False)}
+    g = Server(eng, max_atoms=4096, max_mols=256, repack_every=4, graphs=True)
+    got = {r.rid: r for r in g.run(_copy(reqs), realtime=False)}
+    torch.cuda.synchronize()
+    print(f"graph captures {g.graph_captures}, replays {g.graph_replays}, iterations {g.iters}")
+    assert g.graph_replays > g.graph_captures > 0
+    assert sorted(got) == sorted(ref)
+    for rid, d in ref.items():
+        x = got[rid]
+        assert x.error == d.error and x.steps_done == d.steps_done, rid
+        assert x.energy == d.energy, rid
+        if d.kind == "sp":
+            assert np.array_equal(x.forces, d.forces), rid
+        else:
+            assert np.array_equal(x.positions, d.positions), rid
