@@ -652,22 +652,30 @@ def main():
     flops = mlp_flops_per_atom(eng.D, int(model.hidden[0])) * N
     achieved = flops / (mlp_avg_ms * 1e-3) / 1e12
     peak = MLP_PEAK_TFLOPS[args.precision]
+    # the MLP stage is one launch per tile size in use (64-row tiles in whole
+    # rounds over the CUs, then 32-row tiles for the rest); its time is the
+    # span of those launches on their stream
+    mlp_launches = [f"mlp_mfma_kernel<{eng.mfma.mode}, {r}>" for r, n in
+                    ((64, ws.mlp_ws["n64"]), (32, ws.mlp_ws["n32"])) if n > 0] \
+        if args.precision != "fp32" else ["mlp_f32 kernels"]
     mlp_roof = {"bound": "mfma" if args.precision != "fp32" else "valu", "kernel": "mlp_mfma_kernel",
+                "launches": mlp_launches,
                 "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
                 "frac": round(achieved / peak, 4), "traffic": None,
                 "flops_per_launch": flops, "avg_launch_ms": round(mlp_avg_ms, 4)}
-    # The dominant kernel: the Coulomb / vdW sweep over the unordered pairs
-    # within 15 A, VALU-bound, ~50 FP32 operations per unordered pair
-    # (forward, potential and force), against the FP32 vector peak.
+    # The Coulomb / vdW sweep over the unordered pairs within 15 A, VALU-bound,
+    # ~50 FP32 operations per unordered pair (forward, potential and force),
+    # against the FP32 vector peak.
     P_c = coulomb_pairs(x, box)
     coul_avg_ms = float(res["coul_ms"].mean())
     coul_flops = COUL_FLOPS_PER_PAIR * P_c
     coul_achieved = coul_flops / (coul_avg_ms * 1e-3) / 1e12
-    roofline = {"bound": "valu", "kernel": "coulomb_kernel", "achieved": round(coul_achieved, 3),
-                "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(coul_achieved / FP32_VALU_PEAK_TFLOPS, 4), "traffic": None,
-                "flops_per_launch": coul_flops, "pairs_15A": P_c,
-                "avg_launch_ms": round(coul_avg_ms, 4)}
+    coul_roof = {"bound": "valu", "kernel": "coulomb_kernel", "launches": ["coulomb_kernel"],
+                 "achieved": round(coul_achieved, 3),
+                 "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+                 "frac": round(coul_achieved / FP32_VALU_PEAK_TFLOPS, 4), "traffic": None,
+                 "flops_per_launch": coul_flops, "pairs_15A": P_c,
+                 "avg_launch_ms": round(coul_avg_ms, 4)}
     # HBM bytes per launch come from two rocprofv3 counter runs (FETCH_SIZE,
     # WRITE_SIZE) calibrated on a plain copy; they cannot be counted live here
     tpath = os.path.join(ROOT, "profiles", "water_box_12288_bf16_hbm_traffic.json")
@@ -675,14 +683,16 @@ def main():
         tr = json.load(open(tpath))
         kern = {k.split("<")[0]: v for k, v in tr["kernels"].items()}
         if "coulomb_kernel" in kern:
-            roofline["traffic"] = kern["coulomb_kernel"]["bytes"]
-            roofline["traffic_source"] = tr["source"]
-        if args.precision == "bf16":
+            coul_roof["traffic"] = kern["coulomb_kernel"]["bytes"]
+            coul_roof["traffic_source"] = tr["source"]
+        if tr.get("mlp_precision") == args.precision:
             mlp_roof["traffic"] = tr["mlp_stage_bytes"]
             mlp_roof["traffic_source"] = tr["source"]
+    # the headline roofline is the stage that takes the longest per step
+    roofline = dict(mlp_roof if mlp_avg_ms >= coul_avg_ms else coul_roof)
     bdesc = descriptor_bytes(N, eng.D, P_r, P_r3)
     by_stage = {
-        "coulomb": dict(roofline),
+        "coulomb": coul_roof,
         "mlp": mlp_roof,
         "descriptors": {"bound": "hbm", "unit": "GB/s", "peak": 8000.0, "bytes_per_launch": bdesc,
                         "achieved": round(bdesc / (stages["descriptors"] * 1e-3) / 1e9, 1)},

@@ -166,3 +166,72 @@ def test_captured_step_matches_eager(cuda, dtype):
         assert abs(lg - le) <= 1e-5 * abs(le), (k, le, lg)
     for a, b in zip(eager.parameters(), graphed.parameters()):
         assert torch.allclose(a, b, rtol=1e-5, atol=1e-7)
+
+
+# ---------------------------------------------------------------- config [4] shape
+@functools.lru_cache(maxsize=None)
+def _setup_config4():
+    """The force-matching workload at its stated shape: 768-512-512-512-1
+    teacher and student nets (C/H/N/O, softplus), 16 chain molecules of
+    20-60 atoms built as in the serving stream (seed 4)."""
+    sizes = np.random.default_rng(4).integers(20, 61, size=16)
+    Z, x, ms = W.pack(W.chain_set(sizes, seed=4))
+    sp = O.SymParams(ELES)
+    lists = O.build_lists(x, Z, ms, sp)
+    teacher = BPEEModel(ELES, dsf_alpha=0.2, seed=1, hidden=[512, 512, 512])
+    student = BPEEModel(ELES, dsf_alpha=0.2, seed=2, hidden=[512, 512, 512])
+    assert student.sym.D == 768
+    xt = torch.tensor(x, requires_grad=True)
+    Et = O.bp_energies(xt, Z, ms, sp, O.layers_torch(teacher.energy_nets), teacher.alpha, lists)
+    (g,) = torch.autograd.grad(Et.sum(), xt)
+    B, N = len(ms) - 1, len(Z)
+    w_e, w_f = np.full(B, 1.0 / B), np.full(B, 0.5 / (3 * N))
+    return Z, x, ms, sp, lists, student, Et.detach().numpy(), -g.numpy(), w_e, w_f
+
+
+@pytest.mark.parametrize("dtype,tol", [("fp32", 2e-4), ("bf16", 3e-2)])
+def test_config4_gradient_vs_oracle(cuda, dtype, tol):
+    from tensormol_amd.train import ForceMatchingTrainer
+    Z, x, ms, sp, lists, student, E_ref, F_ref, w_e, w_f = _setup_config4()
+    tr = ForceMatchingTrainer(student, mlp_dtype=dtype)
+    tb = tr.batch(Z, ms, E_ref, F_ref, w_e, w_f)
+    L, grads = tr.gradients(tb, torch.tensor(x, dtype=torch.float64, device=cuda))
+    L_ref, g_ref = _oracle_grad(student, Z, x, ms, sp, lists, E_ref, F_ref, w_e, w_f)
+    gv = np.concatenate([g.double().cpu().numpy().ravel() for g in grads])
+    rv = np.concatenate([g.ravel() for g in g_ref])
+    err = np.linalg.norm(gv - rv) / np.linalg.norm(rv)
+    print(f"config [4] {dtype}: {len(Z)} atoms, loss {float(L):.6e} vs oracle {L_ref:.6e}; "
+          f"gradient rel err {err:.2e}")
+    assert abs(float(L) - L_ref) <= tol * abs(L_ref)
+    assert err < tol
+
+
+def test_config4_gradient_finite_differences(cuda):
+    """32 random weights of the 768-512-512-512-1 student: GPU gradient vs
+    central differences of the FP64 oracle loss."""
+    from tensormol_amd.train import ForceMatchingTrainer
+    Z, x, ms, sp, lists, student, E_ref, F_ref, w_e, w_f = _setup_config4()
+    tr = ForceMatchingTrainer(student)
+    tb = tr.batch(Z, ms, E_ref, F_ref, w_e, w_f)
+    _, grads = tr.gradients(tb, torch.tensor(x, dtype=torch.float64, device=cuda))
+    grads = [g.double().cpu().numpy() for g in grads]
+    gmax = max(np.abs(g).max() for g in grads)
+    rng = np.random.default_rng(6)
+    h = 1e-5
+    worst = 0.0
+    for _ in range(32):
+        k = int(rng.integers(len(grads)))
+        idx = tuple(int(rng.integers(n)) for n in grads[k].shape)
+        vals = []
+        for sgn in (1.0, -1.0):
+            lp = O.layers_torch(student.energy_nets)
+            flat = [t for e in student.eles for (Wt, bt) in lp[e] for t in (Wt, bt)]
+            with torch.no_grad():
+                flat[k][idx] += sgn * h
+            Lp, _, _ = O.force_matching_loss(x, Z, ms, sp, lp, student.alpha, E_ref, F_ref, w_e,
+                                             w_f, lists)
+            vals.append(float(Lp.detach()))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        worst = max(worst, abs(grads[k][idx] - fd) / gmax)
+    print(f"config [4], 32 weights: max |g - fd| / max|g| = {worst:.2e}")
+    assert worst < 1e-4
