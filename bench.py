@@ -670,7 +670,8 @@ def main():
     coul_avg_ms = float(res["coul_ms"].mean())
     coul_flops = COUL_FLOPS_PER_PAIR * P_c
     coul_achieved = coul_flops / (coul_avg_ms * 1e-3) / 1e12
-    coul_roof = {"bound": "valu", "kernel": "coulomb_kernel", "launches": ["coulomb_kernel"],
+    coul_kernel = "coulomb_cluster_kernel" if eng.coulomb_tables else "coulomb_kernel"
+    coul_roof = {"bound": "valu", "kernel": coul_kernel, "launches": [coul_kernel],
                  "achieved": round(coul_achieved, 3),
                  "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
                  "frac": round(coul_achieved / FP32_VALU_PEAK_TFLOPS, 4), "traffic": None,
@@ -678,12 +679,12 @@ def main():
                  "avg_launch_ms": round(coul_avg_ms, 4)}
     # HBM bytes per launch come from two rocprofv3 counter runs (FETCH_SIZE,
     # WRITE_SIZE) calibrated on a plain copy; they cannot be counted live here
-    tpath = os.path.join(ROOT, "profiles", "water_box_12288_bf16_hbm_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", f"water_box_12288_{args.precision}_hbm_traffic.json")
     if os.path.exists(tpath) and N == 12288:
         tr = json.load(open(tpath))
         kern = {k.split("<")[0]: v for k, v in tr["kernels"].items()}
-        if "coulomb_kernel" in kern:
-            coul_roof["traffic"] = kern["coulomb_kernel"]["bytes"]
+        if coul_kernel in kern:
+            coul_roof["traffic"] = kern[coul_kernel]["bytes"]
             coul_roof["traffic_source"] = tr["source"]
         if tr.get("mlp_precision") == args.precision:
             mlp_roof["traffic"] = tr["mlp_stage_bytes"]

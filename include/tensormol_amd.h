@@ -209,6 +209,37 @@ int tm_coulomb(int N, const float* xq, const int* elem, const int* mol, const in
                int add_ecc, int has_vdw, int nele, const double* c6, const double* r0, float* xqc,
                int* elc, double* phi, double* grad, double* eatom, int* flags, void* stream);
 
+/* Tables of the pair functions tm_coulomb_cluster reads (host only, no device
+ * work): nint intervals of width h = rc_ee / (nint - 1) (A), interval k
+ * spanning [k h - off, (k + 1) h - off] with off in [0, h) chosen so that a
+ * node lands on the elu matching radius; float4 cubic pieces (c0, c1, c2, c3)
+ * of c0 + c1 t + c2 t^2 + c3 t^3, t in [0, 1), fitted in float64 to the values
+ * and r-derivatives at the interval ends.  scale (host double[2]) receives
+ * 1/h and off/h: r maps to the table coordinate r / h + off / h.  Table 0 is
+ * the DSF-elu kernel K(r) (RawSymFunc.py:1307-1359; zero when add_ecc is 0);
+ * tables 1.. are the vdW energies V(r) of the element pairs (a <= b) in the
+ * order (0,0), (0,1), ..., (0,nele-1), (1,1), ... (RawSymFunc.py:1361-1414,
+ * with its double A->bohr scaling; zero when has_vdw is 0).  out: host
+ * float[4 * (1 + nele (nele + 1) / 2) * nint]; consts, width, c6, r0 as for
+ * tm_coulomb. */
+int tm_coulomb_table(const double* consts, double rc_ee, double width, int add_ecc, int has_vdw,
+                     int nele, const double* c6, const double* r0, int nint, float* out,
+                     double* scale);
+
+/* The tm_coulomb sweep for one system on a cell grid (molc NULL), from the
+ * cell-ordered xqc / elc that tm_charges writes, with the pair functions
+ * from tm_coulomb_table's tables (device copy; at most 3072 float4 entries,
+ * nint >= 16; inv_h, u0 = its scale) and float32 lane sums added in float64.
+ * Same outputs as tm_coulomb (phi, grad, eatom by atom index; flags as its
+ * range overflow); every centre's sums are formed in an order fixed by the
+ * inputs. */
+int tm_coulomb_cluster(int N, const double* box, const int* cell_start, const int* cell_atoms,
+                       const int* cell_of, int nx, int ny, int nz, double csx, double csy,
+                       double csz, double ox, double oy, double oz, int periodic, double rc_ee,
+                       int nele, const float* table, int nint, double inv_h, double u0,
+                       const float* xqc, const int* elc, double* phi, double* grad, double* eatom,
+                       int* flags, void* stream);
+
 /* u_i = phi_i - mean_m(phi): dE_cc/dqraw_i through the neutrality step. */
 int tm_charge_upstream(int N, const int* mol, const double* phi, const double* phimean, float* u,
                        void* stream);

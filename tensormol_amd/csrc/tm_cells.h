@@ -285,3 +285,29 @@ __device__ __forceinline__ void tm_sweep_pf(int i, const int* __restrict__ mol,
     cur = nxt;
   }
 }
+
+// Upper bound of the candidate ranges one centre's sweep can need (the
+// ranges tm_build_ranges keeps): stencil rows (dy, dz) whose cell-gap distance
+// max(|d| - 1, 0) cs is within the trimmed cutoff (the band distance the sweep
+// uses is never smaller), two ranges per row where a periodic row can wrap.
+inline int tm_range_bound(double rc, const double cs[3], const int nc[3], int periodic) {
+  const double r = rc + (double)TM_TRIM_MARGIN;
+  int span[2];
+  int full[2];
+  for (int d = 1; d < 3; ++d) {
+    int k = (int)ceil(rc / cs[d]) + 1;
+    full[d - 1] = periodic && 2 * k + 1 >= nc[d];
+    span[d - 1] = full[d - 1] ? nc[d] : 2 * k + 1;
+  }
+  auto gap = [&](int axis, int o) {
+    int d = full[axis] ? (o <= nc[axis + 1] / 2 ? o : o - nc[axis + 1]) : o - span[axis] / 2;
+    return fmax(0.0, (double)(abs(d) - 1)) * cs[axis + 1];
+  };
+  int rows = 0;
+  for (int oz = 0; oz < span[1]; ++oz)
+    for (int oy = 0; oy < span[0]; ++oy) {
+      double gy = gap(0, oy), gz = gap(1, oz);
+      rows += gy * gy + gz * gz < r * r;
+    }
+  return periodic ? 2 * rows : rows;
+}

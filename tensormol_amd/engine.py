@@ -29,6 +29,9 @@ from .model import BPEEModel
 
 _P = _lib.ptr
 
+# float4 table entries the clustered Coulomb sweep holds in LDS (48 KiB)
+CL_TABLE_ENTRIES = 3072
+
 
 def nl_threshold(rc):
     """First float64 s for which sqrt(s) + 1e-13 < rc is false (MolEmb.cpp:1214-1218).
@@ -84,7 +87,7 @@ class BPEEEngine:
     """
 
     def __init__(self, model: BPEEModel, device="cuda", precision=None, ldr=128, lda=48,
-                 check_overflow=True):
+                 check_overflow=True, coulomb_tables=True):
         self.model = model
         self.device = torch.device(device)
         self._dev_index = self.device.index   # None: the current device at each call
@@ -106,6 +109,15 @@ class BPEEEngine:
         self.r0 = np.ascontiguousarray(r0, dtype=np.float64)
         self.vdw_width = model.poly_width * BOHRPERA
         self.hid = np.array(model.hidden, dtype=np.int32)
+        # One large system on a cell grid takes the Coulomb sweep with
+        # tabulated pair functions (256 intervals over the cutoff; up to four
+        # elements, 44 KiB of tables).
+        self.coulomb_tables = bool(coulomb_tables)
+        ntab = 1 + self.nele * (self.nele + 1) // 2
+        self.cl_nint = 256
+        if ntab * self.cl_nint > CL_TABLE_ENTRIES:
+            self.coulomb_tables = False
+        self._cl_tables = {}
         self._pack_f32()
         if self.precision != "fp32":
             from . import mlp_mfma
@@ -358,6 +370,25 @@ class BPEEEngine:
         else:
             self.mfma.run(ws)
 
+    def _cl_table(self):
+        """(device tables, 1/h, off/h) of the Coulomb kernel and the
+        element-pair vdW energies (float4 cubic pieces, built on the host in
+        float64) for the current AddEcc setting."""
+        add_ecc = int(bool(PARAMS["AddEcc"]))
+        t = self._cl_tables.get(add_ecc)
+        if t is None:
+            ntab = 1 + self.nele * (self.nele + 1) // 2
+            host = np.zeros(4 * ntab * self.cl_nint, dtype=np.float32)
+            scale = np.zeros(2, dtype=np.float64)
+            self.lib.tm_coulomb_table(self.dsf.ctypes.data_as(_lib.P), self.model.ee_off,
+                                      self.vdw_width, add_ecc, int(self.model.has_vdw), self.nele,
+                                      self.c6.ctypes.data_as(_lib.P),
+                                      self.r0.ctypes.data_as(_lib.P), self.cl_nint,
+                                      host.ctypes.data_as(_lib.P), scale.ctypes.data_as(_lib.P))
+            t = (torch.tensor(host, device=self.device), float(scale[0]), float(scale[1]))
+            self._cl_tables[add_ecc] = t
+        return t
+
     def electrostatics(self, ws, pos, timer=None):
         """Charges (neutral per molecule), the Coulomb / vdW sweep and the
         molecule means of the potential.  timer: optional pair of callables run
@@ -382,15 +413,24 @@ class BPEEEngine:
         else:
             L.tm_charges(ws.N, _P(ws.mol), _P(ms), _P(ws.qmean), _P(ws.qraw), _P(pos),
                          _P(ws.xq), _P(ws.q), _P(ws.qx), None, None, None, None, st)
+        table = self._cl_table() if (ws.use_cells and self.coulomb_tables) else None
         if timer is not None:
             timer[0]()
-        L.tm_coulomb(ws.N, None if ws.use_cells else _P(ws.xq), _P(ws.elem), _P(ws.mol), _P(ws.mol_start), _P(ws.box),
-                     g[0], g[1], g[2], g[3], g[5], g[6], g[7], g[8], g[9], g[10], g[11],
-                     g[12], g[13], g[14], ws.periodic, self.dsf.ctypes.data_as(_lib.P), self.model.ee_off, self.vdw_width,
-                     int(bool(PARAMS["AddEcc"])), int(self.model.has_vdw), self.nele,
-                     self.c6.ctypes.data_as(_lib.P), self.r0.ctypes.data_as(_lib.P),
-                     _P(ws.xqc), _P(ws.elc), _P(ws.phi), _P(ws.gradc), _P(ws.eatom), _P(ws.flags),
-                     st)
+        if table is not None:
+            L.tm_coulomb_cluster(ws.N, _P(ws.box), g[0], g[1], g[2], g[6], g[7], g[8], g[9], g[10],
+                                 g[11], g[12], g[13], g[14], ws.periodic, self.model.ee_off,
+                                 self.nele, _P(table[0]), self.cl_nint, table[1], table[2],
+                                 _P(ws.xqc), _P(ws.elc), _P(ws.phi), _P(ws.gradc), _P(ws.eatom),
+                                 _P(ws.flags), st)
+        else:
+            L.tm_coulomb(ws.N, None if ws.use_cells else _P(ws.xq), _P(ws.elem), _P(ws.mol), _P(ws.mol_start), _P(ws.box),
+                         g[0], g[1], g[2], g[3], g[5], g[6], g[7], g[8], g[9], g[10], g[11],
+                         g[12], g[13], g[14], ws.periodic, self.dsf.ctypes.data_as(_lib.P),
+                         self.model.ee_off, self.vdw_width,
+                         int(bool(PARAMS["AddEcc"])), int(self.model.has_vdw), self.nele,
+                         self.c6.ctypes.data_as(_lib.P), self.r0.ctypes.data_as(_lib.P),
+                         _P(ws.xqc), _P(ws.elc), _P(ws.phi), _P(ws.gradc), _P(ws.eatom),
+                         _P(ws.flags), st)
         if timer is not None:
             timer[1]()
         L.tm_mol_sum_f64(ws.B, _P(ws.mol_start), 1, _P(ws.phi), 1, _P(ws.phimean), 1, _P(ws.red), ws.N, st)

@@ -17,9 +17,9 @@ SYSTEMS = [("water64", "water"), ("chains", "chno"), ("waterbox10", "water"), ("
            ("morphine", "chno"), ("water10", "water"), ("peptide188", "chno"), ("johnsonmols", "chno")]
 
 
-def _engine(modelname, precision="fp32"):
+def _engine(modelname, precision="fp32", **kw):
     from tensormol_amd.engine import BPEEEngine
-    return BPEEEngine(H.model_for(modelname), precision=precision)
+    return BPEEEngine(H.model_for(modelname), precision=precision, **kw)
 
 
 def _prepare(eng, name, dev):
@@ -183,7 +183,7 @@ def test_fused_abi_paths_bit_identical(cuda, name, modelname):
     tm_mol_sum_f32 to rounding."""
     from tensormol_amd._lib import P, ptr
     from tensormol_amd.params import PARAMS
-    eng = _engine(modelname, "fp32")
+    eng = _engine(modelname, "fp32", coulomb_tables=False)   # the analytic tm_coulomb sweep
     ws, pos = _prepare(eng, name, cuda)
     assert ws.use_cells
     eng.run(ws, pos)
