@@ -670,7 +670,7 @@ def main():
     coul_avg_ms = float(res["coul_ms"].mean())
     coul_flops = COUL_FLOPS_PER_PAIR * P_c
     coul_achieved = coul_flops / (coul_avg_ms * 1e-3) / 1e12
-    coul_kernel = "coulomb_cluster_kernel" if eng.coulomb_tables else "coulomb_kernel"
+    coul_kernel = "coulomb_tab_kernel" if eng.coulomb_tables else "coulomb_kernel"
     coul_roof = {"bound": "valu", "kernel": coul_kernel, "launches": [coul_kernel],
                  "achieved": round(coul_achieved, 3),
                  "peak": FP32_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",

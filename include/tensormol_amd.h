@@ -209,7 +209,7 @@ int tm_coulomb(int N, const float* xq, const int* elem, const int* mol, const in
                int add_ecc, int has_vdw, int nele, const double* c6, const double* r0, float* xqc,
                int* elc, double* phi, double* grad, double* eatom, int* flags, void* stream);
 
-/* Tables of the pair functions tm_coulomb_cluster reads (host only, no device
+/* Tables of the pair functions tm_coulomb_tab reads (host only, no device
  * work): nint intervals of width h = rc_ee / (nint - 1) (A), interval k
  * spanning [k h - off, (k + 1) h - off] with off in [0, h) chosen so that a
  * node lands on the elu matching radius; float4 cubic pieces (c0, c1, c2, c3)
@@ -233,7 +233,7 @@ int tm_coulomb_table(const double* consts, double rc_ee, double width, int add_e
  * Same outputs as tm_coulomb (phi, grad, eatom by atom index; flags as its
  * range overflow); every centre's sums are formed in an order fixed by the
  * inputs. */
-int tm_coulomb_cluster(int N, const double* box, const int* cell_start, const int* cell_atoms,
+int tm_coulomb_tab(int N, const double* box, const int* cell_start, const int* cell_atoms,
                        const int* cell_of, int nx, int ny, int nz, double csx, double csy,
                        double csz, double ox, double oy, double oz, int periodic, double rc_ee,
                        int nele, const float* table, int nint, double inv_h, double u0,

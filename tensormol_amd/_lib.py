@@ -42,7 +42,7 @@ SIGNATURES = {
     "tm_coulomb": [I, P, P, P, P, P, P, P, P, P, P, I, I, I, D, D, D, D, D, D, I, P, D, D, I, I, I, P, P, P, P,
                    P, P, P, P, P],
     "tm_coulomb_table": [P, D, D, I, I, I, P, P, I, P, P],
-    "tm_coulomb_cluster": [I, P, P, P, P, I, I, I, D, D, D, D, D, D, I, D, I, P, I, D, D, P, P, P, P,
+    "tm_coulomb_tab": [I, P, P, P, P, I, I, I, D, D, D, D, D, D, I, D, I, P, I, D, D, P, P, P, P,
                            P, P, P],
     "tm_vv_drift": [I, D, P, P, P, P],
     "tm_vv_kick": [I, D, P, P, P, P, P, P],

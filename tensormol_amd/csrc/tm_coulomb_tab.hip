@@ -266,30 +266,30 @@ int tm_coulomb_table(const double* consts, double rc_ee, double width, int add_e
   return TM_OK;
 }
 
-int tm_coulomb_cluster(int N, const double* box, const int* cell_start, const int* cell_atoms,
+int tm_coulomb_tab(int N, const double* box, const int* cell_start, const int* cell_atoms,
                        const int* cell_of, int nx, int ny, int nz, double csx, double csy,
                        double csz, double ox, double oy, double oz, int periodic, double rc_ee,
                        int nele, const float* table, int nint, double inv_h, double u0,
                        const float* xqc, const int* elc, double* phi, double* grad, double* eatom,
                        int* flags, void* stream) {
-  TM_REQUIRE(nele > 0 && nele <= TM_MAX_ELE, "tm_coulomb_cluster: bad nele %d", nele);
+  TM_REQUIRE(nele > 0 && nele <= TM_MAX_ELE, "tm_coulomb_tab: bad nele %d", nele);
   const int ntab = 1 + nele * (nele + 1) / 2;
   TM_REQUIRE(nint >= 16 && (long)ntab * nint <= CL_TABLE_MAX,
-             "tm_coulomb_cluster: %d tables of %d intervals exceed %d entries", ntab, nint,
+             "tm_coulomb_tab: %d tables of %d intervals exceed %d entries", ntab, nint,
              CL_TABLE_MAX);
   TM_REQUIRE(cell_start != nullptr && cell_atoms != nullptr && cell_of != nullptr &&
                  table != nullptr && xqc != nullptr && elc != nullptr,
-             "tm_coulomb_cluster: null argument");
-  TM_REQUIRE(!periodic || box != nullptr, "tm_coulomb_cluster: periodic without box");
+             "tm_coulomb_tab: null argument");
+  TM_REQUIRE(!periodic || box != nullptr, "tm_coulomb_tab: periodic without box");
   TM_REQUIRE(nx > 0 && ny > 0 && nz > 0 && csx > 0 && csy > 0 && csz > 0,
-             "tm_coulomb_cluster: bad grid");
+             "tm_coulomb_tab: bad grid");
   TM_REQUIRE(inv_h > 0.0 && u0 >= 0.0 && u0 < 1.0 && (rc_ee * inv_h + u0) < nint,
-             "tm_coulomb_cluster: table scale (%g, %g) does not cover the cutoff", inv_h, u0);
+             "tm_coulomb_tab: table scale (%g, %g) does not cover the cutoff", inv_h, u0);
   {
     const double cs[3] = {csx, csy, csz};
     const int nc[3] = {nx, ny, nz};
     TM_REQUIRE(tm_range_bound(rc_ee, cs, nc, periodic) <= CT_RANGES,
-               "tm_coulomb_cluster: cells of %.2f A are too small for a %.1f A cutoff (a sweep "
+               "tm_coulomb_tab: cells of %.2f A are too small for a %.1f A cutoff (a sweep "
                "could need %d candidate ranges, room for %d)", fmin(csx, fmin(csy, csz)), rc_ee,
                tm_range_bound(rc_ee, cs, nc, periodic), CT_RANGES);
   }

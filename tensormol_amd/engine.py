@@ -417,7 +417,7 @@ class BPEEEngine:
         if timer is not None:
             timer[0]()
         if table is not None:
-            L.tm_coulomb_cluster(ws.N, _P(ws.box), g[0], g[1], g[2], g[6], g[7], g[8], g[9], g[10],
+            L.tm_coulomb_tab(ws.N, _P(ws.box), g[0], g[1], g[2], g[6], g[7], g[8], g[9], g[10],
                                  g[11], g[12], g[13], g[14], ws.periodic, self.model.ee_off,
                                  self.nele, _P(table[0]), self.cl_nint, table[1], table[2],
                                  _P(ws.xqc), _P(ws.elc), _P(ws.phi), _P(ws.gradc), _P(ws.eatom),

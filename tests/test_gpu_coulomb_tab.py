@@ -1,4 +1,4 @@
-"""The tabulated Coulomb / vdW sweep (tm_coulomb_cluster) against the analytic
+"""The tabulated Coulomb / vdW sweep (tm_coulomb_tab) against the analytic
 sweep (tm_coulomb) and the FP64 oracle.
 
 Systems on a cell grid: the periodic 1,000-water box (edge 31.06 A), the open
@@ -9,7 +9,7 @@ Systems on a cell grid: the periodic 1,000-water box (edge 31.06 A), the open
   grad:  rms |diff| < 2e-6 Ha/A, max < 2e-5
   eatom: summed Coulomb and vdW energies within 1e-7 Ha per atom
 The oracle gates (per-atom energy 2e-6 Ha, force rms 1e-4 Ha/A) run through the
-engine, which takes the clustered sweep on these systems.
+engine, which takes the tabulated sweep on these systems.
 """
 import numpy as np
 import pytest
@@ -40,7 +40,7 @@ def _run(name, modelname, tables, dev):
 
 @pytest.mark.parametrize("name,modelname", [("waterbox10", "water"), ("chno_small", "chno"),
                                             ("box4096", "water")])
-def test_cluster_sweep_matches_analytic_sweep(cuda, name, modelname):
+def test_tab_sweep_matches_analytic_sweep(cuda, name, modelname):
     eng, ws, _ = _run(name, modelname, True, cuda)
     assert eng.coulomb_tables
     ref_eng, ref, _ = _run(name, modelname, False, cuda)
@@ -59,7 +59,7 @@ def test_cluster_sweep_matches_analytic_sweep(cuda, name, modelname):
 
 
 @pytest.mark.parametrize("name", ["waterbox10", "box4096"])
-def test_cluster_sweep_deterministic(cuda, name):
+def test_tab_sweep_deterministic(cuda, name):
     """Two evaluations give the same bits: every sum is formed in an order
     fixed by the inputs."""
     eng, ws, pos = _run(name, "water", True, cuda)
@@ -71,7 +71,7 @@ def test_cluster_sweep_deterministic(cuda, name):
 
 
 @pytest.mark.parametrize("name,modelname", [("waterbox10", "water"), ("chno_small", "chno")])
-def test_cluster_sweep_vs_oracle(cuda, name, modelname):
+def test_tab_sweep_vs_oracle(cuda, name, modelname):
     eng, ws, _ = _run(name, modelname, True, cuda)
     r = eng.results(ws)
     ref = H.oracle_eval(name, modelname)
