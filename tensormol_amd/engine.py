@@ -29,7 +29,7 @@ from .model import BPEEModel
 
 _P = _lib.ptr
 
-# float4 table entries the clustered Coulomb sweep holds in LDS (48 KiB)
+# largest table (float4 entries, 48 KiB) the tabulated Coulomb sweep accepts
 CL_TABLE_ENTRIES = 3072
 
 

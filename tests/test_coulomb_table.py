@@ -1,4 +1,4 @@
-"""The pair-function tables of the clustered Coulomb sweep (tm_coulomb_table, a
+"""The pair-function tables of the tabulated Coulomb sweep (tm_coulomb_table, a
 host routine of the C-ABI library, so these run without a GPU) against the
 oracle's float64 DSF-elu and vdW kernels (RawSymFunc.py:1307-1414).
 
