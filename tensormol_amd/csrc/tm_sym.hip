@@ -27,8 +27,7 @@
 namespace {
 
 constexpr int SYM_WAVES = 4;
-constexpr int MAX_ROW = 128;      // radial row entries staged per wave (= max row stride)
-constexpr int MAX_ANG = 48;       // angular sub-list entries per centre
+constexpr int MAX_ROW = 256;      // widest radial row (row stride) the kernels stage in LDS
 constexpr int NR_MAX = 64;        // radial shells
 constexpr int NA = 8;             // angular shells (fixed by the lane mapping)
 constexpr int NT = 8;             // angular theta points
@@ -85,26 +84,29 @@ __device__ __forceinline__ void powz_pair(float b, const SymArgs& a, float& Az, 
   dAz = b > 0.f ? a.zeta * Az * __builtin_amdgcn_rcpf(b) : 0.f;
 }
 
+template <int RC>
 struct RowLds {
   static constexpr bool has_fc = true, has_fa = true, has_el = true;
-  float4 d[SYM_WAVES][MAX_ROW];      // (x_j - x_i, r)
-  float fc[SYM_WAVES][MAX_ROW];      // radial cutoff function
-  float fa[SYM_WAVES][MAX_ROW];      // angular cutoff function
-  int el[SYM_WAVES][MAX_ROW];
+  float4 d[SYM_WAVES][RC];      // (x_j - x_i, r)
+  float fc[SYM_WAVES][RC];      // radial cutoff function
+  float fa[SYM_WAVES][RC];      // angular cutoff function
+  int el[SYM_WAVES][RC];
 };
 // The forward's row: the angular cutoff is formed once per triple instead,
 // which keeps its LDS small enough for 7 waves per SIMD.
+template <int RC>
 struct RowLdsFwd {
   static constexpr bool has_fc = true, has_fa = false, has_el = false;
-  float4 d[SYM_WAVES][MAX_ROW];
-  float fc[SYM_WAVES][MAX_ROW];
+  float4 d[SYM_WAVES][RC];
+  float fc[SYM_WAVES][RC];
 };
 // The adjoint's row: its radial pass forms fc and reads the element itself
 // (lane = neighbour), which keeps its LDS small enough for 6 waves per SIMD.
+template <int RC>
 struct RowLdsAdj {
   static constexpr bool has_fc = false, has_fa = true, has_el = false;
-  float4 d[SYM_WAVES][MAX_ROW];
-  float fa[SYM_WAVES][MAX_ROW];
+  float4 d[SYM_WAVES][RC];
+  float fa[SYM_WAVES][RC];
 };
 
 __device__ __forceinline__ float cutoff_fc(float k, float r) { return 0.5f * (__cosf(k * r) + 1.0f); }
@@ -180,7 +182,7 @@ __device__ __forceinline__ void decode_triple(int t, const int* seg, int nele, i
   p = -1;
 }
 
-template <int ZI>
+template <int ZI, int RC>
 __global__ void __launch_bounds__(SYM_WAVES * 64)
 sym_fwd_kernel(int N, const double* __restrict__ pos, const double* __restrict__ box,
                const int* __restrict__ mol, const int* __restrict__ elem,
@@ -189,14 +191,14 @@ sym_fwd_kernel(int N, const double* __restrict__ pos, const double* __restrict__
                const int* __restrict__ acnt, const int* __restrict__ aseg, int ldr, int lda,
                SymArgs a, float* __restrict__ G, int ldg, int* __restrict__ rev,
                int* __restrict__ flags) {
-  __shared__ RowLdsFwd L;
+  __shared__ RowLdsFwd<RC> L;
   __shared__ float4 tq[SYM_WAVES][64];        // per triple: (cos theta, sin theta, fc_j fc_k, -)
   __shared__ float4 tR[SYM_WAVES][64][NA / 4];   // per triple: exp(-eta (ravg - Rs_s)^2), s = 0..7
   __shared__ int segs[SYM_WAVES][2][TM_MAX_ELE + 1];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i = blockIdx.x * SYM_WAVES + w;
   if (i >= N) return;   // only wave-level synchronisation below
-  const int n = min(ncnt[i], MAX_ROW);
+  const int n = min(ncnt[i], RC);
   if (lane <= a.nele) {
     segs[w][0][lane] = min(rseg[(long)i * (a.nele + 1) + lane], n);
     segs[w][1][lane] = aseg[(long)i * (a.nele + 1) + lane];
@@ -231,7 +233,7 @@ sym_fwd_kernel(int N, const double* __restrict__ pos, const double* __restrict__
   // ends, a butterfly over g sums the 8 groups and lane (t, g) writes shell
   // s = g.  Triples are ordered by element pair (blocks ea <= eb), so each
   // segment is a contiguous range of the enumeration.
-  const int m = min(acnt[i], MAX_ANG);
+  const int m = min(acnt[i], lda);
   const int* arow = ang + (long)i * lda;
   const int ntrip = m * (m - 1) / 2;
   const int off = a.nele * a.nr;
@@ -330,7 +332,7 @@ sym_fwd_kernel(int N, const double* __restrict__ pos, const double* __restrict__
 }
 
 // dE/dG (two contributions combined: gE + u_i * gQ) -> w slots
-template <int ZI>
+template <int ZI, int RC>
 __global__ void __launch_bounds__(SYM_WAVES * 64) __attribute__((amdgpu_waves_per_eu(4)))
 sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__ box,
                const int* __restrict__ mol, const int* __restrict__ elem,
@@ -340,16 +342,16 @@ sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__
                SymArgs a, const float* __restrict__ gE, const float* __restrict__ gQ,
                const float* __restrict__ u, const double* __restrict__ phi,
                const double* __restrict__ phimean, int ldg, float4* __restrict__ wslot) {
-  __shared__ RowLdsAdj L;
+  __shared__ RowLdsAdj<RC> L;
   extern __shared__ float gdyn[];   // SYM_WAVES * D floats: dE/dG of each wave's centre
   float* gw = gdyn + (threadIdx.x >> 6) * a.D;
-  __shared__ float wacc[3][SYM_WAVES][MAX_ROW];    // w per neighbour slot (x, y, z planes)
+  __shared__ float wacc[3][SYM_WAVES][RC];    // w per neighbour slot (x, y, z planes)
   __shared__ float cab[2][3][SYM_WAVES][64];       // per triple: contributions to j (0) and k (1)
   __shared__ int segs[SYM_WAVES][2][TM_MAX_ELE + 1];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i = blockIdx.x * SYM_WAVES + w;
   if (i >= N) return;
-  const int n = min(ncnt[i], MAX_ROW);
+  const int n = min(ncnt[i], RC);
   if (lane <= a.nele) {
     segs[w][0][lane] = min(rseg[(long)i * (a.nele + 1) + lane], n);
     segs[w][1][lane] = aseg[(long)i * (a.nele + 1) + lane];
@@ -385,7 +387,7 @@ sym_adj_kernel(int N, const double* __restrict__ pos, const double* __restrict__
   }
   __syncwarp();
   // ---------------- angular: lane = triple, contributions staged per chunk
-  const int m = min(acnt[i], MAX_ANG);
+  const int m = min(acnt[i], lda);
   const int* arow = ang + (long)i * lda;
   const int ntrip = m * (m - 1) / 2;
   const int off = a.nele * a.nr;
@@ -584,13 +586,14 @@ __global__ void force_gather_kernel(int N, int ldr, const int* __restrict__ ncnt
 // of features; the lane layout and the write-once order are the forward's.
 // Used for the weight gradient of a force loss: with u = dLoss/dF,
 // dLoss/dW = -d/dW sum_i (dE_i/dG_i)(W) . V_i.
+template <int RC>
 struct JvpLds {
-  float4 t[SYM_WAVES][MAX_ROW];     // (du_j - du_i, dr_j = d_j . (du_j - du_i) / r_j)
-  float dfc[SYM_WAVES][MAX_ROW];    // d fc_r / dr
-  float dfa[SYM_WAVES][MAX_ROW];    // d fc_a / dr
+  float4 t[SYM_WAVES][RC];     // (du_j - du_i, dr_j = d_j . (du_j - du_i) / r_j)
+  float dfc[SYM_WAVES][RC];    // d fc_r / dr
+  float dfa[SYM_WAVES][RC];    // d fc_a / dr
 };
 
-template <int ZI>
+template <int ZI, int RC>
 __global__ void __launch_bounds__(SYM_WAVES * 64)
 sym_jvp_kernel(int N, const double* __restrict__ pos, const double* __restrict__ box,
                const int* __restrict__ mol, const int* __restrict__ elem,
@@ -598,8 +601,8 @@ sym_jvp_kernel(int N, const double* __restrict__ pos, const double* __restrict__
                const int* __restrict__ rseg, const int* __restrict__ ang,
                const int* __restrict__ acnt, const int* __restrict__ aseg, int ldr, int lda,
                SymArgs a, const double* __restrict__ du, float* __restrict__ V, int ldg) {
-  __shared__ RowLds L;
-  __shared__ JvpLds T;
+  __shared__ RowLds<RC> L;
+  __shared__ JvpLds<RC> T;
   __shared__ float tc[SYM_WAVES][64], tsn[SYM_WAVES][64], tcs[SYM_WAVES][64];
   __shared__ float tf[SYM_WAVES][64], tdf[SYM_WAVES][64], tdc[SYM_WAVES][64], tdr[SYM_WAVES][64];
   __shared__ float tR[SYM_WAVES][NA][64], tRp[SYM_WAVES][NA][64];
@@ -608,7 +611,7 @@ sym_jvp_kernel(int N, const double* __restrict__ pos, const double* __restrict__
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i = blockIdx.x * SYM_WAVES + w;
   if (i >= N) return;   // only wave-level synchronisation below
-  const int n = min(ncnt[i], MAX_ROW);
+  const int n = min(ncnt[i], RC);
   if (lane <= a.nele) {
     segs[w][0][lane] = min(rseg[(long)i * (a.nele + 1) + lane], n);
     segs[w][1][lane] = aseg[(long)i * (a.nele + 1) + lane];
@@ -651,7 +654,7 @@ sym_jvp_kernel(int N, const double* __restrict__ pos, const double* __restrict__
     }
   }
   // ---------------- angular: phase A lane = triple, phase B lane = (t, s)
-  const int m = min(acnt[i], MAX_ANG);
+  const int m = min(acnt[i], lda);
   const int* arow = ang + (long)i * lda;
   const int ntrip = m * (m - 1) / 2;
   const int off = a.nele * a.nr;
@@ -765,11 +768,13 @@ int tm_sym_forward(int N, const double* pos, const double* box, const int* mol, 
   if (st) return st;
   TM_REQUIRE(ldg >= a.D, "tm_sym_forward: ldg %d < D %d", ldg, a.D);
   TM_REQUIRE(rev == nullptr || flags != nullptr, "tm_sym_forward: rev needs flags");
-  TM_REQUIRE(ldr <= MAX_ROW && lda <= MAX_ANG, "tm_sym_forward: strides %d/%d above %d/%d",
-             ldr, lda, MAX_ROW, MAX_ANG);
+  TM_REQUIRE(ldr > 0 && ldr <= MAX_ROW && lda > 0 && lda <= ldr,
+             "tm_sym_forward: strides %d/%d (need 0 < lda <= ldr <= %d)", ldr, lda, MAX_ROW);
   if (N == 0) return TM_OK;
   hipStream_t s = (hipStream_t)stream;
-  auto k = a.zeta_int == 8 ? sym_fwd_kernel<8> : sym_fwd_kernel<0>;
+  // rows up to 128 entries stage in the smaller LDS layout (more waves per CU)
+  auto k = ldr <= 128 ? (a.zeta_int == 8 ? sym_fwd_kernel<8, 128> : sym_fwd_kernel<0, 128>)
+                      : (a.zeta_int == 8 ? sym_fwd_kernel<8, 256> : sym_fwd_kernel<0, 256>);
   k<<<tm_div_up(N, SYM_WAVES), SYM_WAVES * 64, 0, s>>>(N, pos, box, mol, elem, nbr, ncnt, rseg,
                                                        ang, acnt, aseg, ldr, lda, a, G, ldg, rev,
                                                        flags);
@@ -790,11 +795,12 @@ int tm_sym_adjoint(int N, const double* pos, const double* box, const int* mol, 
   TM_REQUIRE(ldg >= a.D, "tm_sym_adjoint: ldg %d < D %d", ldg, a.D);
   TM_REQUIRE(gQ == nullptr || u != nullptr || (phi != nullptr && phimean != nullptr),
              "tm_sym_adjoint: gQ needs u or phi and phimean");
-  TM_REQUIRE(ldr <= MAX_ROW && lda <= MAX_ANG, "tm_sym_adjoint: strides %d/%d above %d/%d", ldr,
-             lda, MAX_ROW, MAX_ANG);
+  TM_REQUIRE(ldr > 0 && ldr <= MAX_ROW && lda > 0 && lda <= ldr,
+             "tm_sym_adjoint: strides %d/%d (need 0 < lda <= ldr <= %d)", ldr, lda, MAX_ROW);
   if (N == 0) return TM_OK;
   hipStream_t s = (hipStream_t)stream;
-  auto k = a.zeta_int == 8 ? sym_adj_kernel<8> : sym_adj_kernel<0>;
+  auto k = ldr <= 128 ? (a.zeta_int == 8 ? sym_adj_kernel<8, 128> : sym_adj_kernel<0, 128>)
+                      : (a.zeta_int == 8 ? sym_adj_kernel<8, 256> : sym_adj_kernel<0, 256>);
   k<<<tm_div_up(N, SYM_WAVES), SYM_WAVES * 64, sizeof(float) * SYM_WAVES * a.D, s>>>(
       N, pos, box, mol, elem, nbr, ncnt, rseg, ang, acnt, aseg, ldr, lda, a, gE, gQ, u, phi,
       phimean, ldg, (float4*)wslot);
@@ -811,11 +817,12 @@ int tm_sym_jvp(int N, const double* pos, const double* box, const int* mol, cons
   int st = fill_args(a, nele, nr, eta, zeta, rr, ra, rs_r, rs_a, thetas);
   if (st) return st;
   TM_REQUIRE(ldg >= a.D, "tm_sym_jvp: ldg %d < D %d", ldg, a.D);
-  TM_REQUIRE(ldr <= MAX_ROW && lda <= MAX_ANG, "tm_sym_jvp: strides %d/%d above %d/%d", ldr, lda,
-             MAX_ROW, MAX_ANG);
+  TM_REQUIRE(ldr > 0 && ldr <= MAX_ROW && lda > 0 && lda <= ldr,
+             "tm_sym_jvp: strides %d/%d (need 0 < lda <= ldr <= %d)", ldr, lda, MAX_ROW);
   if (N == 0) return TM_OK;
   hipStream_t s = (hipStream_t)stream;
-  auto k = a.zeta_int == 8 ? sym_jvp_kernel<8> : sym_jvp_kernel<0>;
+  auto k = ldr <= 128 ? (a.zeta_int == 8 ? sym_jvp_kernel<8, 128> : sym_jvp_kernel<0, 128>)
+                      : (a.zeta_int == 8 ? sym_jvp_kernel<8, 256> : sym_jvp_kernel<0, 256>);
   k<<<tm_div_up(N, SYM_WAVES), SYM_WAVES * 64, 0, s>>>(N, pos, box, mol, elem, nbr, ncnt, rseg,
                                                        ang, acnt, aseg, ldr, lda, a, du, V, ldg);
   TM_LAUNCH_CHECK();

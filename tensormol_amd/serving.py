@@ -27,11 +27,18 @@ A request found finished in 4 that step 1 did not foresee (a converged FIRE
 run, a failure) is already in the next batch: it rides along once more and
 that step's output for it is discarded, so its result is the state at the
 step that finished it.
+A request whose neighbour rows reach the engine's capacity (a collapsed
+geometry) is not failed: it is run again from its inputs by a second server
+whose engine has wider rows (256 / 128 by default), in a thread of its own on
+a stream of its own, while the batch loop goes on; its answer is that run's.
+The rows of both engines hold the same neighbours in the same order, so a
+request that fits both gives the same result on either.
 Per-request state never leaves the device while the request is active, and
 every per-molecule quantity (descriptors, networks, Coulomb, reductions,
 integrators) depends only on the molecule's own atoms, so a request gives the
 same result alone and inside any mix.
 """
+import threading
 import time
 from dataclasses import dataclass, field
 
@@ -64,6 +71,7 @@ class Request:
     t_done: float = None
     converged: bool = None
     error: str = None       # set when the request failed (its molecule only)
+    rescued: bool = False   # answered by the wide-row server after a row overflow
     # server bookkeeping: atom offset and molecule slot in the current packed batch
     _off: int = field(default=-1, repr=False)
     _mol: int = field(default=-1, repr=False)
@@ -82,7 +90,7 @@ class Server:
 
     def __init__(self, engine, max_atoms=65536, max_mols=8192, hbm_budget_bytes=None, T=300.0,
                  dt=0.5, fire_dt0=1.0, fire_fmax=5e-3, fire_maxstep=0.1, repack_every=1,
-                 graphs=False):
+                 graphs=False, rescue_rows=(256, 128)):
         """repack_every: with K > 1, a batch whose membership changes only by
         departures and arrivals is repacked at most every K iterations; in
         between, the previous layout is relaunched with departed requests as
@@ -91,8 +99,14 @@ class Server:
         graphs: a relaunched layout's step (every kernel of the evaluation and
         the integrators) is captured into a HIP graph on its first relaunch
         and replayed for the following ones, so an unchanged batch costs one
-        graph launch instead of one launch per kernel."""
+        graph launch instead of one launch per kernel.
+        rescue_rows: (ldr, lda) of the engine that re-runs requests whose rows
+        overflow this engine's (None: such requests fail)."""
         self.eng = engine
+        self.rescue_rows = None
+        if rescue_rows is not None and (rescue_rows[0] > engine.ldr or rescue_rows[1] > engine.lda):
+            self.rescue_rows = (int(rescue_rows[0]), int(rescue_rows[1]))
+        self._rescue = None        # the wide-row server, made on the first overflow
         self.repack_every = max(1, int(repack_every))
         self.use_graphs = bool(graphs)
         self._layout = 0           # incremented by every repack
@@ -277,7 +291,8 @@ class Server:
         side.wait_stream(cur)
         g = torch.cuda.CUDAGraph()
         with torch.cuda.stream(side):
-            g.capture_begin()
+            # thread-local capture: the wide-row server's thread may launch meanwhile
+            g.capture_begin(capture_error_mode="thread_local")
             try:
                 self._step()
             finally:
@@ -343,6 +358,12 @@ class Server:
             if r._retired:      # finished at the previous step; this output is discarded
                 continue
             r.steps_done += 1
+            if bad[k] and row_full[k] and self.rescue_rows is not None:
+                if self._rescue is None:
+                    self._rescue = _RowRescue(self)
+                self._rescue.submit(r, t0)   # answered by the wide-row server
+                r._retired = True
+                continue
             if bad[k]:
                 r.error = ("neighbour row at capacity (collapsed geometry)" if row_full[k]
                            else "non-finite energy or forces")
@@ -430,6 +451,8 @@ class Server:
                 self.atom_steps += nxt["N"]
             if inflight is not None:
                 self._finish(inflight, t0, fthr2, done)
+                if self._rescue is not None:
+                    done.extend(self._rescue.take())
                 if on_iter is not None:
                     on_iter(self, it)
             inflight = nxt
@@ -441,10 +464,84 @@ class Server:
                     time.sleep(min(wait, 0.001))
         if inflight is not None:
             self._finish(inflight, t0, fthr2, done)
+        if self._rescue is not None:
+            done.extend(self._rescue.close())
+            self._rescue = None
         self.active = []
         self.wall = time.perf_counter() - t0
         self.iters = it
         return done
+
+
+class _RowRescue:
+    """Requests whose neighbour rows overflowed, run again from their inputs on
+    a server whose engine has wider rows, in a thread of its own on a stream
+    of its own.  The original request is answered with that run's result (its
+    latency runs to the moment the answer is handed back)."""
+
+    def __init__(self, server):
+        from .engine import BPEEEngine
+        e = server.eng
+        ldr, lda = server.rescue_rows
+        eng = BPEEEngine(e.model, device=e.device, precision=e.precision, ldr=ldr, lda=lda,
+                         check_overflow=e.check_overflow)
+        fdt0, fmax, fmaxstep = server.fire
+        self.server = Server(eng, max_atoms=server.max_atoms, max_mols=min(server.max_mols, 1024),
+                             T=server.T, dt=server.dt, fire_dt0=fdt0, fire_fmax=fmax,
+                             fire_maxstep=fmaxstep, rescue_rows=None)
+        self.cv = threading.Condition()
+        self.queue = []
+        self.out = []
+        self.closing = False
+        self.error = None
+        self.stream = torch.cuda.Stream(device=eng.device)
+        self.thread = threading.Thread(target=self._loop, daemon=True)
+        self.thread.start()
+
+    def submit(self, orig, t0):
+        copy = Request(orig.rid, orig.Z, orig.xyz, orig.kind, orig.nsteps, 0.0, seed=orig.seed)
+        with self.cv:
+            self.queue.append((orig, copy, t0))
+            self.cv.notify()
+
+    def _loop(self):
+        try:
+            with torch.cuda.device(self.server.eng.device), torch.cuda.stream(self.stream):
+                while True:
+                    with self.cv:
+                        while not self.queue and not self.closing:
+                            self.cv.wait()
+                        if not self.queue:
+                            return
+                        items, self.queue = self.queue, []
+                    self.server.run([c for _, c, _ in items], realtime=False)
+                    now = time.perf_counter()
+                    with self.cv:
+                        for orig, c, t0 in items:
+                            orig.energy, orig.forces, orig.positions = c.energy, c.forces, c.positions
+                            orig.steps_done, orig.converged, orig.error = (c.steps_done, c.converged,
+                                                                          c.error)
+                            orig.rescued = True
+                            orig.t_done = now - t0
+                            self.out.append(orig)
+        except BaseException as ex:   # reported by close()
+            self.error = ex
+
+    def take(self):
+        """Requests answered since the last call."""
+        with self.cv:
+            out, self.out = self.out, []
+        return out
+
+    def close(self):
+        """Finish the queued requests; returns those not yet taken."""
+        with self.cv:
+            self.closing = True
+            self.cv.notify()
+        self.thread.join()
+        if self.error is not None:
+            raise RuntimeError("wide-row server failed") from self.error
+        return self.take()
 
 
 def latency_stats(done):

@@ -171,7 +171,7 @@ def test_failure_stays_with_its_molecule(served):
     normal = [r for r in reqs if r.kind == "sp"][:2] + [r for r in reqs if r.kind == "nvt"][:2]
     Zd, xd = _dense_molecule()
     bad = Request(10 ** 6, Zd, xd, "nvt", 6, 0.0, seed=1)
-    s2 = Server(eng, max_atoms=1024, max_mols=8)
+    s2 = Server(eng, max_atoms=1024, max_mols=8, rescue_rows=None)
     out = {r.rid: r for r in s2.run(_copy(normal) + [bad], realtime=False)}
     assert out[bad.rid].error is not None and out[bad.rid].steps_done == 1
     assert out[bad.rid].positions is None
@@ -182,6 +182,65 @@ def test_failure_stays_with_its_molecule(served):
             np.testing.assert_array_equal(o.forces, d.forces)
         else:
             np.testing.assert_array_equal(o.positions, d.positions)
+
+
+def test_row_overflow_rescued_by_wide_rows(served):
+    """A molecule whose angular rows overflow the 48-entry rows is answered by
+    the wide-row (256 / 128) server: the same result as that server gives it
+    alone, bit for bit, while its batch-mates keep their results."""
+    from tensormol_amd.engine import BPEEEngine
+    from tensormol_amd.serving import Request, Server
+    model, eng, srv, reqs, done = served
+    normal = [r for r in reqs if r.kind == "sp"][:2] + [r for r in reqs if r.kind == "opt"][:2]
+    Zd, xd = _dense_molecule()
+    dense = [Request(10 ** 6, Zd, xd, "nvt", 6, 0.0, seed=1),
+             Request(10 ** 6 + 1, Zd, xd, "sp", 1, 0.0)]
+    s2 = Server(eng, max_atoms=1024, max_mols=8)
+    out = {r.rid: r for r in s2.run(_copy(normal) + _copy(dense), realtime=False)}
+    torch.cuda.synchronize()
+    assert sorted(out) == sorted([r.rid for r in normal] + [r.rid for r in dense])
+    for r in normal:
+        o, d = out[r.rid], done[r.rid]
+        assert o.error is None and not o.rescued
+        if r.kind == "sp":
+            np.testing.assert_array_equal(o.forces, d.forces)
+        else:
+            np.testing.assert_array_equal(o.positions, d.positions)
+    wide = Server(BPEEEngine(model, precision=eng.precision, ldr=256, lda=128), max_atoms=1024,
+                  max_mols=8, rescue_rows=None)
+    alone = {r.rid: r for r in wide.run(_copy(dense), realtime=False)}
+    torch.cuda.synchronize()
+    for r in dense:
+        o, a = out[r.rid], alone[r.rid]
+        assert o.rescued and o.error is None and a.error is None, (o.error, a.error)
+        assert o.steps_done == r.nsteps and o.energy == a.energy
+        if r.kind == "sp":
+            np.testing.assert_array_equal(o.forces, a.forces)
+        else:
+            np.testing.assert_array_equal(o.positions, a.positions)
+    assert o.t_done is not None
+
+
+def test_rescued_single_point_vs_oracle(cuda):
+    """The collapsed molecule's energy and forces from the wide-row server (FP32
+    path) against the FP64 oracle."""
+    from tensormol_amd.engine import BPEEEngine
+    from tensormol_amd.serving import Request, Server
+    model = H.chno_model()
+    Zd, xd = _dense_molecule()
+    srv = Server(BPEEEngine(model, precision="fp32"), max_atoms=256, max_mols=4)
+    (r,) = srv.run([Request(7, Zd, xd, "sp", 1, 0.0)], realtime=False)
+    torch.cuda.synchronize()
+    assert r.rescued and r.error is None
+    ref = _oracle(model, Zd, xd)
+    dE = abs(r.energy - ref["Etotal"][0])
+    F_ref = -ref["gradient"] * JOULEPERHARTREE
+    frms = np.sqrt(((r.forces - F_ref) ** 2).mean()) / JOULEPERHARTREE
+    fscale = np.sqrt((F_ref ** 2).mean()) / JOULEPERHARTREE
+    print(f"dense 60-atom ball: |dE| {dE:.2e} Ha ({dE / len(Zd):.2e} per atom), force rms err "
+          f"{frms:.2e} Ha/A (rms {fscale:.2e})")
+    assert dE / len(Zd) < 2e-6
+    assert frms < 1e-4 * max(1.0, fscale)
 
 
 def test_lazy_repack_same_results(served):
