@@ -248,7 +248,7 @@ def run_sphere(args):
     frms = float(np.sqrt(((r["gradient"] - ref["gradient"]) ** 2).mean()))
     dE = float(abs(r["Etotal"][0] - ref["Etotal"][0]))
     log(f"oracle {cpu_s:.1f} s: force rms err {frms:.3e} Ha/A, |dE| {dE:.3e} Ha")
-    eng32 = BPEEEngine(model, precision="fp32", check_overflow=False)
+    eng32 = BPEEEngine(model, precision="fp32", lda=128, check_overflow=False)
     ws32 = eng32.prepare(Z, ms, None, pos0=x)
     with torch.cuda.stream(side):
         eng32.run(ws32, pos)
@@ -447,7 +447,10 @@ def run_serving(args, stream):
 
     log = lambda m: print(f"[serve] {m}", file=sys.stderr, flush=True)
     model = BPEEModel([1, 6, 7, 8], dsf_alpha=0.2, seed=0)
-    eng = BPEEEngine(model, precision=args.precision, check_overflow=False)
+    # angular rows of 128 entries (the loops run over the actual counts, so a
+    # normal molecule pays nothing for them); requests that still overflow are
+    # answered by the server's wide-row (256 / 128) engine
+    eng = BPEEEngine(model, precision=args.precision, lda=128, check_overflow=False)
     srv = Server(eng, max_atoms=args.serve_atoms, max_mols=8192,
                  hbm_budget_bytes=args.serve_hbm_gib * 2 ** 30,
                  repack_every=args.serve_repack_every)
@@ -457,6 +460,7 @@ def run_serving(args, stream):
     lat = latency_stats(done)
     rt_rate = atom_steps(done) / srv.wall
     rt_iters, rt_wall = srv.iters, srv.wall
+    rt_rescued = srv.rescued   # row overflows answered by the wide-row server
     log(f"realtime: {len(done)} requests in {rt_wall:.2f} s, {rt_iters} iterations, "
         f"p50 {lat['p50_ms']:.2f} ms p99 {lat['p99_ms']:.2f} ms, {rt_rate:.4g} atom-steps/s")
     # saturated: the first pass at full batch size runs ~20 % slower (warm-up of
@@ -475,7 +479,7 @@ def run_serving(args, stream):
                          dsf_alpha=model.dsf_alpha, has_vdw=model.has_vdw)
         errs.append(float(np.sqrt(((r.forces / 2625499.638 + ref["gradient"]) ** 2).mean())))
     # plain FP32 path: continuous batching (separates the batching and MFMA gains) ...
-    eng32 = BPEEEngine(model, precision="fp32", check_overflow=False)
+    eng32 = BPEEEngine(model, precision="fp32", lda=128, check_overflow=False)
     srv32 = Server(eng32, max_atoms=args.serve_atoms, max_mols=8192,
                    hbm_budget_bytes=args.serve_hbm_gib * 2 ** 30)
     srv32.run(mk(), realtime=False)   # untimed first pass, as for the BF16 server
@@ -515,6 +519,7 @@ def run_serving(args, stream):
         "repack_every": args.serve_repack_every,
         "capacity_atoms": srv.max_atoms,
         "failed_requests_by_kind": failed,
+        "rescued_requests": rt_rescued,
         "force_rms_err_vs_fp64_max": max(errs) if errs else None,
         "force_check_sample": f"first {len(errs)} single-point requests",
     }
@@ -637,7 +642,7 @@ def main():
     # plain FP32 path (VALU GEMMs, every intermediate in HBM) on the same box
     fp32_value = None
     if not args.no_fp32 and args.precision != "fp32":
-        eng32 = BPEEEngine(model, precision="fp32", check_overflow=False)
+        eng32 = BPEEEngine(model, precision="fp32", lda=128, check_overflow=False)
         ws32 = eng32.prepare(Z, np.array([0, N]), box[None])
         # same warmup, step count and HIP-graph replay as the timed run: the
         # random-weight system heats, so a shorter or earlier segment is cheaper

@@ -29,8 +29,8 @@ that step's output for it is discarded, so its result is the state at the
 step that finished it.
 A request whose neighbour rows reach the engine's capacity (a collapsed
 geometry) is not failed: it is run again from its inputs by a second server
-whose engine has wider rows (256 / 128 by default), in a thread of its own on
-a stream of its own, while the batch loop goes on; its answer is that run's.
+whose engine has wider rows (256 / 128 by default), on a stream of its own,
+one of its iterations after each of the batch loop's; its answer is that run's.
 The rows of both engines hold the same neighbours in the same order, so a
 request that fits both gives the same result on either.
 Per-request state never leaves the device while the request is active, and
@@ -38,7 +38,6 @@ every per-molecule quantity (descriptors, networks, Coulomb, reductions,
 integrators) depends only on the molecule's own atoms, so a request gives the
 same result alone and inside any mix.
 """
-import threading
 import time
 from dataclasses import dataclass, field
 
@@ -107,6 +106,7 @@ class Server:
         if rescue_rows is not None and (rescue_rows[0] > engine.ldr or rescue_rows[1] > engine.lda):
             self.rescue_rows = (int(rescue_rows[0]), int(rescue_rows[1]))
         self._rescue = None        # the wide-row server, made on the first overflow
+        self.rescued = 0           # requests answered by it in the last run
         self.repack_every = max(1, int(repack_every))
         self.use_graphs = bool(graphs)
         self._layout = 0           # incremented by every repack
@@ -291,8 +291,7 @@ class Server:
         side.wait_stream(cur)
         g = torch.cuda.CUDAGraph()
         with torch.cuda.stream(side):
-            # thread-local capture: the wide-row server's thread may launch meanwhile
-            g.capture_begin(capture_error_mode="thread_local")
+            g.capture_begin()
             try:
                 self._step()
             finally:
@@ -392,6 +391,16 @@ class Server:
     def run(self, requests, realtime=True, max_iters=10 ** 7, on_iter=None):
         """Serve `requests`.  With realtime, arrivals follow the wall clock;
         otherwise every request is available at once."""
+        gen = self.iterate(requests, realtime, max_iters, on_iter)
+        while True:
+            try:
+                next(gen)
+            except StopIteration as stop:
+                return stop.value
+
+    def iterate(self, requests, realtime=True, max_iters=10 ** 7, on_iter=None):
+        """run() as a generator: yields after every iteration of the batch loop
+        and returns the finished requests."""
         pending = sorted(requests, key=lambda r: r.t_arrival)
         for r in pending:
             if len(r.Z) > self.max_atoms:
@@ -451,10 +460,11 @@ class Server:
                 self.atom_steps += nxt["N"]
             if inflight is not None:
                 self._finish(inflight, t0, fthr2, done)
-                if self._rescue is not None:
-                    done.extend(self._rescue.take())
                 if on_iter is not None:
                     on_iter(self, it)
+            if self._rescue is not None:
+                self._rescue.step()
+                done.extend(self._rescue.take())
             inflight = nxt
             if inflight is None:
                 if pi >= len(pending):
@@ -462,10 +472,13 @@ class Server:
                 wait = pending[pi].t_arrival - (time.perf_counter() - t0)
                 if realtime and wait > 0:
                     time.sleep(min(wait, 0.001))
+            yield it
         if inflight is not None:
             self._finish(inflight, t0, fthr2, done)
+        self.rescued = 0
         if self._rescue is not None:
             done.extend(self._rescue.close())
+            self.rescued = self._rescue.count
             self._rescue = None
         self.active = []
         self.wall = time.perf_counter() - t0
@@ -475,9 +488,9 @@ class Server:
 
 class _RowRescue:
     """Requests whose neighbour rows overflowed, run again from their inputs on
-    a server whose engine has wider rows, in a thread of its own on a stream
-    of its own.  The original request is answered with that run's result (its
-    latency runs to the moment the answer is handed back)."""
+    a server whose engine has wider rows, on a stream of its own.  Its batch
+    loop advances one iteration per call of step(); the original request is
+    answered with that run's result (its latency runs to that moment)."""
 
     def __init__(self, server):
         from .engine import BPEEEngine
@@ -486,61 +499,56 @@ class _RowRescue:
         eng = BPEEEngine(e.model, device=e.device, precision=e.precision, ldr=ldr, lda=lda,
                          check_overflow=e.check_overflow)
         fdt0, fmax, fmaxstep = server.fire
+        # its batches change only by departures: relaunched layouts replay as graphs
         self.server = Server(eng, max_atoms=server.max_atoms, max_mols=min(server.max_mols, 1024),
                              T=server.T, dt=server.dt, fire_dt0=fdt0, fire_fmax=fmax,
-                             fire_maxstep=fmaxstep, rescue_rows=None)
-        self.cv = threading.Condition()
-        self.queue = []
-        self.out = []
-        self.closing = False
-        self.error = None
+                             fire_maxstep=fmaxstep, rescue_rows=None, repack_every=4,
+                             graphs=True)
         self.stream = torch.cuda.Stream(device=eng.device)
-        self.thread = threading.Thread(target=self._loop, daemon=True)
-        self.thread.start()
+        self.queue = []     # (original, copy, t0) waiting for the next batch
+        self.batch = None   # those of the batch being run
+        self.gen = None
+        self.out = []
+        self.count = 0
 
     def submit(self, orig, t0):
         copy = Request(orig.rid, orig.Z, orig.xyz, orig.kind, orig.nsteps, 0.0, seed=orig.seed)
-        with self.cv:
-            self.queue.append((orig, copy, t0))
-            self.cv.notify()
+        self.queue.append((orig, copy, t0))
+        self.count += 1
 
-    def _loop(self):
-        try:
-            with torch.cuda.device(self.server.eng.device), torch.cuda.stream(self.stream):
-                while True:
-                    with self.cv:
-                        while not self.queue and not self.closing:
-                            self.cv.wait()
-                        if not self.queue:
-                            return
-                        items, self.queue = self.queue, []
-                    self.server.run([c for _, c, _ in items], realtime=False)
-                    now = time.perf_counter()
-                    with self.cv:
-                        for orig, c, t0 in items:
-                            orig.energy, orig.forces, orig.positions = c.energy, c.forces, c.positions
-                            orig.steps_done, orig.converged, orig.error = (c.steps_done, c.converged,
-                                                                          c.error)
-                            orig.rescued = True
-                            orig.t_done = now - t0
-                            self.out.append(orig)
-        except BaseException as ex:   # reported by close()
-            self.error = ex
+    def step(self):
+        """One iteration of the wide-row batch loop (a new batch of the queued
+        requests when none is running)."""
+        if self.gen is None:
+            if not self.queue:
+                return
+            self.batch, self.queue = self.queue, []
+            self.gen = self.server.iterate([c for _, c, _ in self.batch], realtime=False)
+        with torch.cuda.stream(self.stream):
+            try:
+                next(self.gen)
+                return
+            except StopIteration:
+                pass
+        now = time.perf_counter()
+        for orig, c, t0 in self.batch:
+            orig.energy, orig.forces, orig.positions = c.energy, c.forces, c.positions
+            orig.steps_done, orig.converged, orig.error = c.steps_done, c.converged, c.error
+            orig.rescued = True
+            orig.t_done = now - t0
+            self.out.append(orig)
+        self.gen = self.batch = None
 
     def take(self):
         """Requests answered since the last call."""
-        with self.cv:
-            out, self.out = self.out, []
+        out, self.out = self.out, []
         return out
 
     def close(self):
-        """Finish the queued requests; returns those not yet taken."""
-        with self.cv:
-            self.closing = True
-            self.cv.notify()
-        self.thread.join()
-        if self.error is not None:
-            raise RuntimeError("wide-row server failed") from self.error
+        """Runs the remaining requests to the end; returns those not yet taken."""
+        while self.gen is not None or self.queue:
+            self.step()
+        torch.cuda.current_stream(self.server.eng.device).wait_stream(self.stream)
         return self.take()
 
 
