@@ -135,6 +135,39 @@ int tm_sym_jvp(int N, const double* pos, const double* box, const int* mol, cons
                float rr, float ra, const float* rs_r, const float* rs_a, const float* thetas,
                const double* du, float* V, int ldg, void* stream);
 
+/* ---- training-step MLP kernels ------------------------------------------
+ * The MLP half of the force-matching step's second-order graph
+ * (loss_op_EandG / train_step_EandG, TFMolInstanceDirect.py:4889-4903,
+ * 5288-5348, differentiating tf.gradients(E, xyz), :5761), written out as
+ * GEMMs: forward, the input gradient J = dy/dG, the tangent along
+ * V = dLoss/dJ, the reverse pass and the weight gradients
+ * (tensormol_amd/train.py lists the sequence).
+ *
+ * C = op(A) op(B) on v_mfma_f32_16x16x32_bf16: FP32 operands in memory,
+ * rounded to BF16 when staged, FP32 accumulation.  A(m,k) = ta ? A[k*lda+m] :
+ * A[m*lda+k]; B(k,n) = tb ? B[n*ldb+k] : B[k*ldb+n].  epi 0: C = acc (+ bias[n]
+ * when bias is given); 1: z = acc + bias[n], C = log(1 + e^{alpha z})/alpha,
+ * aux = sigmoid(alpha z); 2: C = acc, aux = acc * s (s, aux indexed as C).
+ * nsplit > 1 (epi 0, no bias): K is split over nsplit workgroup layers whose
+ * partial products go to work (nsplit * M * ldc floats) and are added in
+ * split order. */
+int tm_gemm_bf16(int M, int N, int K, const float* A, int lda, int ta, const float* B, int ldb,
+                 int tb, float* C, int ldc, int epi, const float* bias, const float* s,
+                 float* aux, float alpha, int nsplit, float* work, void* stream);
+
+/* Elementwise steps between those GEMMs over rows r < M, columns n < H (every
+ * [M][H] operand with row stride ld): op 0 out = s * w4[n]; 1 out = s * x;
+ * 2 out = s * ybar[r] * w4[n] + alpha s (1 - s) zt w4[n];
+ * 3 out = s * x + alpha s (1 - s) zt hd. */
+int tm_train_elem(int op, int M, int H, int ld, float alpha, const float* s, const float* x,
+                  const float* zt, const float* hd, const float* w4, const float* ybar, float* out,
+                  void* stream);
+
+/* Row gather (scatter = 0: out[r*ldo + c] = in[idx[r]*ldi + c]) or scatter
+ * (scatter = 1: out[idx[r]*ldo + c] = in[r*ldi + c]) of M rows, c < cols. */
+int tm_rows(int scatter, int M, int cols, const int* idx, const float* in, int ldi, float* out,
+            int ldo, void* stream);
+
 int tm_force_gather(int N, int ldr, const int* ncnt, const int* rev, const float* wslot,
                     const double* extra, double scale, double* grad, double* force,
                     void* stream);

@@ -53,6 +53,9 @@ SIGNATURES = {
     "tm_fire_dots": [I, P, P, P, P],
     "tm_fire_update": [I, I, P, I, P, P, P, P, P, P, P, D, I, D, D, D, D, D, P],
     "tm_mlp_mfma": [I, I, I, I, P, P, I, P, P, P, F, I, I, P, P, P],
+    "tm_gemm_bf16": [I, I, I, P, I, I, P, I, I, P, I, I, P, P, P, F, I, P, P],
+    "tm_train_elem": [I, I, I, I, F, P, P, P, P, P, P, P, P],
+    "tm_rows": [I, I, I, P, P, I, P, I, P],
     "tm_mlp_mfma_pack_info": [I, I, I, I, P],
 }
 

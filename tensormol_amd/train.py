@@ -15,7 +15,24 @@ graph.  Here that graph is split along its natural seam:
   dL/dW                                  torch double backward through J
 
 so the descriptor Jacobian is never stored: the adjoint and its transpose are
-recomputed from the neighbour lists.  The loss is
+recomputed from the neighbour lists.
+
+With mlp_dtype "bf16" the MLP half runs on hand-written MFMA kernels
+(tm_gemm_bf16, BF16 operands, FP32 accumulation) instead of torch autograd:
+per element, with rows r of its atoms, layers z_l = h_{l-1} W_l + b_l,
+h_l = act(z_l), s_l = act'(z_l) = sigmoid(a z_l), y = h_3 w_4 + b_4,
+  forward:    3 GEMMs with the bias / softplus epilogue, 1 for y;
+  J = dy/dG:  zd_3 = s_3 * w_4, hd_{l-1} = zd_l W_l^T, zd_{l-1} = s_{l-1} * hd_{l-1},
+              J = zd_1 W_1^T (3 GEMMs);
+  then F = -(dG/dx)^T J, the loss, ybar = dLoss/dy and V = dLoss/dJ (HIP
+  adjoint and tangent of the descriptors);
+  tangent:    zt_l = ht_{l-1} W_l, ht_l = s_l * zt_l, ht_0 = V (3 GEMMs);
+  reverse:    zb_3 = s_3 * ybar w_4 + a s_3 (1 - s_3) zt_3 w_4, hb_{l-1} = zb_l W_l^T,
+              zb_{l-1} = s_{l-1} * hb_{l-1} + a s (1 - s) zt_{l-1} hd_{l-1} (2 GEMMs);
+  gradients:  dW_l, db_l = [h_{l-1}, 1; ht_{l-1}, 0]^T [zb_l; zd_l], dw_4, db_4 =
+              [h_3, 1; ht_3, 0]^T [ybar; 1] (4 GEMMs over the stacked rows).
+That is the reverse pass of Phi = sum_r (ybar_r y_r + J_r . V_r), whose weight
+gradient is the loss's (V held fixed).  The loss is
 L = sum_b w_e[b] (E_b - E*_b)^2 + sum_i w_f[b(i)] |F_i - F*_i|^2 (F = -dE/dx,
 Ha/A); loss_op_EandG is w_e = EnergyScalar/2 (maxatom/natom_b)^2, w_f =
 GradScalar/2 (maxatom/natom_b)^2, and MSE(E) + 0.5 MSE(F) is w_e = 1/B,
@@ -27,7 +44,12 @@ import numpy as np
 import torch
 from torch.autograd.function import once_differentiable
 
+import ctypes as _ctypes
+
+from . import _lib
 from .engine import BPEEEngine
+
+_lib_ptr = _lib.ptr
 
 
 class DescriptorAdjoint(torch.autograd.Function):
@@ -74,9 +96,13 @@ class ForceMatchingTrainer:
     def __init__(self, model, device="cuda", mlp_dtype="fp32", lr=1e-4, ldr=128, lda=48,
                  capturable=False):
         """capturable: keep the optimizer's state on the device so that a whole
-        step can be captured into a HIP graph (capture())."""
-        if mlp_dtype not in ("fp32", "bf16"):
-            raise ValueError(f"mlp_dtype {mlp_dtype!r}: use 'fp32' or 'bf16'")
+        step can be captured into a HIP graph (capture()).
+        mlp_dtype: "fp32" (torch, FP32 GEMMs), "bf16" (hand-written MFMA kernels,
+        BF16 operands) or "bf16_torch" (torch with BF16-rounded operands in FP32
+        GEMMs, the same arithmetic up to the rounding of the backward GEMMs'
+        gradient operands and the summation order)."""
+        if mlp_dtype not in ("fp32", "bf16", "bf16_torch"):
+            raise ValueError(f"mlp_dtype {mlp_dtype!r}: use 'fp32', 'bf16' or 'bf16_torch'")
         self.model = model
         self.eng = BPEEEngine(model, device=device, precision="fp32", ldr=ldr, lda=lda)
         self.device = self.eng.device
@@ -116,7 +142,7 @@ class ForceMatchingTrainer:
     def _mlp(self, h, layers):
         a = self.alpha
         for k, (W, b) in enumerate(layers):
-            if self.mlp_dtype == "bf16":
+            if self.mlp_dtype == "bf16_torch":
                 # BF16 operands: their products are exact in FP32, so an FP32 GEMM
                 # of the rounded operands is the arithmetic of a BF16 MFMA GEMM
                 # with FP32 accumulation (up to summation order)
@@ -150,16 +176,159 @@ class ForceMatchingTrainer:
         return L, E, F
 
     def gradients(self, tb, pos):
+        if self.mlp_dtype == "bf16":
+            L, _, _, grads = self._mfma_step(tb, pos)
+            return L, grads
         L, E, F = self.loss(tb, pos)
         return L, torch.autograd.grad(L, self.parameters())
 
     def step(self, tb, pos, check=True):
         """One Adam step; returns the loss before the update."""
         self.opt.zero_grad(set_to_none=True)
-        L, _, _ = self.loss(tb, pos, check=check)
-        L.backward()
+        if self.mlp_dtype == "bf16":
+            L, _, _, grads = self._mfma_step(tb, pos, check=check)
+            for p, g in zip(self.parameters(), grads):
+                p.grad = g
+        else:
+            L, _, _ = self.loss(tb, pos, check=check)
+            L.backward()
         self.opt.step()
         return L
+
+    # ------------------------------------------------ MFMA path (mlp_dtype "bf16")
+    def _mfma_buffers(self, tb):
+        """Per-element device buffers of a batch (made once per batch).  Row
+        stride ld = width + 16; the column after the last feature of the
+        stacked inputs holds 1 for the primal rows and 0 for the tangent rows
+        (the bias column of the weight-gradient GEMMs)."""
+        bufs = getattr(tb, "_mfma", None)
+        if bufs is not None:
+            return bufs
+        dev, D = self.device, self.eng.D
+        H = [int(W.shape[1]) for (W, _) in self.params[self.model.eles[0]]][:3]
+        assert H[0] == H[1] == H[2], "the MFMA training path needs equal hidden widths"
+        H = H[0]
+        f32 = dict(dtype=torch.float32, device=dev)
+        ldx, ldh = D + 16, H + 16
+        bufs = {}
+        for e, idx in tb.idx.items():
+            M = int(idx.numel())
+            b = {"M": M, "idx": idx.to(torch.int32).contiguous(), "ldx": ldx, "ldh": ldh, "H": H}
+            b["X0"] = torch.zeros(2 * M, ldx, **f32)
+            b["X0"][:M, D] = 1.0
+            for k in ("H1", "H2", "H3"):
+                b[k] = torch.zeros(2 * M, ldh, **f32)
+                b[k][:M, H] = 1.0
+            for k in ("ZZ1", "ZZ2", "ZZ3"):
+                b[k] = torch.zeros(2 * M, ldh, **f32)
+            for k in ("S1", "S2", "S3", "ZT1", "ZT2", "ZT3", "HD1", "HD2", "HB"):
+                b[k] = torch.zeros(M, ldh, **f32)
+            b["Y"] = torch.zeros(M, 1, **f32)
+            b["YB"] = torch.ones(2 * M, 1, **f32)
+            b["Je"] = torch.zeros(M, D, **f32)
+            b["gW"] = [torch.zeros(D + 1, H, **f32), torch.zeros(H + 1, H, **f32),
+                       torch.zeros(H + 1, H, **f32), torch.zeros(H + 1, 1, **f32)]
+            b["nsplit"] = max(1, min(8, (2 * M) // 512))
+            b["work"] = torch.zeros(b["nsplit"] * (D + 1) * H, **f32)
+            bufs[e] = b
+        tb._mfma = bufs
+        return bufs
+
+    def _gemm(self, M, N, K, A, lda, ta, B, ldb, tb_, C, ldc, epi=0, bias=None, s=None, aux=None,
+              nsplit=1, work=None):
+        self.eng.lib.tm_gemm_bf16(M, N, K, A, lda, ta, B, ldb, tb_, C, ldc, epi,
+                                  None if bias is None else _lib_ptr(bias), s, aux, self.alpha,
+                                  nsplit, None if work is None else _lib_ptr(work),
+                                  self.eng._stream())
+
+    def _mfma_step(self, tb, pos, check=True):
+        """(L, E, F, gradients in parameters() order) of the BF16 MFMA path."""
+        eng, ws = self.eng, tb.ws
+        lib, st = eng.lib, eng._stream()
+        P = _lib_ptr
+        off = lambda t, rows, ld: _ctypes.c_void_p(t.data_ptr() + 4 * rows * ld)
+        eng.build_lists(ws, pos)
+        if check:
+            eng.check_flags(ws)
+        eng.descriptors(ws, pos)
+        N, D, a = tb.N, eng.D, self.alpha
+        bufs = self._mfma_buffers(tb)
+        f32 = dict(dtype=torch.float32, device=self.device)
+        eps = torch.zeros(N, 1, **f32)
+        J = torch.zeros(N, D, **f32)
+        for e, b in bufs.items():
+            M, ldx, ldh, H, idx = b["M"], b["ldx"], b["ldh"], b["H"], b["idx"]
+            (W1, b1), (W2, b2), (W3, b3), (W4, b4) = self.params[e]
+            lib.tm_rows(0, M, D, P(idx), P(ws.G), D, P(b["X0"]), ldx, st)
+            # forward
+            prev, ldp, Kp = b["X0"], ldx, D
+            for (W, bb, Hk, Sk) in ((W1, b1, "H1", "S1"), (W2, b2, "H2", "S2"), (W3, b3, "H3", "S3")):
+                self._gemm(M, H, Kp, P(prev), ldp, 0, P(W), H, 0, P(b[Hk]), ldh, 1, bias=bb,
+                           aux=P(b[Sk]))
+                prev, ldp, Kp = b[Hk], ldh, H
+            self._gemm(M, 1, H, P(b["H3"]), ldh, 0, P(W4), 1, 0, P(b["Y"]), 1, 0, bias=b4)
+            lib.tm_rows(1, M, 1, P(idx), P(b["Y"]), 1, P(eps), 1, st)
+            # J chain: zd_3 = s_3 w_4 (rows M.. of ZZ3), then down to J
+            lib.tm_train_elem(0, M, H, ldh, a, P(b["S3"]), None, None, None, P(W4), None,
+                              off(b["ZZ3"], M, ldh), st)
+            for (Wn, zz_in, hd, S, zz_out) in ((W3, "ZZ3", "HD2", "S2", "ZZ2"),
+                                               (W2, "ZZ2", "HD1", "S1", "ZZ1")):
+                self._gemm(M, H, H, off(b[zz_in], M, ldh), ldh, 0, P(Wn), H, 1, P(b[hd]), ldh)
+                lib.tm_train_elem(1, M, H, ldh, a, P(b[S]), P(b[hd]), None, None, None, None,
+                                  off(b[zz_out], M, ldh), st)
+            self._gemm(M, D, H, off(b["ZZ1"], M, ldh), ldh, 0, P(W1), H, 1, P(b["Je"]), D)
+            lib.tm_rows(1, M, D, P(idx), P(b["Je"]), D, P(J), D, st)
+        E = torch.zeros(tb.B, dtype=torch.float64, device=self.device).index_add(
+            0, tb.mol, eps[:, 0].double())
+        grad = torch.empty(N, 3, dtype=torch.float64, device=self.device)
+        eng.descriptor_vjp(ws, pos, J, grad)
+        F = -grad
+        wf = tb.w_f[tb.mol][:, None]
+        L = (tb.w_e * (E - tb.E_ref) ** 2).sum() + (wf * (F - tb.F_ref) ** 2).sum()
+        ybar = (2.0 * tb.w_e * (E - tb.E_ref))[tb.mol].float().reshape(N, 1).contiguous()
+        gg = (-2.0 * wf * (F - tb.F_ref)).contiguous()   # dLoss/dgrad
+        V = torch.empty(N, D, **f32)
+        eng.descriptor_jvp(ws, pos, gg, V)
+        grads = []
+        for e in self.model.eles:
+            b = bufs.get(e)
+            if b is None:   # no atoms of this element in the batch
+                grads += [torch.zeros_like(t) for (W, bb) in self.params[e] for t in (W, bb)]
+                continue
+            M, ldx, ldh, H, idx = b["M"], b["ldx"], b["ldh"], b["H"], b["idx"]
+            (W1, b1), (W2, b2), (W3, b3), (W4, b4) = self.params[e]
+            lib.tm_rows(0, M, D, P(idx), P(V), D, off(b["X0"], M, ldx), ldx, st)
+            lib.tm_rows(0, M, 1, P(idx), P(ybar), 1, P(b["YB"]), 1, st)
+            # tangent
+            prev, ldp, Kp = off(b["X0"], M, ldx), ldx, D
+            for (W, Hk, Sk, Zk) in ((W1, "H1", "S1", "ZT1"), (W2, "H2", "S2", "ZT2"),
+                                    (W3, "H3", "S3", "ZT3")):
+                self._gemm(M, H, Kp, prev, ldp, 0, P(W), H, 0, P(b[Zk]), ldh, 2, s=P(b[Sk]),
+                           aux=off(b[Hk], M, ldh))
+                prev, ldp, Kp = off(b[Hk], M, ldh), ldh, H
+            # reverse
+            lib.tm_train_elem(2, M, H, ldh, a, P(b["S3"]), None, P(b["ZT3"]), None, P(W4),
+                              P(b["YB"]), P(b["ZZ3"]), st)
+            for (Wn, zz_in, S, zt, hd, zz_out) in ((W3, "ZZ3", "S2", "ZT2", "HD2", "ZZ2"),
+                                                   (W2, "ZZ2", "S1", "ZT1", "HD1", "ZZ1")):
+                self._gemm(M, H, H, P(b[zz_in]), ldh, 0, P(Wn), H, 1, P(b["HB"]), ldh)
+                lib.tm_train_elem(3, M, H, ldh, a, P(b[S]), P(b["HB"]), P(b[zt]), P(b[hd]),
+                                  None, None, P(b[zz_out]), st)
+            # weight gradients over the stacked rows (bias from the ones column)
+            ns, wk = b["nsplit"], b["work"]
+            gW1, gW2, gW3, gw4 = b["gW"]
+            self._gemm(D + 1, H, 2 * M, P(b["X0"]), ldx, 1, P(b["ZZ1"]), ldh, 0, P(gW1), H,
+                       nsplit=ns, work=wk)
+            self._gemm(H + 1, H, 2 * M, P(b["H1"]), ldh, 1, P(b["ZZ2"]), ldh, 0, P(gW2), H,
+                       nsplit=ns, work=wk)
+            self._gemm(H + 1, H, 2 * M, P(b["H2"]), ldh, 1, P(b["ZZ3"]), ldh, 0, P(gW3), H,
+                       nsplit=ns, work=wk)
+            self._gemm(H + 1, 1, 2 * M, P(b["H3"]), ldh, 1, P(b["YB"]), 1, 0, P(gw4), 1,
+                       nsplit=ns, work=wk)
+            grads += [gW1[:D], gW1[D], gW2[:H], gW2[H], gW3[:H], gW3[H], gw4[:H], gw4[H]]
+        # parameters() order: element by element, (W, b) per layer; shapes as the parameters
+        grads = [g.reshape(p.shape).clone() for g, p in zip(grads, self.parameters())]
+        return L, E, F, grads
 
     def capture(self, tb, pos, warmup=3):
         """Capture one whole step (lists, descriptors, MLP, double backward,
@@ -180,7 +349,5 @@ class ForceMatchingTrainer:
         g = torch.cuda.CUDAGraph()
         self.opt.zero_grad(set_to_none=True)
         with torch.cuda.graph(g):
-            L, _, _ = self.loss(tb, pos, check=False)
-            L.backward()
-            self.opt.step()
+            L = self.step(tb, pos, check=False)
         return g, L

@@ -100,6 +100,26 @@ def test_training_gradient_vs_oracle(cuda, dtype, tol):
     assert err < tol
 
 
+def test_mfma_gradient_vs_torch_bf16(cuda):
+    """The hand-written MFMA path against torch on the same BF16-rounded forward
+    operands (the MFMA path also rounds the backward GEMMs' gradient operands):
+    the two weight gradients agree to 1e-2 of their norm, losses to 1e-4."""
+    from tensormol_amd.train import ForceMatchingTrainer
+    Z, x, ms, sp, lists, student, E_ref, F_ref, w_e, w_f = _setup()
+    pos = torch.tensor(x, dtype=torch.float64, device=cuda)
+    out = {}
+    for dt in ("bf16", "bf16_torch"):
+        tr = ForceMatchingTrainer(student, mlp_dtype=dt)
+        tb = tr.batch(Z, ms, E_ref, F_ref, w_e, w_f)
+        L, grads = tr.gradients(tb, pos)
+        out[dt] = (float(L), np.concatenate([g.double().cpu().numpy().ravel() for g in grads]))
+    (La, ga), (Lb, gb) = out["bf16"], out["bf16_torch"]
+    err = np.linalg.norm(ga - gb) / np.linalg.norm(gb)
+    print(f"MFMA vs torch bf16: loss {La:.6e} / {Lb:.6e}, gradient rel diff {err:.2e}")
+    assert abs(La - Lb) <= 1e-4 * abs(Lb)
+    assert err < 1e-2
+
+
 def test_training_gradient_finite_differences(cuda):
     """32 random weights: GPU gradient vs central differences of the FP64 loss."""
     from tensormol_amd.train import ForceMatchingTrainer

@@ -1,0 +1,132 @@
+"""The training step's MLP kernels through the C ABI: the BF16-operand MFMA GEMM
+against a plain PyTorch FP32 GEMM of the same BF16-rounded operands (every
+transpose combination, sizes that are not tile multiples, each epilogue, split
+K), the elementwise steps against their formulas, and the row gather /
+scatter.  The products of BF16 operands are exact in FP32, so the GEMM differs
+from the reference only by summation order: tolerance 1e-5 of sum |a||b|."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _lib():
+    from tensormol_amd import _lib
+    return _lib.lib(), _lib.ptr
+
+
+def _r(t):
+    return t.to(torch.bfloat16).float()
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("M,N,K", [(70, 130, 45), (64, 64, 32), (1, 513, 300), (257, 1, 96)])
+def test_gemm_plain_and_bias(cuda, ta, tb, M, N, K):
+    L, P = _lib()
+    g = torch.Generator(device="cpu").manual_seed(M * 7 + N * 3 + K)
+    A = torch.randn(M, K, generator=g).to(cuda)
+    B = torch.randn(K, N, generator=g).to(cuda)
+    bias = torch.randn(N, generator=g).to(cuda)
+    As = A.t().contiguous() if ta else A          # stored layout
+    Bs = B.t().contiguous() if tb else B
+    lda = As.shape[1]
+    ldb = Bs.shape[1]
+    ref = _r(A) @ _r(B)
+    scale = (_r(A).abs() @ _r(B).abs()).clamp_min(1e-6)
+    for bz in (None, bias):
+        C = torch.full((M, N + 3), float("nan"), device=cuda)
+        st = torch.cuda.current_stream().cuda_stream
+        L.tm_gemm_bf16(M, N, K, P(As), lda, ta, P(Bs), ldb, tb, P(C), N + 3, 0, P(bz), None,
+                       None, 1.0, 1, None, st)
+        torch.cuda.synchronize()
+        r = ref + (0 if bz is None else bias)
+        err = ((C[:, :N] - r).abs() / scale).max().item()
+        assert err < 1e-5, (ta, tb, M, N, K, bz is not None, err)
+        assert torch.isnan(C[:, N:]).all()   # nothing written past N
+
+
+def test_gemm_split_k(cuda):
+    L, P = _lib()
+    g = torch.Generator(device="cpu").manual_seed(5)
+    M, N, K = 513, 96, 4000
+    A = torch.randn(K, M, generator=g).to(cuda)   # stored transposed (A(m, k) = A[k][m])
+    B = torch.randn(K, N, generator=g).to(cuda)
+    ref = _r(A).t() @ _r(B)
+    scale = (_r(A).abs().t() @ _r(B).abs())
+    outs = []
+    for ns in (1, 3, 8):
+        C = torch.zeros(M, N, device=cuda)
+        work = torch.zeros(ns * M * N, device=cuda)
+        L.tm_gemm_bf16(M, N, K, P(A), M, 1, P(B), N, 0, P(C), N, 0, None, None, None, 1.0, ns,
+                       P(work), torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        assert ((C - ref).abs() / scale).max().item() < 1e-5
+        outs.append(C)
+    # the same split gives the same bits on a second call
+    C2 = torch.zeros(M, N, device=cuda)
+    work = torch.zeros(8 * M * N, device=cuda)
+    L.tm_gemm_bf16(M, N, K, P(A), M, 1, P(B), N, 0, P(C2), N, 0, None, None, None, 1.0, 8,
+                   P(work), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    assert torch.equal(C2, outs[-1])
+
+
+def test_gemm_softplus_and_tangent_epilogues(cuda):
+    L, P = _lib()
+    g = torch.Generator(device="cpu").manual_seed(11)
+    M, N, K, a = 100, 80, 72, 3.0
+    A = torch.randn(M, K, generator=g).to(cuda) * 0.3
+    B = torch.randn(K, N, generator=g).to(cuda) * 0.3
+    bias = torch.randn(N, generator=g).to(cuda)
+    s = torch.rand(M, N, generator=g).to(cuda)
+    st = torch.cuda.current_stream().cuda_stream
+    z = _r(A) @ _r(B) + bias
+    C = torch.zeros(M, N, device=cuda)
+    aux = torch.zeros(M, N, device=cuda)
+    L.tm_gemm_bf16(M, N, K, P(A), K, 0, P(B), N, 0, P(C), N, 1, P(bias), None, P(aux), a, 1, None,
+                   st)
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(C.cpu(), (torch.nn.functional.softplus(a * z) / a).cpu(),
+                               rtol=1e-4, atol=1e-5)
+    np.testing.assert_allclose(aux.cpu(), torch.sigmoid(a * z).cpu(), rtol=1e-4, atol=1e-5)
+    C.zero_(); aux.zero_()
+    L.tm_gemm_bf16(M, N, K, P(A), K, 0, P(B), N, 0, P(C), N, 2, None, P(s), P(aux), 1.0, 1, None,
+                   st)
+    torch.cuda.synchronize()
+    zt = _r(A) @ _r(B)
+    np.testing.assert_allclose(C.cpu(), zt.cpu(), rtol=1e-4, atol=1e-5)
+    np.testing.assert_allclose(aux.cpu(), (zt * s).cpu(), rtol=1e-4, atol=1e-5)
+
+
+def test_train_elem_ops(cuda):
+    L, P = _lib()
+    g = torch.Generator(device="cpu").manual_seed(3)
+    M, H, ld, a = 37, 50, 66, 2.5
+    mk = lambda: torch.randn(M, ld, generator=g).to(cuda)
+    s, x, zt, hd = torch.rand(M, ld, generator=g).to(cuda), mk(), mk(), mk()
+    w4, yb = torch.randn(H, generator=g).to(cuda), torch.randn(M, generator=g).to(cuda)
+    st = torch.cuda.current_stream().cuda_stream
+    S, X, ZT, HD = s[:, :H], x[:, :H], zt[:, :H], hd[:, :H]
+    ref = {0: S * w4, 1: S * X, 2: S * yb[:, None] * w4 + a * S * (1 - S) * ZT * w4,
+           3: S * X + a * S * (1 - S) * ZT * HD}
+    for op, r in ref.items():
+        out = torch.zeros(M, ld, device=cuda)
+        L.tm_train_elem(op, M, H, ld, a, P(s), P(x), P(zt), P(hd), P(w4), P(yb), P(out), st)
+        torch.cuda.synchronize()
+        np.testing.assert_allclose(out[:, :H].cpu(), r.cpu(), rtol=1e-6, atol=1e-6)
+        assert (out[:, H:] == 0).all()
+
+
+def test_rows_gather_scatter(cuda):
+    L, P = _lib()
+    idx = torch.tensor([5, 0, 3, 9], dtype=torch.int32, device=cuda)
+    src = torch.arange(10 * 7, dtype=torch.float32, device=cuda).reshape(10, 7)
+    out = torch.zeros(4, 9, device=cuda)
+    st = torch.cuda.current_stream().cuda_stream
+    L.tm_rows(0, 4, 7, P(idx), P(src), 7, P(out), 9, st)
+    back = torch.zeros(10, 7, device=cuda)
+    L.tm_rows(1, 4, 7, P(idx), P(out), 9, P(back), 7, st)
+    torch.cuda.synchronize()
+    assert torch.equal(out[:, :7], src[idx.long()])
+    assert torch.equal(back[idx.long()], src[idx.long()])
