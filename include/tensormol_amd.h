@@ -155,6 +155,29 @@ int tm_gemm_bf16(int M, int N, int K, const float* A, int lda, int ta, const flo
                  int tb, float* C, int ldc, int epi, const float* bias, const float* s,
                  float* aux, float alpha, int nsplit, float* work, void* stream);
 
+/* One GEMM of a grouped call; fields as the arguments of tm_gemm_bf16. */
+typedef struct tm_gemm_group {
+  int M, N, K;
+  const float* A;
+  int lda;
+  const float* B;
+  int ldb;
+  float* C;
+  int ldc;
+  const float* bias;
+  const float* s;
+  float* aux;
+  float* work;
+} tm_gemm_group;
+
+/* ngroups independent GEMMs (groups: host array) with the same operand forms,
+ * epilogue and split count, in one launch per 16 of them: the per-element
+ * networks of one layer (and the four weight-gradient GEMMs of every element)
+ * run side by side instead of one after another.  Each result equals the
+ * corresponding tm_gemm_bf16 call bit for bit. */
+int tm_gemm_bf16_grouped(int ngroups, const tm_gemm_group* groups, int ta, int tb, int epi,
+                         float alpha, int nsplit, void* stream);
+
 /* Elementwise steps between those GEMMs over rows r < M, columns n < H (every
  * [M][H] operand with row stride ld): op 0 out = s * w4[n]; 1 out = s * x;
  * 2 out = s * ybar[r] * w4[n] + alpha s (1 - s) zt w4[n];

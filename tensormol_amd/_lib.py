@@ -54,6 +54,7 @@ SIGNATURES = {
     "tm_fire_update": [I, I, P, I, P, P, P, P, P, P, P, D, I, D, D, D, D, D, P],
     "tm_mlp_mfma": [I, I, I, I, P, P, I, P, P, P, F, I, I, P, P, P],
     "tm_gemm_bf16": [I, I, I, P, I, I, P, I, I, P, I, I, P, P, P, F, I, P, P],
+    "tm_gemm_bf16_grouped": [I, P, I, I, I, F, I, P],
     "tm_train_elem": [I, I, I, I, F, P, P, P, P, P, P, P, P],
     "tm_rows": [I, I, I, P, P, I, P, I, P],
     "tm_mlp_mfma_pack_info": [I, I, I, I, P],
@@ -94,6 +95,12 @@ class _Lib:
                 raise RuntimeError(f"{name} failed ({st}): {msg}")
             return st
         return call
+
+
+class GemmGroup(ctypes.Structure):
+    """tm_gemm_group: one GEMM of a tm_gemm_bf16_grouped call."""
+    _fields_ = [("M", I), ("N", I), ("K", I), ("A", P), ("lda", I), ("B", P), ("ldb", I),
+                ("C", P), ("ldc", I), ("bias", P), ("s", P), ("aux", P), ("work", P)]
 
 
 _LIB = None

@@ -20,6 +20,9 @@
 // Each weight gradient is one GEMM over the stacked rows [h; ht]^T [zb; zd],
 // with a column of ones appended to h (zeros to ht) so that the bias gradient
 // comes out as the last row.
+#include <algorithm>
+
+#include "tensormol_amd.h"
 #include "tm_common.h"
 
 namespace {
@@ -27,112 +30,185 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-constexpr int GT = 64;        // C tile edge per workgroup
-constexpr int GK = 32;        // K step (one v_mfma_f32_16x16x32_bf16)
-constexpr int LROW = GK + 8;  // LDS row of a staged operand, in bf16 (padded)
+constexpr int GT = 128;       // C tile edge per workgroup (four waves of 64 x 64)
+constexpr int GK = 64;        // K per staged step (two v_mfma_f32_16x16x32_bf16 deep)
+constexpr int LROW = GK + 8;  // LDS row of a staged operand, in bf16 (144 B, padded)
+constexpr int MAXG = 16;      // GEMMs per grouped launch
+constexpr int NXCD = 8;       // XCDs; workgroups are dealt to them round-robin
 
 enum { EPI_STORE = 0, EPI_SOFTPLUS = 1, EPI_TANGENT = 2 };
 
-struct GemmArgs {
-  int M, N, K;
+// One GEMM of a grouped launch: the caller's tm_gemm_group and its tiling.
+struct GemmDev {
   const float* A;
-  int lda, ta;        // A(m, k) = ta ? A[k * lda + m] : A[m * lda + k]
   const float* B;
-  int ldb, tb;        // B(k, n) = tb ? B[n * ldb + k] : B[k * ldb + n]
-  float* C;
-  int ldc;
-  int epi;
-  const float* bias;  // [N] (EPI_STORE: may be NULL; EPI_SOFTPLUS: required)
-  const float* s;     // EPI_TANGENT: aux = acc * s (s[m * ldc + n])
-  float* aux;         // EPI_SOFTPLUS: sigmoid(a z); EPI_TANGENT: acc * s
-  float alpha;
-  int kchunk;         // K range of one split (blockIdx.z)
-  long cstride;       // element offset between the splits' partial outputs
+  float* C;            // where the GEMM kernel writes: out, or the split partials
+  float* out;
+  const float* bias;
+  const float* s;
+  float* aux;
+  long cstride;        // floats between the splits' partial outputs
+  int M, N, K, lda, ldb, ldc;
+  int kchunk;          // K range of one split (a multiple of GK)
+  int tiles_n;         // column tiles
+  int tile0;           // first linear tile of this GEMM
 };
 
-__device__ __forceinline__ unsigned short to_bf16(float x) {
+struct GemmBatch {
+  GemmDev g[MAXG];
+  int ngroups, ntiles, per_xcd, epi;
+  float alpha;
+};
+
+__device__ __forceinline__ unsigned bf16_bits(float x) {
   return __builtin_bit_cast(unsigned short, (__bf16)x);
 }
 
-// C tile (GT x GT) of one workgroup, four waves of 32 x 32 (2 x 2 MFMA tiles).
-// Operands are staged through LDS as BF16, A as [m][k] and B as [n][k], so
-// every MFMA fragment is one 16-byte read: lane l takes row l % 16 and k
-// elements 8 (l / 16) .. 8 (l / 16) + 7; the accumulator holds rows
-// 4 (l / 16) .. + 3 of column l % 16.
-__global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
-  __shared__ unsigned short As[GT][LROW];
-  __shared__ unsigned short Bs[GT][LROW];
-  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-  const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
-  const int k_begin = blockIdx.z * g.kchunk;
-  const int k_end = min(g.K, k_begin + g.kchunk);
-  const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
-  floatx4 acc[2][2];
+__device__ __forceinline__ uint4 pack8(const float* v) {
+  return make_uint4(bf16_bits(v[0]) | (bf16_bits(v[1]) << 16),
+                    bf16_bits(v[2]) | (bf16_bits(v[3]) << 16),
+                    bf16_bits(v[4]) | (bf16_bits(v[5]) << 16),
+                    bf16_bits(v[6]) | (bf16_bits(v[7]) << 16));
+}
+
+// One operand's share of a K step (GT rows x GK), 32 values per thread, held
+// in registers from its global load to its LDS store.  Both forms leave the
+// operand in LDS as S[row][k] in BF16, so an MFMA fragment (8 consecutive k
+// of one row) is one 16-byte read, and each thread's LDS writes are 16 bytes.
+// KCONT, element (r, k) at X[r * ld + k]: thread t takes k = 8 (t % 8) .. + 7
+// of rows t / 8 + 32 i (two 16-byte loads per row).
+// Otherwise element (r, k) at X[k * ld + r]: thread t takes row t % 128 at
+// k = 8 (t / 128 + 2 i) .. + 7, one 4-byte load per k, so a wave's load reads
+// 256 contiguous bytes.
+template <bool KCONT>
+struct Stage {
+  float v[32];
+
+  __device__ __forceinline__ void load(const float* __restrict__ X, long ld, int rows, int r0,
+                                       int k0, int kend, int tid) {
+    if (KCONT) {
+      const int k = k0 + (tid & 7) * 8;
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < 4; ++i) {
+        const int r = r0 + (tid >> 3) + 32 * i;
+        const float* p = X + (long)r * ld + k;
+        float* d = v + 8 * i;
+        if (r < rows && k + 8 <= kend && (reinterpret_cast<uintptr_t>(p) & 15) == 0) {
+          const float4 f0 = reinterpret_cast<const float4*>(p)[0];
+          const float4 f1 = reinterpret_cast<const float4*>(p)[1];
+          d[0] = f0.x; d[1] = f0.y; d[2] = f0.z; d[3] = f0.w;
+          d[4] = f1.x; d[5] = f1.y; d[6] = f1.z; d[7] = f1.w;
+        } else {
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = k_begin; k0 < k_end; k0 += GK) {
-    // stage A (GT x GK) and B (GK x GT): 8 elements per thread each
-    if (!g.ta) {   // contiguous along k
-      const int r = tid >> 2, kk = (tid & 3) * 8;
-      const int m = m0 + r;
-      unsigned short v[8];
-#pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const int k = k0 + kk + i;
-        v[i] = to_bf16((m < g.M && k < k_end) ? g.A[(long)m * g.lda + k] : 0.f);
+          for (int q = 0; q < 8; ++q) d[q] = (r < rows && k + q < kend) ? p[q] : 0.f;
+        }
       }
+    } else {
+      const int r = r0 + (tid & 127);
+      const float* p = X + r;
+      if (k0 + GK <= kend) {
+        if (r < rows) {
 #pragma unroll
-      for (int i = 0; i < 8; ++i) As[r][kk + i] = v[i];
-    } else {       // contiguous along m
-      const int kk = tid >> 3, r = (tid & 7) * 8;
-      const int k = k0 + kk;
+          for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const int m = m0 + r + i;
-        As[r + i][kk] = to_bf16((m < g.M && k < k_end) ? g.A[(long)k * g.lda + m] : 0.f);
+            for (int q = 0; q < 8; ++q)
+              v[8 * i + q] = p[(long)(k0 + 8 * ((tid >> 7) + 2 * i) + q) * ld];
+        } else {
+#pragma unroll
+          for (int q = 0; q < 32; ++q) v[q] = 0.f;
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            const int k = k0 + 8 * ((tid >> 7) + 2 * i) + q;
+            v[8 * i + q] = (r < rows && k < kend) ? p[(long)k * ld] : 0.f;
+          }
       }
     }
-    if (g.tb) {    // contiguous along k
-      const int c = tid >> 2, kk = (tid & 3) * 8;
-      const int n = n0 + c;
-#pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const int k = k0 + kk + i;
-        Bs[c][kk + i] = to_bf16((n < g.N && k < k_end) ? g.B[(long)n * g.ldb + k] : 0.f);
-      }
-    } else {       // contiguous along n
-      const int kk = tid >> 3, c = (tid & 7) * 8;
-      const int k = k0 + kk;
-#pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const int n = n0 + c + i;
-        Bs[c + i][kk] = to_bf16((n < g.N && k < k_end) ? g.B[(long)k * g.ldb + n] : 0.f);
-      }
-    }
-    __syncthreads();
-    const int fr = lane & 15, fk = (lane >> 4) * 8;
-    uint4 a[2], b[2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i) a[i] = *reinterpret_cast<const uint4*>(&As[wm + 16 * i + fr][fk]);
-#pragma unroll
-    for (int j = 0; j < 2; ++j) b[j] = *reinterpret_cast<const uint4*>(&Bs[wn + 16 * j + fr][fk]);
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a[i]),
-                                                             __builtin_bit_cast(bf16x8, b[j]),
-                                                             acc[i][j], 0, 0, 0);
-    __syncthreads();
   }
-  float* C = g.C + (long)blockIdx.z * g.cstride;
-  const float a = g.alpha;
+
+  __device__ __forceinline__ void store(unsigned short (*S)[LROW], int tid) const {
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 4; ++i) {
+      const int row = KCONT ? (tid >> 3) + 32 * i : (tid & 127);
+      const int col = KCONT ? (tid & 7) * 8 : 8 * ((tid >> 7) + 2 * i);
+      *reinterpret_cast<uint4*>(&S[row][col]) = pack8(v + 8 * i);
+    }
+  }
+};
+
+// C tile (GT x GT) of one GEMM of the batch per workgroup, four waves of
+// 64 x 64 (4 x 4 MFMA tiles); blockIdx.y is the K split.  Operands are staged
+// through two LDS buffers as BF16 [row][k]; MFMA fragment: lane l takes row
+// l % 16 and k elements 8 (l / 16) .. + 7; the accumulator holds rows
+// 4 (l / 16) .. + 3 of column l % 16.  The next K step's operands are loaded
+// into registers while the current one is multiplied.
+// Workgroups are dealt to the XCDs round-robin, so workgroup w takes tile
+// (w % 8) * per_xcd + w / 8: each XCD runs a contiguous run of tiles, and the
+// column tiles of a row block (which share their A rows) meet in one L2.
+template <bool TA, bool TB>
+__global__ void __launch_bounds__(256) gemm_bf16_kernel(const GemmBatch gb) {
+  __shared__ __align__(16) unsigned short As[2][GT][LROW];
+  __shared__ __align__(16) unsigned short Bs[2][GT][LROW];
+  const int t = (blockIdx.x % NXCD) * gb.per_xcd + blockIdx.x / NXCD;
+  if (t >= gb.ntiles) return;
+  int gi = 0;
+  while (gi + 1 < gb.ngroups && t >= gb.g[gi + 1].tile0) ++gi;
+  const GemmDev& g = gb.g[gi];
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  const int local = t - g.tile0;
+  const int m0 = (local / g.tiles_n) * GT, n0 = (local % g.tiles_n) * GT;
+  const int k_begin = blockIdx.y * g.kchunk;
+  const int k_end = min(g.K, k_begin + g.kchunk);
+  const int wm = (w >> 1) * 64, wn = (w & 1) * 64;
+  floatx4 acc[4][4];
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+  Stage<!TA> sa;   // A(m, k): contiguous along k unless transposed
+  Stage<TB> sb;    // B(k, n) as rows n: contiguous along k when transposed
+  if (k_begin < k_end) {
+    sa.load(g.A, g.lda, g.M, m0, k_begin, k_end, tid);
+    sb.load(g.B, g.ldb, g.N, n0, k_begin, k_end, tid);
+  }
+  const int fr = lane & 15, fk = (lane >> 4) * 8;
+  int buf = 0;
+  for (int k0 = k_begin; k0 < k_end; k0 += GK, buf ^= 1) {
+    sa.store(As[buf], tid);
+    sb.store(Bs[buf], tid);
+    __syncthreads();
+    if (k0 + GK < k_end) {
+      sa.load(g.A, g.lda, g.M, m0, k0 + GK, k_end, tid);
+      sb.load(g.B, g.ldb, g.N, n0, k0 + GK, k_end, tid);
+    }
+#pragma unroll
+    for (int kk = 0; kk < GK; kk += 32) {
+      uint4 a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        a[i] = *reinterpret_cast<const uint4*>(&As[buf][wm + 16 * i + fr][kk + fk]);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        b[j] = *reinterpret_cast<const uint4*>(&Bs[buf][wn + 16 * j + fr][kk + fk]);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+              __builtin_bit_cast(bf16x8, a[i]), __builtin_bit_cast(bf16x8, b[j]), acc[i][j], 0,
+              0, 0);
+    }
+  }
+  float* C = g.C + (long)blockIdx.y * g.cstride;
+  const float a = gb.alpha;
+  const int epi = gb.epi;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
       const int n = n0 + wn + 16 * j + (lane & 15);
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
@@ -140,12 +216,12 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
         if (m >= g.M || n >= g.N) continue;
         const long o = (long)m * g.ldc + n;
         float v = acc[i][j][q];
-        if (g.epi == EPI_SOFTPLUS) {
+        if (epi == EPI_SOFTPLUS) {
           const float z = a * (v + g.bias[n]);
           // log(1 + e^z) / a, and its derivative sigmoid(z), without overflow
           C[o] = (fmaxf(z, 0.f) + log1pf(__expf(-fabsf(z)))) / a;
           g.aux[o] = 1.0f / (1.0f + __expf(-z));
-        } else if (g.epi == EPI_TANGENT) {
+        } else if (epi == EPI_TANGENT) {
           C[o] = v;
           g.aux[o] = v * g.s[o];
         } else {
@@ -155,15 +231,16 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
     }
 }
 
-// C = sum over the splits of their partial outputs, in split order.
-__global__ void split_sum_kernel(int M, int N, int ldc, int nsplit, long cstride,
-                                 const float* __restrict__ part, float* __restrict__ C) {
+// out = sum over the splits of their partial outputs, in split order; one
+// GEMM of the batch per blockIdx.y.
+__global__ void split_sum_kernel(const GemmBatch gb, int nsplit) {
+  const GemmDev& g = gb.g[blockIdx.y];
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)M * N) return;
-  const long o = (t / N) * ldc + t % N;
+  if (t >= (long)g.M * g.N) return;
+  const long o = (t / g.N) * g.ldc + t % g.N;
   float v = 0.f;
-  for (int z = 0; z < nsplit; ++z) v += part[z * cstride + o];
-  C[o] = v;
+  for (int z = 0; z < nsplit; ++z) v += g.C[z * g.cstride + o];
+  g.out[o] = v;
 }
 
 // Elementwise steps between the GEMMs (row r < M, column n < H; ld = row stride
@@ -211,41 +288,73 @@ __global__ void rows_kernel(int scatter, int M, int cols, const int* __restrict_
 
 extern "C" {
 
+int tm_gemm_bf16_grouped(int ngroups, const tm_gemm_group* groups, int ta, int tb, int epi,
+                         float alpha, int nsplit, void* stream) {
+  TM_REQUIRE(ngroups >= 0 && (ngroups == 0 || groups != nullptr),
+             "tm_gemm_bf16_grouped: bad group list");
+  TM_REQUIRE(epi >= EPI_STORE && epi <= EPI_TANGENT, "tm_gemm_bf16: bad epilogue %d", epi);
+  TM_REQUIRE(nsplit >= 1 && nsplit <= 64, "tm_gemm_bf16: nsplit %d outside 1..64", nsplit);
+  TM_REQUIRE(nsplit == 1 || epi == EPI_STORE,
+             "tm_gemm_bf16: split K needs the plain epilogue");
+  TM_REQUIRE(epi != EPI_SOFTPLUS || alpha > 0.f, "tm_gemm_bf16: softplus needs alpha > 0");
+  for (int i = 0; i < ngroups; ++i) {
+    const tm_gemm_group& q = groups[i];
+    TM_REQUIRE(q.M >= 0 && q.N >= 0 && q.K >= 0, "tm_gemm_bf16: negative size in GEMM %d", i);
+    if (q.M == 0 || q.N == 0) continue;   // nothing to write, no operand read
+    TM_REQUIRE(epi != EPI_SOFTPLUS || (q.bias != nullptr && q.aux != nullptr),
+               "tm_gemm_bf16: the softplus epilogue needs bias and aux (GEMM %d)", i);
+    TM_REQUIRE(epi != EPI_TANGENT || (q.s != nullptr && q.aux != nullptr),
+               "tm_gemm_bf16: the tangent epilogue needs s and aux (GEMM %d)", i);
+    TM_REQUIRE(nsplit == 1 || (q.bias == nullptr && q.work != nullptr),
+               "tm_gemm_bf16: split K needs a workspace and no bias (GEMM %d)", i);
+  }
+  hipStream_t st = (hipStream_t)stream;
+  auto kern = ta ? (tb ? gemm_bf16_kernel<true, true> : gemm_bf16_kernel<true, false>)
+                 : (tb ? gemm_bf16_kernel<false, true> : gemm_bf16_kernel<false, false>);
+  for (int g0 = 0; g0 < ngroups;) {
+    GemmBatch gb;
+    gb.ngroups = 0;
+    gb.ntiles = 0;
+    gb.epi = epi;
+    gb.alpha = alpha;
+    long max_mn = 0;
+    for (; g0 < ngroups && gb.ngroups < MAXG; ++g0) {
+      const tm_gemm_group& q = groups[g0];
+      if (q.M == 0 || q.N == 0) continue;
+      GemmDev& d = gb.g[gb.ngroups++];
+      d.A = q.A; d.B = q.B; d.out = q.C;
+      d.C = nsplit > 1 ? q.work : q.C;
+      d.bias = q.bias; d.s = q.s; d.aux = q.aux;
+      d.M = q.M; d.N = q.N; d.K = q.K;
+      d.lda = q.lda; d.ldb = q.ldb; d.ldc = q.ldc;
+      d.cstride = (long)q.M * q.ldc;
+      const int steps = (q.K + GK - 1) / GK;
+      d.kchunk = steps == 0 ? GK : ((steps + nsplit - 1) / nsplit) * GK;
+      d.tiles_n = tm_div_up(q.N, GT);
+      d.tile0 = gb.ntiles;
+      gb.ntiles += d.tiles_n * tm_div_up(q.M, GT);
+      max_mn = std::max(max_mn, (long)q.M * q.N);
+    }
+    if (gb.ngroups == 0) continue;
+    gb.per_xcd = tm_div_up(gb.ntiles, NXCD);
+    kern<<<dim3(NXCD * gb.per_xcd, nsplit), 256, 0, st>>>(gb);
+    TM_LAUNCH_CHECK();
+    if (nsplit > 1) {
+      split_sum_kernel<<<dim3(tm_div_up(max_mn, 256), gb.ngroups), 256, 0, st>>>(gb, nsplit);
+      TM_LAUNCH_CHECK();
+    }
+  }
+  return TM_OK;
+}
+
 int tm_gemm_bf16(int M, int N, int K, const float* A, int lda, int ta, const float* B, int ldb,
                  int tb, float* C, int ldc, int epi, const float* bias, const float* s,
                  float* aux, float alpha, int nsplit, float* work, void* stream) {
-  TM_REQUIRE(M >= 0 && N >= 0 && K >= 0, "tm_gemm_bf16: negative size");
-  TM_REQUIRE(epi >= EPI_STORE && epi <= EPI_TANGENT, "tm_gemm_bf16: bad epilogue %d", epi);
-  TM_REQUIRE(epi != EPI_SOFTPLUS || (bias != nullptr && aux != nullptr && alpha > 0.f),
-             "tm_gemm_bf16: the softplus epilogue needs bias, aux and alpha > 0");
-  TM_REQUIRE(epi != EPI_TANGENT || (s != nullptr && aux != nullptr),
-             "tm_gemm_bf16: the tangent epilogue needs s and aux");
-  TM_REQUIRE(nsplit >= 1 && (nsplit == 1 || (epi == EPI_STORE && bias == nullptr &&
-                                             work != nullptr)),
-             "tm_gemm_bf16: split K needs the plain epilogue and a workspace");
-  if (M == 0 || N == 0) return TM_OK;
-  hipStream_t st = (hipStream_t)stream;
-  GemmArgs g;
-  g.M = M; g.N = N; g.K = K;
-  g.A = A; g.lda = lda; g.ta = ta;
-  g.B = B; g.ldb = ldb; g.tb = tb;
-  g.C = nsplit > 1 ? work : C;
-  g.ldc = ldc;
-  g.epi = epi; g.bias = bias; g.s = s; g.aux = aux; g.alpha = alpha;
-  // whole K steps per split
-  const int steps = (K + GK - 1) / GK;
-  g.kchunk = ((steps + nsplit - 1) / nsplit) * GK;
-  const int nz = K == 0 ? 1 : (K + g.kchunk - 1) / g.kchunk;
-  g.cstride = (long)M * ldc;
-  dim3 grid(tm_div_up(N, GT), tm_div_up(M, GT), nz);
-  gemm_bf16_kernel<<<grid, 256, 0, st>>>(g);
-  TM_LAUNCH_CHECK();
-  if (nsplit > 1) {
-    split_sum_kernel<<<tm_div_up((long)M * N, 256), 256, 0, st>>>(M, N, ldc, nz, g.cstride,
-                                                                  work, C);
-    TM_LAUNCH_CHECK();
-  }
-  return TM_OK;
+  tm_gemm_group q;
+  q.M = M; q.N = N; q.K = K;
+  q.A = A; q.lda = lda; q.B = B; q.ldb = ldb; q.C = C; q.ldc = ldc;
+  q.bias = bias; q.s = s; q.aux = aux; q.work = work;
+  return tm_gemm_bf16_grouped(1, &q, ta, tb, epi, alpha, nsplit, stream);
 }
 
 int tm_train_elem(int op, int M, int H, int ld, float alpha, const float* s, const float* x,

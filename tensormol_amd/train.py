@@ -228,56 +228,75 @@ class ForceMatchingTrainer:
             b["Je"] = torch.zeros(M, D, **f32)
             b["gW"] = [torch.zeros(D + 1, H, **f32), torch.zeros(H + 1, H, **f32),
                        torch.zeros(H + 1, H, **f32), torch.zeros(H + 1, 1, **f32)]
-            b["nsplit"] = max(1, min(8, (2 * M) // 512))
-            b["work"] = torch.zeros(b["nsplit"] * (D + 1) * H, **f32)
             bufs[e] = b
+        # the weight-gradient GEMMs of every element run in one grouped launch
+        # with a common K split (K = 2M rows); each keeps its own partials
+        ns = max(1, min(8, (max(2 * b["M"] for b in bufs.values()) + 2047) // 2048))
+        for b in bufs.values():
+            b["nsplit"] = ns
+            b["work"] = [torch.zeros(ns * (D + 1) * H, **f32), torch.zeros(ns * (H + 1) * H, **f32),
+                         torch.zeros(ns * (H + 1) * H, **f32), torch.zeros(ns * (H + 1), **f32)]
         tb._mfma = bufs
         return bufs
 
-    def _gemm(self, M, N, K, A, lda, ta, B, ldb, tb_, C, ldc, epi=0, bias=None, s=None, aux=None,
-              nsplit=1, work=None):
-        self.eng.lib.tm_gemm_bf16(M, N, K, A, lda, ta, B, ldb, tb_, C, ldc, epi,
-                                  None if bias is None else _lib_ptr(bias), s, aux, self.alpha,
-                                  nsplit, None if work is None else _lib_ptr(work),
-                                  self.eng._stream())
+    def _ggemm(self, groups, ta, tb_, epi=0, nsplit=1):
+        """One tm_gemm_bf16_grouped call over tuples (M, N, K, A, lda, B, ldb,
+        C, ldc, bias, s, aux, work) of device addresses (ints) or None."""
+        arr = (_lib.GemmGroup * len(groups))(*[_lib.GemmGroup(*g) for g in groups])
+        self.eng.lib.tm_gemm_bf16_grouped(len(groups), _ctypes.cast(arr, _ctypes.c_void_p), ta,
+                                          tb_, epi, self.alpha, nsplit, self.eng._stream())
 
     def _mfma_step(self, tb, pos, check=True):
-        """(L, E, F, gradients in parameters() order) of the BF16 MFMA path."""
+        """(L, E, F, gradients in parameters() order) of the BF16 MFMA path.
+        Each GEMM stage runs for every element in one grouped launch; the
+        elementwise steps and row gathers between them run per element."""
         eng, ws = self.eng, tb.ws
         lib, st = eng.lib, eng._stream()
         P = _lib_ptr
-        off = lambda t, rows, ld: _ctypes.c_void_p(t.data_ptr() + 4 * rows * ld)
+        ad = lambda t, rows=0, ld=0: t.data_ptr() + 4 * rows * ld   # row `rows` of t
+        off = lambda t, rows, ld: _ctypes.c_void_p(ad(t, rows, ld))
         eng.build_lists(ws, pos)
         if check:
             eng.check_flags(ws)
         eng.descriptors(ws, pos)
         N, D, a = tb.N, eng.D, self.alpha
         bufs = self._mfma_buffers(tb)
+        items = [(b, self.params[e]) for e, b in bufs.items()]
+        H = items[0][0]["H"]
         f32 = dict(dtype=torch.float32, device=self.device)
         eps = torch.zeros(N, 1, **f32)
         J = torch.zeros(N, D, **f32)
-        for e, b in bufs.items():
-            M, ldx, ldh, H, idx = b["M"], b["ldx"], b["ldh"], b["H"], b["idx"]
-            (W1, b1), (W2, b2), (W3, b3), (W4, b4) = self.params[e]
-            lib.tm_rows(0, M, D, P(idx), P(ws.G), D, P(b["X0"]), ldx, st)
-            # forward
-            prev, ldp, Kp = b["X0"], ldx, D
-            for (W, bb, Hk, Sk) in ((W1, b1, "H1", "S1"), (W2, b2, "H2", "S2"), (W3, b3, "H3", "S3")):
-                self._gemm(M, H, Kp, P(prev), ldp, 0, P(W), H, 0, P(b[Hk]), ldh, 1, bias=bb,
-                           aux=P(b[Sk]))
-                prev, ldp, Kp = b[Hk], ldh, H
-            self._gemm(M, 1, H, P(b["H3"]), ldh, 0, P(W4), 1, 0, P(b["Y"]), 1, 0, bias=b4)
-            lib.tm_rows(1, M, 1, P(idx), P(b["Y"]), 1, P(eps), 1, st)
-            # J chain: zd_3 = s_3 w_4 (rows M.. of ZZ3), then down to J
-            lib.tm_train_elem(0, M, H, ldh, a, P(b["S3"]), None, None, None, P(W4), None,
+        for b, _ in items:
+            lib.tm_rows(0, b["M"], D, P(b["idx"]), P(ws.G), D, P(b["X0"]), b["ldx"], st)
+        # forward: z_l = h_{l-1} W_l + b_l, h_l = act(z_l), s_l = act'(z_l)
+        layers = (("X0", "H1", "S1", "ZT1"), ("H1", "H2", "S2", "ZT2"), ("H2", "H3", "S3", "ZT3"))
+        for l, (hin, hout, sk, _) in enumerate(layers):
+            grp = []
+            for b, prm in items:
+                ldp, Kp = (b["ldx"], D) if l == 0 else (b["ldh"], H)
+                grp.append((b["M"], H, Kp, ad(b[hin]), ldp, ad(prm[l][0]), H, ad(b[hout]), b["ldh"],
+                            ad(prm[l][1]), None, ad(b[sk]), None))
+            self._ggemm(grp, 0, 0, 1)
+        self._ggemm([(b["M"], 1, H, ad(b["H3"]), b["ldh"], ad(prm[3][0]), 1, ad(b["Y"]), 1,
+                      ad(prm[3][1]), None, None, None) for b, prm in items], 0, 0, 0)
+        # J chain: zd_3 = s_3 w_4 (rows M.. of ZZ3), then down to J
+        for b, prm in items:
+            M, ldh = b["M"], b["ldh"]
+            lib.tm_rows(1, M, 1, P(b["idx"]), P(b["Y"]), 1, P(eps), 1, st)
+            lib.tm_train_elem(0, M, H, ldh, a, P(b["S3"]), None, None, None, P(prm[3][0]), None,
                               off(b["ZZ3"], M, ldh), st)
-            for (Wn, zz_in, hd, S, zz_out) in ((W3, "ZZ3", "HD2", "S2", "ZZ2"),
-                                               (W2, "ZZ2", "HD1", "S1", "ZZ1")):
-                self._gemm(M, H, H, off(b[zz_in], M, ldh), ldh, 0, P(Wn), H, 1, P(b[hd]), ldh)
-                lib.tm_train_elem(1, M, H, ldh, a, P(b[S]), P(b[hd]), None, None, None, None,
+        for (l, zz_in, hd, sk, zz_out) in ((2, "ZZ3", "HD2", "S2", "ZZ2"),
+                                           (1, "ZZ2", "HD1", "S1", "ZZ1")):
+            self._ggemm([(b["M"], H, H, ad(b[zz_in], b["M"], b["ldh"]), b["ldh"], ad(prm[l][0]), H,
+                          ad(b[hd]), b["ldh"], None, None, None, None) for b, prm in items], 0, 1)
+            for b, _ in items:
+                M, ldh = b["M"], b["ldh"]
+                lib.tm_train_elem(1, M, H, ldh, a, P(b[sk]), P(b[hd]), None, None, None, None,
                                   off(b[zz_out], M, ldh), st)
-            self._gemm(M, D, H, off(b["ZZ1"], M, ldh), ldh, 0, P(W1), H, 1, P(b["Je"]), D)
-            lib.tm_rows(1, M, D, P(idx), P(b["Je"]), D, P(J), D, st)
+        self._ggemm([(b["M"], D, H, ad(b["ZZ1"], b["M"], b["ldh"]), b["ldh"], ad(prm[0][0]), H,
+                      ad(b["Je"]), D, None, None, None, None) for b, prm in items], 0, 1)
+        for b, _ in items:
+            lib.tm_rows(1, b["M"], D, P(b["idx"]), P(b["Je"]), D, P(J), D, st)
         E = torch.zeros(tb.B, dtype=torch.float64, device=self.device).index_add(
             0, tb.mol, eps[:, 0].double())
         grad = torch.empty(N, 3, dtype=torch.float64, device=self.device)
@@ -289,42 +308,53 @@ class ForceMatchingTrainer:
         gg = (-2.0 * wf * (F - tb.F_ref)).contiguous()   # dLoss/dgrad
         V = torch.empty(N, D, **f32)
         eng.descriptor_jvp(ws, pos, gg, V)
+        for b, _ in items:
+            M, ldx = b["M"], b["ldx"]
+            lib.tm_rows(0, M, D, P(b["idx"]), P(V), D, off(b["X0"], M, ldx), ldx, st)
+            lib.tm_rows(0, M, 1, P(b["idx"]), P(ybar), 1, P(b["YB"]), 1, st)
+        # tangent along V: zt_l = ht_{l-1} W_l, ht_l = s_l zt_l (rows M.. of H_l)
+        for l, (hin, hout, sk, zt) in enumerate(layers):
+            grp = []
+            for b, prm in items:
+                M, ldh = b["M"], b["ldh"]
+                ldp, Kp = (b["ldx"], D) if l == 0 else (ldh, H)
+                grp.append((M, H, Kp, ad(b[hin], M, ldp), ldp, ad(prm[l][0]), H, ad(b[zt]), ldh,
+                            None, ad(b[sk]), ad(b[hout], M, ldh), None))
+            self._ggemm(grp, 0, 0, 2)
+        # reverse
+        for b, prm in items:
+            lib.tm_train_elem(2, b["M"], H, b["ldh"], a, P(b["S3"]), None, P(b["ZT3"]), None,
+                              P(prm[3][0]), P(b["YB"]), P(b["ZZ3"]), st)
+        for (l, zz_in, sk, zt, hd, zz_out) in ((2, "ZZ3", "S2", "ZT2", "HD2", "ZZ2"),
+                                               (1, "ZZ2", "S1", "ZT1", "HD1", "ZZ1")):
+            self._ggemm([(b["M"], H, H, ad(b[zz_in]), b["ldh"], ad(prm[l][0]), H, ad(b["HB"]),
+                          b["ldh"], None, None, None, None) for b, prm in items], 0, 1)
+            for b, _ in items:
+                lib.tm_train_elem(3, b["M"], H, b["ldh"], a, P(b[sk]), P(b["HB"]), P(b[zt]),
+                                  P(b[hd]), None, None, P(b[zz_out]), st)
+        # weight gradients over the stacked rows (bias from the ones column),
+        # every layer of every element in one grouped launch
+        grp = []
+        for b, _ in items:
+            M2, ldx, ldh = 2 * b["M"], b["ldx"], b["ldh"]
+            gW1, gW2, gW3, gw4 = b["gW"]
+            wk = b["work"]
+            grp += [(D + 1, H, M2, ad(b["X0"]), ldx, ad(b["ZZ1"]), ldh, ad(gW1), H, None, None,
+                     None, ad(wk[0])),
+                    (H + 1, H, M2, ad(b["H1"]), ldh, ad(b["ZZ2"]), ldh, ad(gW2), H, None, None,
+                     None, ad(wk[1])),
+                    (H + 1, H, M2, ad(b["H2"]), ldh, ad(b["ZZ3"]), ldh, ad(gW3), H, None, None,
+                     None, ad(wk[2])),
+                    (H + 1, 1, M2, ad(b["H3"]), ldh, ad(b["YB"]), 1, ad(gw4), 1, None, None, None,
+                     ad(wk[3]))]
+        self._ggemm(grp, 1, 0, 0, nsplit=items[0][0]["nsplit"])
         grads = []
         for e in self.model.eles:
             b = bufs.get(e)
             if b is None:   # no atoms of this element in the batch
                 grads += [torch.zeros_like(t) for (W, bb) in self.params[e] for t in (W, bb)]
                 continue
-            M, ldx, ldh, H, idx = b["M"], b["ldx"], b["ldh"], b["H"], b["idx"]
-            (W1, b1), (W2, b2), (W3, b3), (W4, b4) = self.params[e]
-            lib.tm_rows(0, M, D, P(idx), P(V), D, off(b["X0"], M, ldx), ldx, st)
-            lib.tm_rows(0, M, 1, P(idx), P(ybar), 1, P(b["YB"]), 1, st)
-            # tangent
-            prev, ldp, Kp = off(b["X0"], M, ldx), ldx, D
-            for (W, Hk, Sk, Zk) in ((W1, "H1", "S1", "ZT1"), (W2, "H2", "S2", "ZT2"),
-                                    (W3, "H3", "S3", "ZT3")):
-                self._gemm(M, H, Kp, prev, ldp, 0, P(W), H, 0, P(b[Zk]), ldh, 2, s=P(b[Sk]),
-                           aux=off(b[Hk], M, ldh))
-                prev, ldp, Kp = off(b[Hk], M, ldh), ldh, H
-            # reverse
-            lib.tm_train_elem(2, M, H, ldh, a, P(b["S3"]), None, P(b["ZT3"]), None, P(W4),
-                              P(b["YB"]), P(b["ZZ3"]), st)
-            for (Wn, zz_in, S, zt, hd, zz_out) in ((W3, "ZZ3", "S2", "ZT2", "HD2", "ZZ2"),
-                                                   (W2, "ZZ2", "S1", "ZT1", "HD1", "ZZ1")):
-                self._gemm(M, H, H, P(b[zz_in]), ldh, 0, P(Wn), H, 1, P(b["HB"]), ldh)
-                lib.tm_train_elem(3, M, H, ldh, a, P(b[S]), P(b["HB"]), P(b[zt]), P(b[hd]),
-                                  None, None, P(b[zz_out]), st)
-            # weight gradients over the stacked rows (bias from the ones column)
-            ns, wk = b["nsplit"], b["work"]
             gW1, gW2, gW3, gw4 = b["gW"]
-            self._gemm(D + 1, H, 2 * M, P(b["X0"]), ldx, 1, P(b["ZZ1"]), ldh, 0, P(gW1), H,
-                       nsplit=ns, work=wk)
-            self._gemm(H + 1, H, 2 * M, P(b["H1"]), ldh, 1, P(b["ZZ2"]), ldh, 0, P(gW2), H,
-                       nsplit=ns, work=wk)
-            self._gemm(H + 1, H, 2 * M, P(b["H2"]), ldh, 1, P(b["ZZ3"]), ldh, 0, P(gW3), H,
-                       nsplit=ns, work=wk)
-            self._gemm(H + 1, 1, 2 * M, P(b["H3"]), ldh, 1, P(b["YB"]), 1, 0, P(gw4), 1,
-                       nsplit=ns, work=wk)
             grads += [gW1[:D], gW1[D], gW2[:H], gW2[H], gW3[:H], gW3[H], gw4[:H], gw4[H]]
         # parameters() order: element by element, (W, b) per layer; shapes as the parameters
         grads = [g.reshape(p.shape).clone() for g, p in zip(grads, self.parameters())]

@@ -130,3 +130,64 @@ def test_rows_gather_scatter(cuda):
     torch.cuda.synchronize()
     assert torch.equal(out[:, :7], src[idx.long()])
     assert torch.equal(back[idx.long()], src[idx.long()])
+
+
+@pytest.mark.parametrize("ta,tb,epi,ns", [(0, 0, 1, 1), (0, 1, 0, 1), (1, 0, 0, 4), (0, 0, 2, 1)])
+def test_gemm_grouped_equals_single_calls(cuda, ta, tb, epi, ns):
+    """17 GEMMs of mixed shapes (one empty; more than one launch's worth) in
+    one grouped call give the bits of 17 single calls, and match the FP32
+    GEMM of the rounded operands."""
+    from tensormol_amd import _lib as TL
+    L, P = _lib()
+    g = torch.Generator(device="cpu").manual_seed(17 + 4 * ta + 2 * tb + epi)
+    st = torch.cuda.current_stream().cuda_stream
+    shapes = [(300, 256, 200), (129, 1, 70), (0, 64, 64), (40, 513, 130), (700, 128, 1000)]
+    shapes = [shapes[i % len(shapes)] for i in range(17)]
+    a = 2.0
+    cases, groups, singles = [], [], []
+    for (M, N, K) in shapes:
+        A = torch.randn(M, K, generator=g).to(cuda) * 0.2
+        B = torch.randn(K, N, generator=g).to(cuda) * 0.2
+        As = A.t().contiguous() if ta else A
+        Bs = B.t().contiguous() if tb else B
+        lda, ldb = (max(M, 1) if ta else K), (K if tb else N)
+        bias = torch.randn(N, generator=g).to(cuda) if epi == 1 else None
+        s = torch.rand(M, N, generator=g).to(cuda) if epi == 2 else None
+        outs = []
+        for _ in range(2):
+            C = torch.zeros(M, N, device=cuda)
+            aux = torch.zeros(M, N, device=cuda) if epi else None
+            work = torch.zeros(ns * M * N + 1, device=cuda) if ns > 1 else None
+            outs.append((C, aux, work))
+        cases.append((M, N, K, A, B, As, Bs, lda, ldb, bias, s, outs))
+        ad = lambda t: None if t is None else t.data_ptr()
+        C, aux, work = outs[0]
+        groups.append(TL.GemmGroup(M, N, K, ad(As), lda, ad(Bs), ldb, ad(C), N, ad(bias), ad(s),
+                                   ad(aux), ad(work)))
+        C, aux, work = outs[1]
+        singles.append((M, N, K, P(As), lda, ta, P(Bs), ldb, tb, P(C), N, epi, P(bias), P(s),
+                        P(aux), a, ns, P(work), st))
+    arr = (TL.GemmGroup * len(groups))(*groups)
+    import ctypes
+    L.tm_gemm_bf16_grouped(len(groups), ctypes.cast(arr, ctypes.c_void_p), ta, tb, epi, a, ns, st)
+    for args in singles:
+        L.tm_gemm_bf16(*args)
+    torch.cuda.synchronize()
+    for (M, N, K, A, B, As, Bs, lda, ldb, bias, s, outs) in cases:
+        (C, aux, _), (C1, aux1, _) = outs
+        assert torch.equal(C, C1), (M, N, K)
+        if aux is not None:
+            assert torch.equal(aux, aux1), (M, N, K)
+        if M == 0:
+            continue
+        z = _r(A) @ _r(B)
+        scale = (_r(A).abs() @ _r(B).abs()).clamp_min(1e-6)
+        if epi == 1:
+            z = z + bias
+            np.testing.assert_allclose(C.cpu(), (torch.nn.functional.softplus(a * z) / a).cpu(),
+                                       rtol=1e-4, atol=1e-5)
+            np.testing.assert_allclose(aux.cpu(), torch.sigmoid(a * z).cpu(), rtol=1e-4, atol=1e-5)
+        else:
+            assert ((C - z).abs() / scale).max().item() < 1e-5, (M, N, K)
+            if epi == 2:
+                np.testing.assert_allclose(aux.cpu(), (C * s).cpu(), rtol=1e-6, atol=1e-7)
