@@ -76,56 +76,52 @@ __device__ __forceinline__ uint4 pack8(const float* v) {
 // operand in LDS as S[row][k] in BF16, so an MFMA fragment (8 consecutive k
 // of one row) is one 16-byte read, and each thread's LDS writes are 16 bytes.
 // KCONT, element (r, k) at X[r * ld + k]: thread t takes k = 8 (t % 8) .. + 7
-// of rows t / 8 + 32 i (two 16-byte loads per row).
+// of rows t / 8 + 32 i (VEC: two 16-byte loads per row, which needs ld % 4 == 0
+// and a 16-byte aligned X).
 // Otherwise element (r, k) at X[k * ld + r]: thread t takes row t % 128 at
 // k = 8 (t / 128 + 2 i) .. + 7, one 4-byte load per k, so a wave's load reads
 // 256 contiguous bytes.
-template <bool KCONT>
+// Loads are branch-free: a row past the operand's last is read as its last
+// row (it only feeds output rows or columns that are never stored), and in
+// the last, partial K step a k past kend is read at kend - 1 and replaced by
+// zero.  Every address stays inside the operand.
+template <bool KCONT, bool VEC>
 struct Stage {
   float v[32];
 
+  template <bool TAIL>
   __device__ __forceinline__ void load(const float* __restrict__ X, long ld, int rows, int r0,
                                        int k0, int kend, int tid) {
     if (KCONT) {
       const int k = k0 + (tid & 7) * 8;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int r = r0 + (tid >> 3) + 32 * i;
-        const float* p = X + (long)r * ld + k;
+        const int r = min(r0 + (tid >> 3) + 32 * i, rows - 1);
+        const float* p = X + (long)r * ld;
         float* d = v + 8 * i;
-        if (r < rows && k + 8 <= kend && (reinterpret_cast<uintptr_t>(p) & 15) == 0) {
-          const float4 f0 = reinterpret_cast<const float4*>(p)[0];
-          const float4 f1 = reinterpret_cast<const float4*>(p)[1];
+        if (VEC && !TAIL) {
+          const float4 f0 = *reinterpret_cast<const float4*>(p + k);
+          const float4 f1 = *reinterpret_cast<const float4*>(p + k + 4);
           d[0] = f0.x; d[1] = f0.y; d[2] = f0.z; d[3] = f0.w;
           d[4] = f1.x; d[5] = f1.y; d[6] = f1.z; d[7] = f1.w;
         } else {
 #pragma unroll
-          for (int q = 0; q < 8; ++q) d[q] = (r < rows && k + q < kend) ? p[q] : 0.f;
+          for (int q = 0; q < 8; ++q) {
+            const float x = p[TAIL ? min(k + q, kend - 1) : k + q];
+            d[q] = (!TAIL || k + q < kend) ? x : 0.f;
+          }
         }
       }
     } else {
-      const int r = r0 + (tid & 127);
-      const float* p = X + r;
-      if (k0 + GK <= kend) {
-        if (r < rows) {
+      const float* p = X + min(r0 + (tid & 127), rows - 1);
 #pragma unroll
-          for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int q = 0; q < 8; ++q)
-              v[8 * i + q] = p[(long)(k0 + 8 * ((tid >> 7) + 2 * i) + q) * ld];
-        } else {
-#pragma unroll
-          for (int q = 0; q < 32; ++q) v[q] = 0.f;
+        for (int q = 0; q < 8; ++q) {
+          const int k = k0 + 8 * ((tid >> 7) + 2 * i) + q;
+          const float x = p[(long)(TAIL ? min(k, kend - 1) : k) * ld];
+          v[8 * i + q] = (!TAIL || k < kend) ? x : 0.f;
         }
-      } else {
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-          for (int q = 0; q < 8; ++q) {
-            const int k = k0 + 8 * ((tid >> 7) + 2 * i) + q;
-            v[8 * i + q] = (r < rows && k < kend) ? p[(long)k * ld] : 0.f;
-          }
-      }
     }
   }
 
@@ -148,7 +144,7 @@ struct Stage {
 // Workgroups are dealt to the XCDs round-robin, so workgroup w takes tile
 // (w % 8) * per_xcd + w / 8: each XCD runs a contiguous run of tiles, and the
 // column tiles of a row block (which share their A rows) meet in one L2.
-template <bool TA, bool TB>
+template <bool TA, bool TB, bool VEC>
 __global__ void __launch_bounds__(256) gemm_bf16_kernel(const GemmBatch gb) {
   __shared__ __align__(16) unsigned short As[2][GT][LROW];
   __shared__ __align__(16) unsigned short Bs[2][GT][LROW];
@@ -162,28 +158,32 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(const GemmBatch gb) {
   const int m0 = (local / g.tiles_n) * GT, n0 = (local % g.tiles_n) * GT;
   const int k_begin = blockIdx.y * g.kchunk;
   const int k_end = min(g.K, k_begin + g.kchunk);
+  const int nsteps = k_end > k_begin ? (k_end - k_begin + GK - 1) / GK : 0;
   const int wm = (w >> 1) * 64, wn = (w & 1) * 64;
   floatx4 acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-  Stage<!TA> sa;   // A(m, k): contiguous along k unless transposed
-  Stage<TB> sb;    // B(k, n) as rows n: contiguous along k when transposed
-  if (k_begin < k_end) {
-    sa.load(g.A, g.lda, g.M, m0, k_begin, k_end, tid);
-    sb.load(g.B, g.ldb, g.N, n0, k_begin, k_end, tid);
-  }
+  Stage<!TA, VEC> sa;   // A(m, k): contiguous along k unless transposed
+  Stage<TB, VEC> sb;    // B(k, n) as rows n: contiguous along k when transposed
+  auto load = [&](int k0) {
+    if (k0 + GK <= k_end) {
+      sa.template load<false>(g.A, g.lda, g.M, m0, k0, k_end, tid);
+      sb.template load<false>(g.B, g.ldb, g.N, n0, k0, k_end, tid);
+    } else {
+      sa.template load<true>(g.A, g.lda, g.M, m0, k0, k_end, tid);
+      sb.template load<true>(g.B, g.ldb, g.N, n0, k0, k_end, tid);
+    }
+  };
+  if (nsteps > 0) load(k_begin);
   const int fr = lane & 15, fk = (lane >> 4) * 8;
-  int buf = 0;
-  for (int k0 = k_begin; k0 < k_end; k0 += GK, buf ^= 1) {
+  for (int step = 0; step < nsteps; ++step) {
+    const int buf = step & 1;
     sa.store(As[buf], tid);
     sb.store(Bs[buf], tid);
     __syncthreads();
-    if (k0 + GK < k_end) {
-      sa.load(g.A, g.lda, g.M, m0, k0 + GK, k_end, tid);
-      sb.load(g.B, g.ldb, g.N, n0, k0 + GK, k_end, tid);
-    }
+    if (step + 1 < nsteps) load(k_begin + (step + 1) * GK);
 #pragma unroll
     for (int kk = 0; kk < GK; kk += 32) {
       uint4 a[4], b[4];
@@ -202,33 +202,61 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(const GemmBatch gb) {
               0, 0);
     }
   }
+  // Epilogue.  Every input it reads (bias, s) is loaded before the first
+  // store: a load behind a store to a pointer that may alias it would wait
+  // for that store, one memory latency per element.
   float* C = g.C + (long)blockIdx.y * g.cstride;
   const float a = gb.alpha;
   const int epi = gb.epi;
+  const int mrow = m0 + wm + (lane >> 4) * 4, ncol = n0 + wn + (lane & 15);
+  float bn[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    bn[j] = (g.bias != nullptr && ncol + 16 * j < g.N) ? g.bias[ncol + 16 * j] : 0.f;
+  if (epi == EPI_TANGENT) {
+    floatx4 sv[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int m = mrow + 16 * i + q, n = ncol + 16 * j;
+          sv[i][j][q] = (m < g.M && n < g.N) ? g.s[(long)m * g.ldc + n] : 0.f;
+        }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int m = mrow + 16 * i + q, n = ncol + 16 * j;
+          if (m >= g.M || n >= g.N) continue;
+          const long o = (long)m * g.ldc + n;
+          C[o] = acc[i][j][q];
+          g.aux[o] = acc[i][j][q] * sv[i][j][q];
+        }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int n = n0 + wn + 16 * j + (lane & 15);
+    for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int m = m0 + wm + 16 * i + (lane >> 4) * 4 + q;
+        const int m = mrow + 16 * i + q, n = ncol + 16 * j;
         if (m >= g.M || n >= g.N) continue;
         const long o = (long)m * g.ldc + n;
-        float v = acc[i][j][q];
+        const float v = acc[i][j][q] + bn[j];
         if (epi == EPI_SOFTPLUS) {
-          const float z = a * (v + g.bias[n]);
+          const float z = a * v;
           // log(1 + e^z) / a, and its derivative sigmoid(z), without overflow
           C[o] = (fmaxf(z, 0.f) + log1pf(__expf(-fabsf(z)))) / a;
           g.aux[o] = 1.0f / (1.0f + __expf(-z));
-        } else if (epi == EPI_TANGENT) {
-          C[o] = v;
-          g.aux[o] = v * g.s[o];
         } else {
-          C[o] = g.bias != nullptr ? v + g.bias[n] : v;
+          C[o] = v;
         }
       }
-    }
 }
 
 // out = sum over the splits of their partial outputs, in split order; one
@@ -309,8 +337,21 @@ int tm_gemm_bf16_grouped(int ngroups, const tm_gemm_group* groups, int ta, int t
                "tm_gemm_bf16: split K needs a workspace and no bias (GEMM %d)", i);
   }
   hipStream_t st = (hipStream_t)stream;
-  auto kern = ta ? (tb ? gemm_bf16_kernel<true, true> : gemm_bf16_kernel<true, false>)
-                 : (tb ? gemm_bf16_kernel<false, true> : gemm_bf16_kernel<false, false>);
+  // 16-byte loads of the operands stored along k when every one allows them
+  bool vec = true;
+  for (int i = 0; i < ngroups; ++i) {
+    const tm_gemm_group& q = groups[i];
+    if (q.M == 0 || q.N == 0) continue;
+    if (!ta) vec = vec && q.lda % 4 == 0 && (reinterpret_cast<uintptr_t>(q.A) & 15) == 0;
+    if (tb) vec = vec && q.ldb % 4 == 0 && (reinterpret_cast<uintptr_t>(q.B) & 15) == 0;
+  }
+  using Kern = void (*)(const GemmBatch);
+  const Kern kerns[2][2][2] = {
+      {{gemm_bf16_kernel<false, false, false>, gemm_bf16_kernel<false, false, true>},
+       {gemm_bf16_kernel<false, true, false>, gemm_bf16_kernel<false, true, true>}},
+      {{gemm_bf16_kernel<true, false, false>, gemm_bf16_kernel<true, false, true>},
+       {gemm_bf16_kernel<true, true, false>, gemm_bf16_kernel<true, true, true>}}};
+  const Kern kern = kerns[ta ? 1 : 0][tb ? 1 : 0][vec ? 1 : 0];
   for (int g0 = 0; g0 < ngroups;) {
     GemmBatch gb;
     gb.ngroups = 0;

@@ -112,7 +112,9 @@ class ForceMatchingTrainer:
         self.params = {e: [(torch.tensor(np.asarray(W), **f32).requires_grad_(),
                             torch.tensor(np.asarray(b), **f32).requires_grad_())
                            for (W, b) in model.energy_nets[e]] for e in model.eles}
-        self.opt = torch.optim.Adam(self.parameters(), lr=lr, capturable=bool(capturable))
+        # one fused update kernel for every parameter on the GPU
+        self.opt = torch.optim.Adam(self.parameters(), lr=lr, capturable=bool(capturable),
+                                    fused=self.device.type == "cuda")
 
     def parameters(self):
         """Flat list: elements in model order, layers in order, (W, b) each."""
@@ -186,7 +188,7 @@ class ForceMatchingTrainer:
         """One Adam step; returns the loss before the update."""
         self.opt.zero_grad(set_to_none=True)
         if self.mlp_dtype == "bf16":
-            L, _, _, grads = self._mfma_step(tb, pos, check=check)
+            L, _, _, grads = self._mfma_step(tb, pos, check=check, views=True)
             for p, g in zip(self.parameters(), grads):
                 p.grad = g
         else:
@@ -246,10 +248,12 @@ class ForceMatchingTrainer:
         self.eng.lib.tm_gemm_bf16_grouped(len(groups), _ctypes.cast(arr, _ctypes.c_void_p), ta,
                                           tb_, epi, self.alpha, nsplit, self.eng._stream())
 
-    def _mfma_step(self, tb, pos, check=True):
+    def _mfma_step(self, tb, pos, check=True, views=False):
         """(L, E, F, gradients in parameters() order) of the BF16 MFMA path.
         Each GEMM stage runs for every element in one grouped launch; the
-        elementwise steps and row gathers between them run per element."""
+        elementwise steps and row gathers between them run per element.
+        views=True returns the gradients as views of the batch's buffers (the
+        next step overwrites them) instead of copies."""
         eng, ws = self.eng, tb.ws
         lib, st = eng.lib, eng._stream()
         P = _lib_ptr
@@ -357,7 +361,9 @@ class ForceMatchingTrainer:
             gW1, gW2, gW3, gw4 = b["gW"]
             grads += [gW1[:D], gW1[D], gW2[:H], gW2[H], gW3[:H], gW3[H], gw4[:H], gw4[H]]
         # parameters() order: element by element, (W, b) per layer; shapes as the parameters
-        grads = [g.reshape(p.shape).clone() for g, p in zip(grads, self.parameters())]
+        grads = [g.view(p.shape) for g, p in zip(grads, self.parameters())]
+        if not views:
+            grads = [g.clone() for g in grads]
         return L, E, F, grads
 
     def capture(self, tb, pos, warmup=3):
