@@ -155,7 +155,18 @@ int tm_gemm_bf16(int M, int N, int K, const float* A, int lda, int ta, const flo
                  int tb, float* C, int ldc, int epi, const float* bias, const float* s,
                  float* aux, float alpha, int nsplit, float* work, void* stream);
 
-/* One GEMM of a grouped call; fields as the arguments of tm_gemm_bf16. */
+/* One GEMM of a grouped call; fields as the arguments of tm_gemm_bf16, plus
+ * the fused elementwise steps of the double backward (x1, x2, out2; NULL when
+ * unused):
+ *   epi 1 with out2: out2 = aux * x1[n]                  (zd_3 = s_3 w_4)
+ *   epi 2 with out2: out2 = s * x2[m] * x1[n] + alpha s (1 - s) * acc * x1[n]
+ *                    (zb_3 from the last tangent layer, x2 = ybar, x1 = w_4);
+ *                    C may be NULL with epi 2
+ *   epi 3:           C = s * acc + alpha s (1 - s) * x1 * x2
+ *                    (zb_{l-1} = s hb + a s (1 - s) zt hd; s, x1, x2 indexed
+ *                    as C)
+ * The formulas are those of tm_train_elem ops 0, 2 and 3 (op 1 is epi 2's
+ * aux). */
 typedef struct tm_gemm_group {
   int M, N, K;
   const float* A;
@@ -168,6 +179,9 @@ typedef struct tm_gemm_group {
   const float* s;
   float* aux;
   float* work;
+  const float* x1;
+  const float* x2;
+  float* out2;
 } tm_gemm_group;
 
 /* ngroups independent GEMMs (groups: host array) with the same operand forms,
@@ -190,6 +204,20 @@ int tm_train_elem(int op, int M, int H, int ld, float alpha, const float* s, con
  * (scatter = 1: out[idx[r]*ldo + c] = in[r*ldi + c]) of M rows, c < cols. */
 int tm_rows(int scatter, int M, int cols, const int* idx, const float* in, int ldi, float* out,
             int ldo, void* stream);
+
+/* One row gather / scatter of a grouped call; fields as tm_rows' arguments. */
+typedef struct tm_rows_group {
+  int M, cols;
+  const int* idx;
+  const float* in;
+  int ldi;
+  float* out;
+  int ldo;
+} tm_rows_group;
+
+/* ngroups tm_rows calls (groups: host array; the per-element row sets of one
+ * step) in one launch per 16 of them. */
+int tm_rows_grouped(int ngroups, const tm_rows_group* groups, int scatter, void* stream);
 
 int tm_force_gather(int N, int ldr, const int* ncnt, const int* rev, const float* wslot,
                     const double* extra, double scale, double* grad, double* force,

@@ -57,6 +57,7 @@ SIGNATURES = {
     "tm_gemm_bf16_grouped": [I, P, I, I, I, F, I, P],
     "tm_train_elem": [I, I, I, I, F, P, P, P, P, P, P, P, P],
     "tm_rows": [I, I, I, P, P, I, P, I, P],
+    "tm_rows_grouped": [I, P, I, P],
     "tm_mlp_mfma_pack_info": [I, I, I, I, P],
 }
 
@@ -100,7 +101,13 @@ class _Lib:
 class GemmGroup(ctypes.Structure):
     """tm_gemm_group: one GEMM of a tm_gemm_bf16_grouped call."""
     _fields_ = [("M", I), ("N", I), ("K", I), ("A", P), ("lda", I), ("B", P), ("ldb", I),
-                ("C", P), ("ldc", I), ("bias", P), ("s", P), ("aux", P), ("work", P)]
+                ("C", P), ("ldc", I), ("bias", P), ("s", P), ("aux", P), ("work", P),
+                ("x1", P), ("x2", P), ("out2", P)]
+
+
+class RowsGroup(ctypes.Structure):
+    """tm_rows_group: one row gather / scatter of a tm_rows_grouped call."""
+    _fields_ = [("M", I), ("cols", I), ("idx", P), ("inp", P), ("ldi", I), ("out", P), ("ldo", I)]
 
 
 _LIB = None

@@ -36,7 +36,7 @@ constexpr int LROW = GK + 8;  // LDS row of a staged operand, in bf16 (144 B, pa
 constexpr int MAXG = 16;      // GEMMs per grouped launch
 constexpr int NXCD = 8;       // XCDs; workgroups are dealt to them round-robin
 
-enum { EPI_STORE = 0, EPI_SOFTPLUS = 1, EPI_TANGENT = 2 };
+enum { EPI_STORE = 0, EPI_SOFTPLUS = 1, EPI_TANGENT = 2, EPI_REVERSE = 3 };
 
 // One GEMM of a grouped launch: the caller's tm_gemm_group and its tiling.
 struct GemmDev {
@@ -47,6 +47,9 @@ struct GemmDev {
   const float* bias;
   const float* s;
   float* aux;
+  const float* x1;
+  const float* x2;
+  float* out2;
   long cstride;        // floats between the splits' partial outputs
   int M, N, K, lda, ldb, ldc;
   int kchunk;          // K range of one split (a multiple of GK)
@@ -202,44 +205,46 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(const GemmBatch gb) {
               0, 0);
     }
   }
-  // Epilogue.  Every input it reads (bias, s) is loaded before the first
-  // store: a load behind a store to a pointer that may alias it would wait
-  // for that store, one memory latency per element.
-  float* C = g.C + (long)blockIdx.y * g.cstride;
+  // Epilogue, in four passes of 16 values per lane (one row block i each).
+  // Each pass loads every input it reads before its first store: a load
+  // behind a store to a pointer that may alias it would wait for that store,
+  // one memory latency per element.
+  //  EPI_STORE    C = acc + bias[n]
+  //  EPI_SOFTPLUS z = a (acc + bias[n]), C = log(1 + e^z) / a, aux = sigmoid(z);
+  //               out2 (if given) = aux * x1[n]
+  //  EPI_TANGENT  C (if given) = acc, aux = acc * s; out2 (if given) =
+  //               s * x2[m] * x1[n] + a s (1 - s) * acc * x1[n]
+  //  EPI_REVERSE  C = s * acc + a s (1 - s) * x1 * x2  (s, x1, x2 indexed as C)
+  float* C = g.C != nullptr ? g.C + (long)blockIdx.y * g.cstride : nullptr;
   const float a = gb.alpha;
   const int epi = gb.epi;
   const int mrow = m0 + wm + (lane >> 4) * 4, ncol = n0 + wn + (lane & 15);
-  float bn[4];
+  float bn[4], wn4[4];
 #pragma unroll
-  for (int j = 0; j < 4; ++j)
-    bn[j] = (g.bias != nullptr && ncol + 16 * j < g.N) ? g.bias[ncol + 16 * j] : 0.f;
-  if (epi == EPI_TANGENT) {
-    floatx4 sv[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int m = mrow + 16 * i + q, n = ncol + 16 * j;
-          sv[i][j][q] = (m < g.M && n < g.N) ? g.s[(long)m * g.ldc + n] : 0.f;
-        }
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int m = mrow + 16 * i + q, n = ncol + 16 * j;
-          if (m >= g.M || n >= g.N) continue;
-          const long o = (long)m * g.ldc + n;
-          C[o] = acc[i][j][q];
-          g.aux[o] = acc[i][j][q] * sv[i][j][q];
-        }
-    return;
+  for (int j = 0; j < 4; ++j) {
+    const bool ok = ncol + 16 * j < g.N;
+    bn[j] = (g.bias != nullptr && ok) ? g.bias[ncol + 16 * j] : 0.f;
+    wn4[j] = (g.out2 != nullptr && ok) ? g.x1[ncol + 16 * j] : 0.f;
   }
+  const float ia = 1.0f / a;
+  // one row block (16 values per lane); called with constant i so that the
+  // accumulators stay in registers
+  auto block = [&](const int i, const floatx4 (&acc_i)[4]) __attribute__((always_inline)) {
+    float sv[4][4], u1[4][4], u2[4][4], ym[4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+    for (int q = 0; q < 4; ++q) {
+      const int m = mrow + 16 * i + q;
+      ym[q] = (epi == EPI_TANGENT && g.out2 != nullptr && m < g.M) ? g.x2[m] : 0.f;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int n = ncol + 16 * j;
+        const bool ok = m < g.M && n < g.N;
+        const long o = (long)m * g.ldc + n;
+        sv[j][q] = (ok && (epi == EPI_TANGENT || epi == EPI_REVERSE)) ? g.s[o] : 0.f;
+        u1[j][q] = (ok && epi == EPI_REVERSE) ? g.x1[o] : 0.f;
+        u2[j][q] = (ok && epi == EPI_REVERSE) ? g.x2[o] : 0.f;
+      }
+    }
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
@@ -247,16 +252,33 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(const GemmBatch gb) {
         const int m = mrow + 16 * i + q, n = ncol + 16 * j;
         if (m >= g.M || n >= g.N) continue;
         const long o = (long)m * g.ldc + n;
-        const float v = acc[i][j][q] + bn[j];
+        const float v = acc_i[j][q];
         if (epi == EPI_SOFTPLUS) {
-          const float z = a * v;
-          // log(1 + e^z) / a, and its derivative sigmoid(z), without overflow
-          C[o] = (fmaxf(z, 0.f) + log1pf(__expf(-fabsf(z)))) / a;
-          g.aux[o] = 1.0f / (1.0f + __expf(-z));
+          const float z = a * (v + bn[j]);
+          // log(1 + e^z) / a and its derivative sigmoid(z), without overflow
+          const float e = __expf(-fabsf(z));
+          C[o] = (fmaxf(z, 0.f) + __logf(1.0f + e)) * ia;
+          const float sg = z >= 0.f ? __fdividef(1.0f, 1.0f + e) : __fdividef(e, 1.0f + e);
+          g.aux[o] = sg;
+          if (g.out2 != nullptr) g.out2[o] = sg * wn4[j];
+        } else if (epi == EPI_TANGENT) {
+          const float sg = sv[j][q];
+          if (C != nullptr) C[o] = v;
+          g.aux[o] = v * sg;
+          if (g.out2 != nullptr)
+            g.out2[o] = sg * ym[q] * wn4[j] + a * sg * (1.0f - sg) * v * wn4[j];
+        } else if (epi == EPI_REVERSE) {
+          const float sg = sv[j][q];
+          C[o] = sg * v + a * sg * (1.0f - sg) * u1[j][q] * u2[j][q];
         } else {
-          C[o] = v;
+          C[o] = v + bn[j];
         }
       }
+  };
+  block(0, acc[0]);
+  block(1, acc[1]);
+  block(2, acc[2]);
+  block(3, acc[3]);
 }
 
 // out = sum over the splits of their partial outputs, in split order; one
@@ -312,6 +334,21 @@ __global__ void rows_kernel(int scatter, int M, int cols, const int* __restrict_
   else out[(long)r * ldo + c] = in[(long)idx[r] * ldi + c];
 }
 
+// rows_kernel over several (idx, in, out) sets at once, one per blockIdx.y.
+struct RowsBatch {
+  tm_rows_group r[MAXG];
+  int scatter;
+};
+
+__global__ void rows_grouped_kernel(const RowsBatch rb) {
+  const tm_rows_group& d = rb.r[blockIdx.y];
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)d.M * d.cols) return;
+  const int r = (int)(t / d.cols), c = (int)(t % d.cols);
+  if (rb.scatter) d.out[(long)d.idx[r] * d.ldo + c] = d.in[(long)r * d.ldi + c];
+  else d.out[(long)r * d.ldo + c] = d.in[(long)d.idx[r] * d.ldi + c];
+}
+
 }  // namespace
 
 extern "C" {
@@ -320,7 +357,7 @@ int tm_gemm_bf16_grouped(int ngroups, const tm_gemm_group* groups, int ta, int t
                          float alpha, int nsplit, void* stream) {
   TM_REQUIRE(ngroups >= 0 && (ngroups == 0 || groups != nullptr),
              "tm_gemm_bf16_grouped: bad group list");
-  TM_REQUIRE(epi >= EPI_STORE && epi <= EPI_TANGENT, "tm_gemm_bf16: bad epilogue %d", epi);
+  TM_REQUIRE(epi >= EPI_STORE && epi <= EPI_REVERSE, "tm_gemm_bf16: bad epilogue %d", epi);
   TM_REQUIRE(nsplit >= 1 && nsplit <= 64, "tm_gemm_bf16: nsplit %d outside 1..64", nsplit);
   TM_REQUIRE(nsplit == 1 || epi == EPI_STORE,
              "tm_gemm_bf16: split K needs the plain epilogue");
@@ -333,6 +370,16 @@ int tm_gemm_bf16_grouped(int ngroups, const tm_gemm_group* groups, int ta, int t
                "tm_gemm_bf16: the softplus epilogue needs bias and aux (GEMM %d)", i);
     TM_REQUIRE(epi != EPI_TANGENT || (q.s != nullptr && q.aux != nullptr),
                "tm_gemm_bf16: the tangent epilogue needs s and aux (GEMM %d)", i);
+    TM_REQUIRE(epi != EPI_REVERSE || (q.s != nullptr && q.x1 != nullptr && q.x2 != nullptr &&
+                                      q.C != nullptr),
+               "tm_gemm_bf16: the reverse epilogue needs s, x1, x2 and C (GEMM %d)", i);
+    TM_REQUIRE(q.out2 == nullptr ||
+                   (epi == EPI_SOFTPLUS && q.x1 != nullptr) ||
+                   (epi == EPI_TANGENT && q.x1 != nullptr && q.x2 != nullptr),
+               "tm_gemm_bf16: out2 needs the softplus (x1) or tangent (x1, x2) epilogue "
+               "(GEMM %d)", i);
+    TM_REQUIRE(q.C != nullptr || epi == EPI_TANGENT,
+               "tm_gemm_bf16: C may be NULL only with the tangent epilogue (GEMM %d)", i);
     TM_REQUIRE(nsplit == 1 || (q.bias == nullptr && q.work != nullptr),
                "tm_gemm_bf16: split K needs a workspace and no bias (GEMM %d)", i);
   }
@@ -366,6 +413,7 @@ int tm_gemm_bf16_grouped(int ngroups, const tm_gemm_group* groups, int ta, int t
       d.A = q.A; d.B = q.B; d.out = q.C;
       d.C = nsplit > 1 ? q.work : q.C;
       d.bias = q.bias; d.s = q.s; d.aux = q.aux;
+      d.x1 = q.x1; d.x2 = q.x2; d.out2 = q.out2;
       d.M = q.M; d.N = q.N; d.K = q.K;
       d.lda = q.lda; d.ldb = q.ldb; d.ldc = q.ldc;
       d.cstride = (long)q.M * q.ldc;
@@ -395,6 +443,7 @@ int tm_gemm_bf16(int M, int N, int K, const float* A, int lda, int ta, const flo
   q.M = M; q.N = N; q.K = K;
   q.A = A; q.lda = lda; q.B = B; q.ldb = ldb; q.C = C; q.ldc = ldc;
   q.bias = bias; q.s = s; q.aux = aux; q.work = work;
+  q.x1 = nullptr; q.x2 = nullptr; q.out2 = nullptr;
   return tm_gemm_bf16_grouped(1, &q, ta, tb, epi, alpha, nsplit, stream);
 }
 
@@ -415,6 +464,28 @@ int tm_rows(int scatter, int M, int cols, const int* idx, const float* in, int l
   rows_kernel<<<tm_div_up((long)M * cols, 256), 256, 0, (hipStream_t)stream>>>(
       scatter, M, cols, idx, in, ldi, out, ldo);
   TM_LAUNCH_CHECK();
+  return TM_OK;
+}
+
+int tm_rows_grouped(int ngroups, const tm_rows_group* groups, int scatter, void* stream) {
+  TM_REQUIRE(ngroups >= 0 && (ngroups == 0 || groups != nullptr),
+             "tm_rows_grouped: bad group list");
+  for (int g0 = 0; g0 < ngroups;) {
+    RowsBatch rb;
+    rb.scatter = scatter;
+    int n = 0;
+    long most = 0;
+    for (; g0 < ngroups && n < MAXG; ++g0) {
+      const tm_rows_group& q = groups[g0];
+      TM_REQUIRE(q.M >= 0 && q.cols >= 0, "tm_rows_grouped: negative size (set %d)", g0);
+      if ((long)q.M * q.cols == 0) continue;
+      rb.r[n++] = q;
+      most = std::max(most, (long)q.M * q.cols);
+    }
+    if (n == 0) continue;
+    rows_grouped_kernel<<<dim3(tm_div_up(most, 256), n), 256, 0, (hipStream_t)stream>>>(rb);
+    TM_LAUNCH_CHECK();
+  }
   return TM_OK;
 }
 

@@ -223,7 +223,7 @@ class ForceMatchingTrainer:
                 b[k][:M, H] = 1.0
             for k in ("ZZ1", "ZZ2", "ZZ3"):
                 b[k] = torch.zeros(2 * M, ldh, **f32)
-            for k in ("S1", "S2", "S3", "ZT1", "ZT2", "ZT3", "HD1", "HD2", "HB"):
+            for k in ("S1", "S2", "S3", "ZT1", "ZT2", "ZT3", "HD1", "HD2"):
                 b[k] = torch.zeros(M, ldh, **f32)
             b["Y"] = torch.zeros(M, 1, **f32)
             b["YB"] = torch.ones(2 * M, 1, **f32)
@@ -242,65 +242,68 @@ class ForceMatchingTrainer:
         return bufs
 
     def _ggemm(self, groups, ta, tb_, epi=0, nsplit=1):
-        """One tm_gemm_bf16_grouped call over tuples (M, N, K, A, lda, B, ldb,
-        C, ldc, bias, s, aux, work) of device addresses (ints) or None."""
-        arr = (_lib.GemmGroup * len(groups))(*[_lib.GemmGroup(*g) for g in groups])
+        """One tm_gemm_bf16_grouped call over dicts of tm_gemm_group fields
+        (device addresses as ints; absent fields are NULL)."""
+        arr = (_lib.GemmGroup * len(groups))(*[_lib.GemmGroup(**g) for g in groups])
         self.eng.lib.tm_gemm_bf16_grouped(len(groups), _ctypes.cast(arr, _ctypes.c_void_p), ta,
                                           tb_, epi, self.alpha, nsplit, self.eng._stream())
 
+    def _grows(self, sets, scatter):
+        """One tm_rows_grouped call over (M, cols, idx, in, ldi, out, ldo)."""
+        arr = (_lib.RowsGroup * len(sets))(*[_lib.RowsGroup(*r) for r in sets])
+        self.eng.lib.tm_rows_grouped(len(sets), _ctypes.cast(arr, _ctypes.c_void_p), scatter,
+                                     self.eng._stream())
+
     def _mfma_step(self, tb, pos, check=True, views=False):
         """(L, E, F, gradients in parameters() order) of the BF16 MFMA path.
-        Each GEMM stage runs for every element in one grouped launch; the
-        elementwise steps and row gathers between them run per element.
+        Each GEMM stage runs for every element in one grouped launch, with
+        the elementwise steps between them in the GEMM epilogues; the row
+        gathers and scatters of every element also take one launch each.
         views=True returns the gradients as views of the batch's buffers (the
         next step overwrites them) instead of copies."""
         eng, ws = self.eng, tb.ws
-        lib, st = eng.lib, eng._stream()
-        P = _lib_ptr
-        ad = lambda t, rows=0, ld=0: t.data_ptr() + 4 * rows * ld   # row `rows` of t
-        off = lambda t, rows, ld: _ctypes.c_void_p(ad(t, rows, ld))
         eng.build_lists(ws, pos)
         if check:
             eng.check_flags(ws)
         eng.descriptors(ws, pos)
-        N, D, a = tb.N, eng.D, self.alpha
+        N, D = tb.N, eng.D
         bufs = self._mfma_buffers(tb)
         items = [(b, self.params[e]) for e, b in bufs.items()]
         H = items[0][0]["H"]
+        ad = lambda t, rows=0, ld=0: t.data_ptr() + 4 * rows * ld   # row `rows` of t
         f32 = dict(dtype=torch.float32, device=self.device)
         eps = torch.zeros(N, 1, **f32)
         J = torch.zeros(N, D, **f32)
-        for b, _ in items:
-            lib.tm_rows(0, b["M"], D, P(b["idx"]), P(ws.G), D, P(b["X0"]), b["ldx"], st)
-        # forward: z_l = h_{l-1} W_l + b_l, h_l = act(z_l), s_l = act'(z_l)
+        self._grows([(b["M"], D, ad(b["idx"]), ad(ws.G), D, ad(b["X0"]), b["ldx"])
+                     for b, _ in items], 0)
+        # forward: z_l = h_{l-1} W_l + b_l, h_l = act(z_l), s_l = act'(z_l); the
+        # last layer also starts the J chain, zd_3 = s_3 w_4 (rows M.. of ZZ3)
         layers = (("X0", "H1", "S1", "ZT1"), ("H1", "H2", "S2", "ZT2"), ("H2", "H3", "S3", "ZT3"))
         for l, (hin, hout, sk, _) in enumerate(layers):
             grp = []
             for b, prm in items:
-                ldp, Kp = (b["ldx"], D) if l == 0 else (b["ldh"], H)
-                grp.append((b["M"], H, Kp, ad(b[hin]), ldp, ad(prm[l][0]), H, ad(b[hout]), b["ldh"],
-                            ad(prm[l][1]), None, ad(b[sk]), None))
+                M, ldh = b["M"], b["ldh"]
+                ldp, Kp = (b["ldx"], D) if l == 0 else (ldh, H)
+                g = dict(M=M, N=H, K=Kp, A=ad(b[hin]), lda=ldp, B=ad(prm[l][0]), ldb=H,
+                         C=ad(b[hout]), ldc=ldh, bias=ad(prm[l][1]), aux=ad(b[sk]))
+                if l == 2:
+                    g.update(x1=ad(prm[3][0]), out2=ad(b["ZZ3"], M, ldh))
+                grp.append(g)
             self._ggemm(grp, 0, 0, 1)
-        self._ggemm([(b["M"], 1, H, ad(b["H3"]), b["ldh"], ad(prm[3][0]), 1, ad(b["Y"]), 1,
-                      ad(prm[3][1]), None, None, None) for b, prm in items], 0, 0, 0)
-        # J chain: zd_3 = s_3 w_4 (rows M.. of ZZ3), then down to J
-        for b, prm in items:
-            M, ldh = b["M"], b["ldh"]
-            lib.tm_rows(1, M, 1, P(b["idx"]), P(b["Y"]), 1, P(eps), 1, st)
-            lib.tm_train_elem(0, M, H, ldh, a, P(b["S3"]), None, None, None, P(prm[3][0]), None,
-                              off(b["ZZ3"], M, ldh), st)
+        self._ggemm([dict(M=b["M"], N=1, K=H, A=ad(b["H3"]), lda=b["ldh"], B=ad(prm[3][0]),
+                          ldb=1, C=ad(b["Y"]), ldc=1, bias=ad(prm[3][1])) for b, prm in items],
+                    0, 0, 0)
+        self._grows([(b["M"], 1, ad(b["idx"]), ad(b["Y"]), 1, ad(eps), 1) for b, _ in items], 1)
+        # J chain: hd_{l-1} = zd_l W_l^T, zd_{l-1} = s_{l-1} hd_{l-1}, J = zd_1 W_1^T
         for (l, zz_in, hd, sk, zz_out) in ((2, "ZZ3", "HD2", "S2", "ZZ2"),
                                            (1, "ZZ2", "HD1", "S1", "ZZ1")):
-            self._ggemm([(b["M"], H, H, ad(b[zz_in], b["M"], b["ldh"]), b["ldh"], ad(prm[l][0]), H,
-                          ad(b[hd]), b["ldh"], None, None, None, None) for b, prm in items], 0, 1)
-            for b, _ in items:
-                M, ldh = b["M"], b["ldh"]
-                lib.tm_train_elem(1, M, H, ldh, a, P(b[sk]), P(b[hd]), None, None, None, None,
-                                  off(b[zz_out], M, ldh), st)
-        self._ggemm([(b["M"], D, H, ad(b["ZZ1"], b["M"], b["ldh"]), b["ldh"], ad(prm[0][0]), H,
-                      ad(b["Je"]), D, None, None, None, None) for b, prm in items], 0, 1)
-        for b, _ in items:
-            lib.tm_rows(1, b["M"], D, P(b["idx"]), P(b["Je"]), D, P(J), D, st)
+            self._ggemm([dict(M=b["M"], N=H, K=H, A=ad(b[zz_in], b["M"], b["ldh"]), lda=b["ldh"],
+                              B=ad(prm[l][0]), ldb=H, C=ad(b[hd]), ldc=b["ldh"], s=ad(b[sk]),
+                              aux=ad(b[zz_out], b["M"], b["ldh"])) for b, prm in items], 0, 1, 2)
+        self._ggemm([dict(M=b["M"], N=D, K=H, A=ad(b["ZZ1"], b["M"], b["ldh"]), lda=b["ldh"],
+                          B=ad(prm[0][0]), ldb=H, C=ad(b["Je"]), ldc=D) for b, prm in items],
+                    0, 1, 0)
+        self._grows([(b["M"], D, ad(b["idx"]), ad(b["Je"]), D, ad(J), D) for b, _ in items], 1)
         E = torch.zeros(tb.B, dtype=torch.float64, device=self.device).index_add(
             0, tb.mol, eps[:, 0].double())
         grad = torch.empty(N, 3, dtype=torch.float64, device=self.device)
@@ -312,30 +315,29 @@ class ForceMatchingTrainer:
         gg = (-2.0 * wf * (F - tb.F_ref)).contiguous()   # dLoss/dgrad
         V = torch.empty(N, D, **f32)
         eng.descriptor_jvp(ws, pos, gg, V)
-        for b, _ in items:
-            M, ldx = b["M"], b["ldx"]
-            lib.tm_rows(0, M, D, P(b["idx"]), P(V), D, off(b["X0"], M, ldx), ldx, st)
-            lib.tm_rows(0, M, 1, P(b["idx"]), P(ybar), 1, P(b["YB"]), 1, st)
-        # tangent along V: zt_l = ht_{l-1} W_l, ht_l = s_l zt_l (rows M.. of H_l)
+        self._grows([(b["M"], D, ad(b["idx"]), ad(V), D, ad(b["X0"], b["M"], b["ldx"]), b["ldx"])
+                     for b, _ in items] +
+                    [(b["M"], 1, ad(b["idx"]), ad(ybar), 1, ad(b["YB"]), 1) for b, _ in items], 0)
+        # tangent along V: zt_l = ht_{l-1} W_l, ht_l = s_l zt_l (rows M.. of H_l);
+        # the last layer also starts the reverse pass,
+        # zb_3 = s_3 ybar w_4 + a s_3 (1 - s_3) zt_3 w_4 (rows ..M of ZZ3)
         for l, (hin, hout, sk, zt) in enumerate(layers):
             grp = []
             for b, prm in items:
                 M, ldh = b["M"], b["ldh"]
                 ldp, Kp = (b["ldx"], D) if l == 0 else (ldh, H)
-                grp.append((M, H, Kp, ad(b[hin], M, ldp), ldp, ad(prm[l][0]), H, ad(b[zt]), ldh,
-                            None, ad(b[sk]), ad(b[hout], M, ldh), None))
+                g = dict(M=M, N=H, K=Kp, A=ad(b[hin], M, ldp), lda=ldp, B=ad(prm[l][0]), ldb=H,
+                         C=ad(b[zt]), ldc=ldh, s=ad(b[sk]), aux=ad(b[hout], M, ldh))
+                if l == 2:
+                    g.update(C=None, x1=ad(prm[3][0]), x2=ad(b["YB"]), out2=ad(b["ZZ3"]))
+                grp.append(g)
             self._ggemm(grp, 0, 0, 2)
-        # reverse
-        for b, prm in items:
-            lib.tm_train_elem(2, b["M"], H, b["ldh"], a, P(b["S3"]), None, P(b["ZT3"]), None,
-                              P(prm[3][0]), P(b["YB"]), P(b["ZZ3"]), st)
+        # reverse: zb_{l-1} = s_{l-1} (zb_l W_l^T) + a s_{l-1} (1 - s_{l-1}) zt_{l-1} hd_{l-1}
         for (l, zz_in, sk, zt, hd, zz_out) in ((2, "ZZ3", "S2", "ZT2", "HD2", "ZZ2"),
                                                (1, "ZZ2", "S1", "ZT1", "HD1", "ZZ1")):
-            self._ggemm([(b["M"], H, H, ad(b[zz_in]), b["ldh"], ad(prm[l][0]), H, ad(b["HB"]),
-                          b["ldh"], None, None, None, None) for b, prm in items], 0, 1)
-            for b, _ in items:
-                lib.tm_train_elem(3, b["M"], H, b["ldh"], a, P(b[sk]), P(b["HB"]), P(b[zt]),
-                                  P(b[hd]), None, None, P(b[zz_out]), st)
+            self._ggemm([dict(M=b["M"], N=H, K=H, A=ad(b[zz_in]), lda=b["ldh"], B=ad(prm[l][0]),
+                              ldb=H, C=ad(b[zz_out]), ldc=b["ldh"], s=ad(b[sk]), x1=ad(b[zt]),
+                              x2=ad(b[hd])) for b, prm in items], 0, 1, 3)
         # weight gradients over the stacked rows (bias from the ones column),
         # every layer of every element in one grouped launch
         grp = []
@@ -343,14 +345,13 @@ class ForceMatchingTrainer:
             M2, ldx, ldh = 2 * b["M"], b["ldx"], b["ldh"]
             gW1, gW2, gW3, gw4 = b["gW"]
             wk = b["work"]
-            grp += [(D + 1, H, M2, ad(b["X0"]), ldx, ad(b["ZZ1"]), ldh, ad(gW1), H, None, None,
-                     None, ad(wk[0])),
-                    (H + 1, H, M2, ad(b["H1"]), ldh, ad(b["ZZ2"]), ldh, ad(gW2), H, None, None,
-                     None, ad(wk[1])),
-                    (H + 1, H, M2, ad(b["H2"]), ldh, ad(b["ZZ3"]), ldh, ad(gW3), H, None, None,
-                     None, ad(wk[2])),
-                    (H + 1, 1, M2, ad(b["H3"]), ldh, ad(b["YB"]), 1, ad(gw4), 1, None, None, None,
-                     ad(wk[3]))]
+            for (Mo, No, A, lda, B, ldb, G, w) in (
+                    (D + 1, H, b["X0"], ldx, b["ZZ1"], ldh, gW1, wk[0]),
+                    (H + 1, H, b["H1"], ldh, b["ZZ2"], ldh, gW2, wk[1]),
+                    (H + 1, H, b["H2"], ldh, b["ZZ3"], ldh, gW3, wk[2]),
+                    (H + 1, 1, b["H3"], ldh, b["YB"], 1, gw4, wk[3])):
+                grp.append(dict(M=Mo, N=No, K=M2, A=ad(A), lda=lda, B=ad(B), ldb=ldb, C=ad(G),
+                                ldc=No, work=ad(w)))
         self._ggemm(grp, 1, 0, 0, nsplit=items[0][0]["nsplit"])
         grads = []
         for e in self.model.eles:

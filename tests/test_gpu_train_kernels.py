@@ -191,3 +191,72 @@ def test_gemm_grouped_equals_single_calls(cuda, ta, tb, epi, ns):
             assert ((C - z).abs() / scale).max().item() < 1e-5, (M, N, K)
             if epi == 2:
                 np.testing.assert_allclose(aux.cpu(), (C * s).cpu(), rtol=1e-6, atol=1e-7)
+
+
+def test_gemm_fused_elementwise_epilogues(cuda):
+    """The epilogues that carry the elementwise steps of the double backward
+    against tm_train_elem's formulas on the FP32 GEMM of the rounded
+    operands: softplus + out2 = sigmoid * w4[n] (op 0), tangent + out2 (op 2,
+    C not written), and the reverse epilogue (op 3)."""
+    import ctypes
+    from tensormol_amd import _lib as TL
+    L, P = _lib()
+    g = torch.Generator(device="cpu").manual_seed(23)
+    M, N, K, a, ld = 150, 140, 96, 2.5, 144
+    st = torch.cuda.current_stream().cuda_stream
+    A = torch.randn(M, K, generator=g).to(cuda) * 0.3
+    B = torch.randn(K, N, generator=g).to(cuda) * 0.3
+    bias = torch.randn(N, generator=g).to(cuda)
+    s = torch.rand(M, ld, generator=g).to(cuda)
+    w4 = torch.randn(N, generator=g).to(cuda)
+    yb = torch.randn(M, generator=g).to(cuda)
+    zt, hd = torch.randn(M, ld, generator=g).to(cuda), torch.randn(M, ld, generator=g).to(cuda)
+    acc = _r(A) @ _r(B)
+    S = s[:, :N]
+    ad = lambda t: t.data_ptr()
+
+    def run(epi, **f):
+        grp = TL.GemmGroup(M=M, N=N, K=K, A=ad(A), lda=K, B=ad(B), ldb=N, ldc=ld, **f)
+        arr = (TL.GemmGroup * 1)(grp)
+        L.tm_gemm_bf16_grouped(1, ctypes.cast(arr, ctypes.c_void_p), 0, 0, epi, a, 1, st)
+        torch.cuda.synchronize()
+
+    C, aux, out2 = (torch.full((M, ld), 7.0, device=cuda) for _ in range(3))
+    run(1, C=ad(C), bias=ad(bias), aux=ad(aux), x1=ad(w4), out2=ad(out2))
+    sg = torch.sigmoid(a * (acc + bias))
+    np.testing.assert_allclose(aux[:, :N].cpu(), sg.cpu(), rtol=1e-4, atol=1e-5)
+    np.testing.assert_allclose(out2[:, :N].cpu(), (sg * w4).cpu(), rtol=1e-4, atol=1e-5)
+    assert (out2[:, N:] == 7.0).all()
+    C, aux, out2 = (torch.full((M, ld), 7.0, device=cuda) for _ in range(3))
+    run(2, C=None, s=ad(s), aux=ad(aux), x1=ad(w4), x2=ad(yb), out2=ad(out2))
+    ref2 = S * yb[:, None] * w4 + a * S * (1 - S) * acc * w4
+    np.testing.assert_allclose(aux[:, :N].cpu(), (acc * S).cpu(), rtol=1e-4, atol=1e-5)
+    np.testing.assert_allclose(out2[:, :N].cpu(), ref2.cpu(), rtol=1e-4, atol=1e-5)
+    assert (C == 7.0).all()
+    C = torch.full((M, ld), 7.0, device=cuda)
+    run(3, C=ad(C), s=ad(s), x1=ad(zt), x2=ad(hd))
+    ref3 = S * acc + a * S * (1 - S) * zt[:, :N] * hd[:, :N]
+    np.testing.assert_allclose(C[:, :N].cpu(), ref3.cpu(), rtol=1e-4, atol=1e-5)
+    assert (C[:, N:] == 7.0).all()
+
+
+def test_rows_grouped_equals_single_calls(cuda):
+    import ctypes
+    from tensormol_amd import _lib as TL
+    L, P = _lib()
+    st = torch.cuda.current_stream().cuda_stream
+    src = torch.arange(40 * 9, dtype=torch.float32, device=cuda).reshape(40, 9)
+    sets = []
+    outs = []
+    for k, idx in enumerate(([5, 0, 3, 9], [1, 2], [], [39, 20, 7])):
+        it = torch.tensor(idx, dtype=torch.int32, device=cuda)
+        o = torch.zeros(max(len(idx), 1), 11, device=cuda)
+        outs.append((it, o))
+        sets.append(TL.RowsGroup(len(idx), 9, it.data_ptr(), src.data_ptr(), 9, o.data_ptr(), 11))
+    arr = (TL.RowsGroup * len(sets))(*sets)
+    L.tm_rows_grouped(len(sets), ctypes.cast(arr, ctypes.c_void_p), 0, st)
+    torch.cuda.synchronize()
+    for it, o in outs:
+        if it.numel():
+            assert torch.equal(o[:it.numel(), :9], src[it.long()])
+            assert (o[:, 9:] == 0).all()
