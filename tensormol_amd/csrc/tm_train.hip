@@ -53,6 +53,7 @@ struct GemmDev {
   long cstride;        // floats between the splits' partial outputs
   int M, N, K, lda, ldb, ldc;
   int kchunk;          // K range of one split (a multiple of GK)
+  int vec_out;         // every row-indexed epilogue array takes 16-byte accesses
   int tiles_n;         // column tiles
   int tile0;           // first linear tile of this GEMM
 };
@@ -138,6 +139,98 @@ struct Stage {
   }
 };
 
+constexpr int CLD = GT + 4;   // LDS row of the staged C tile, in floats
+
+// Four consecutive values of row o .. o + 3 (nv of them valid).
+__device__ __forceinline__ void ld4(const float* __restrict__ p, long o, int nv, bool vec,
+                                    float (&v)[4]) {
+  if (vec && nv == 4) {
+    const float4 f = *reinterpret_cast<const float4*>(p + o);
+    v[0] = f.x; v[1] = f.y; v[2] = f.z; v[3] = f.w;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = k < nv ? p[o + k] : 0.f;
+  }
+}
+
+__device__ __forceinline__ void st4(float* __restrict__ p, long o, int nv, bool vec,
+                                    const float (&v)[4]) {
+  if (vec && nv == 4) {
+    *reinterpret_cast<float4*>(p + o) = make_float4(v[0], v[1], v[2], v[3]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (k < nv) p[o + k] = v[k];
+  }
+}
+
+// Epilogue of one C tile staged in LDS (Cs[row * CLD + col]): thread t takes
+// columns 4 (t % 32) .. + 3 of rows t / 32 + 8 r.  The outputs (C, aux, out2)
+// never overlap the inputs (bias, s, x1, x2), which restrict states, so a
+// row's loads need not wait for the previous row's stores.
+//  EPI_STORE    C = acc + bias[n]
+//  EPI_SOFTPLUS z = a (acc + bias[n]), C = log(1 + e^z) / a, aux = sigmoid(z);
+//               out2 (if given) = aux * x1[n]
+//  EPI_TANGENT  C (if given) = acc, aux = acc * s; out2 (if given) =
+//               s * x2[m] * x1[n] + a s (1 - s) * acc * x1[n]
+//  EPI_REVERSE  C = s * acc + a s (1 - s) * x1 * x2  (s, x1, x2 indexed as C)
+__device__ __forceinline__ void epilogue_tile(
+    const float* Cs, int m0, int n0, int M, int N, long ldc, int epi, float a, bool vec,
+    float* __restrict__ C, float* __restrict__ aux, float* __restrict__ out2,
+    const float* __restrict__ bias, const float* __restrict__ s,
+    const float* __restrict__ x1, const float* __restrict__ x2, int tid) {
+  const int c = (tid & 31) * 4, n = n0 + c;
+  if (n >= N) return;
+  const int nv = min(4, N - n);
+  float bv[4], wv[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    bv[k] = (bias != nullptr && k < nv) ? bias[n + k] : 0.f;
+    wv[k] = (out2 != nullptr && k < nv) ? x1[n + k] : 0.f;
+  }
+  const float ia = 1.0f / a;
+#pragma unroll 2
+  for (int r = tid >> 5; r < GT; r += 8) {
+    const int m = m0 + r;
+    if (m >= M) break;
+    const long o = (long)m * ldc + n;
+    float v[4], sv[4] = {0.f, 0.f, 0.f, 0.f}, u1[4] = {0.f, 0.f, 0.f, 0.f},
+                u2[4] = {0.f, 0.f, 0.f, 0.f}, r0[4], r1[4], r2[4];
+    const float4 cv = *reinterpret_cast<const float4*>(Cs + r * CLD + c);
+    v[0] = cv.x; v[1] = cv.y; v[2] = cv.z; v[3] = cv.w;
+    if (epi == EPI_TANGENT || epi == EPI_REVERSE) ld4(s, o, nv, vec, sv);
+    if (epi == EPI_REVERSE) {
+      ld4(x1, o, nv, vec, u1);
+      ld4(x2, o, nv, vec, u2);
+    }
+    const float ym = (epi == EPI_TANGENT && out2 != nullptr) ? x2[m] : 0.f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (epi == EPI_SOFTPLUS) {
+        const float z = a * (v[k] + bv[k]);
+        // log(1 + e^z) / a and its derivative sigmoid(z), without overflow
+        const float e = __expf(-fabsf(z));
+        r0[k] = (fmaxf(z, 0.f) + __logf(1.0f + e)) * ia;
+        r1[k] = z >= 0.f ? __fdividef(1.0f, 1.0f + e) : __fdividef(e, 1.0f + e);
+        r2[k] = r1[k] * wv[k];
+      } else if (epi == EPI_TANGENT) {
+        const float sg = sv[k];
+        r0[k] = v[k];
+        r1[k] = v[k] * sg;
+        r2[k] = sg * ym * wv[k] + a * sg * (1.0f - sg) * v[k] * wv[k];
+      } else if (epi == EPI_REVERSE) {
+        const float sg = sv[k];
+        r0[k] = sg * v[k] + a * sg * (1.0f - sg) * u1[k] * u2[k];
+      } else {
+        r0[k] = v[k] + bv[k];
+      }
+    }
+    if (C != nullptr) st4(C, o, nv, vec, r0);
+    if (epi == EPI_SOFTPLUS || epi == EPI_TANGENT) st4(aux, o, nv, vec, r1);
+    if (out2 != nullptr) st4(out2, o, nv, vec, r2);
+  }
+}
+
 // C tile (GT x GT) of one GEMM of the batch per workgroup, four waves of
 // 64 x 64 (4 x 4 MFMA tiles); blockIdx.y is the K split.  Operands are staged
 // through two LDS buffers as BF16 [row][k]; MFMA fragment: lane l takes row
@@ -149,8 +242,10 @@ struct Stage {
 // column tiles of a row block (which share their A rows) meet in one L2.
 template <bool TA, bool TB, bool VEC>
 __global__ void __launch_bounds__(256) gemm_bf16_kernel(const GemmBatch gb) {
-  __shared__ __align__(16) unsigned short As[2][GT][LROW];
-  __shared__ __align__(16) unsigned short Bs[2][GT][LROW];
+  // two staged K steps of A and B, and after the K loop the C tile
+  __shared__ __align__(16) unsigned char lds[2 * 2 * GT * LROW * 2];
+  auto As = reinterpret_cast<unsigned short (*)[GT][LROW]>(lds);
+  auto Bs = reinterpret_cast<unsigned short (*)[GT][LROW]>(lds + 2 * GT * LROW * 2);
   const int t = (blockIdx.x % NXCD) * gb.per_xcd + blockIdx.x / NXCD;
   if (t >= gb.ntiles) return;
   int gi = 0;
@@ -205,80 +300,23 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(const GemmBatch gb) {
               0, 0);
     }
   }
-  // Epilogue, in four passes of 16 values per lane (one row block i each).
-  // Each pass loads every input it reads before its first store: a load
-  // behind a store to a pointer that may alias it would wait for that store,
-  // one memory latency per element.
-  //  EPI_STORE    C = acc + bias[n]
-  //  EPI_SOFTPLUS z = a (acc + bias[n]), C = log(1 + e^z) / a, aux = sigmoid(z);
-  //               out2 (if given) = aux * x1[n]
-  //  EPI_TANGENT  C (if given) = acc, aux = acc * s; out2 (if given) =
-  //               s * x2[m] * x1[n] + a s (1 - s) * acc * x1[n]
-  //  EPI_REVERSE  C = s * acc + a s (1 - s) * x1 * x2  (s, x1, x2 indexed as C)
-  float* C = g.C != nullptr ? g.C + (long)blockIdx.y * g.cstride : nullptr;
-  const float a = gb.alpha;
-  const int epi = gb.epi;
-  const int mrow = m0 + wm + (lane >> 4) * 4, ncol = n0 + wn + (lane & 15);
-  float bn[4], wn4[4];
+  // Epilogue: the C tile goes through LDS, then epilogue_tile() applies the
+  // epilogue in a rolled loop with coalesced rows (a fully unrolled per-lane
+  // epilogue is tens of KB of straight-line code that every workgroup
+  // fetches once, which costs more than the tile's arithmetic).
+  __syncthreads();   // every wave is done with the staged operands
+  float* Cs = reinterpret_cast<float*>(lds);
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const bool ok = ncol + 16 * j < g.N;
-    bn[j] = (g.bias != nullptr && ok) ? g.bias[ncol + 16 * j] : 0.f;
-    wn4[j] = (g.out2 != nullptr && ok) ? g.x1[ncol + 16 * j] : 0.f;
-  }
-  const float ia = 1.0f / a;
-  // one row block (16 values per lane); called with constant i so that the
-  // accumulators stay in registers
-  auto block = [&](const int i, const floatx4 (&acc_i)[4]) __attribute__((always_inline)) {
-    float sv[4][4], u1[4][4], u2[4][4], ym[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int m = mrow + 16 * i + q;
-      ym[q] = (epi == EPI_TANGENT && g.out2 != nullptr && m < g.M) ? g.x2[m] : 0.f;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int n = ncol + 16 * j;
-        const bool ok = m < g.M && n < g.N;
-        const long o = (long)m * g.ldc + n;
-        sv[j][q] = (ok && (epi == EPI_TANGENT || epi == EPI_REVERSE)) ? g.s[o] : 0.f;
-        u1[j][q] = (ok && epi == EPI_REVERSE) ? g.x1[o] : 0.f;
-        u2[j][q] = (ok && epi == EPI_REVERSE) ? g.x2[o] : 0.f;
-      }
-    }
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int m = mrow + 16 * i + q, n = ncol + 16 * j;
-        if (m >= g.M || n >= g.N) continue;
-        const long o = (long)m * g.ldc + n;
-        const float v = acc_i[j][q];
-        if (epi == EPI_SOFTPLUS) {
-          const float z = a * (v + bn[j]);
-          // log(1 + e^z) / a and its derivative sigmoid(z), without overflow
-          const float e = __expf(-fabsf(z));
-          C[o] = (fmaxf(z, 0.f) + __logf(1.0f + e)) * ia;
-          const float sg = z >= 0.f ? __fdividef(1.0f, 1.0f + e) : __fdividef(e, 1.0f + e);
-          g.aux[o] = sg;
-          if (g.out2 != nullptr) g.out2[o] = sg * wn4[j];
-        } else if (epi == EPI_TANGENT) {
-          const float sg = sv[j][q];
-          if (C != nullptr) C[o] = v;
-          g.aux[o] = v * sg;
-          if (g.out2 != nullptr)
-            g.out2[o] = sg * ym[q] * wn4[j] + a * sg * (1.0f - sg) * v * wn4[j];
-        } else if (epi == EPI_REVERSE) {
-          const float sg = sv[j][q];
-          C[o] = sg * v + a * sg * (1.0f - sg) * u1[j][q] * u2[j][q];
-        } else {
-          C[o] = v + bn[j];
-        }
-      }
-  };
-  block(0, acc[0]);
-  block(1, acc[1]);
-  block(2, acc[2]);
-  block(3, acc[3]);
+      for (int q = 0; q < 4; ++q)
+        Cs[(wm + 16 * i + (lane >> 4) * 4 + q) * CLD + wn + 16 * j + (lane & 15)] = acc[i][j][q];
+  __syncthreads();
+  epilogue_tile(Cs, m0, n0, g.M, g.N, g.ldc, gb.epi, gb.alpha, g.vec_out != 0,
+                g.C != nullptr ? g.C + (long)blockIdx.y * g.cstride : nullptr, g.aux, g.out2,
+                g.bias, g.s, g.x1, g.x2, tid);
 }
 
 // out = sum over the splits of their partial outputs, in split order; one
@@ -340,13 +378,14 @@ struct RowsBatch {
   int scatter;
 };
 
+// Four rows per 256-thread block, 64 lanes per row walking its columns.
 __global__ void rows_grouped_kernel(const RowsBatch rb) {
   const tm_rows_group& d = rb.r[blockIdx.y];
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)d.M * d.cols) return;
-  const int r = (int)(t / d.cols), c = (int)(t % d.cols);
-  if (rb.scatter) d.out[(long)d.idx[r] * d.ldo + c] = d.in[(long)r * d.ldi + c];
-  else d.out[(long)r * d.ldo + c] = d.in[(long)d.idx[r] * d.ldi + c];
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= d.M) return;
+  const long src = rb.scatter ? (long)r * d.ldi : (long)d.idx[r] * d.ldi;
+  const long dst = rb.scatter ? (long)d.idx[r] * d.ldo : (long)r * d.ldo;
+  for (int c = threadIdx.x & 63; c < d.cols; c += 64) d.out[dst + c] = d.in[src + c];
 }
 
 }  // namespace
@@ -414,6 +453,12 @@ int tm_gemm_bf16_grouped(int ngroups, const tm_gemm_group* groups, int ta, int t
       d.C = nsplit > 1 ? q.work : q.C;
       d.bias = q.bias; d.s = q.s; d.aux = q.aux;
       d.x1 = q.x1; d.x2 = q.x2; d.out2 = q.out2;
+      {
+        auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+        const bool rev = epi == EPI_REVERSE;
+        d.vec_out = q.ldc % 4 == 0 && al(d.C) && al(q.aux) && al(q.out2) && al(q.s) &&
+                    (!rev || (al(q.x1) && al(q.x2)));
+      }
       d.M = q.M; d.N = q.N; d.K = q.K;
       d.lda = q.lda; d.ldb = q.ldb; d.ldc = q.ldc;
       d.cstride = (long)q.M * q.ldc;
@@ -474,16 +519,16 @@ int tm_rows_grouped(int ngroups, const tm_rows_group* groups, int scatter, void*
     RowsBatch rb;
     rb.scatter = scatter;
     int n = 0;
-    long most = 0;
+    int most = 0;
     for (; g0 < ngroups && n < MAXG; ++g0) {
       const tm_rows_group& q = groups[g0];
       TM_REQUIRE(q.M >= 0 && q.cols >= 0, "tm_rows_grouped: negative size (set %d)", g0);
       if ((long)q.M * q.cols == 0) continue;
       rb.r[n++] = q;
-      most = std::max(most, (long)q.M * q.cols);
+      most = std::max(most, q.M);
     }
     if (n == 0) continue;
-    rows_grouped_kernel<<<dim3(tm_div_up(most, 256), n), 256, 0, (hipStream_t)stream>>>(rb);
+    rows_grouped_kernel<<<dim3(tm_div_up(most, 4), n), 256, 0, (hipStream_t)stream>>>(rb);
     TM_LAUNCH_CHECK();
   }
   return TM_OK;
