@@ -188,7 +188,9 @@ typedef struct tm_gemm_group {
  * epilogue and split count, in one launch per 16 of them: the per-element
  * networks of one layer (and the four weight-gradient GEMMs of every element)
  * run side by side instead of one after another.  Each result equals the
- * corresponding tm_gemm_bf16 call bit for bit. */
+ * corresponding tm_gemm_bf16 call bit for bit.  The outputs of a GEMM (C,
+ * aux, out2, work) must not overlap any input of the call (A, B, bias, s, x1,
+ * x2 of any GEMM) or each other. */
 int tm_gemm_bf16_grouped(int ngroups, const tm_gemm_group* groups, int ta, int tb, int epi,
                          float alpha, int nsplit, void* stream);
 
