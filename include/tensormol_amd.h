@@ -150,7 +150,7 @@ int tm_sym_jvp(int N, const double* pos, const double* box, const int* mol, cons
  * aux = sigmoid(alpha z); 2: C = acc, aux = acc * s (s, aux indexed as C).
  * nsplit > 1 (epi 0, no bias): K is split over nsplit workgroup layers whose
  * partial products go to work (nsplit * M * ldc floats) and are added in
- * split order. */
+ * split order.  C, aux and work must not overlap A, B, bias or s. */
 int tm_gemm_bf16(int M, int N, int K, const float* A, int lda, int ta, const float* B, int ldb,
                  int tb, float* C, int ldc, int epi, const float* bias, const float* s,
                  float* aux, float alpha, int nsplit, float* work, void* stream);
