@@ -354,6 +354,19 @@ def run_train(args):
             losses += [float(v) for v in gl]
         rec.update(ms_per_step=1e3 * wall / args.train_steps, atoms_per_s=N * args.train_steps / wall,
                    hip_graph=not args.train_no_graph, loss_first=losses[0], loss_last=losses[-1])
+        if dtype == "bf16":
+            # the GEMM stage (every grouped GEMM launch of a step), timed live with
+            # HIP events on the launch stream over eager steps
+            tr.gemm_timing = []
+            nst = 5
+            for _ in range(nst):
+                tr.step(tb, pos)
+            torch.cuda.synchronize()
+            gms = sum(a_.elapsed_time(b_) for a_, b_, _ in tr.gemm_timing) / nst
+            gfl = sum(f for _, _, f in tr.gemm_timing) / nst
+            rec.update(gemm_ms_per_step=gms, gemm_flops_per_step=gfl,
+                       gemm_launches_per_step=len(tr.gemm_timing) // nst)
+            tr.gemm_timing = None
         out[dtype] = rec
         log(f"{dtype}: {rec['ms_per_step']:.2f} ms/step ({'graph replay' if rec['hip_graph'] else 'eager'}), "
             f"eager {rec['eager_ms_per_step']:.2f} ms/step, {rec['atoms_per_s']:.4g} atoms/s, "
@@ -361,8 +374,8 @@ def run_train(args):
     # gradient check on the first 16 molecules against the FP64 oracle's double backward
     k = 16
     Zs, xs, mss = Z[:ms[k]], x[:ms[k]], ms[:k + 1]
-    ls = O.build_lists(xs, Zs, mss, sp)
     w_e, w_f = np.full(k, 1.0 / k), np.full(k, 0.5 / (3 * len(Zs)))
+    ls = O.build_lists(xs, Zs, mss, sp)
     lt = O.layers_torch(student.energy_nets, requires_grad=True)
     L, _, _ = O.force_matching_loss(xs, Zs, mss, sp, lt, student.alpha, E_ref[:k], F_ref[:ms[k]],
                                     w_e, w_f, ls)
@@ -376,6 +389,19 @@ def run_train(args):
         gv = np.concatenate([t.double().cpu().numpy().ravel() for t in gg])
         errs[dtype] = float(np.linalg.norm(gv - rv) / np.linalg.norm(rv))
     log(f"weight-gradient rel err vs FP64 oracle (16 molecules): {errs}")
+    # CPU baseline: the oracle's FP64 loss and weight gradient (double backward,
+    # neighbour lists included) of the whole batch, one step without the update
+    t0 = time.perf_counter()
+    lsf = O.build_lists(x, Z, ms, sp)
+    ltf = O.layers_torch(student.energy_nets, requires_grad=True)
+    Lf, _, _ = O.force_matching_loss(x, Z, ms, sp, ltf, student.alpha, E_ref, F_ref,
+                                     np.full(B, 1.0 / B), np.full(B, 0.5 / (3 * N)), lsf)
+    torch.autograd.grad(Lf, [t for e in eles for (Wt, bt) in ltf[e] for t in (Wt, bt)])
+    cpu_s = time.perf_counter() - t0
+    bf = out["bf16"]
+    gtf = bf["gemm_flops_per_step"] / (bf["gemm_ms_per_step"] * 1e-3) / 1e12
+    log(f"GEMM stage {bf['gemm_ms_per_step']:.3f} ms/step over {bf['gemm_launches_per_step']} "
+        f"launches, {gtf:.1f} TFLOP/s; oracle loss + gradient of {N} atoms in {cpu_s:.1f} s")
     return {
         "metric": "force-matching training atoms/s (E+F loss, weight gradient by double backward)",
         "value": round(out["bf16"]["atoms_per_s"], 1), "unit": "atoms/s", "n_gpus": 1,
@@ -389,6 +415,20 @@ def run_train(args):
         "fp32_mlp": {k_: (round(v, 4) if isinstance(v, float) else v) for k_, v in out["fp32"].items()},
         "loss_first_last_bf16": [out["bf16"]["loss_first"], out["bf16"]["loss_last"]],
         "weight_grad_rel_err_vs_fp64": errs,
+        "roofline": {"bound": "mfma", "kernel": "gemm_bf16_kernel",
+                     "achieved": round(gtf, 2), "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": round(gtf / 2500.0, 4), "traffic": None,
+                     "flops_per_step": bf["gemm_flops_per_step"],
+                     "gemm_ms_per_step": round(bf["gemm_ms_per_step"], 4),
+                     "launches_per_step": bf["gemm_launches_per_step"],
+                     "note": "the step's grouped GEMMs (forward, J chain, tangent, reverse, weight "
+                             "gradients with their split sum): algorithmic 2MNK flops over the "
+                             "launches' event-timed duration"},
+        "cpu_baseline": {"value": round(N / cpu_s, 1), "unit": "atoms/s",
+                         "cores": torch.get_num_threads(), "kind": "port",
+                         "sample": f"one step's FP64 loss and weight gradient (double backward, "
+                                   f"neighbour lists included, no update) of the whole batch "
+                                   f"({N} atoms) by the oracle, {cpu_s:.1f} s"},
     }
 
 

@@ -108,6 +108,9 @@ class ForceMatchingTrainer:
         self.device = self.eng.device
         self.alpha = float(model.alpha)
         self.mlp_dtype = mlp_dtype
+        # a list to collect (start event, end event, flops) of every GEMM launch
+        # of the MFMA path (eager steps only), or None
+        self.gemm_timing = None
         f32 = dict(dtype=torch.float32, device=self.device)
         self.params = {e: [(torch.tensor(np.asarray(W), **f32).requires_grad_(),
                             torch.tensor(np.asarray(b), **f32).requires_grad_())
@@ -245,8 +248,17 @@ class ForceMatchingTrainer:
         """One tm_gemm_bf16_grouped call over dicts of tm_gemm_group fields
         (device addresses as ints; absent fields are NULL)."""
         arr = (_lib.GemmGroup * len(groups))(*[_lib.GemmGroup(**g) for g in groups])
-        self.eng.lib.tm_gemm_bf16_grouped(len(groups), _ctypes.cast(arr, _ctypes.c_void_p), ta,
-                                          tb_, epi, self.alpha, nsplit, self.eng._stream())
+        call = lambda: self.eng.lib.tm_gemm_bf16_grouped(
+            len(groups), _ctypes.cast(arr, _ctypes.c_void_p), ta, tb_, epi, self.alpha, nsplit,
+            self.eng._stream())
+        if self.gemm_timing is None:
+            call()
+            return
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record()
+        call()
+        ev[1].record()
+        self.gemm_timing.append((ev[0], ev[1], sum(2 * g["M"] * g["N"] * g["K"] for g in groups)))
 
     def _grows(self, sets, scatter):
         """One tm_rows_grouped call over (M, cols, idx, in, ldi, out, ldo)."""
