@@ -169,30 +169,34 @@ int tm_gemm_bf16(int M, int N, int K, const float* A, int lda, int ta, const flo
  * aux). */
 typedef struct tm_gemm_group {
   int M, N, K;
-  const float* A;
+  const void* A;       /* FP32, or BF16 with io bit 0 */
   int lda;
-  const float* B;
+  const void* B;       /* as A */
   int ldb;
-  float* C;
+  void* C;             /* FP32, or BF16 with io bit 1 */
   int ldc;
   const float* bias;
   const float* s;
-  float* aux;
+  void* aux;           /* FP32, or BF16 with io bit 2 */
   float* work;
   const float* x1;
   const float* x2;
-  float* out2;
+  void* out2;          /* FP32, or BF16 with io bit 3 */
 } tm_gemm_group;
 
 /* ngroups independent GEMMs (groups: host array) with the same operand forms,
  * epilogue and split count, in one launch per 16 of them: the per-element
  * networks of one layer (and the four weight-gradient GEMMs of every element)
  * run side by side instead of one after another.  Each result equals the
- * corresponding tm_gemm_bf16 call bit for bit.  The outputs of a GEMM (C,
+ * corresponding tm_gemm_bf16 call bit for bit.  io selects storage types for
+ * the whole call: bit 0 A and B are BF16 (the staged bits: a BF16 operand
+ * gives the result of its FP32 value exactly), bits 1-3 C, aux, out2 are
+ * written as BF16 (rounded to nearest even; split K needs an FP32 C); bias, s,
+ * x1, x2 and work are FP32.  The outputs of a GEMM (C,
  * aux, out2, work) must not overlap any input of the call (A, B, bias, s, x1,
  * x2 of any GEMM) or each other. */
 int tm_gemm_bf16_grouped(int ngroups, const tm_gemm_group* groups, int ta, int tb, int epi,
-                         float alpha, int nsplit, void* stream);
+                         int io, float alpha, int nsplit, void* stream);
 
 /* Elementwise steps between those GEMMs over rows r < M, columns n < H (every
  * [M][H] operand with row stride ld): op 0 out = s * w4[n]; 1 out = s * x;
@@ -207,19 +211,31 @@ int tm_train_elem(int op, int M, int H, int ld, float alpha, const float* s, con
 int tm_rows(int scatter, int M, int cols, const int* idx, const float* in, int ldi, float* out,
             int ldo, void* stream);
 
-/* One row gather / scatter of a grouped call; fields as tm_rows' arguments. */
+/* One row gather / scatter of a grouped call; fields as tm_rows' arguments;
+ * out_bf16: out is BF16 (the FP32 input rounded to nearest even). */
 typedef struct tm_rows_group {
   int M, cols;
   const int* idx;
   const float* in;
   int ldi;
-  float* out;
+  void* out;
   int ldo;
+  int out_bf16;
 } tm_rows_group;
 
 /* ngroups tm_rows calls (groups: host array; the per-element row sets of one
  * step) in one launch per 16 of them. */
 int tm_rows_grouped(int ngroups, const tm_rows_group* groups, int scatter, void* stream);
+
+/* out[0 .. n) = BF16 of in (round to nearest even) for ngroups arrays (host
+ * array of records), one launch per 16: the weights of one training step,
+ * cast once for every GEMM that reads them. */
+typedef struct tm_cast_group {
+  const float* in;
+  void* out;
+  long n;
+} tm_cast_group;
+int tm_cast_bf16_grouped(int ngroups, const tm_cast_group* groups, void* stream);
 
 int tm_force_gather(int N, int ldr, const int* ncnt, const int* rev, const float* wslot,
                     const double* extra, double scale, double* grad, double* force,

@@ -54,10 +54,11 @@ SIGNATURES = {
     "tm_fire_update": [I, I, P, I, P, P, P, P, P, P, P, D, I, D, D, D, D, D, P],
     "tm_mlp_mfma": [I, I, I, I, P, P, I, P, P, P, F, I, I, P, P, P],
     "tm_gemm_bf16": [I, I, I, P, I, I, P, I, I, P, I, I, P, P, P, F, I, P, P],
-    "tm_gemm_bf16_grouped": [I, P, I, I, I, F, I, P],
+    "tm_gemm_bf16_grouped": [I, P, I, I, I, I, F, I, P],
     "tm_train_elem": [I, I, I, I, F, P, P, P, P, P, P, P, P],
     "tm_rows": [I, I, I, P, P, I, P, I, P],
     "tm_rows_grouped": [I, P, I, P],
+    "tm_cast_bf16_grouped": [I, P, P],
     "tm_mlp_mfma_pack_info": [I, I, I, I, P],
 }
 
@@ -107,7 +108,13 @@ class GemmGroup(ctypes.Structure):
 
 class RowsGroup(ctypes.Structure):
     """tm_rows_group: one row gather / scatter of a tm_rows_grouped call."""
-    _fields_ = [("M", I), ("cols", I), ("idx", P), ("inp", P), ("ldi", I), ("out", P), ("ldo", I)]
+    _fields_ = [("M", I), ("cols", I), ("idx", P), ("inp", P), ("ldi", I), ("out", P), ("ldo", I),
+                ("out_bf16", I)]
+
+
+class CastGroup(ctypes.Structure):
+    """tm_cast_group: one array of a tm_cast_bf16_grouped call."""
+    _fields_ = [("inp", P), ("out", P), ("n", L)]
 
 
 _LIB = None

@@ -21,6 +21,7 @@
 // with a column of ones appended to h (zeros to ht) so that the bias gradient
 // comes out as the last row.
 #include <algorithm>
+#include <type_traits>
 
 #include "tensormol_amd.h"
 #include "tm_common.h"
@@ -60,7 +61,7 @@ struct GemmDev {
 
 struct GemmBatch {
   GemmDev g[MAXG];
-  int ngroups, ntiles, per_xcd, epi;
+  int ngroups, ntiles, per_xcd, epi, io;
   float alpha;
 };
 
@@ -139,6 +140,61 @@ struct Stage {
   }
 };
 
+// The same staging for operands stored as BF16 (the bits the FP32 path
+// would stage): KCONT with VEC is one 16-byte load per row, stored as is.
+template <bool KCONT, bool VEC>
+struct StageBF {
+  uint4 v[4];
+
+  template <bool TAIL>
+  __device__ __forceinline__ void load(const unsigned short* __restrict__ X, long ld, int rows,
+                                       int r0, int k0, int kend, int tid) {
+    if (KCONT) {
+      const int k = k0 + (tid & 7) * 8;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = min(r0 + (tid >> 3) + 32 * i, rows - 1);
+        const unsigned short* p = X + (long)r * ld;
+        if (VEC && !TAIL) {
+          v[i] = *reinterpret_cast<const uint4*>(p + k);
+        } else {
+          unsigned h[8];
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            const unsigned x = p[TAIL ? min(k + q, kend - 1) : k + q];
+            h[q] = (!TAIL || k + q < kend) ? x : 0u;
+          }
+          v[i] = make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16),
+                            h[6] | (h[7] << 16));
+        }
+      }
+    } else {
+      const unsigned short* p = X + min(r0 + (tid & 127), rows - 1);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        unsigned h[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int k = k0 + 8 * ((tid >> 7) + 2 * i) + q;
+          const unsigned x = p[(long)(TAIL ? min(k, kend - 1) : k) * ld];
+          h[q] = (!TAIL || k < kend) ? x : 0u;
+        }
+        v[i] = make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16),
+                          h[6] | (h[7] << 16));
+      }
+    }
+  }
+
+  __device__ __forceinline__ void store(unsigned short (*S)[LROW], int tid) const {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int row = KCONT ? (tid >> 3) + 32 * i : (tid & 127);
+      const int col = KCONT ? (tid & 7) * 8 : 8 * ((tid >> 7) + 2 * i);
+      *reinterpret_cast<uint4*>(&S[row][col]) = v[i];
+    }
+  }
+};
+
 constexpr int CLD = GT + 4;   // LDS row of the staged C tile, in floats
 
 // Four consecutive values of row o .. o + 3 (nv of them valid).
@@ -164,6 +220,19 @@ __device__ __forceinline__ void st4(float* __restrict__ p, long o, int nv, bool 
   }
 }
 
+// st4 for a BF16 array (round to nearest even, as the staging does).
+__device__ __forceinline__ void st4h(unsigned short* __restrict__ p, long o, int nv, bool vec,
+                                     const float (&v)[4]) {
+  if (vec && nv == 4) {
+    *reinterpret_cast<uint2*>(p + o) = make_uint2(bf16_bits(v[0]) | (bf16_bits(v[1]) << 16),
+                                                  bf16_bits(v[2]) | (bf16_bits(v[3]) << 16));
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (k < nv) p[o + k] = (unsigned short)bf16_bits(v[k]);
+  }
+}
+
 // Epilogue of one C tile staged in LDS (Cs[row * CLD + col]): thread t takes
 // columns 4 (t % 32) .. + 3 of rows t / 32 + 8 r.  The outputs (C, aux, out2)
 // never overlap the inputs (bias, s, x1, x2), which restrict states, so a
@@ -175,7 +244,7 @@ __device__ __forceinline__ void st4(float* __restrict__ p, long o, int nv, bool 
 //               s * x2[m] * x1[n] + a s (1 - s) * acc * x1[n]
 //  EPI_REVERSE  C = s * acc + a s (1 - s) * x1 * x2  (s, x1, x2 indexed as C)
 __device__ __forceinline__ void epilogue_tile(
-    const float* Cs, int m0, int n0, int M, int N, long ldc, int epi, float a, bool vec,
+    const float* Cs, int m0, int n0, int M, int N, long ldc, int epi, float a, bool vec, int io,
     float* __restrict__ C, float* __restrict__ aux, float* __restrict__ out2,
     const float* __restrict__ bias, const float* __restrict__ s,
     const float* __restrict__ x1, const float* __restrict__ x2, int tid) {
@@ -225,9 +294,19 @@ __device__ __forceinline__ void epilogue_tile(
         r0[k] = v[k] + bv[k];
       }
     }
-    if (C != nullptr) st4(C, o, nv, vec, r0);
-    if (epi == EPI_SOFTPLUS || epi == EPI_TANGENT) st4(aux, o, nv, vec, r1);
-    if (out2 != nullptr) st4(out2, o, nv, vec, r2);
+    // io bits 1, 2, 3: C, aux, out2 stored as BF16
+    if (C != nullptr) {
+      if (io & 2) st4h(reinterpret_cast<unsigned short*>(C), o, nv, vec, r0);
+      else st4(C, o, nv, vec, r0);
+    }
+    if (epi == EPI_SOFTPLUS || epi == EPI_TANGENT) {
+      if (io & 4) st4h(reinterpret_cast<unsigned short*>(aux), o, nv, vec, r1);
+      else st4(aux, o, nv, vec, r1);
+    }
+    if (out2 != nullptr) {
+      if (io & 8) st4h(reinterpret_cast<unsigned short*>(out2), o, nv, vec, r2);
+      else st4(out2, o, nv, vec, r2);
+    }
   }
 }
 
@@ -240,7 +319,7 @@ __device__ __forceinline__ void epilogue_tile(
 // Workgroups are dealt to the XCDs round-robin, so workgroup w takes tile
 // (w % 8) * per_xcd + w / 8: each XCD runs a contiguous run of tiles, and the
 // column tiles of a row block (which share their A rows) meet in one L2.
-template <bool TA, bool TB, bool VEC>
+template <bool TA, bool TB, bool VEC, bool BF>
 __global__ void __launch_bounds__(256) gemm_bf16_kernel(const GemmBatch gb) {
   // two staged K steps of A and B, and after the K loop the C tile
   __shared__ __align__(16) unsigned char lds[2 * 2 * GT * LROW * 2];
@@ -263,15 +342,22 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(const GemmBatch gb) {
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-  Stage<!TA, VEC> sa;   // A(m, k): contiguous along k unless transposed
-  Stage<TB, VEC> sb;    // B(k, n) as rows n: contiguous along k when transposed
+  // A(m, k): contiguous along k unless transposed; B(k, n) as rows n:
+  // contiguous along k when transposed.  BF: both stored as BF16.
+  typedef typename std::conditional<BF, StageBF<!TA, VEC>, Stage<!TA, VEC>>::type SA;
+  typedef typename std::conditional<BF, StageBF<TB, VEC>, Stage<TB, VEC>>::type SB;
+  typedef typename std::conditional<BF, unsigned short, float>::type Elem;
+  SA sa;
+  SB sb;
+  const Elem* Ap = reinterpret_cast<const Elem*>(g.A);
+  const Elem* Bp = reinterpret_cast<const Elem*>(g.B);
   auto load = [&](int k0) {
     if (k0 + GK <= k_end) {
-      sa.template load<false>(g.A, g.lda, g.M, m0, k0, k_end, tid);
-      sb.template load<false>(g.B, g.ldb, g.N, n0, k0, k_end, tid);
+      sa.template load<false>(Ap, g.lda, g.M, m0, k0, k_end, tid);
+      sb.template load<false>(Bp, g.ldb, g.N, n0, k0, k_end, tid);
     } else {
-      sa.template load<true>(g.A, g.lda, g.M, m0, k0, k_end, tid);
-      sb.template load<true>(g.B, g.ldb, g.N, n0, k0, k_end, tid);
+      sa.template load<true>(Ap, g.lda, g.M, m0, k0, k_end, tid);
+      sb.template load<true>(Bp, g.ldb, g.N, n0, k0, k_end, tid);
     }
   };
   if (nsteps > 0) load(k_begin);
@@ -314,7 +400,7 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(const GemmBatch gb) {
       for (int q = 0; q < 4; ++q)
         Cs[(wm + 16 * i + (lane >> 4) * 4 + q) * CLD + wn + 16 * j + (lane & 15)] = acc[i][j][q];
   __syncthreads();
-  epilogue_tile(Cs, m0, n0, g.M, g.N, g.ldc, gb.epi, gb.alpha, g.vec_out != 0,
+  epilogue_tile(Cs, m0, n0, g.M, g.N, g.ldc, gb.epi, gb.alpha, g.vec_out != 0, gb.io,
                 g.C != nullptr ? g.C + (long)blockIdx.y * g.cstride : nullptr, g.aux, g.out2,
                 g.bias, g.s, g.x1, g.x2, tid);
 }
@@ -385,7 +471,27 @@ __global__ void rows_grouped_kernel(const RowsBatch rb) {
   if (r >= d.M) return;
   const long src = rb.scatter ? (long)r * d.ldi : (long)d.idx[r] * d.ldi;
   const long dst = rb.scatter ? (long)d.idx[r] * d.ldo : (long)r * d.ldo;
-  for (int c = threadIdx.x & 63; c < d.cols; c += 64) d.out[dst + c] = d.in[src + c];
+  if (d.out_bf16) {
+    unsigned short* out = static_cast<unsigned short*>(d.out);
+    for (int c = threadIdx.x & 63; c < d.cols; c += 64)
+      out[dst + c] = (unsigned short)bf16_bits(d.in[src + c]);
+  } else {
+    float* out = static_cast<float*>(d.out);
+    for (int c = threadIdx.x & 63; c < d.cols; c += 64) out[dst + c] = d.in[src + c];
+  }
+}
+
+// out = BF16 of in (round to nearest even) for several arrays, one per blockIdx.y.
+struct CastBatch {
+  tm_cast_group c[MAXG];
+};
+
+__global__ void cast_bf16_kernel(const CastBatch cb) {
+  const tm_cast_group& d = cb.c[blockIdx.y];
+  unsigned short* out = static_cast<unsigned short*>(d.out);
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < d.n;
+       t += (long)gridDim.x * blockDim.x)
+    out[t] = (unsigned short)bf16_bits(d.in[t]);
 }
 
 }  // namespace
@@ -393,13 +499,14 @@ __global__ void rows_grouped_kernel(const RowsBatch rb) {
 extern "C" {
 
 int tm_gemm_bf16_grouped(int ngroups, const tm_gemm_group* groups, int ta, int tb, int epi,
-                         float alpha, int nsplit, void* stream) {
+                         int io, float alpha, int nsplit, void* stream) {
   TM_REQUIRE(ngroups >= 0 && (ngroups == 0 || groups != nullptr),
              "tm_gemm_bf16_grouped: bad group list");
   TM_REQUIRE(epi >= EPI_STORE && epi <= EPI_REVERSE, "tm_gemm_bf16: bad epilogue %d", epi);
+  TM_REQUIRE((io & ~15) == 0, "tm_gemm_bf16: bad io flags %d", io);
   TM_REQUIRE(nsplit >= 1 && nsplit <= 64, "tm_gemm_bf16: nsplit %d outside 1..64", nsplit);
-  TM_REQUIRE(nsplit == 1 || epi == EPI_STORE,
-             "tm_gemm_bf16: split K needs the plain epilogue");
+  TM_REQUIRE(nsplit == 1 || (epi == EPI_STORE && (io & 2) == 0),
+             "tm_gemm_bf16: split K needs the plain epilogue and an FP32 C");
   TM_REQUIRE(epi != EPI_SOFTPLUS || alpha > 0.f, "tm_gemm_bf16: softplus needs alpha > 0");
   for (int i = 0; i < ngroups; ++i) {
     const tm_gemm_group& q = groups[i];
@@ -423,41 +530,51 @@ int tm_gemm_bf16_grouped(int ngroups, const tm_gemm_group* groups, int ta, int t
                "tm_gemm_bf16: split K needs a workspace and no bias (GEMM %d)", i);
   }
   hipStream_t st = (hipStream_t)stream;
+  const bool bf = (io & 1) != 0;
+  auto aligned = [](const void* p, int bytes) {
+    return (reinterpret_cast<uintptr_t>(p) & (uintptr_t)(bytes - 1)) == 0;
+  };
   // 16-byte loads of the operands stored along k when every one allows them
   bool vec = true;
+  const int per16 = bf ? 8 : 4;   // elements per 16 bytes
   for (int i = 0; i < ngroups; ++i) {
     const tm_gemm_group& q = groups[i];
     if (q.M == 0 || q.N == 0) continue;
-    if (!ta) vec = vec && q.lda % 4 == 0 && (reinterpret_cast<uintptr_t>(q.A) & 15) == 0;
-    if (tb) vec = vec && q.ldb % 4 == 0 && (reinterpret_cast<uintptr_t>(q.B) & 15) == 0;
+    if (!ta) vec = vec && q.lda % per16 == 0 && aligned(q.A, 16);
+    if (tb) vec = vec && q.ldb % per16 == 0 && aligned(q.B, 16);
   }
   using Kern = void (*)(const GemmBatch);
-  const Kern kerns[2][2][2] = {
-      {{gemm_bf16_kernel<false, false, false>, gemm_bf16_kernel<false, false, true>},
-       {gemm_bf16_kernel<false, true, false>, gemm_bf16_kernel<false, true, true>}},
-      {{gemm_bf16_kernel<true, false, false>, gemm_bf16_kernel<true, false, true>},
-       {gemm_bf16_kernel<true, true, false>, gemm_bf16_kernel<true, true, true>}}};
-  const Kern kern = kerns[ta ? 1 : 0][tb ? 1 : 0][vec ? 1 : 0];
+#define TM_GEMM_K(a, b)                                                                  \
+  {gemm_bf16_kernel<a, b, false, false>, gemm_bf16_kernel<a, b, false, true>,            \
+   gemm_bf16_kernel<a, b, true, false>, gemm_bf16_kernel<a, b, true, true>}
+  const Kern kerns[2][2][4] = {{TM_GEMM_K(false, false), TM_GEMM_K(false, true)},
+                               {TM_GEMM_K(true, false), TM_GEMM_K(true, true)}};
+#undef TM_GEMM_K
+  const Kern kern = kerns[ta ? 1 : 0][tb ? 1 : 0][(vec ? 2 : 0) + (bf ? 1 : 0)];
   for (int g0 = 0; g0 < ngroups;) {
     GemmBatch gb;
     gb.ngroups = 0;
     gb.ntiles = 0;
     gb.epi = epi;
+    gb.io = io;
     gb.alpha = alpha;
     long max_mn = 0;
     for (; g0 < ngroups && gb.ngroups < MAXG; ++g0) {
       const tm_gemm_group& q = groups[g0];
       if (q.M == 0 || q.N == 0) continue;
       GemmDev& d = gb.g[gb.ngroups++];
-      d.A = q.A; d.B = q.B; d.out = q.C;
-      d.C = nsplit > 1 ? q.work : q.C;
-      d.bias = q.bias; d.s = q.s; d.aux = q.aux;
-      d.x1 = q.x1; d.x2 = q.x2; d.out2 = q.out2;
+      d.A = static_cast<const float*>(q.A);
+      d.B = static_cast<const float*>(q.B);
+      d.out = static_cast<float*>(q.C);
+      d.C = nsplit > 1 ? q.work : static_cast<float*>(q.C);
+      d.bias = q.bias; d.s = q.s; d.aux = static_cast<float*>(q.aux);
+      d.x1 = q.x1; d.x2 = q.x2; d.out2 = static_cast<float*>(q.out2);
       {
-        auto al = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+        // 16-byte epilogue accesses: FP32 arrays 16-byte aligned, BF16 outputs 8-byte
         const bool rev = epi == EPI_REVERSE;
-        d.vec_out = q.ldc % 4 == 0 && al(d.C) && al(q.aux) && al(q.out2) && al(q.s) &&
-                    (!rev || (al(q.x1) && al(q.x2)));
+        d.vec_out = q.ldc % 4 == 0 && aligned(d.C, (io & 2) && nsplit == 1 ? 8 : 16) &&
+                    aligned(q.aux, (io & 4) ? 8 : 16) && aligned(q.out2, (io & 8) ? 8 : 16) &&
+                    aligned(q.s, 16) && (!rev || (aligned(q.x1, 16) && aligned(q.x2, 16)));
       }
       d.M = q.M; d.N = q.N; d.K = q.K;
       d.lda = q.lda; d.ldb = q.ldb; d.ldc = q.ldc;
@@ -489,7 +606,7 @@ int tm_gemm_bf16(int M, int N, int K, const float* A, int lda, int ta, const flo
   q.A = A; q.lda = lda; q.B = B; q.ldb = ldb; q.C = C; q.ldc = ldc;
   q.bias = bias; q.s = s; q.aux = aux; q.work = work;
   q.x1 = nullptr; q.x2 = nullptr; q.out2 = nullptr;
-  return tm_gemm_bf16_grouped(1, &q, ta, tb, epi, alpha, nsplit, stream);
+  return tm_gemm_bf16_grouped(1, &q, ta, tb, epi, 0, alpha, nsplit, stream);
 }
 
 int tm_train_elem(int op, int M, int H, int ld, float alpha, const float* s, const float* x,
@@ -529,6 +646,27 @@ int tm_rows_grouped(int ngroups, const tm_rows_group* groups, int scatter, void*
     }
     if (n == 0) continue;
     rows_grouped_kernel<<<dim3(tm_div_up(most, 4), n), 256, 0, (hipStream_t)stream>>>(rb);
+    TM_LAUNCH_CHECK();
+  }
+  return TM_OK;
+}
+
+int tm_cast_bf16_grouped(int ngroups, const tm_cast_group* groups, void* stream) {
+  TM_REQUIRE(ngroups >= 0 && (ngroups == 0 || groups != nullptr),
+             "tm_cast_bf16_grouped: bad group list");
+  for (int g0 = 0; g0 < ngroups;) {
+    CastBatch cb;
+    int n = 0;
+    long most = 0;
+    for (; g0 < ngroups && n < MAXG; ++g0) {
+      TM_REQUIRE(groups[g0].n >= 0, "tm_cast_bf16_grouped: negative size (array %d)", g0);
+      if (groups[g0].n == 0) continue;
+      cb.c[n++] = groups[g0];
+      most = std::max(most, groups[g0].n);
+    }
+    if (n == 0) continue;
+    const int blocks = (int)std::min<long>(tm_div_up(most, 256), 1024);
+    cast_bf16_kernel<<<dim3(blocks, n), 256, 0, (hipStream_t)stream>>>(cb);
     TM_LAUNCH_CHECK();
   }
   return TM_OK;

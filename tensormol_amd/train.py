@@ -205,7 +205,10 @@ class ForceMatchingTrainer:
         """Per-element device buffers of a batch (made once per batch).  Row
         stride ld = width + 16; the column after the last feature of the
         stacked inputs holds 1 for the primal rows and 0 for the tangent rows
-        (the bias column of the weight-gradient GEMMs)."""
+        (the bias column of the weight-gradient GEMMs).  Arrays that only
+        GEMMs read (stacked inputs X0, activations H, upstream rows ZZ, the
+        weights' copies Wb, YBb) are BF16: the GEMMs round their operands to
+        BF16 anyway, so storing them rounded gives the same products."""
         bufs = getattr(tb, "_mfma", None)
         if bufs is not None:
             return bufs
@@ -214,23 +217,26 @@ class ForceMatchingTrainer:
         assert H[0] == H[1] == H[2], "the MFMA training path needs equal hidden widths"
         H = H[0]
         f32 = dict(dtype=torch.float32, device=dev)
+        b16 = dict(dtype=torch.bfloat16, device=dev)
         ldx, ldh = D + 16, H + 16
         bufs = {}
         for e, idx in tb.idx.items():
             M = int(idx.numel())
             b = {"M": M, "idx": idx.to(torch.int32).contiguous(), "ldx": ldx, "ldh": ldh, "H": H}
-            b["X0"] = torch.zeros(2 * M, ldx, **f32)
+            b["X0"] = torch.zeros(2 * M, ldx, **b16)
             b["X0"][:M, D] = 1.0
             for k in ("H1", "H2", "H3"):
-                b[k] = torch.zeros(2 * M, ldh, **f32)
+                b[k] = torch.zeros(2 * M, ldh, **b16)
                 b[k][:M, H] = 1.0
             for k in ("ZZ1", "ZZ2", "ZZ3"):
-                b[k] = torch.zeros(2 * M, ldh, **f32)
+                b[k] = torch.zeros(2 * M, ldh, **b16)
             for k in ("S1", "S2", "S3", "ZT1", "ZT2", "ZT3", "HD1", "HD2"):
                 b[k] = torch.zeros(M, ldh, **f32)
             b["Y"] = torch.zeros(M, 1, **f32)
-            b["YB"] = torch.ones(2 * M, 1, **f32)
+            b["YB"] = torch.zeros(M, 1, **f32)      # ybar (the last tangent layer's epilogue)
+            b["YBb"] = torch.ones(2 * M, 1, **b16)  # [ybar; 1] (the w_4 gradient GEMM)
             b["Je"] = torch.zeros(M, D, **f32)
+            b["Wb"] = [torch.zeros(W.shape, **b16) for (W, _) in self.params[e]]
             b["gW"] = [torch.zeros(D + 1, H, **f32), torch.zeros(H + 1, H, **f32),
                        torch.zeros(H + 1, H, **f32), torch.zeros(H + 1, 1, **f32)]
             bufs[e] = b
@@ -244,13 +250,14 @@ class ForceMatchingTrainer:
         tb._mfma = bufs
         return bufs
 
-    def _ggemm(self, groups, ta, tb_, epi=0, nsplit=1):
+    def _ggemm(self, groups, ta, tb_, epi=0, io=1, nsplit=1):
         """One tm_gemm_bf16_grouped call over dicts of tm_gemm_group fields
-        (device addresses as ints; absent fields are NULL)."""
+        (device addresses as ints; absent fields are NULL); io: the call's
+        storage types (bit 0 BF16 operands, bits 1-3 BF16 C / aux / out2)."""
         arr = (_lib.GemmGroup * len(groups))(*[_lib.GemmGroup(**g) for g in groups])
         call = lambda: self.eng.lib.tm_gemm_bf16_grouped(
-            len(groups), _ctypes.cast(arr, _ctypes.c_void_p), ta, tb_, epi, self.alpha, nsplit,
-            self.eng._stream())
+            len(groups), _ctypes.cast(arr, _ctypes.c_void_p), ta, tb_, epi, io, self.alpha,
+            nsplit, self.eng._stream())
         if self.gemm_timing is None:
             call()
             return
@@ -261,10 +268,18 @@ class ForceMatchingTrainer:
         self.gemm_timing.append((ev[0], ev[1], sum(2 * g["M"] * g["N"] * g["K"] for g in groups)))
 
     def _grows(self, sets, scatter):
-        """One tm_rows_grouped call over (M, cols, idx, in, ldi, out, ldo)."""
+        """One tm_rows_grouped call over (M, cols, idx, in, ldi, out, ldo,
+        out_bf16)."""
         arr = (_lib.RowsGroup * len(sets))(*[_lib.RowsGroup(*r) for r in sets])
         self.eng.lib.tm_rows_grouped(len(sets), _ctypes.cast(arr, _ctypes.c_void_p), scatter,
                                      self.eng._stream())
+
+    def _cast(self, pairs):
+        """One tm_cast_bf16_grouped call: each (FP32 tensor, BF16 tensor) pair."""
+        arr = (_lib.CastGroup * len(pairs))(
+            *[_lib.CastGroup(x.data_ptr(), y.data_ptr(), x.numel()) for x, y in pairs])
+        self.eng.lib.tm_cast_bf16_grouped(len(pairs), _ctypes.cast(arr, _ctypes.c_void_p),
+                                          self.eng._stream())
 
     def _mfma_step(self, tb, pos, check=True, views=False):
         """(L, E, F, gradients in parameters() order) of the BF16 MFMA path.
@@ -282,11 +297,15 @@ class ForceMatchingTrainer:
         bufs = self._mfma_buffers(tb)
         items = [(b, self.params[e]) for e, b in bufs.items()]
         H = items[0][0]["H"]
-        ad = lambda t, rows=0, ld=0: t.data_ptr() + 4 * rows * ld   # row `rows` of t
+        ad = lambda t, rows=0, ld=0: t.data_ptr() + t.element_size() * rows * ld   # row `rows`
         f32 = dict(dtype=torch.float32, device=self.device)
-        eps = torch.zeros(N, 1, **f32)
-        J = torch.zeros(N, D, **f32)
-        self._grows([(b["M"], D, ad(b["idx"]), ad(ws.G), D, ad(b["X0"]), b["ldx"])
+        # every atom has an element of the model (prepare() checks), so the
+        # per-element scatters below write every row of these
+        eps = torch.empty(N, 1, **f32)
+        J = torch.empty(N, D, **f32)
+        # this step's weights as BF16 (one launch), then the descriptors as BF16 rows
+        self._cast([(prm[l][0], b["Wb"][l]) for b, prm in items for l in range(4)])
+        self._grows([(b["M"], D, ad(b["idx"]), ad(ws.G), D, ad(b["X0"]), b["ldx"], 1)
                      for b, _ in items], 0)
         # forward: z_l = h_{l-1} W_l + b_l, h_l = act(z_l), s_l = act'(z_l); the
         # last layer also starts the J chain, zd_3 = s_3 w_4 (rows M.. of ZZ3)
@@ -296,26 +315,29 @@ class ForceMatchingTrainer:
             for b, prm in items:
                 M, ldh = b["M"], b["ldh"]
                 ldp, Kp = (b["ldx"], D) if l == 0 else (ldh, H)
-                g = dict(M=M, N=H, K=Kp, A=ad(b[hin]), lda=ldp, B=ad(prm[l][0]), ldb=H,
+                g = dict(M=M, N=H, K=Kp, A=ad(b[hin]), lda=ldp, B=ad(b["Wb"][l]), ldb=H,
                          C=ad(b[hout]), ldc=ldh, bias=ad(prm[l][1]), aux=ad(b[sk]))
                 if l == 2:
                     g.update(x1=ad(prm[3][0]), out2=ad(b["ZZ3"], M, ldh))
                 grp.append(g)
-            self._ggemm(grp, 0, 0, 1)
-        self._ggemm([dict(M=b["M"], N=1, K=H, A=ad(b["H3"]), lda=b["ldh"], B=ad(prm[3][0]),
+            self._ggemm(grp, 0, 0, 1, io=1 | 2 | 8)   # H_l, zd_3 BF16; s_l FP32
+        self._ggemm([dict(M=b["M"], N=1, K=H, A=ad(b["H3"]), lda=b["ldh"], B=ad(b["Wb"][3]),
                           ldb=1, C=ad(b["Y"]), ldc=1, bias=ad(prm[3][1])) for b, prm in items],
                     0, 0, 0)
-        self._grows([(b["M"], 1, ad(b["idx"]), ad(b["Y"]), 1, ad(eps), 1) for b, _ in items], 1)
+        self._grows([(b["M"], 1, ad(b["idx"]), ad(b["Y"]), 1, ad(eps), 1, 0) for b, _ in items],
+                    1)
         # J chain: hd_{l-1} = zd_l W_l^T, zd_{l-1} = s_{l-1} hd_{l-1}, J = zd_1 W_1^T
         for (l, zz_in, hd, sk, zz_out) in ((2, "ZZ3", "HD2", "S2", "ZZ2"),
                                            (1, "ZZ2", "HD1", "S1", "ZZ1")):
             self._ggemm([dict(M=b["M"], N=H, K=H, A=ad(b[zz_in], b["M"], b["ldh"]), lda=b["ldh"],
-                              B=ad(prm[l][0]), ldb=H, C=ad(b[hd]), ldc=b["ldh"], s=ad(b[sk]),
-                              aux=ad(b[zz_out], b["M"], b["ldh"])) for b, prm in items], 0, 1, 2)
+                              B=ad(b["Wb"][l]), ldb=H, C=ad(b[hd]), ldc=b["ldh"], s=ad(b[sk]),
+                              aux=ad(b[zz_out], b["M"], b["ldh"])) for b, prm in items], 0, 1, 2,
+                        io=1 | 4)   # hd FP32, zd BF16
         self._ggemm([dict(M=b["M"], N=D, K=H, A=ad(b["ZZ1"], b["M"], b["ldh"]), lda=b["ldh"],
-                          B=ad(prm[0][0]), ldb=H, C=ad(b["Je"]), ldc=D) for b, prm in items],
+                          B=ad(b["Wb"][0]), ldb=H, C=ad(b["Je"]), ldc=D) for b, prm in items],
                     0, 1, 0)
-        self._grows([(b["M"], D, ad(b["idx"]), ad(b["Je"]), D, ad(J), D) for b, _ in items], 1)
+        self._grows([(b["M"], D, ad(b["idx"]), ad(b["Je"]), D, ad(J), D, 0) for b, _ in items],
+                    1)
         E = torch.zeros(tb.B, dtype=torch.float64, device=self.device).index_add(
             0, tb.mol, eps[:, 0].double())
         grad = torch.empty(N, 3, dtype=torch.float64, device=self.device)
@@ -327,9 +349,11 @@ class ForceMatchingTrainer:
         gg = (-2.0 * wf * (F - tb.F_ref)).contiguous()   # dLoss/dgrad
         V = torch.empty(N, D, **f32)
         eng.descriptor_jvp(ws, pos, gg, V)
-        self._grows([(b["M"], D, ad(b["idx"]), ad(V), D, ad(b["X0"], b["M"], b["ldx"]), b["ldx"])
-                     for b, _ in items] +
-                    [(b["M"], 1, ad(b["idx"]), ad(ybar), 1, ad(b["YB"]), 1) for b, _ in items], 0)
+        self._grows([(b["M"], D, ad(b["idx"]), ad(V), D, ad(b["X0"], b["M"], b["ldx"]), b["ldx"],
+                      1) for b, _ in items] +
+                    [(b["M"], 1, ad(b["idx"]), ad(ybar), 1, ad(b["YB"]), 1, 0) for b, _ in items] +
+                    [(b["M"], 1, ad(b["idx"]), ad(ybar), 1, ad(b["YBb"]), 1, 1) for b, _ in items],
+                    0)
         # tangent along V: zt_l = ht_{l-1} W_l, ht_l = s_l zt_l (rows M.. of H_l);
         # the last layer also starts the reverse pass,
         # zb_3 = s_3 ybar w_4 + a s_3 (1 - s_3) zt_3 w_4 (rows ..M of ZZ3)
@@ -338,18 +362,18 @@ class ForceMatchingTrainer:
             for b, prm in items:
                 M, ldh = b["M"], b["ldh"]
                 ldp, Kp = (b["ldx"], D) if l == 0 else (ldh, H)
-                g = dict(M=M, N=H, K=Kp, A=ad(b[hin], M, ldp), lda=ldp, B=ad(prm[l][0]), ldb=H,
+                g = dict(M=M, N=H, K=Kp, A=ad(b[hin], M, ldp), lda=ldp, B=ad(b["Wb"][l]), ldb=H,
                          C=ad(b[zt]), ldc=ldh, s=ad(b[sk]), aux=ad(b[hout], M, ldh))
                 if l == 2:
                     g.update(C=None, x1=ad(prm[3][0]), x2=ad(b["YB"]), out2=ad(b["ZZ3"]))
                 grp.append(g)
-            self._ggemm(grp, 0, 0, 2)
+            self._ggemm(grp, 0, 0, 2, io=1 | 4 | 8)   # zt FP32; ht_l, zb_3 BF16
         # reverse: zb_{l-1} = s_{l-1} (zb_l W_l^T) + a s_{l-1} (1 - s_{l-1}) zt_{l-1} hd_{l-1}
         for (l, zz_in, sk, zt, hd, zz_out) in ((2, "ZZ3", "S2", "ZT2", "HD2", "ZZ2"),
                                                (1, "ZZ2", "S1", "ZT1", "HD1", "ZZ1")):
-            self._ggemm([dict(M=b["M"], N=H, K=H, A=ad(b[zz_in]), lda=b["ldh"], B=ad(prm[l][0]),
+            self._ggemm([dict(M=b["M"], N=H, K=H, A=ad(b[zz_in]), lda=b["ldh"], B=ad(b["Wb"][l]),
                               ldb=H, C=ad(b[zz_out]), ldc=b["ldh"], s=ad(b[sk]), x1=ad(b[zt]),
-                              x2=ad(b[hd])) for b, prm in items], 0, 1, 3)
+                              x2=ad(b[hd])) for b, prm in items], 0, 1, 3, io=1 | 2)   # zb BF16
         # weight gradients over the stacked rows (bias from the ones column),
         # every layer of every element in one grouped launch
         grp = []
@@ -361,7 +385,7 @@ class ForceMatchingTrainer:
                     (D + 1, H, b["X0"], ldx, b["ZZ1"], ldh, gW1, wk[0]),
                     (H + 1, H, b["H1"], ldh, b["ZZ2"], ldh, gW2, wk[1]),
                     (H + 1, H, b["H2"], ldh, b["ZZ3"], ldh, gW3, wk[2]),
-                    (H + 1, 1, b["H3"], ldh, b["YB"], 1, gw4, wk[3])):
+                    (H + 1, 1, b["H3"], ldh, b["YBb"], 1, gw4, wk[3])):
                 grp.append(dict(M=Mo, N=No, K=M2, A=ad(A), lda=lda, B=ad(B), ldb=ldb, C=ad(G),
                                 ldc=No, work=ad(w)))
         self._ggemm(grp, 1, 0, 0, nsplit=items[0][0]["nsplit"])

@@ -169,7 +169,8 @@ def test_gemm_grouped_equals_single_calls(cuda, ta, tb, epi, ns):
                         P(aux), a, ns, P(work), st))
     arr = (TL.GemmGroup * len(groups))(*groups)
     import ctypes
-    L.tm_gemm_bf16_grouped(len(groups), ctypes.cast(arr, ctypes.c_void_p), ta, tb, epi, a, ns, st)
+    L.tm_gemm_bf16_grouped(len(groups), ctypes.cast(arr, ctypes.c_void_p), ta, tb, epi, 0, a, ns,
+                           st)
     for args in singles:
         L.tm_gemm_bf16(*args)
     torch.cuda.synchronize()
@@ -218,7 +219,7 @@ def test_gemm_fused_elementwise_epilogues(cuda):
     def run(epi, **f):
         grp = TL.GemmGroup(M=M, N=N, K=K, A=ad(A), lda=K, B=ad(B), ldb=N, ldc=ld, **f)
         arr = (TL.GemmGroup * 1)(grp)
-        L.tm_gemm_bf16_grouped(1, ctypes.cast(arr, ctypes.c_void_p), 0, 0, epi, a, 1, st)
+        L.tm_gemm_bf16_grouped(1, ctypes.cast(arr, ctypes.c_void_p), 0, 0, epi, 0, a, 1, st)
         torch.cuda.synchronize()
 
     C, aux, out2 = (torch.full((M, ld), 7.0, device=cuda) for _ in range(3))
@@ -260,3 +261,72 @@ def test_rows_grouped_equals_single_calls(cuda):
         if it.numel():
             assert torch.equal(o[:it.numel(), :9], src[it.long()])
             assert (o[:, 9:] == 0).all()
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("M,N,K,pad", [(200, 136, 200, 8), (70, 130, 45, 3)])
+def test_gemm_bf16_storage_equals_fp32_staging(cuda, ta, tb, M, N, K, pad):
+    """Operands stored as BF16 (io bit 0) give the bits of the same operands in
+    FP32, which the kernel rounds to BF16 when it stages them (16-byte loads
+    with the padded strides, element loads with the odd ones); outputs written
+    as BF16 (io bits 1-3) are the FP32 outputs rounded to nearest even."""
+    import ctypes
+    from tensormol_amd import _lib as TL
+    L, P = _lib()
+    g = torch.Generator(device="cpu").manual_seed(100 + M + 7 * ta + 3 * tb)
+    st = torch.cuda.current_stream().cuda_stream
+    A = torch.randn(M, K, generator=g).to(cuda) * 0.3
+    B = torch.randn(K, N, generator=g).to(cuda) * 0.3
+    bias = torch.randn(N, generator=g).to(cuda)
+    w4 = torch.randn(N, generator=g).to(cuda)
+    As = A.t() if ta else A
+    Bs = B.t() if tb else B
+    lda, ldb = As.shape[1] + pad, Bs.shape[1] + pad
+    A32 = torch.zeros(As.shape[0], lda, device=cuda); A32[:, :As.shape[1]] = As
+    B32 = torch.zeros(Bs.shape[0], ldb, device=cuda); B32[:, :Bs.shape[1]] = Bs
+    A16, B16 = A32.bfloat16(), B32.bfloat16()
+    ldc = N + pad
+
+    def run(Ap, Bp, io, epi):
+        outs = [torch.zeros(M, ldc, device=cuda, dtype=torch.bfloat16 if io & bit else torch.float32)
+                for bit in (2, 4, 8)]
+        f = dict(M=M, N=N, K=K, A=Ap.data_ptr(), lda=lda, B=Bp.data_ptr(), ldb=ldb,
+                 C=outs[0].data_ptr(), ldc=ldc)
+        if epi == 1:
+            f.update(bias=bias.data_ptr(), aux=outs[1].data_ptr(), x1=w4.data_ptr(),
+                     out2=outs[2].data_ptr())
+        arr = (TL.GemmGroup * 1)(TL.GemmGroup(**f))
+        L.tm_gemm_bf16_grouped(1, ctypes.cast(arr, ctypes.c_void_p), ta, tb, epi, io, 1.5, 1, st)
+        torch.cuda.synchronize()
+        return outs
+
+    for epi in (0, 1):
+        ref = run(A32, B32, 0, epi)
+        same = run(A16, B16, 1, epi)
+        for r, o in zip(ref, same):
+            assert torch.equal(r, o)
+        half = run(A16, B16, 1 | 2 | 4 | 8, epi)
+        for r, o in zip(ref, half):
+            assert torch.equal(r.bfloat16(), o)
+
+
+def test_rows_bf16_and_cast(cuda):
+    import ctypes
+    from tensormol_amd import _lib as TL
+    L, P = _lib()
+    st = torch.cuda.current_stream().cuda_stream
+    g = torch.Generator(device="cpu").manual_seed(9)
+    src = torch.randn(40, 9, generator=g).to(cuda)
+    it = torch.tensor([5, 0, 3, 39], dtype=torch.int32, device=cuda)
+    out = torch.zeros(4, 12, dtype=torch.bfloat16, device=cuda)
+    arr = (TL.RowsGroup * 1)(TL.RowsGroup(4, 9, it.data_ptr(), src.data_ptr(), 9, out.data_ptr(), 12, 1))
+    L.tm_rows_grouped(1, ctypes.cast(arr, ctypes.c_void_p), 0, st)
+    xs = [torch.randn(n, generator=g).to(cuda) * 10 for n in (1, 1000, 70000)]
+    ys = [torch.zeros(x.numel(), dtype=torch.bfloat16, device=cuda) for x in xs]
+    carr = (TL.CastGroup * 3)(*[TL.CastGroup(x.data_ptr(), y.data_ptr(), x.numel()) for x, y in zip(xs, ys)])
+    L.tm_cast_bf16_grouped(3, ctypes.cast(carr, ctypes.c_void_p), st)
+    torch.cuda.synchronize()
+    assert torch.equal(out[:, :9], src[it.long()].bfloat16())
+    assert (out[:, 9:] == 0).all()
+    for x, y in zip(xs, ys):
+        assert torch.equal(y, x.bfloat16())
