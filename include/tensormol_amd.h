@@ -237,6 +237,17 @@ typedef struct tm_cast_group {
 } tm_cast_group;
 int tm_cast_bf16_grouped(int ngroups, const tm_cast_group* groups, void* stream);
 
+/* The loss side of the force-matching step (loss_op_EandG,
+ * TFMolInstanceDirect.py:4889-4903): per molecule m of atoms
+ * [mol_start[m], mol_start[m+1]), E[m] = sum of the atom energies eps,
+ * ybar[i] = 2 w_e[m] (E[m] - E_ref[m]) (dLoss/dy of each atom), F = -grad,
+ * gg[i] = -2 w_f[m] (F_i - F_ref_i) (dLoss/dgrad, [N][3]), and the loss
+ * L[0] = sum_m w_e[m] (E[m] - E_ref[m])^2 + sum_i w_f[m] |F_i - F_ref_i|^2 in
+ * FP64 (part: B doubles of workspace).  Deterministic: fixed summation order. */
+int tm_fm_loss(int B, const int* mol_start, const float* eps, const double* E_ref,
+               const double* w_e, const double* grad, const double* F_ref, const double* w_f,
+               double* E, float* ybar, double* gg, double* part, double* L, void* stream);
+
 int tm_force_gather(int N, int ldr, const int* ncnt, const int* rev, const float* wslot,
                     const double* extra, double scale, double* grad, double* force,
                     void* stream);

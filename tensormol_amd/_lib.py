@@ -59,6 +59,7 @@ SIGNATURES = {
     "tm_rows": [I, I, I, P, P, I, P, I, P],
     "tm_rows_grouped": [I, P, I, P],
     "tm_cast_bf16_grouped": [I, P, P],
+    "tm_fm_loss": [I, P, P, P, P, P, P, P, P, P, P, P, P, P],
     "tm_mlp_mfma_pack_info": [I, I, I, I, P],
 }
 

@@ -494,6 +494,56 @@ __global__ void cast_bf16_kernel(const CastBatch cb) {
     out[t] = (unsigned short)bf16_bits(d.in[t]);
 }
 
+// Force-matching loss terms, one wave per molecule m (atoms s .. e - 1):
+//   E[m] = sum of eps over its atoms (FP64, lane sums then a fixed butterfly),
+//   dE = E[m] - E_ref[m], ybar[i] = 2 w_e[m] dE,
+//   F = -grad, gg[i] = -2 w_f[m] (F_i - F_ref_i) (= dLoss/dgrad),
+//   part[m] = w_e[m] dE^2 + sum_i w_f[m] |F_i - F_ref_i|^2.
+__global__ void fm_loss_kernel(int B, const int* __restrict__ mol_start,
+                               const float* __restrict__ eps, const double* __restrict__ E_ref,
+                               const double* __restrict__ w_e, const double* __restrict__ grad,
+                               const double* __restrict__ F_ref, const double* __restrict__ w_f,
+                               double* __restrict__ E, float* __restrict__ ybar,
+                               double* __restrict__ gg, double* __restrict__ part) {
+  const int m = blockIdx.x, lane = threadIdx.x;
+  if (m >= B) return;
+  const int s = mol_start[m], e = mol_start[m + 1];
+  double esum = 0.0;
+  for (int i = s + lane; i < e; i += 64) esum += (double)eps[i];
+  esum = tm_wave_sum_d(esum);
+  const double dE = esum - E_ref[m], we = w_e[m], wf = w_f[m];
+  const float yb = (float)(2.0 * we * dE);
+  double fsum = 0.0;
+  for (int i = s + lane; i < e; i += 64) {
+    ybar[i] = yb;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double d = -grad[3 * (long)i + c] - F_ref[3 * (long)i + c];
+      gg[3 * (long)i + c] = -2.0 * wf * d;
+      fsum += wf * d * d;
+    }
+  }
+  fsum = tm_wave_sum_d(fsum);
+  if (lane == 0) {
+    E[m] = esum;
+    part[m] = we * dE * dE + fsum;
+  }
+}
+
+// L = sum of part[0 .. B) in a fixed order (one workgroup).
+__global__ void fm_loss_sum_kernel(int B, const double* __restrict__ part, double* __restrict__ L) {
+  __shared__ double red[256];
+  double v = 0.0;
+  for (int m = threadIdx.x; m < B; m += 256) v += part[m];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) L[0] = red[0];
+}
+
 }  // namespace
 
 extern "C" {
@@ -648,6 +698,21 @@ int tm_rows_grouped(int ngroups, const tm_rows_group* groups, int scatter, void*
     rows_grouped_kernel<<<dim3(tm_div_up(most, 4), n), 256, 0, (hipStream_t)stream>>>(rb);
     TM_LAUNCH_CHECK();
   }
+  return TM_OK;
+}
+
+int tm_fm_loss(int B, const int* mol_start, const float* eps, const double* E_ref,
+               const double* w_e, const double* grad, const double* F_ref, const double* w_f,
+               double* E, float* ybar, double* gg, double* part, double* L, void* stream) {
+  TM_REQUIRE(B >= 0, "tm_fm_loss: negative molecule count");
+  hipStream_t st = (hipStream_t)stream;
+  if (B > 0) {
+    fm_loss_kernel<<<B, 64, 0, st>>>(B, mol_start, eps, E_ref, w_e, grad, F_ref, w_f, E, ybar,
+                                     gg, part);
+    TM_LAUNCH_CHECK();
+  }
+  fm_loss_sum_kernel<<<1, 256, 0, st>>>(B, part, L);
+  TM_LAUNCH_CHECK();
   return TM_OK;
 }
 

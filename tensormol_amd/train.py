@@ -338,15 +338,20 @@ class ForceMatchingTrainer:
                     0, 1, 0)
         self._grows([(b["M"], D, ad(b["idx"]), ad(b["Je"]), D, ad(J), D, 0) for b, _ in items],
                     1)
-        E = torch.zeros(tb.B, dtype=torch.float64, device=self.device).index_add(
-            0, tb.mol, eps[:, 0].double())
-        grad = torch.empty(N, 3, dtype=torch.float64, device=self.device)
+        f64 = dict(dtype=torch.float64, device=self.device)
+        grad = torch.empty(N, 3, **f64)
         eng.descriptor_vjp(ws, pos, J, grad)
+        # molecule energies, the loss, ybar = dLoss/dy and gg = dLoss/dgrad in one
+        # deterministic launch pair
+        E, part, Lt = torch.empty(tb.B, **f64), torch.empty(tb.B, **f64), torch.empty(1, **f64)
+        ybar = torch.empty(N, 1, **f32)
+        gg = torch.empty(N, 3, **f64)
+        P = _lib_ptr
+        eng.lib.tm_fm_loss(tb.B, P(ws.mol_start), P(eps), P(tb.E_ref), P(tb.w_e), P(grad),
+                           P(tb.F_ref), P(tb.w_f), P(E), P(ybar), P(gg), P(part), P(Lt),
+                           eng._stream())
+        L = Lt[0]
         F = -grad
-        wf = tb.w_f[tb.mol][:, None]
-        L = (tb.w_e * (E - tb.E_ref) ** 2).sum() + (wf * (F - tb.F_ref) ** 2).sum()
-        ybar = (2.0 * tb.w_e * (E - tb.E_ref))[tb.mol].float().reshape(N, 1).contiguous()
-        gg = (-2.0 * wf * (F - tb.F_ref)).contiguous()   # dLoss/dgrad
         V = torch.empty(N, D, **f32)
         eng.descriptor_jvp(ws, pos, gg, V)
         self._grows([(b["M"], D, ad(b["idx"]), ad(V), D, ad(b["X0"], b["M"], b["ldx"]), b["ldx"],

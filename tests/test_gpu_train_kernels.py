@@ -330,3 +330,36 @@ def test_rows_bf16_and_cast(cuda):
     assert (out[:, 9:] == 0).all()
     for x, y in zip(xs, ys):
         assert torch.equal(y, x.bfloat16())
+
+
+def test_fm_loss_against_torch_formulas(cuda):
+    """tm_fm_loss (molecule energies, loss, ybar, dLoss/dgrad) against the same
+    formulas in FP64 torch on ragged molecules, one of them empty."""
+    L, P = _lib()
+    g = torch.Generator(device="cpu").manual_seed(31)
+    sizes = [3, 70, 1, 0, 130, 64]
+    ms = torch.tensor([0] + list(np.cumsum(sizes)), dtype=torch.int32)
+    N, B = int(ms[-1]), len(sizes)
+    mol = torch.repeat_interleave(torch.arange(B), torch.tensor(sizes))
+    eps = torch.randn(N, 1, generator=g)
+    E_ref, w_e, w_f = torch.randn(B, generator=g, dtype=torch.float64), \
+        torch.rand(B, generator=g, dtype=torch.float64), torch.rand(B, generator=g, dtype=torch.float64)
+    grad, F_ref = torch.randn(N, 3, generator=g, dtype=torch.float64), torch.randn(N, 3, generator=g, dtype=torch.float64)
+    E_t = torch.zeros(B, dtype=torch.float64).index_add(0, mol, eps[:, 0].double())
+    F = -grad
+    wf = w_f[mol][:, None]
+    L_t = (w_e * (E_t - E_ref) ** 2).sum() + (wf * (F - F_ref) ** 2).sum()
+    ybar_t = (2.0 * w_e * (E_t - E_ref))[mol].float().reshape(N, 1)
+    gg_t = -2.0 * wf * (F - F_ref)
+    d = lambda t: t.to(cuda).contiguous()
+    E, part, Lo = (torch.empty(n, dtype=torch.float64, device=cuda) for n in (B, B, 1))
+    ybar = torch.empty(N, 1, device=cuda)
+    gg = torch.empty(N, 3, dtype=torch.float64, device=cuda)
+    ms_d, eps_d, Er, we, gr, Fr, wfd = map(d, (ms, eps, E_ref, w_e, grad, F_ref, w_f))
+    L.tm_fm_loss(B, P(ms_d), P(eps_d), P(Er), P(we), P(gr), P(Fr), P(wfd), P(E), P(ybar), P(gg),
+                 P(part), P(Lo), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(E.cpu().numpy(), E_t.numpy(), rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(Lo.cpu().numpy()[0], L_t.item(), rtol=1e-12)
+    np.testing.assert_allclose(ybar.cpu().numpy(), ybar_t.numpy(), rtol=1e-6)
+    np.testing.assert_allclose(gg.cpu().numpy(), gg_t.numpy(), rtol=1e-12, atol=1e-15)
