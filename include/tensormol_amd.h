@@ -182,6 +182,8 @@ typedef struct tm_gemm_group {
   const float* x1;
   const float* x2;
   void* out2;          /* FP32, or BF16 with io bit 3 */
+  const int* crow;     /* epi 0 without split K: row m of the product goes to row
+                          crow[m] of C (a scatter), or NULL */
 } tm_gemm_group;
 
 /* ngroups independent GEMMs (groups: host array) with the same operand forms,

@@ -104,7 +104,7 @@ class GemmGroup(ctypes.Structure):
     """tm_gemm_group: one GEMM of a tm_gemm_bf16_grouped call."""
     _fields_ = [("M", I), ("N", I), ("K", I), ("A", P), ("lda", I), ("B", P), ("ldb", I),
                 ("C", P), ("ldc", I), ("bias", P), ("s", P), ("aux", P), ("work", P),
-                ("x1", P), ("x2", P), ("out2", P)]
+                ("x1", P), ("x2", P), ("out2", P), ("crow", P)]
 
 
 class RowsGroup(ctypes.Structure):

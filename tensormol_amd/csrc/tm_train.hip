@@ -51,6 +51,7 @@ struct GemmDev {
   const float* x1;
   const float* x2;
   float* out2;
+  const int* crow;     // output row of GEMM row m (plain epilogue), or NULL
   long cstride;        // floats between the splits' partial outputs
   int M, N, K, lda, ldb, ldc;
   int kchunk;          // K range of one split (a multiple of GK)
@@ -245,6 +246,7 @@ __device__ __forceinline__ void st4h(unsigned short* __restrict__ p, long o, int
 //  EPI_REVERSE  C = s * acc + a s (1 - s) * x1 * x2  (s, x1, x2 indexed as C)
 __device__ __forceinline__ void epilogue_tile(
     const float* Cs, int m0, int n0, int M, int N, long ldc, int epi, float a, bool vec, int io,
+    const int* __restrict__ crow,
     float* __restrict__ C, float* __restrict__ aux, float* __restrict__ out2,
     const float* __restrict__ bias, const float* __restrict__ s,
     const float* __restrict__ x1, const float* __restrict__ x2, int tid) {
@@ -262,7 +264,7 @@ __device__ __forceinline__ void epilogue_tile(
   for (int r = tid >> 5; r < GT; r += 8) {
     const int m = m0 + r;
     if (m >= M) break;
-    const long o = (long)m * ldc + n;
+    const long o = (long)(crow != nullptr ? crow[m] : m) * ldc + n;
     float v[4], sv[4] = {0.f, 0.f, 0.f, 0.f}, u1[4] = {0.f, 0.f, 0.f, 0.f},
                 u2[4] = {0.f, 0.f, 0.f, 0.f}, r0[4], r1[4], r2[4];
     const float4 cv = *reinterpret_cast<const float4*>(Cs + r * CLD + c);
@@ -400,7 +402,7 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(const GemmBatch gb) {
       for (int q = 0; q < 4; ++q)
         Cs[(wm + 16 * i + (lane >> 4) * 4 + q) * CLD + wn + 16 * j + (lane & 15)] = acc[i][j][q];
   __syncthreads();
-  epilogue_tile(Cs, m0, n0, g.M, g.N, g.ldc, gb.epi, gb.alpha, g.vec_out != 0, gb.io,
+  epilogue_tile(Cs, m0, n0, g.M, g.N, g.ldc, gb.epi, gb.alpha, g.vec_out != 0, gb.io, g.crow,
                 g.C != nullptr ? g.C + (long)blockIdx.y * g.cstride : nullptr, g.aux, g.out2,
                 g.bias, g.s, g.x1, g.x2, tid);
 }
@@ -578,6 +580,8 @@ int tm_gemm_bf16_grouped(int ngroups, const tm_gemm_group* groups, int ta, int t
                "tm_gemm_bf16: C may be NULL only with the tangent epilogue (GEMM %d)", i);
     TM_REQUIRE(nsplit == 1 || (q.bias == nullptr && q.work != nullptr),
                "tm_gemm_bf16: split K needs a workspace and no bias (GEMM %d)", i);
+    TM_REQUIRE(q.crow == nullptr || (epi == EPI_STORE && nsplit == 1),
+               "tm_gemm_bf16: a row map needs the plain epilogue without split K (GEMM %d)", i);
   }
   hipStream_t st = (hipStream_t)stream;
   const bool bf = (io & 1) != 0;
@@ -619,6 +623,7 @@ int tm_gemm_bf16_grouped(int ngroups, const tm_gemm_group* groups, int ta, int t
       d.C = nsplit > 1 ? q.work : static_cast<float*>(q.C);
       d.bias = q.bias; d.s = q.s; d.aux = static_cast<float*>(q.aux);
       d.x1 = q.x1; d.x2 = q.x2; d.out2 = static_cast<float*>(q.out2);
+      d.crow = q.crow;
       {
         // 16-byte epilogue accesses: FP32 arrays 16-byte aligned, BF16 outputs 8-byte
         const bool rev = epi == EPI_REVERSE;
@@ -655,7 +660,7 @@ int tm_gemm_bf16(int M, int N, int K, const float* A, int lda, int ta, const flo
   q.M = M; q.N = N; q.K = K;
   q.A = A; q.lda = lda; q.B = B; q.ldb = ldb; q.C = C; q.ldc = ldc;
   q.bias = bias; q.s = s; q.aux = aux; q.work = work;
-  q.x1 = nullptr; q.x2 = nullptr; q.out2 = nullptr;
+  q.x1 = nullptr; q.x2 = nullptr; q.out2 = nullptr; q.crow = nullptr;
   return tm_gemm_bf16_grouped(1, &q, ta, tb, epi, 0, alpha, nsplit, stream);
 }
 

@@ -232,10 +232,8 @@ class ForceMatchingTrainer:
                 b[k] = torch.zeros(2 * M, ldh, **b16)
             for k in ("S1", "S2", "S3", "ZT1", "ZT2", "ZT3", "HD1", "HD2"):
                 b[k] = torch.zeros(M, ldh, **f32)
-            b["Y"] = torch.zeros(M, 1, **f32)
             b["YB"] = torch.zeros(M, 1, **f32)      # ybar (the last tangent layer's epilogue)
             b["YBb"] = torch.ones(2 * M, 1, **b16)  # [ybar; 1] (the w_4 gradient GEMM)
-            b["Je"] = torch.zeros(M, D, **f32)
             b["Wb"] = [torch.zeros(W.shape, **b16) for (W, _) in self.params[e]]
             b["gW"] = [torch.zeros(D + 1, H, **f32), torch.zeros(H + 1, H, **f32),
                        torch.zeros(H + 1, H, **f32), torch.zeros(H + 1, 1, **f32)]
@@ -300,7 +298,7 @@ class ForceMatchingTrainer:
         ad = lambda t, rows=0, ld=0: t.data_ptr() + t.element_size() * rows * ld   # row `rows`
         f32 = dict(dtype=torch.float32, device=self.device)
         # every atom has an element of the model (prepare() checks), so the
-        # per-element scatters below write every row of these
+        # per-element GEMMs below write every row of these through their row maps
         eps = torch.empty(N, 1, **f32)
         J = torch.empty(N, D, **f32)
         # this step's weights as BF16 (one launch), then the descriptors as BF16 rows
@@ -321,11 +319,10 @@ class ForceMatchingTrainer:
                     g.update(x1=ad(prm[3][0]), out2=ad(b["ZZ3"], M, ldh))
                 grp.append(g)
             self._ggemm(grp, 0, 0, 1, io=1 | 2 | 8)   # H_l, zd_3 BF16; s_l FP32
+        # atom energies y = h_3 w_4 + b_4, written to each atom's row of eps
         self._ggemm([dict(M=b["M"], N=1, K=H, A=ad(b["H3"]), lda=b["ldh"], B=ad(b["Wb"][3]),
-                          ldb=1, C=ad(b["Y"]), ldc=1, bias=ad(prm[3][1])) for b, prm in items],
-                    0, 0, 0)
-        self._grows([(b["M"], 1, ad(b["idx"]), ad(b["Y"]), 1, ad(eps), 1, 0) for b, _ in items],
-                    1)
+                          ldb=1, C=ad(eps), ldc=1, bias=ad(prm[3][1]), crow=ad(b["idx"]))
+                     for b, prm in items], 0, 0, 0)
         # J chain: hd_{l-1} = zd_l W_l^T, zd_{l-1} = s_{l-1} hd_{l-1}, J = zd_1 W_1^T
         for (l, zz_in, hd, sk, zz_out) in ((2, "ZZ3", "HD2", "S2", "ZZ2"),
                                            (1, "ZZ2", "HD1", "S1", "ZZ1")):
@@ -333,11 +330,10 @@ class ForceMatchingTrainer:
                               B=ad(b["Wb"][l]), ldb=H, C=ad(b[hd]), ldc=b["ldh"], s=ad(b[sk]),
                               aux=ad(b[zz_out], b["M"], b["ldh"])) for b, prm in items], 0, 1, 2,
                         io=1 | 4)   # hd FP32, zd BF16
+        # J rows written to each atom's row of J
         self._ggemm([dict(M=b["M"], N=D, K=H, A=ad(b["ZZ1"], b["M"], b["ldh"]), lda=b["ldh"],
-                          B=ad(b["Wb"][0]), ldb=H, C=ad(b["Je"]), ldc=D) for b, prm in items],
-                    0, 1, 0)
-        self._grows([(b["M"], D, ad(b["idx"]), ad(b["Je"]), D, ad(J), D, 0) for b, _ in items],
-                    1)
+                          B=ad(b["Wb"][0]), ldb=H, C=ad(J), ldc=D, crow=ad(b["idx"]))
+                     for b, prm in items], 0, 1, 0)
         f64 = dict(dtype=torch.float64, device=self.device)
         grad = torch.empty(N, 3, **f64)
         eng.descriptor_vjp(ws, pos, J, grad)

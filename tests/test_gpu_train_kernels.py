@@ -363,3 +363,31 @@ def test_fm_loss_against_torch_formulas(cuda):
     np.testing.assert_allclose(Lo.cpu().numpy()[0], L_t.item(), rtol=1e-12)
     np.testing.assert_allclose(ybar.cpu().numpy(), ybar_t.numpy(), rtol=1e-6)
     np.testing.assert_allclose(gg.cpu().numpy(), gg_t.numpy(), rtol=1e-12, atol=1e-15)
+
+
+def test_gemm_row_map_scatters_rows(cuda):
+    """crow: row m of the product lands in row crow[m] of C (plain epilogue,
+    with bias), the rows not in the map untouched; equal to the unmapped
+    result scattered."""
+    import ctypes
+    from tensormol_amd import _lib as TL
+    L, P = _lib()
+    g = torch.Generator(device="cpu").manual_seed(41)
+    M, N, K = 150, 70, 96
+    A = torch.randn(M, K, generator=g).to(cuda)
+    B = torch.randn(K, N, generator=g).to(cuda)
+    bias = torch.randn(N, generator=g).to(cuda)
+    rows = torch.randperm(400, generator=g)[:M].to(torch.int32).to(cuda)
+    st = torch.cuda.current_stream().cuda_stream
+    C0 = torch.zeros(M, N, device=cuda)
+    C1 = torch.full((400, N), 3.0, device=cuda)
+    for C, crow in ((C0, None), (C1, rows)):
+        f = dict(M=M, N=N, K=K, A=A.data_ptr(), lda=K, B=B.data_ptr(), ldb=N, C=C.data_ptr(),
+                 ldc=N, bias=bias.data_ptr(), crow=None if crow is None else crow.data_ptr())
+        arr = (TL.GemmGroup * 1)(TL.GemmGroup(**f))
+        L.tm_gemm_bf16_grouped(1, ctypes.cast(arr, ctypes.c_void_p), 0, 0, 0, 0, 1.0, 1, st)
+    torch.cuda.synchronize()
+    assert torch.equal(C1[rows.long()], C0)
+    mask = torch.ones(400, dtype=torch.bool, device=cuda)
+    mask[rows.long()] = False
+    assert (C1[mask] == 3.0).all()
