@@ -603,10 +603,10 @@ sym_jvp_kernel(int N, const double* __restrict__ pos, const double* __restrict__
                SymArgs a, const double* __restrict__ du, float* __restrict__ V, int ldg) {
   __shared__ RowLds<RC> L;
   __shared__ JvpLds<RC> T;
-  __shared__ float tc[SYM_WAVES][64], tsn[SYM_WAVES][64], tcs[SYM_WAVES][64];
-  __shared__ float tf[SYM_WAVES][64], tdf[SYM_WAVES][64], tdc[SYM_WAVES][64], tdr[SYM_WAVES][64];
-  __shared__ float tR[SYM_WAVES][NA][64], tRp[SYM_WAVES][NA][64];
-  __shared__ int tp[SYM_WAVES][64];
+  // per triple: (cos theta, sin theta, cos/sin guarded, fc_j fc_k), (d(fc_j fc_k),
+  // d cos theta, d ravg, -), R_s and dR_s/d ravg for s = 0..7
+  __shared__ float4 tqa[SYM_WAVES][64], tqb[SYM_WAVES][64];
+  __shared__ float4 tR[SYM_WAVES][64][NA / 4], tRp[SYM_WAVES][64][NA / 4];
   __shared__ int segs[SYM_WAVES][2][TM_MAX_ELE + 1];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i = blockIdx.x * SYM_WAVES + w;
@@ -653,20 +653,47 @@ sym_jvp_kernel(int N, const double* __restrict__ pos, const double* __restrict__
       }
     }
   }
-  // ---------------- angular: phase A lane = triple, phase B lane = (t, s)
+  // ---------------- angular
+  // The forward's layout (sym_fwd_kernel): phase A lane = triple stages the
+  // triple's geometry and its tangents in LDS; phase B lane = (theta point t,
+  // triple group g) takes every 8th triple of an element-pair segment, forms
+  // (1 + cos(theta - theta_t))^zeta and its derivative once, and adds
+  //   X R_s + Y R'_s,  X = A' v' c' F + A F',  Y = A r' F
+  // to one accumulator per shell s (A = (1 + v)^zeta, v = cos(theta - theta_t),
+  // c', r', F' the tangents of cos theta, ravg and fc_j fc_k); a butterfly
+  // over g at each segment end, lane (t, g) writes shell s = g.
   const int m = min(acnt[i], lda);
   const int* arow = ang + (long)i * lda;
   const int ntrip = m * (m - 1) / 2;
   const int off = a.nele * a.nr;
-  const int ot = lane >> 3, os = lane & 7;
+  const int ot = lane >> 3, g = lane & 7;
   const float ct = a.cos_t[ot], st = a.sin_t[ot];
-  int next_p = 0, cur_p = -1;
-  float acc = 0.f;
+  const int* sg = segs[w][1];
+  float acc[NA];
+#pragma unroll
+  for (int s = 0; s < NA; ++s) acc[s] = 0.f;
+  int pdone = 0;   // segments [0, pdone) are written
+  auto flush = [&](int p) {
+    float v[NA];
+#pragma unroll
+    for (int s = 0; s < NA; ++s) {
+      float x = acc[s];
+      x += __shfl_xor(x, 1, 64);
+      x += __shfl_xor(x, 2, 64);
+      x += __shfl_xor(x, 4, 64);
+      v[s] = x;
+      acc[s] = 0.f;
+    }
+    float out = v[0];
+#pragma unroll
+    for (int s = 1; s < NA; ++s) out = g == s ? v[s] : out;
+    Vi[off + p * 64 + ot * NA + g] = a.two_1mz * out;
+  };
   for (int base = 0; base < ntrip; base += 64) {
     int t = base + lane;
     if (t < ntrip) {
       int ja, kb, p;
-      decode_triple(t, segs[w][1], a.nele, ja, kb, p);
+      decode_triple(t, sg, a.nele, ja, kb, p);
       int qj = arow[ja], qk = arow[kb];
       float4 dj = L.d[w][qj], dk = L.d[w][qk];
       float4 uj = T.t[w][qj], uk = T.t[w][qk];
@@ -680,52 +707,55 @@ sym_jvp_kernel(int N, const double* __restrict__ pos, const double* __restrict__
                   dj.z * uk.z) * inv - c * (uj.w * irj + uk.w * irk);
       float fj = L.fa[w][qj], fk = L.fa[w][qk];
       float ravg = 0.5f * (rj + rk);
-      tc[w][lane] = c;
-      tsn[w][lane] = sn;
-      tcs[w][lane] = sn > 1e-6f ? c * __builtin_amdgcn_rcpf(sn) : 0.f;
-      tf[w][lane] = fj * fk;
-      tdf[w][lane] = T.dfa[w][qj] * uj.w * fk + fj * T.dfa[w][qk] * uk.w;
-      tdc[w][lane] = dc;
-      tdr[w][lane] = 0.5f * (uj.w + uk.w);
-      tp[w][lane] = p;
+      tqa[w][lane] = make_float4(c, sn, sn > 1e-6f ? c * __builtin_amdgcn_rcpf(sn) : 0.f,
+                                 fj * fk);
+      tqb[w][lane] = make_float4(T.dfa[w][qj] * uj.w * fk + fj * T.dfa[w][qk] * uk.w, dc,
+                                 0.5f * (uj.w + uk.w), 0.f);
+      float R[NA], Rp[NA];
 #pragma unroll
       for (int s = 0; s < NA; ++s) {
         float x = ravg - a.rs_a[s];
-        float R = __expf(-a.eta * x * x);
-        tR[w][s][lane] = R;
-        tRp[w][s][lane] = -2.0f * a.eta * x * R;
+        R[s] = __expf(-a.eta * x * x);
+        Rp[s] = -2.0f * a.eta * x * R[s];
       }
+      tR[w][lane][0] = make_float4(R[0], R[1], R[2], R[3]);
+      tR[w][lane][1] = make_float4(R[4], R[5], R[6], R[7]);
+      tRp[w][lane][0] = make_float4(Rp[0], Rp[1], Rp[2], Rp[3]);
+      tRp[w][lane][1] = make_float4(Rp[4], Rp[5], Rp[6], Rp[7]);
     }
     __syncwarp();
-    int cnt = min(64, ntrip - base);
-    for (int u = 0; u < cnt; ++u) {
-      int p = tp[w][u];
-      if (p != cur_p) {
-        if (cur_p >= 0) {
-          for (; next_p < cur_p; ++next_p) Vi[off + next_p * 64 + lane] = 0.f;
-          Vi[off + cur_p * 64 + lane] = acc;
-          next_p = cur_p + 1;
+    const int cnt = min(64, ntrip - base);
+    int T0 = 0, p = 0;   // first triple of segment p
+    for (int ea = 0; ea < a.nele; ++ea) {
+      const int la = sg[ea + 1] - sg[ea];
+      for (int eb = ea; eb < a.nele; ++eb, ++p) {
+        const int lb = sg[eb + 1] - sg[eb];
+        const int Te = T0 + (ea == eb ? la * (la - 1) / 2 : la * lb);
+        const int lo = max(T0 - base, 0), hi = min(Te - base, cnt);   // chunk-local part
+        for (int u = lo + ((g - lo) & 7); u < hi; u += 8) {
+          const float4 qa = tqa[w][u], qb = tqb[w][u];
+          float Az, dAz;
+          powz_pair<ZI>(1.0f + (qa.x * ct + qa.y * st), a, Az, dAz);
+          const float dv = ct - qa.z * st;
+          const float X = dAz * dv * qb.y * qa.w + Az * qb.x;
+          const float Y = Az * qb.z * qa.w;
+          const float4 r0 = tR[w][u][0], r1 = tR[w][u][1];
+          const float4 p0 = tRp[w][u][0], p1 = tRp[w][u][1];
+          acc[0] += X * r0.x + Y * p0.x; acc[1] += X * r0.y + Y * p0.y;
+          acc[2] += X * r0.z + Y * p0.z; acc[3] += X * r0.w + Y * p0.w;
+          acc[4] += X * r1.x + Y * p1.x; acc[5] += X * r1.y + Y * p1.y;
+          acc[6] += X * r1.z + Y * p1.z; acc[7] += X * r1.w + Y * p1.w;
         }
-        cur_p = p;
-        acc = 0.f;
+        if (p >= pdone && Te <= base + cnt) {   // segment complete (wave-uniform)
+          flush(p);
+          pdone = p + 1;
+        }
+        T0 = Te;
       }
-      float v = tc[w][u] * ct + tsn[w][u] * st;
-      float b = 1.0f + v;
-      float Az, dAz;
-      powz_pair<ZI>(b, a, Az, dAz);
-      float dv = ct - tcs[w][u] * st;
-      float R = tR[w][os][u], F = tf[w][u];
-      acc += a.two_1mz * (dAz * dv * tdc[w][u] * R * F +
-                          Az * (tRp[w][os][u] * tdr[w][u] * F + R * tdf[w][u]));
     }
     __syncwarp();
   }
-  if (cur_p >= 0) {
-    for (; next_p < cur_p; ++next_p) Vi[off + next_p * 64 + lane] = 0.f;
-    Vi[off + cur_p * 64 + lane] = acc;
-    next_p = cur_p + 1;
-  }
-  for (; next_p < a.nelep; ++next_p) Vi[off + next_p * 64 + lane] = 0.f;
+  for (int p = pdone; p < a.nelep; ++p) Vi[off + p * 64 + lane] = 0.f;
 }
 
 int fill_args(SymArgs& a, int nele, int nr, float eta, float zeta, float rr, float ra,
