@@ -590,7 +590,6 @@ template <int RC>
 struct JvpLds {
   float4 t[SYM_WAVES][RC];     // (du_j - du_i, dr_j = d_j . (du_j - du_i) / r_j)
   float dfc[SYM_WAVES][RC];    // d fc_r / dr
-  float dfa[SYM_WAVES][RC];    // d fc_a / dr
 };
 
 template <int ZI, int RC>
@@ -601,12 +600,14 @@ sym_jvp_kernel(int N, const double* __restrict__ pos, const double* __restrict__
                const int* __restrict__ rseg, const int* __restrict__ ang,
                const int* __restrict__ acnt, const int* __restrict__ aseg, int ldr, int lda,
                SymArgs a, const double* __restrict__ du, float* __restrict__ V, int ldg) {
-  __shared__ RowLds<RC> L;
+  // the forward's row (the angular cutoff and its derivative are formed per
+  // triple), which with the per-triple staging below fits 4 waves per SIMD
+  __shared__ RowLdsFwd<RC> L;
   __shared__ JvpLds<RC> T;
   // per triple: (cos theta, sin theta, cos/sin guarded, fc_j fc_k), (d(fc_j fc_k),
-  // d cos theta, d ravg, -), R_s and dR_s/d ravg for s = 0..7
+  // d cos theta, d ravg, ravg), R_s for s = 0..7
   __shared__ float4 tqa[SYM_WAVES][64], tqb[SYM_WAVES][64];
-  __shared__ float4 tR[SYM_WAVES][64][NA / 4], tRp[SYM_WAVES][64][NA / 4];
+  __shared__ float4 tR[SYM_WAVES][64][NA / 4];
   __shared__ int segs[SYM_WAVES][2][TM_MAX_ELE + 1];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i = blockIdx.x * SYM_WAVES + w;
@@ -628,7 +629,6 @@ sym_jvp_kernel(int N, const double* __restrict__ pos, const double* __restrict__
       const float4 d = L.d[w][q];
       T.t[w][q] = make_float4(tx, ty, tz, (d.x * tx + d.y * ty + d.z * tz) * __builtin_amdgcn_rcpf(d.w));
       T.dfc[w][q] = -0.5f * kr * __sinf(kr * d.w);
-      T.dfa[w][q] = -0.5f * ka * __sinf(ka * d.w);
     }
   }
   __syncwarp();
@@ -705,23 +705,21 @@ sym_jvp_kernel(int N, const double* __restrict__ pos, const double* __restrict__
       float sn = sin_between(dj, dk, inv);
       float dc = (uj.x * dk.x + uj.y * dk.y + uj.z * dk.z + dj.x * uk.x + dj.y * uk.y +
                   dj.z * uk.z) * inv - c * (uj.w * irj + uk.w * irk);
-      float fj = L.fa[w][qj], fk = L.fa[w][qk];
+      const float fj = cutoff_fc(ka, rj), fk = cutoff_fc(ka, rk);
+      const float dfj = cutoff_dfc(ka, rj), dfk = cutoff_dfc(ka, rk);
       float ravg = 0.5f * (rj + rk);
       tqa[w][lane] = make_float4(c, sn, sn > 1e-6f ? c * __builtin_amdgcn_rcpf(sn) : 0.f,
                                  fj * fk);
-      tqb[w][lane] = make_float4(T.dfa[w][qj] * uj.w * fk + fj * T.dfa[w][qk] * uk.w, dc,
-                                 0.5f * (uj.w + uk.w), 0.f);
-      float R[NA], Rp[NA];
+      tqb[w][lane] = make_float4(dfj * uj.w * fk + fj * dfk * uk.w, dc, 0.5f * (uj.w + uk.w),
+                                 ravg);
+      float R[NA];
 #pragma unroll
       for (int s = 0; s < NA; ++s) {
         float x = ravg - a.rs_a[s];
         R[s] = __expf(-a.eta * x * x);
-        Rp[s] = -2.0f * a.eta * x * R[s];
       }
       tR[w][lane][0] = make_float4(R[0], R[1], R[2], R[3]);
       tR[w][lane][1] = make_float4(R[4], R[5], R[6], R[7]);
-      tRp[w][lane][0] = make_float4(Rp[0], Rp[1], Rp[2], Rp[3]);
-      tRp[w][lane][1] = make_float4(Rp[4], Rp[5], Rp[6], Rp[7]);
     }
     __syncwarp();
     const int cnt = min(64, ntrip - base);
@@ -738,13 +736,11 @@ sym_jvp_kernel(int N, const double* __restrict__ pos, const double* __restrict__
           powz_pair<ZI>(1.0f + (qa.x * ct + qa.y * st), a, Az, dAz);
           const float dv = ct - qa.z * st;
           const float X = dAz * dv * qb.y * qa.w + Az * qb.x;
-          const float Y = Az * qb.z * qa.w;
+          const float Y2 = -2.0f * a.eta * Az * qb.z * qa.w;   // times (ravg - R_s) R_s
           const float4 r0 = tR[w][u][0], r1 = tR[w][u][1];
-          const float4 p0 = tRp[w][u][0], p1 = tRp[w][u][1];
-          acc[0] += X * r0.x + Y * p0.x; acc[1] += X * r0.y + Y * p0.y;
-          acc[2] += X * r0.z + Y * p0.z; acc[3] += X * r0.w + Y * p0.w;
-          acc[4] += X * r1.x + Y * p1.x; acc[5] += X * r1.y + Y * p1.y;
-          acc[6] += X * r1.z + Y * p1.z; acc[7] += X * r1.w + Y * p1.w;
+          const float R[NA] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+#pragma unroll
+          for (int s = 0; s < NA; ++s) acc[s] += R[s] * (X + Y2 * (qb.w - a.rs_a[s]));
         }
         if (p >= pdone && Te <= base + cnt) {   // segment complete (wave-uniform)
           flush(p);
