@@ -76,3 +76,45 @@ def test_binding_argument_counts_match_header():
         params = m.group(1).strip()
         n = 0 if params in ("", "void") else params.count(",") + 1
         assert n == len(args), f"{name}: header has {n} parameters, binding {len(args)}"
+
+
+def test_training_kernel_argument_errors(lib):
+    """The training-step entry points reject bad arguments on the host, before
+    anything is launched (so these run without a GPU)."""
+    from tensormol_amd import _lib as TL
+    fake = 4096   # a non-NULL address that is never dereferenced: every case fails validation
+
+    def call(groups, epi=0, io=0, nsplit=1):
+        arr = (TL.GemmGroup * len(groups))(*groups)
+        lib.tm_gemm_bf16_grouped(len(groups), ctypes.cast(arr, ctypes.c_void_p), 0, 0, epi, io,
+                                 1.0, nsplit, None)
+
+    base = dict(M=8, N=8, K=8, A=fake, lda=8, B=fake, ldb=8, C=fake, ldc=8)
+    with pytest.raises(RuntimeError, match="bad io flags"):
+        call([TL.GemmGroup(**base)], io=16)
+    with pytest.raises(RuntimeError, match="bad epilogue"):
+        call([TL.GemmGroup(**base)], epi=4)
+    with pytest.raises(RuntimeError, match="FP32 C"):
+        call([TL.GemmGroup(**base, work=fake)], io=2, nsplit=2)
+    with pytest.raises(RuntimeError, match="negative size"):
+        call([TL.GemmGroup(**dict(base, M=-1))])
+    with pytest.raises(RuntimeError, match="softplus epilogue needs bias and aux"):
+        call([TL.GemmGroup(**base)], epi=1)
+    with pytest.raises(RuntimeError, match="reverse epilogue"):
+        call([TL.GemmGroup(**base, s=fake)], epi=3)
+    with pytest.raises(RuntimeError, match="row map"):
+        call([TL.GemmGroup(**base, bias=fake, aux=fake, crow=fake)], epi=1)
+    with pytest.raises(RuntimeError, match="out2 needs"):
+        call([TL.GemmGroup(**base, out2=fake)], epi=0)
+    with pytest.raises(RuntimeError, match="workspace"):
+        call([TL.GemmGroup(**base)], nsplit=3)
+    rows = (TL.RowsGroup * 1)(TL.RowsGroup(-1, 4, fake, fake, 4, fake, 4, 0))
+    with pytest.raises(RuntimeError, match="negative size"):
+        lib.tm_rows_grouped(1, ctypes.cast(rows, ctypes.c_void_p), 0, None)
+    casts = (TL.CastGroup * 1)(TL.CastGroup(fake, fake, -5))
+    with pytest.raises(RuntimeError, match="negative size"):
+        lib.tm_cast_bf16_grouped(1, ctypes.cast(casts, ctypes.c_void_p), None)
+    with pytest.raises(RuntimeError, match="negative molecule count"):
+        lib.tm_fm_loss(-1, *([None] * 12), None)
+    # nothing to do is not an error
+    lib.tm_gemm_bf16_grouped(0, None, 0, 0, 0, 0, 1.0, 1, None)
