@@ -239,6 +239,20 @@ typedef struct tm_cast_group {
 } tm_cast_group;
 int tm_cast_bf16_grouped(int ngroups, const tm_cast_group* groups, void* stream);
 
+/* ---- serving ------------------------------------------------------------
+ * Result staging of one batch step: copies nparts device arrays (host array
+ * of records; bytes and offset in whole 4-byte words) into dst at their
+ * offsets in one launch per 16, so the host reads every result of the step
+ * (status flags, energies, forces, positions, row counts) through one
+ * device-to-host copy.  No counterpart in the reference (its loop returns one
+ * molecule's results per TF session run). */
+typedef struct tm_pack_part {
+  const void* src;
+  long bytes;
+  long offset;
+} tm_pack_part;
+int tm_pack_copy(int nparts, const tm_pack_part* parts, void* dst, void* stream);
+
 /* The loss side of the force-matching step (loss_op_EandG,
  * TFMolInstanceDirect.py:4889-4903): per molecule m of atoms
  * [mol_start[m], mol_start[m+1]), E[m] = sum of the atom energies eps,

@@ -60,6 +60,7 @@ SIGNATURES = {
     "tm_rows_grouped": [I, P, I, P],
     "tm_cast_bf16_grouped": [I, P, P],
     "tm_fm_loss": [I, P, P, P, P, P, P, P, P, P, P, P, P, P],
+    "tm_pack_copy": [I, P, P, P],
     "tm_mlp_mfma_pack_info": [I, I, I, I, P],
 }
 
@@ -111,6 +112,11 @@ class RowsGroup(ctypes.Structure):
     """tm_rows_group: one row gather / scatter of a tm_rows_grouped call."""
     _fields_ = [("M", I), ("cols", I), ("idx", P), ("inp", P), ("ldi", I), ("out", P), ("ldo", I),
                 ("out_bf16", I)]
+
+
+class PackPart(ctypes.Structure):
+    """tm_pack_part: one array of a tm_pack_copy call."""
+    _fields_ = [("src", P), ("bytes", L), ("offset", L)]
 
 
 class CastGroup(ctypes.Structure):

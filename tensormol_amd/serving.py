@@ -142,13 +142,13 @@ class Server:
         self.dsum = torch.zeros(b, 3, **f64)
         self.active = []
         self.natoms = 0
-        from .staging import PinnedOutputs, PinnedStager
+        from .staging import PackedOutputs, PinnedStager
         self.stager = PinnedStager(dev, slot_bytes=96 * n + 96 * b + (1 << 16))
-        # two sets of pinned result buffers: the step on the GPU and the one being read
-        spec = {"stat": ((4 + 2 * b,), torch.float64), "force": ((n, 3), torch.float64),
-                "pos": ((n, 3), torch.float64), "ncnt": ((n,), torch.int32),
-                "acnt": ((n,), torch.int32)}
-        self.outs = [PinnedOutputs(spec), PinnedOutputs(spec)]
+        # two sets of result buffers: the step on the GPU and the one being read.
+        # Per step: energies (Ebp + Eext, 24 B a molecule), FIRE |F|^2 rows
+        # (24 B), forces or positions (24 B an atom) and two row counts (8 B)
+        pack = 48 * b + 32 * n + 256
+        self.outs = [PackedOutputs(dev, pack), PackedOutputs(dev, pack)]
 
     def bytes_per_atom(self):
         """Device bytes of one atom's share of the workspace and server state."""
@@ -314,16 +314,17 @@ class Server:
             self.graph_replays += 1
         else:
             self._step()
-        parts = [ws.flags[:4].double(), self.eng.energy_tensor(ws)]
-        if bo:
-            parts.append(self.dsum[bn:bn + bo, 1])
-        outs.fetch("stat", torch.cat(parts))
+        B, N = ws.B, ws.N
         n_dyn = int(self.ms[bn + bo])
-        outs.fetch("force", ws.force[n_dyn:ws.N])
-        outs.fetch("pos", self.pos[:n_dyn])
-        outs.fetch("ncnt", ws.ncnt[:ws.N])
-        outs.fetch("acnt", ws.acnt[:ws.N])
-        outs.mark(self.eng.device)
+        outs.begin()
+        outs.add("Ebp", ws.Ebp[:B])
+        outs.add("Eext", ws.Eext[:B])
+        outs.add("dsum", self.dsum[bn:bn + bo])
+        outs.add("force", ws.force[n_dyn:N])
+        outs.add("pos", self.pos[:n_dyn])
+        outs.add("ncnt", ws.ncnt[:N])
+        outs.add("acnt", ws.acnt[:N])
+        outs.mark()
         return dict(reqs=self.active, ms=self.ms, groups=tuple(self.groups), N=ws.N,
                     mol_np=ws.mol_np, outs=outs)
 
@@ -338,14 +339,16 @@ class Server:
         outs.wait()
         reqs, ms, (bn, bo, _) = batch["reqs"], batch["ms"], batch["groups"]
         B, N = len(reqs), batch["N"]
-        stat = outs.host("stat", 4 + B + bo)
-        flags, E, conv = stat[:4], stat[4:4 + B], stat[4 + B:]
+        # the engine's molecule total (Engine.energy_tensor), same order of adds
+        Eext = outs.host("Eext")
+        E = outs.host("Ebp") + Eext[:, 0] + Eext[:, 1]
+        conv = outs.host("dsum")[:, 1]
         # failures stay with their molecule, since rows, mirror slots and
         # reductions never cross molecules.  A molecule fails when one of its
         # rows reaches capacity (ldr - 1 and lda - 1 neighbours are the most a
         # request may have: a collapsed geometry), judged per molecule so the
         # verdict does not depend on the batch, or when its energy is not finite
-        full = (outs.host("ncnt", N) >= self.eng.ldr) | (outs.host("acnt", N) >= self.eng.lda)
+        full = (outs.host("ncnt") >= self.eng.ldr) | (outs.host("acnt") >= self.eng.lda)
         row_full = np.zeros(B, dtype=bool)
         row_full[np.unique(batch["mol_np"][full])] = True
         bad = row_full | ~np.isfinite(E)
@@ -381,9 +384,9 @@ class Server:
             if r.error is None:
                 r.energy = float(E[k])
                 if r.kind == "sp":
-                    r.forces = outs.host("force", N - n_dyn)[a - n_dyn:b_ - n_dyn].copy()
+                    r.forces = outs.host("force")[a - n_dyn:b_ - n_dyn].copy()
                 else:
-                    r.positions = outs.host("pos", n_dyn)[a:b_].copy()
+                    r.positions = outs.host("pos")[a:b_].copy()
             r.t_done = t_done
             r._retired = True
             done.append(r)

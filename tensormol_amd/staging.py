@@ -87,27 +87,62 @@ class PinnedStager:
         return out
 
 
-class PinnedOutputs:
-    """Named pinned host arrays that device results are copied into without
-    blocking; read them after the event recorded by mark()."""
+class PackedOutputs:
+    """Device results of one step gathered into one device buffer by a single
+    tm_pack_copy launch and brought to pinned host memory by one copy.
 
-    def __init__(self, specs):
-        self.arrs = {k: torch.empty(shape, dtype=dt, pin_memory=True) for k, (shape, dt) in specs.items()}
+    begin() starts a new set; add() appends a contiguous device array under a
+    name (8-byte aligned, in call order); mark() queues the pack launch, the
+    copy of the used bytes and an event on the current stream; after wait(),
+    host(name) is a numpy view of that array's values.
+    """
+
+    def __init__(self, device, nbytes, max_parts=16):
+        from . import _lib
+        self._lib = _lib
+        self.device = device
+        self.size = int(nbytes)
+        self.dev = torch.empty(self.size, dtype=torch.uint8, device=device)
+        self.buf = torch.empty(self.size, dtype=torch.uint8, pin_memory=True)
+        self.np_buf = self.buf.numpy()
+        self.parts = (_lib.PackPart * max_parts)()
+        self.views = {}
+        self.n = 0
+        self.off = 0
         self.event = None
 
-    def fetch(self, name, src):
-        """Queue a copy of the device tensor src into the first rows of `name`."""
-        n = src.shape[0]
-        if n:
-            self.arrs[name][:n].copy_(src, non_blocking=True)
+    def begin(self):
+        self.views = {}
+        self.n = 0
+        self.off = 0
 
-    def mark(self, device):
+    def add(self, name, src):
+        """Queue the device tensor src (contiguous) for the next mark()."""
+        assert src.is_contiguous() and src.device.type == "cuda"
+        nb = src.numel() * src.element_size()
+        if self.off + nb > self.size or self.n == len(self.parts):
+            raise ValueError(f"PackedOutputs: {name} ({nb} bytes) exceeds the buffer")
+        dt = torch.empty(0, dtype=src.dtype).numpy().dtype
+        self.views[name] = (self.off, nb, dt, tuple(src.shape))
+        if nb:
+            p = self.parts[self.n]
+            p.src, p.bytes, p.offset = src.data_ptr(), nb, self.off
+            self.n += 1
+        self.off = (self.off + nb + 7) & ~7
+
+    def mark(self):
+        st = torch.cuda.current_stream(self.device)
+        if self.n:
+            self._lib.lib().tm_pack_copy(self.n, self.parts, self._lib.ptr(self.dev),
+                                         self._lib.ctypes.c_void_p(st.cuda_stream))
+            self.buf[:self.off].copy_(self.dev[:self.off], non_blocking=True)
         self.event = torch.cuda.Event()
-        self.event.record(torch.cuda.current_stream(device))
+        self.event.record(st)
 
     def wait(self):
         if self.event is not None:
             self.event.synchronize()
 
-    def host(self, name, n):
-        return self.arrs[name][:n].numpy()
+    def host(self, name):
+        off, nb, dt, shape = self.views[name]
+        return self.np_buf[off:off + nb].view(dt).reshape(shape)

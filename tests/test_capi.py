@@ -116,5 +116,14 @@ def test_training_kernel_argument_errors(lib):
         lib.tm_cast_bf16_grouped(1, ctypes.cast(casts, ctypes.c_void_p), None)
     with pytest.raises(RuntimeError, match="negative molecule count"):
         lib.tm_fm_loss(-1, *([None] * 12), None)
+    packs = (TL.PackPart * 1)(TL.PackPart(fake, 6, 0))
+    with pytest.raises(RuntimeError, match="whole 4-byte words"):
+        lib.tm_pack_copy(1, ctypes.cast(packs, ctypes.c_void_p), fake, None)
+    packs = (TL.PackPart * 1)(TL.PackPart(None, 8, 0))
+    with pytest.raises(RuntimeError, match="whole 4-byte words"):
+        lib.tm_pack_copy(1, ctypes.cast(packs, ctypes.c_void_p), fake, None)
+    with pytest.raises(RuntimeError, match="bad part list"):
+        lib.tm_pack_copy(1, None, fake, None)
+    lib.tm_pack_copy(0, None, None, None)
     # nothing to do is not an error
     lib.tm_gemm_bf16_grouped(0, None, 0, 0, 0, 0, 1.0, 1, None)

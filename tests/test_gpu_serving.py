@@ -287,3 +287,30 @@ def test_graph_replayed_steps_same_results(served):
             assert np.array_equal(x.forces, d.forces), rid
         else:
             assert np.array_equal(x.positions, d.positions), rid
+
+
+def test_packed_outputs_copy_every_part(cuda):
+    """tm_pack_copy through PackedOutputs: each part lands at its own offset,
+    unchanged, including more parts than one launch takes and empty parts."""
+    from tensormol_amd.staging import PackedOutputs
+    g = torch.Generator(device="cpu").manual_seed(5)
+    srcs = {}
+    for i in range(21):
+        n = int(torch.randint(0, 700, (1,), generator=g))
+        if i % 3 == 0:
+            srcs[f"i{i}"] = torch.randint(-2 ** 31, 2 ** 31 - 1, (n,), generator=g, dtype=torch.int32)
+        else:
+            srcs[f"d{i}"] = torch.randn(n, 3, generator=g, dtype=torch.float64)
+    srcs["empty"] = torch.zeros(0, 3, dtype=torch.float64)
+    total = sum(t.numel() * t.element_size() + 8 for t in srcs.values())
+    outs = PackedOutputs(cuda, total, max_parts=32)
+    for rep in range(2):     # a second, smaller set reuses the buffers
+        dev = {k: v.to(cuda) for k, v in srcs.items()}
+        outs.begin()
+        for k, v in dev.items():
+            outs.add(k, v)
+        outs.mark()
+        outs.wait()
+        for k, v in srcs.items():
+            np.testing.assert_array_equal(outs.host(k), v.numpy(), err_msg=k)
+        srcs = {k: v[: len(v) // 2] for k, v in list(srcs.items())[:7]}
