@@ -92,12 +92,16 @@ class _Lib:
             raise AttributeError(name)
         fn = fns[name]
 
+        err = self.dll.tm_last_error
+
         def call(*args):
             st = fn(*args)
             if st < 0:
-                msg = self.dll.tm_last_error().decode(errors="replace")
+                msg = err().decode(errors="replace")
                 raise RuntimeError(f"{name} failed ({st}): {msg}")
             return st
+        # kept as an instance attribute: later lookups skip __getattr__
+        self.__dict__[name] = call
         return call
 
 

@@ -8,6 +8,7 @@
 //   eta <- eta + dt/(2Q) (KE(v) - (3N+1)/2 kT) + dt/(2Q) (KE(vh) - (3N+1)/2 kT)
 //   v <- (vh + a' dt / 2) / (1 + eta dt / 2)
 // FIRE (Bitzek et al., PRL 97, 170201 (2006)); the reference has no FIRE.
+#include "tensormol_amd.h"
 #include "tm_common.h"
 
 namespace {
@@ -181,6 +182,24 @@ __global__ void fire_move_kernel(int N, const int* __restrict__ mol, int mol0,
   for (int d = 0; d < 3; ++d) x[3 * i + d] += sc * dx[d];
 }
 
+// Result staging of a serving step: several device arrays (whole 4-byte
+// words) concatenated into one buffer, one array per blockIdx.y.
+constexpr int PACK_PARTS = 16;
+
+struct PackBatch {
+  tm_pack_part p[PACK_PARTS];
+};
+
+__global__ void pack_copy_kernel(const PackBatch pb, unsigned* __restrict__ dst) {
+  const tm_pack_part& q = pb.p[blockIdx.y];
+  const unsigned* src = static_cast<const unsigned*>(q.src);
+  unsigned* out = dst + q.offset / 4;
+  const long n = q.bytes / 4;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (long)gridDim.x * blockDim.x)
+    out[t] = src[t];
+}
+
 }  // namespace
 
 extern "C" {
@@ -256,6 +275,31 @@ int tm_fire_update(int N, int B, const int* mol, int mol0, const double* dots_su
   TM_LAUNCH_CHECK();
   fire_move_kernel<<<tm_div_up(N, 256), 256, 0, s>>>(N, mol, mol0, mix, state, f, m, v, x, maxstep);
   TM_LAUNCH_CHECK();
+  return TM_OK;
+}
+
+int tm_pack_copy(int nparts, const tm_pack_part* parts, void* dst, void* stream) {
+  TM_REQUIRE(nparts >= 0 && (nparts == 0 || (parts != nullptr && dst != nullptr)),
+             "tm_pack_copy: bad part list");
+  for (int i = 0; i < nparts; ++i)
+    TM_REQUIRE(parts[i].bytes >= 0 && parts[i].offset >= 0 && parts[i].bytes % 4 == 0 &&
+                   parts[i].offset % 4 == 0 && (parts[i].bytes == 0 || parts[i].src != nullptr),
+               "tm_pack_copy: part %d: sizes and offsets must be whole 4-byte words", i);
+  for (int g0 = 0; g0 < nparts;) {
+    PackBatch pb;
+    int n = 0;
+    long most = 0;
+    for (; g0 < nparts && n < PACK_PARTS; ++g0) {
+      if (parts[g0].bytes == 0) continue;
+      pb.p[n++] = parts[g0];
+      most = most > parts[g0].bytes / 4 ? most : parts[g0].bytes / 4;
+    }
+    if (n == 0) continue;
+    const int blocks = (int)(most > 256 * 255 ? 256 : tm_div_up(most, 256));
+    pack_copy_kernel<<<dim3(blocks, n), 256, 0, (hipStream_t)stream>>>(
+        pb, static_cast<unsigned*>(dst));
+    TM_LAUNCH_CHECK();
+  }
   return TM_OK;
 }
 

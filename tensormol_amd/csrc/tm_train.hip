@@ -546,22 +546,6 @@ __global__ void fm_loss_sum_kernel(int B, const double* __restrict__ part, doubl
   if (threadIdx.x == 0) L[0] = red[0];
 }
 
-// Concatenation of several device arrays (whole 4-byte words) into one
-// buffer, one array per blockIdx.y.
-struct PackBatch {
-  tm_pack_part p[MAXG];
-};
-
-__global__ void pack_copy_kernel(const PackBatch pb, unsigned* __restrict__ dst) {
-  const tm_pack_part& q = pb.p[blockIdx.y];
-  const unsigned* src = static_cast<const unsigned*>(q.src);
-  unsigned* out = dst + q.offset / 4;
-  const long n = q.bytes / 4;
-  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-       t += (long)gridDim.x * blockDim.x)
-    out[t] = src[t];
-}
-
 }  // namespace
 
 extern "C" {
@@ -717,31 +701,6 @@ int tm_rows_grouped(int ngroups, const tm_rows_group* groups, int scatter, void*
     }
     if (n == 0) continue;
     rows_grouped_kernel<<<dim3(tm_div_up(most, 4), n), 256, 0, (hipStream_t)stream>>>(rb);
-    TM_LAUNCH_CHECK();
-  }
-  return TM_OK;
-}
-
-int tm_pack_copy(int nparts, const tm_pack_part* parts, void* dst, void* stream) {
-  TM_REQUIRE(nparts >= 0 && (nparts == 0 || (parts != nullptr && dst != nullptr)),
-             "tm_pack_copy: bad part list");
-  for (int i = 0; i < nparts; ++i)
-    TM_REQUIRE(parts[i].bytes >= 0 && parts[i].offset >= 0 && parts[i].bytes % 4 == 0 &&
-                   parts[i].offset % 4 == 0 && (parts[i].bytes == 0 || parts[i].src != nullptr),
-               "tm_pack_copy: part %d: sizes and offsets must be whole 4-byte words", i);
-  for (int g0 = 0; g0 < nparts;) {
-    PackBatch pb;
-    int n = 0;
-    long most = 0;
-    for (; g0 < nparts && n < MAXG; ++g0) {
-      if (parts[g0].bytes == 0) continue;
-      pb.p[n++] = parts[g0];
-      most = std::max(most, parts[g0].bytes / 4);
-    }
-    if (n == 0) continue;
-    const int blocks = (int)std::min<long>(tm_div_up(most, 256), 256);
-    pack_copy_kernel<<<dim3(blocks, n), 256, 0, (hipStream_t)stream>>>(
-        pb, static_cast<unsigned*>(dst));
     TM_LAUNCH_CHECK();
   }
   return TM_OK;
