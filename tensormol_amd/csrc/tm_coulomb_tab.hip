@@ -31,7 +31,11 @@ namespace {
 
 constexpr double CL_BOHRPERA = 1.889725989;
 constexpr int CT_WAVES = 4;
-constexpr int CT_RANGES = 480;   // as tm_coulomb: 15 A stencil over ~2.3 A cells
+// Candidate ranges per wave: CT_RANGES covers every grid of cells >= 2.3 A at
+// a 15 A cutoff (at most 386 ranges); CT_RANGES_WIDE (tm_coulomb's bound) wider
+// stencils.  The smaller LDS footprint fits one more workgroup per CU.
+constexpr int CT_RANGES = 392;
+constexpr int CT_RANGES_WIDE = 480;
 constexpr int CT_QCAP = 128;     // compacted pair queue per wave
 constexpr int CL_TABLE_MAX = 3072;   // float4 table entries (48 KiB)
 
@@ -76,6 +80,7 @@ __device__ __forceinline__ void ct_pair(const ClArgs& a, const float4* __restric
   acc.gz = fmaf(sc, d.z, acc.gz);
 }
 
+template <int RCAP>
 __global__ void __launch_bounds__(CT_WAVES * 64)
 coulomb_tab_kernel(int N, const float4* __restrict__ xqc, const int* __restrict__ elc,
                    const double* __restrict__ box, CellGrid grid, ClArgs a,
@@ -84,7 +89,7 @@ coulomb_tab_kernel(int N, const float4* __restrict__ xqc, const int* __restrict_
                    int* __restrict__ flags) {
   __shared__ float4 qx[CT_WAVES][CT_QCAP];
   __shared__ int qe[CT_WAVES][CT_QCAP];
-  __shared__ RangeLds<CT_RANGES> rng[CT_WAVES];
+  __shared__ RangeLds<RCAP> rng[CT_WAVES];
   __shared__ int voff[TM_MAX_ELE * TM_MAX_ELE];
   if (threadIdx.x < TM_MAX_ELE * TM_MAX_ELE) {
     const int ea = threadIdx.x / TM_MAX_ELE, eb = threadIdx.x % TM_MAX_ELE;
@@ -285,14 +290,13 @@ int tm_coulomb_tab(int N, const double* box, const int* cell_start, const int* c
              "tm_coulomb_tab: bad grid");
   TM_REQUIRE(inv_h > 0.0 && u0 >= 0.0 && u0 < 1.0 && (rc_ee * inv_h + u0) < nint,
              "tm_coulomb_tab: table scale (%g, %g) does not cover the cutoff", inv_h, u0);
-  {
-    const double cs[3] = {csx, csy, csz};
-    const int nc[3] = {nx, ny, nz};
-    TM_REQUIRE(tm_range_bound(rc_ee, cs, nc, periodic) <= CT_RANGES,
-               "tm_coulomb_tab: cells of %.2f A are too small for a %.1f A cutoff (a sweep "
-               "could need %d candidate ranges, room for %d)", fmin(csx, fmin(csy, csz)), rc_ee,
-               tm_range_bound(rc_ee, cs, nc, periodic), CT_RANGES);
-  }
+  const double cs[3] = {csx, csy, csz};
+  const int nc[3] = {nx, ny, nz};
+  const int need = tm_range_bound(rc_ee, cs, nc, periodic);
+  TM_REQUIRE(need <= CT_RANGES_WIDE,
+             "tm_coulomb_tab: cells of %.2f A are too small for a %.1f A cutoff (a sweep "
+             "could need %d candidate ranges, room for %d)", fmin(csx, fmin(csy, csz)), rc_ee,
+             need, CT_RANGES_WIDE);
   if (N == 0) return TM_OK;
   ClArgs a;
   a.rc = (float)rc_ee;
@@ -308,8 +312,13 @@ int tm_coulomb_tab(int N, const double* box, const int* cell_start, const int* c
   g.cs[0] = (float)csx; g.cs[1] = (float)csy; g.cs[2] = (float)csz;
   g.org[0] = (float)ox; g.org[1] = (float)oy; g.org[2] = (float)oz;
   g.periodic = periodic;
-  coulomb_tab_kernel<<<tm_div_up(N, CT_WAVES), CT_WAVES * 64, 0, (hipStream_t)stream>>>(
-      N, (const float4*)xqc, elc, box, g, a, (const float4*)table, phi, grad, eatom, flags);
+  const dim3 grid(tm_div_up(N, CT_WAVES)), block(CT_WAVES * 64);
+  if (need <= CT_RANGES)
+    coulomb_tab_kernel<CT_RANGES><<<grid, block, 0, (hipStream_t)stream>>>(
+        N, (const float4*)xqc, elc, box, g, a, (const float4*)table, phi, grad, eatom, flags);
+  else
+    coulomb_tab_kernel<CT_RANGES_WIDE><<<grid, block, 0, (hipStream_t)stream>>>(
+        N, (const float4*)xqc, elc, box, g, a, (const float4*)table, phi, grad, eatom, flags);
   TM_LAUNCH_CHECK();
   return TM_OK;
 }
