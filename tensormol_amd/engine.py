@@ -149,6 +149,7 @@ class BPEEEngine:
         dev = self.device
         ws = Workspace()
         ws.ncap, ws.bcap = int(ncap), int(bcap)
+        ws.epoch, ws.argc = 0, {}   # layout version and the launch arguments cached for it
         N, B = max(1, ws.ncap), max(1, ws.bcap)
         i32 = dict(dtype=torch.int32, device=dev)
         f32 = dict(dtype=torch.float32, device=dev)
@@ -261,6 +262,9 @@ class BPEEEngine:
                 ws.mlp_ws = torch.zeros(need, dtype=torch.float32, device=dev)
         else:
             ws.mlp_ws = self.mfma.workspace(ws, up)
+        # every cached launch argument list of this workspace is stale now
+        ws.epoch = getattr(ws, "epoch", 0) + 1
+        ws.argc = {}
         return ws
 
     def _setup_grid(self, ws, pos0, margin):
@@ -304,6 +308,22 @@ class BPEEEngine:
                 _P(ws.molc), n[0], n[1], n[2], c[0], c[1], c[2], o[0], o[1], o[2])
 
     # ---------------------------------------------------------- stages
+    def _call(self, ws, name, key, build):
+        """Call entry point `name` with the argument list build() returns.
+        The list (pointers already converted) is kept on the workspace and
+        reused while the workspace's layout and `key` (whatever else the
+        arguments depend on: positions, stream, settings) are unchanged."""
+        k = (ws.epoch, key)
+        e = ws.argc.get(name)
+        if e is None or e[0] != k:
+            e = (k, tuple(build()))
+            ws.argc[name] = e
+        getattr(self.lib, name.split("#")[0])(*e[1])
+
+    def _raw_stream(self):
+        idx = self._dev_index if self._dev_index is not None else torch.cuda.current_device()
+        return torch._C._cuda_getCurrentRawStream(idx)
+
     def _stream(self):
         """The current stream of the engine's device as a ctypes pointer (read
         without resolving a device object: it is asked for at every launch)."""
@@ -315,21 +335,23 @@ class BPEEEngine:
         (row overflow, non-finite coordinates) are cleared first unless
         clear_flags is False: steps captured together into one graph then
         accumulate them, and a check after the replay sees every step's."""
-        L, st = self.lib, self._stream()
+        key = (pos.data_ptr(), self._raw_stream())
         ws.rev_ok = False   # the mirror slots are filled by the next descriptor forward
         if clear_flags:
-            L.tm_zero_i32(8, _P(ws.flags), st)
+            self._call(ws, "tm_zero_i32", key, lambda: (8, _P(ws.flags), self._stream()))
         if ws.use_cells:
-            o, c, n = ws.grid_origin, ws.grid_cs, ws.grid_nc
-            L.tm_cell_build(ws.N, _P(pos), _P(ws.mol), o[0], o[1], o[2], c[0], c[1], c[2], n[0], n[1],
-                            n[2], ws.periodic, _P(ws.cell_of), _P(ws.cell_count), _P(ws.cell_start),
-                            _P(ws.cell_fill), _P(ws.cell_atoms), _P(ws.cell_pos), _P(ws.posc),
-                            _P(ws.molc), _P(ws.flags), st)
-        g = self._grid_args(ws)
-        L.tm_nl_build(ws.N, _P(pos), _P(ws.elem), _P(ws.mol), _P(ws.mol_start), _P(ws.box), *g,
-                      ws.periodic, self.nele, self.model.sym.rr, self.s_r, self.s_a, self.ldr,
-                      self.lda, _P(ws.nbr), _P(ws.ncnt), _P(ws.rseg), _P(ws.ang), _P(ws.acnt),
-                      _P(ws.aseg), None, _P(ws.flags), st)   # rev: filled by tm_sym_forward
+            def cell_args():
+                o, c, n = ws.grid_origin, ws.grid_cs, ws.grid_nc
+                return (ws.N, _P(pos), _P(ws.mol), o[0], o[1], o[2], c[0], c[1], c[2], n[0], n[1],
+                        n[2], ws.periodic, _P(ws.cell_of), _P(ws.cell_count), _P(ws.cell_start),
+                        _P(ws.cell_fill), _P(ws.cell_atoms), _P(ws.cell_pos), _P(ws.posc),
+                        _P(ws.molc), _P(ws.flags), self._stream())
+            self._call(ws, "tm_cell_build", key, cell_args)
+        self._call(ws, "tm_nl_build", key, lambda: (
+            ws.N, _P(pos), _P(ws.elem), _P(ws.mol), _P(ws.mol_start), _P(ws.box),
+            *self._grid_args(ws), ws.periodic, self.nele, self.model.sym.rr, self.s_r, self.s_a,
+            self.ldr, self.lda, _P(ws.nbr), _P(ws.ncnt), _P(ws.rseg), _P(ws.ang), _P(ws.acnt),
+            _P(ws.aseg), None, _P(ws.flags), self._stream()))   # rev: filled by tm_sym_forward
 
     def check_flags(self, ws):
         """Read the status flags (a synchronisation) and raise on any overflow
@@ -346,12 +368,12 @@ class BPEEEngine:
 
     def descriptors(self, ws, pos):
         sym = self.model.sym
-        self.lib.tm_sym_forward(ws.N, _P(pos), _P(ws.box), _P(ws.mol), _P(ws.elem), _P(ws.nbr),
-                                _P(ws.ncnt), _P(ws.rseg), _P(ws.ang), _P(ws.acnt), _P(ws.aseg),
-                                self.ldr, self.lda, self.nele, sym.nr_r, sym.eta, sym.zeta, sym.rr,
-                                sym.ra, _lib.fptr(self.sym_rs_r), _lib.fptr(self.sym_rs_a),
-                                _lib.fptr(self.sym_th), _P(ws.G), self.D, _P(ws.rev),
-                                _P(ws.flags), self._stream())
+        self._call(ws, "tm_sym_forward", (pos.data_ptr(), self._raw_stream()), lambda: (
+            ws.N, _P(pos), _P(ws.box), _P(ws.mol), _P(ws.elem), _P(ws.nbr), _P(ws.ncnt),
+            _P(ws.rseg), _P(ws.ang), _P(ws.acnt), _P(ws.aseg), self.ldr, self.lda, self.nele,
+            sym.nr_r, sym.eta, sym.zeta, sym.rr, sym.ra, _lib.fptr(self.sym_rs_r),
+            _lib.fptr(self.sym_rs_a), _lib.fptr(self.sym_th), _P(ws.G), self.D, _P(ws.rev),
+            _P(ws.flags), self._stream()))
         ws.rev_ok = True
 
     def networks(self, ws):
@@ -394,8 +416,8 @@ class BPEEEngine:
         molecule means of the potential.  timer: optional pair of callables run
         on the host just before and just after the sweep's launch is queued
         (to record timing events on the launch stream)."""
-        L, st = self.lib, self._stream()
-        g = self._grid_args(ws)
+        key = (pos.data_ptr(), self._raw_stream())
+        st = self._stream
         # One system: the mean of the raw charges is formed inside the charges
         # launch (each block sums it).  A batch of many small molecules takes
         # one block per molecule in a reduction launch first instead, since a
@@ -404,36 +426,45 @@ class BPEEEngine:
         if ws.B == 1:
             ms = ws.mol_start
         else:
-            L.tm_mol_sum_f32(ws.B, _P(ws.mol_start), 1, _P(ws.qraw), 1, _P(ws.qmean), 1,
-                             _P(ws.red), ws.N, st)
+            self._call(ws, "tm_mol_sum_f32", key, lambda: (
+                ws.B, _P(ws.mol_start), 1, _P(ws.qraw), 1, _P(ws.qmean), 1, _P(ws.red), ws.N, st()))
         if ws.use_cells:   # charges written straight into cell order for the sweep
-            L.tm_charges(ws.N, _P(ws.mol), _P(ms), _P(ws.qmean), _P(ws.qraw), _P(pos),
-                         None, _P(ws.q), _P(ws.qx), _P(ws.cell_pos), _P(ws.elem), _P(ws.xqc),
-                         _P(ws.elc), st)
+            self._call(ws, "tm_charges", key, lambda: (
+                ws.N, _P(ws.mol), _P(ms), _P(ws.qmean), _P(ws.qraw), _P(pos), None, _P(ws.q),
+                _P(ws.qx), _P(ws.cell_pos), _P(ws.elem), _P(ws.xqc), _P(ws.elc), st()))
         else:
-            L.tm_charges(ws.N, _P(ws.mol), _P(ms), _P(ws.qmean), _P(ws.qraw), _P(pos),
-                         _P(ws.xq), _P(ws.q), _P(ws.qx), None, None, None, None, st)
+            self._call(ws, "tm_charges", key, lambda: (
+                ws.N, _P(ws.mol), _P(ms), _P(ws.qmean), _P(ws.qraw), _P(pos), _P(ws.xq), _P(ws.q),
+                _P(ws.qx), None, None, None, None, st()))
         table = self._cl_table() if (ws.use_cells and self.coulomb_tables) else None
         if timer is not None:
             timer[0]()
         if table is not None:
-            L.tm_coulomb_tab(ws.N, _P(ws.box), g[0], g[1], g[2], g[6], g[7], g[8], g[9], g[10],
-                                 g[11], g[12], g[13], g[14], ws.periodic, self.model.ee_off,
-                                 self.nele, _P(table[0]), self.cl_nint, table[1], table[2],
-                                 _P(ws.xqc), _P(ws.elc), _P(ws.phi), _P(ws.gradc), _P(ws.eatom),
-                                 _P(ws.flags), st)
+            def tab_args():
+                g = self._grid_args(ws)
+                return (ws.N, _P(ws.box), g[0], g[1], g[2], g[6], g[7], g[8], g[9], g[10], g[11],
+                        g[12], g[13], g[14], ws.periodic, self.model.ee_off, self.nele,
+                        _P(table[0]), self.cl_nint, table[1], table[2], _P(ws.xqc), _P(ws.elc),
+                        _P(ws.phi), _P(ws.gradc), _P(ws.eatom), _P(ws.flags), st())
+            self._call(ws, "tm_coulomb_tab", key + (table[0].data_ptr(),), tab_args)
         else:
-            L.tm_coulomb(ws.N, None if ws.use_cells else _P(ws.xq), _P(ws.elem), _P(ws.mol), _P(ws.mol_start), _P(ws.box),
-                         g[0], g[1], g[2], g[3], g[5], g[6], g[7], g[8], g[9], g[10], g[11],
-                         g[12], g[13], g[14], ws.periodic, self.dsf.ctypes.data_as(_lib.P),
-                         self.model.ee_off, self.vdw_width,
-                         int(bool(PARAMS["AddEcc"])), int(self.model.has_vdw), self.nele,
-                         self.c6.ctypes.data_as(_lib.P), self.r0.ctypes.data_as(_lib.P),
-                         _P(ws.xqc), _P(ws.elc), _P(ws.phi), _P(ws.gradc), _P(ws.eatom),
-                         _P(ws.flags), st)
+            add_ecc = int(bool(PARAMS["AddEcc"]))
+
+            def coul_args():
+                g = self._grid_args(ws)
+                return (ws.N, None if ws.use_cells else _P(ws.xq), _P(ws.elem), _P(ws.mol),
+                        _P(ws.mol_start), _P(ws.box), g[0], g[1], g[2], g[3], g[5], g[6], g[7],
+                        g[8], g[9], g[10], g[11], g[12], g[13], g[14], ws.periodic,
+                        self.dsf.ctypes.data_as(_lib.P), self.model.ee_off, self.vdw_width,
+                        add_ecc, int(self.model.has_vdw), self.nele,
+                        self.c6.ctypes.data_as(_lib.P), self.r0.ctypes.data_as(_lib.P),
+                        _P(ws.xqc), _P(ws.elc), _P(ws.phi), _P(ws.gradc), _P(ws.eatom),
+                        _P(ws.flags), st())
+            self._call(ws, "tm_coulomb", key + (add_ecc,), coul_args)
         if timer is not None:
             timer[1]()
-        L.tm_mol_sum_f64(ws.B, _P(ws.mol_start), 1, _P(ws.phi), 1, _P(ws.phimean), 1, _P(ws.red), ws.N, st)
+        self._call(ws, "tm_mol_sum_f64", key, lambda: (
+            ws.B, _P(ws.mol_start), 1, _P(ws.phi), 1, _P(ws.phimean), 1, _P(ws.red), ws.N, st()))
 
     def _need_rev(self, ws, pos):
         """The force gather reads the mirror slots of the current lists, which
@@ -447,22 +478,26 @@ class BPEEEngine:
         state in the same launch as the gather: the kick of this step with the
         new forces, then the drift of the next (x is updated in place)."""
         self._need_rev(ws, pos)
-        L, st = self.lib, self._stream()
+        key = (pos.data_ptr(), self._raw_stream())
+        st = self._stream
         sym = self.model.sym
-        L.tm_sym_adjoint(ws.N, _P(pos), _P(ws.box), _P(ws.mol), _P(ws.elem), _P(ws.nbr),
-                         _P(ws.ncnt), _P(ws.rseg), _P(ws.ang), _P(ws.acnt), _P(ws.aseg), self.ldr,
-                         self.lda, self.nele, sym.nr_r, sym.eta, sym.zeta, sym.rr, sym.ra,
-                         _lib.fptr(self.sym_rs_r), _lib.fptr(self.sym_rs_a), _lib.fptr(self.sym_th),
-                         _P(ws.JE), _P(ws.JQ), None, _P(ws.phi), _P(ws.phimean), self.D,
-                         _P(ws.wslot), st)
+        self._call(ws, "tm_sym_adjoint#energy", key, lambda: (
+            ws.N, _P(pos), _P(ws.box), _P(ws.mol), _P(ws.elem), _P(ws.nbr), _P(ws.ncnt),
+            _P(ws.rseg), _P(ws.ang), _P(ws.acnt), _P(ws.aseg), self.ldr, self.lda, self.nele,
+            sym.nr_r, sym.eta, sym.zeta, sym.rr, sym.ra, _lib.fptr(self.sym_rs_r),
+            _lib.fptr(self.sym_rs_a), _lib.fptr(self.sym_th), _P(ws.JE), _P(ws.JQ), None,
+            _P(ws.phi), _P(ws.phimean), self.D, _P(ws.wslot), st()))
         if vv is None:
-            L.tm_force_gather(ws.N, self.ldr, _P(ws.ncnt), _P(ws.rev), _P(ws.wslot), _P(ws.gradc),
-                              JOULEPERHARTREE, _P(ws.grad), _P(ws.force), st)
+            self._call(ws, "tm_force_gather#energy", key, lambda: (
+                ws.N, self.ldr, _P(ws.ncnt), _P(ws.rev), _P(ws.wslot), _P(ws.gradc),
+                JOULEPERHARTREE, _P(ws.grad), _P(ws.force), st()))
         else:
             dt, m, v, a, ke, x = vv
-            L.tm_force_gather_vv(ws.N, self.ldr, _P(ws.ncnt), _P(ws.rev), _P(ws.wslot),
-                                 _P(ws.gradc), JOULEPERHARTREE, _P(ws.grad), _P(ws.force),
-                                 float(dt), _P(m), _P(v), _P(a), _P(ke), _P(x), st)
+            vkey = key + (float(dt),) + tuple(0 if t is None else t.data_ptr() for t in (m, v, a, ke, x))
+            self._call(ws, "tm_force_gather_vv", vkey, lambda: (
+                ws.N, self.ldr, _P(ws.ncnt), _P(ws.rev), _P(ws.wslot), _P(ws.gradc),
+                JOULEPERHARTREE, _P(ws.grad), _P(ws.force), float(dt), _P(m), _P(v), _P(a),
+                _P(ke), _P(x), st()))
 
     def descriptor_jvp(self, ws, pos, du, out):
         """out[:N, :D] = dG/dx . du (float32) for per-atom displacements du
