@@ -9,7 +9,8 @@ J/mol/A out).  The list arguments of the reference's positional batch
 protocol are accepted and ignored: the engine rebuilds them on the device.
 
 Weights are the repository's own format (BPEEModel.save/load) found as
-<networks_directory>/<Name>.npz; an unknown name gets seeded weights.
+<networks_directory>/<Name>.npz, else the last TF checkpoint the reference saved under
+<networks_directory>/<Name>/ (weights only); an unknown name gets seeded weights.
 """
 import os
 
@@ -155,9 +156,20 @@ class TFMolManage:
         self.NetType = NetType_
         self.TData = TData_
         path = os.path.join(PARAMS["networks_directory"], f"{Name_}.npz") if Name_ else None
+        chk = None
+        if model is None and Name_ and not (path and os.path.exists(path)):
+            # a network trained by the reference: its last TF checkpoint
+            # (TFInstance.py:94, 246-258), weights only
+            from .tfbundle import find_last_checkpoint
+            chk = find_last_checkpoint(os.path.join(PARAMS["networks_directory"], Name_), Name_)
         if model is None:
             if path and os.path.exists(path):
                 model = BPEEModel.load(path)
+            elif chk is not None:
+                from .tfbundle import model_from_checkpoint
+                model = model_from_checkpoint(chk, TData_.eles if TData_ is not None else None,
+                                              activation_alpha=_activation_alpha(),
+                                              dsf_alpha=PARAMS["DSFAlpha"])
             else:
                 eles = TData_.eles if TData_ is not None else [1, 6, 7, 8]
                 model = BPEEModel(eles, activation_alpha=_activation_alpha(),
