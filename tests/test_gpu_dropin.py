@@ -448,3 +448,46 @@ def test_device_fire_vs_oracle(cuda):
         print(f"FIRE molecule {k}: |dx| {dx:.2e} A, dt gpu {st[k, 0]:.4f} oracle {states[k][0]:.4f}")
         assert dx < 1e-5
         assert abs(st[k, 0] - states[k][0]) < 1e-12 and st[k, 2] == states[k][2]
+
+
+def test_manager_evaluates_reference_checkpoint(cuda, tmp_path):
+    """A network saved by the reference as a TF checkpoint (<networks_directory>/<Name>/
+    <Name>-chk-<step>, variable names of TFMolInstanceDirect.py:5179-5264) is found by
+    TFMolManage(Name) and evaluates as the same weights do: bit-identical to a manager
+    holding the source model, and within the FP32 gates of the oracle."""
+    from tensormol_amd import tfbundle as TB
+    from tensormol_amd.containers import Mol
+    from tensormol_amd.manage import TFMolManage, BPEEInstance
+    from tensormol_amd.params import PARAMS
+    from tests import tfbundle_writer as TW
+    src = H.model_for("water")
+    tensors = {}
+    for Zel in src.eles:
+        for nets, names in zip((src.energy_nets, src.charge_nets),
+                               TB.bpee_variable_names(Zel, len(src.hidden))):
+            for (Wm, b), (wn, bn) in zip(nets[Zel], names):
+                tensors[wn], tensors[bn] = Wm, b
+                tensors[wn + "/Adam"] = np.zeros_like(Wm)
+    d = tmp_path / "water_ref"
+    d.mkdir()
+    TW.write_bundle(str(d / "water_ref-chk-300"), tensors, block_entries=7)
+    nettype = "fc_sqdiff_BP_Direct_EE_ChargeEncode_Update_vdw_DSF_elu_Normalize_Dropout"
+    saved = {k: PARAMS[k] for k in ("networks_directory", "NeuronType", "DSFAlpha")}
+    PARAMS.update(networks_directory=str(tmp_path), NeuronType="softplus", DSFAlpha=src.dsf_alpha)
+    try:
+        mgr = TFMolManage("water_ref", None, False, nettype, False, False)
+    finally:
+        PARAMS.update(saved)
+    assert mgr.model.eles == src.eles and mgr.model.hidden == src.hidden
+    mgr.Instances = BPEEInstance(mgr.model, precision="fp32")
+    ref_mgr = TFMolManage("", None, False, nettype, False, False, model=src)
+    ref_mgr.Instances = BPEEInstance(src, precision="fp32")
+    Z, x = H.system("water10")[:2]
+    out = mgr.EvalBPDirectEEUpdateSingle(Mol(Z, x), 4.6, 3.1, 15.0, True)
+    same = ref_mgr.EvalBPDirectEEUpdateSingle(Mol(Z, x), 4.6, 3.1, 15.0, True)
+    for a, b in zip(out, same):
+        np.testing.assert_array_equal(a, b)
+    sp = O.SymParams(src.eles)
+    ref = O.evaluate(x, Z, np.array([0, len(Z)]), sp, src.energy_nets, src.charge_nets,
+                     alpha=src.alpha, dsf_alpha=src.dsf_alpha, has_vdw=src.has_vdw)
+    assert abs(out[0][0] - ref["Etotal"][0]) < FP32_E_TOL * len(Z)
