@@ -471,6 +471,34 @@ def coulomb_pairs(x, box, rc=15.0):
     return (int(tree.count_neighbors(tree, rc)) - len(x)) // 2
 
 
+def serving_fp32_baselines(args, model, mk, stream, k, log):
+    """Plain FP32 serving baselines: the continuous-batching server on the FP32
+    path (saturated) and one molecule per launch on the first k requests;
+    returns both rates in atom-steps/s."""
+    import torch
+    from tensormol_amd.engine import BPEEEngine
+    from tensormol_amd.serving import Server, atom_steps
+    # plain FP32 path: continuous batching (separates the batching and MFMA gains) ...
+    eng32 = BPEEEngine(model, precision="fp32", lda=128, check_overflow=False)
+    srv32 = Server(eng32, max_atoms=args.serve_atoms, max_mols=8192,
+                   hbm_budget_bytes=args.serve_hbm_gib * 2 ** 30)
+    srv32.run(mk(), realtime=False)   # untimed first pass, as for the BF16 server
+    torch.cuda.synchronize()
+    d32b = srv32.run(mk(), realtime=False)
+    batched32 = atom_steps(d32b) / srv32.wall
+    log(f"batched fp32 saturated: {srv32.wall:.2f} s, {batched32:.4g} atom-steps/s")
+    del srv32
+    # ... and one molecule per launch
+    one = Server(eng32, max_atoms=max(len(z) for z, *_ in stream), max_mols=1)
+    one.run(mk(4), realtime=False)
+    torch.cuda.synchronize()
+    d32 = one.run(mk(k), realtime=False)
+    base_rate = atom_steps(d32) / one.wall
+    log(f"launch-per-molecule fp32: {k} requests, {atom_steps(d32)} atom-steps in {one.wall:.2f} s, "
+        f"{base_rate:.4g} atom-steps/s")
+    return batched32, base_rate
+
+
 def run_serving(args, stream):
     """Serve the request stream: realtime arrivals (latency), all requests at
     t = 0 (saturated throughput), and the plain-FP32 path one molecule per
@@ -518,25 +546,12 @@ def run_serving(args, stream):
                          model.energy_nets, model.charge_nets, alpha=model.alpha,
                          dsf_alpha=model.dsf_alpha, has_vdw=model.has_vdw)
         errs.append(float(np.sqrt(((r.forces / 2625499.638 + ref["gradient"]) ** 2).mean())))
-    # plain FP32 path: continuous batching (separates the batching and MFMA gains) ...
-    eng32 = BPEEEngine(model, precision="fp32", lda=128, check_overflow=False)
-    srv32 = Server(eng32, max_atoms=args.serve_atoms, max_mols=8192,
-                   hbm_budget_bytes=args.serve_hbm_gib * 2 ** 30)
-    srv32.run(mk(), realtime=False)   # untimed first pass, as for the BF16 server
-    torch.cuda.synchronize()
-    d32b = srv32.run(mk(), realtime=False)
-    batched32 = atom_steps(d32b) / srv32.wall
-    log(f"batched fp32 saturated: {srv32.wall:.2f} s, {batched32:.4g} atom-steps/s")
-    del srv32
-    # ... and one molecule per launch
-    one = Server(eng32, max_atoms=max(len(z) for z, *_ in stream), max_mols=1)
-    one.run(mk(4), realtime=False)
-    torch.cuda.synchronize()
     k = args.serve_baseline_requests
-    d32 = one.run(mk(k), realtime=False)
-    base_rate = atom_steps(d32) / one.wall
-    log(f"launch-per-molecule fp32: {k} requests, {atom_steps(d32)} atom-steps in {one.wall:.2f} s, "
-        f"{base_rate:.4g} atom-steps/s")
+    batched32 = base_rate = None
+    if args.serve_no_baselines:
+        k = 0
+    else:
+        batched32, base_rate = serving_fp32_baselines(args, model, mk, stream, k, log)
     natoms = np.array([len(z) for z, *_ in stream])
     failed = {}
     for r in done:
@@ -551,11 +566,11 @@ def run_serving(args, stream):
         "stream_atom_steps_per_s": round(rt_rate, 1), "stream_wall_s": round(rt_wall, 3),
         "stream_iterations": rt_iters,
         "saturated_atom_steps_per_s": round(sat_rate, 1),
-        "launch_per_molecule_fp32_atom_steps_per_s": round(base_rate, 1),
-        "batched_fp32_saturated_atom_steps_per_s": round(batched32, 1),
-        "baseline_sample": f"first {k} requests of the stream, served one molecule per launch "
-                           "by the plain FP32 path",
-        "speedup_vs_launch_per_molecule_fp32": round(sat_rate / base_rate, 2),
+        "launch_per_molecule_fp32_atom_steps_per_s": None if base_rate is None else round(base_rate, 1),
+        "batched_fp32_saturated_atom_steps_per_s": None if batched32 is None else round(batched32, 1),
+        "baseline_sample": (f"first {k} requests of the stream, served one molecule per launch "
+                            "by the plain FP32 path") if base_rate is not None else None,
+        "speedup_vs_launch_per_molecule_fp32": None if base_rate is None else round(sat_rate / base_rate, 2),
         "repack_every": args.serve_repack_every,
         "capacity_atoms": srv.max_atoms,
         "failed_requests_by_kind": failed,
@@ -585,6 +600,8 @@ def main():
     ap.add_argument("--serve-hbm-gib", type=float, default=16.0)
     ap.add_argument("--serve-baseline-requests", type=int, default=128)
     ap.add_argument("--serve-check", type=int, default=8)
+    ap.add_argument("--serve-no-baselines", action="store_true",
+                    help="skip the plain FP32 serving baselines (e.g. to trace the server alone)")
     ap.add_argument("--serve-repack-every", type=int, default=4,
                     help="repack a changed batch at most every K iterations (ghosts between); "
                          "4 measured best for p99 at 2000 req/s")
